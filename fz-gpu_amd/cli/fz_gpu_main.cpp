@@ -1,0 +1,233 @@
+// fz-gpu -- command line drop-in for the reference's `fz-gpu <file> <x> <y> <z> <eb>`
+// (fz.cu:226-237).  With the five positional arguments it prints exactly the three lines
+// of fz.cu:207-209.  Extra behaviour sits behind trailing options:
+//   -o <archive>   write the compressed archive (header + payload + tail)
+//   --literal      byte-identical reference mode (1D sign loss and dropped tail included)
+//   --plain        run the plain per-stage pipeline instead of the fused kernel
+//   --json         print one JSON line of statistics after the three lines
+// and a decompress mode:
+//   fz-gpu -d <archive> <out.f32>
+#include <hip/hip_runtime.h>
+#include <sys/stat.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fz_archive.hpp"
+#include "fz_capi.h"
+
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) {                                                                \
+            std::fprintf(stderr, "fz-gpu: %s failed: %s\n", #x, hipGetErrorString(e_));        \
+            return 2;                                                                          \
+        }                                                                                      \
+    } while (0)
+
+#define FZCHK(x)                                                                               \
+    do {                                                                                       \
+        fz_status s_ = (x);                                                                    \
+        if (s_ != FZ_SUCCESS) {                                                                \
+            std::fprintf(stderr, "fz-gpu: %s: %s\n", #x, fz_status_string(s_));               \
+            return 3;                                                                          \
+        }                                                                                      \
+    } while (0)
+
+static long file_size(const std::string& name)  // fz.cu:19-24
+{
+    struct stat st;
+    return stat(name.c_str(), &st) == 0 ? (long)st.st_size : -1;
+}
+
+static int usage()
+{
+    std::fprintf(stderr,
+                 "usage: fz-gpu <file> <x> <y> <z> <eb> [-o archive] [--literal] [--plain] [--json]\n"
+                 "       fz-gpu -d <archive> <out.f32>\n");
+    return 1;
+}
+
+static int decompress_main(const std::string& arc, const std::string& out)
+{
+    fzar::Header          h;
+    std::vector<uint8_t>  payload;
+    std::vector<uint16_t> tail;
+    if (!fzar::read_archive(arc, h, payload, tail)) {
+        std::fprintf(stderr, "fz-gpu: cannot read archive %s\n", arc.c_str());
+        return 2;
+    }
+    fz_plan plan;
+    FZCHK(fz_plan_create(&plan, h.x, h.y, h.z, h.len, h.mode));
+    fz_sizes sz;
+    FZCHK(fz_plan_sizes(plan, &sz));
+    const uint64_t n = (uint64_t)h.x * h.y * h.z;
+    uint8_t*       d_payload;
+    uint16_t*      d_tail = nullptr;
+    float*         d_out;
+    void*          d_ws;
+    HIPCHK(hipMalloc(&d_payload, sz.payload_max_bytes + 16));
+    HIPCHK(hipMalloc(&d_out, 4 * n + 16));
+    HIPCHK(hipMalloc(&d_ws, sz.workspace_bytes));
+    if (h.tail_len) {
+        HIPCHK(hipMalloc(&d_tail, 2 * h.tail_len));
+        HIPCHK(hipMemcpy(d_tail, tail.data(), 2 * h.tail_len, hipMemcpyHostToDevice));
+    }
+    if (h.payload_bytes) HIPCHK(hipMemcpy(d_payload, payload.data(), h.payload_bytes, hipMemcpyHostToDevice));
+    hipStream_t st;
+    HIPCHK(hipStreamCreate(&st));
+    FZCHK(fz_workspace_init(plan, d_ws, sz.workspace_bytes, st));
+    FZCHK(fz_decompress_f32(plan, d_payload, d_tail, h.ebx2, d_ws, sz.workspace_bytes, d_out, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<float> host(n);
+    HIPCHK(hipMemcpy(host.data(), d_out, 4 * n, hipMemcpyDeviceToHost));
+    FILE* f = std::fopen(out.c_str(), "wb");
+    if (!f || std::fwrite(host.data(), 4, n, f) != n) {
+        std::fprintf(stderr, "fz-gpu: cannot write %s\n", out.c_str());
+        return 2;
+    }
+    std::fclose(f);
+    (void)hipFree(d_payload);
+    (void)hipFree(d_out);
+    (void)hipFree(d_ws);
+    if (d_tail) (void)hipFree(d_tail);
+    (void)hipStreamDestroy(st);
+    fz_plan_destroy(plan);
+    return 0;
+}
+
+int main(int argc, char* argv[])
+{
+    if (argc >= 2 && std::strcmp(argv[1], "-d") == 0) {
+        if (argc != 4) return usage();
+        return decompress_main(argv[2], argv[3]);
+    }
+    if (argc < 6) return usage();
+    const std::string fileName(argv[1]);
+    const int         x = std::stoi(argv[2]), y = std::stoi(argv[3]), z = std::stoi(argv[4]);
+    const double      eb = std::stod(argv[5]);
+    std::string       out_path;
+    int               mode = FZ_MODE_FIXED;
+    bool              plain = false, json = false;
+    for (int i = 6; i < argc; i++) {
+        if (!std::strcmp(argv[i], "-o") && i + 1 < argc)
+            out_path = argv[++i];
+        else if (!std::strcmp(argv[i], "--literal"))
+            mode = FZ_MODE_LITERAL;
+        else if (!std::strcmp(argv[i], "--plain"))
+            plain = true;
+        else if (!std::strcmp(argv[i], "--json"))
+            json = true;
+        else
+            return usage();
+    }
+
+    // fz.cu:117-118, 136: the float count comes from the file size
+    const long fileSizeL = file_size(fileName);
+    if (fileSizeL < 4) {
+        std::fprintf(stderr, "fail to open %s\n", fileName.c_str());
+        return 1;
+    }
+    const int      fileSize = (int)fileSizeL;
+    const uint64_t len = (uint64_t)fileSizeL / sizeof(float);
+    std::vector<float> h_data(len);
+    {
+        FILE* f = std::fopen(fileName.c_str(), "rb");
+        if (!f || std::fread(h_data.data(), 4, len, f) != len) {
+            std::fprintf(stderr, "fail to open %s\n", fileName.c_str());
+            return 1;
+        }
+        std::fclose(f);
+    }
+
+    fz_plan plan;
+    FZCHK(fz_plan_create(&plan, (uint32_t)x, (uint32_t)y, (uint32_t)z, len, mode));
+    fz_sizes sz;
+    FZCHK(fz_plan_sizes(plan, &sz));
+    float*     d_data;
+    uint8_t*   d_payload;
+    uint16_t*  d_tail;
+    fz_result* d_res;
+    void*      d_ws;
+    HIPCHK(hipMalloc(&d_data, 4 * len + 16));
+    HIPCHK(hipMalloc(&d_payload, sz.payload_max_bytes + 16));
+    HIPCHK(hipMalloc(&d_tail, 2 * 2048));
+    HIPCHK(hipMalloc(&d_res, sizeof(fz_result)));
+    HIPCHK(hipMalloc(&d_ws, sz.workspace_bytes));
+    HIPCHK(hipMemcpy(d_data, h_data.data(), 4 * len, hipMemcpyHostToDevice));
+    hipStream_t st;
+    HIPCHK(hipStreamCreate(&st));
+    FZCHK(fz_workspace_init(plan, d_ws, sz.workspace_bytes, st));
+
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, st));
+    if (plain)
+        FZCHK(fz_compress_plain_f32(plan, d_data, eb, d_ws, sz.workspace_bytes, d_payload, d_tail, d_res, st));
+    else
+        FZCHK(fz_compress_f32(plan, d_data, eb, d_ws, sz.workspace_bytes, d_payload, d_tail, d_res, st));
+    HIPCHK(hipEventRecord(e1, st));
+    fz_result r;
+    HIPCHK(hipMemcpyAsync(&r, d_res, sizeof(r), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    if (r.status != FZ_SUCCESS) {
+        std::fprintf(stderr, "fz-gpu: %s\n", fz_status_string((fz_status)r.status));
+        return 3;
+    }
+
+    // fz.cu:207-209 (the size is 4 W + 16 K)
+    const uint64_t csize = r.payload_bytes;
+    std::printf("original size: %d\n", fileSize);
+    std::printf("compressed size: %d\n", (int)csize);
+    std::printf("compression ratio: %f\n", float(fileSize) / float(csize));
+    if (json)
+        std::printf("{\"len\": %llu, \"K\": %llu, \"payload_bytes\": %llu, \"tail_len\": %llu, \"overflow\": %llu, "
+                    "\"vmin\": %.9g, \"vmax\": %.9g, \"ms\": %.6f, \"input_GBps\": %.3f}\n",
+                    (unsigned long long)len, (unsigned long long)r.nblocks, (unsigned long long)r.payload_bytes,
+                    (unsigned long long)r.tail_len, (unsigned long long)r.overflow, r.vmin, r.vmax, ms,
+                    4.0 * len / (ms * 1e-3) / 1e9);
+
+    if (!out_path.empty()) {
+        std::vector<uint8_t>  payload(r.payload_bytes);
+        std::vector<uint16_t> tail(r.tail_len);
+        HIPCHK(hipMemcpy(payload.data(), d_payload, r.payload_bytes, hipMemcpyDeviceToHost));
+        if (r.tail_len) HIPCHK(hipMemcpy(tail.data(), d_tail, 2 * r.tail_len, hipMemcpyDeviceToHost));
+        fzar::Header h{};
+        std::memcpy(h.magic, "FZMI", 4);
+        h.version = 1;
+        h.mode = (uint8_t)mode;
+        h.ndim = (uint8_t)r.ndim;
+        h.x = (uint32_t)x;
+        h.y = (uint32_t)y;
+        h.z = (uint32_t)z;
+        h.len = len;
+        h.eb_rel = eb;
+        h.vmin = r.vmin;
+        h.vmax = r.vmax;
+        h.ebx2 = r.ebx2;
+        h.payload_bytes = r.payload_bytes;
+        h.tail_len = r.tail_len;
+        h.overflow = r.overflow;
+        if (!fzar::write_archive(out_path, h, payload.data(), tail.data())) {
+            std::fprintf(stderr, "fz-gpu: cannot write %s\n", out_path.c_str());
+            return 2;
+        }
+    }
+    (void)hipFree(d_data);
+    (void)hipFree(d_payload);
+    (void)hipFree(d_tail);
+    (void)hipFree(d_res);
+    (void)hipFree(d_ws);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    fz_plan_destroy(plan);
+    return 0;
+}

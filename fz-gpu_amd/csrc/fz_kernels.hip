@@ -1,0 +1,798 @@
+// fz_kernels.hip -- gfx950 kernels of the FZ path.
+//
+//   k_range               value range + eb parameters (fz.cu:137, 149; lorenzo_var.cuh:562-563)
+//   k_tile_compress       dual-quant -> bitshuffle -> flags -> in-tile compaction, one tile per
+//                         workgroup, no cross-workgroup waiting
+//   k_scan_gather         scan of the tile counts + placement of the staged blocks
+//   k_dualquant           plain Lorenzo stage (lorenzo_var.cuh:176-295)
+//   k_bitshuffle          plain bitshuffle + flags stage (fz.cu:61-111)
+//   k_scan_init / k_scan  exclusive u32 sum, single pass (the cub::DeviceScan call, fz.cu:188-192)
+//   k_encode              compaction of flagged 16-byte blocks (fz.cu:49-59)
+//   k_tile_scan, k_decode_unshuffle, k_reconstruct_*   the inverse
+#include <hip/hip_runtime.h>
+
+#include "fz_internal.hpp"
+
+namespace fz {
+
+// A HIP error left in the thread's last-error slot by earlier, unrelated runtime calls must
+// not be reported as a failure of our launch: clear it before launching, read it after.
+#define FZ_LAUNCH(...)                     \
+    do {                                   \
+        (void)hipGetLastError();           \
+        __VA_ARGS__;                       \
+    } while (0)
+
+// 16-byte global accesses at 4-byte alignment (the encoded blocks follow 4 W bytes of flags)
+typedef uint32_t u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
+
+__device__ __forceinline__ void st16(uint8_t* p, uint4 v)
+{
+    *reinterpret_cast<u32x4a4*>(p) = u32x4a4{v.x, v.y, v.z, v.w};
+}
+__device__ __forceinline__ uint4 ld16(const uint8_t* p)
+{
+    const u32x4a4 t = *reinterpret_cast<const u32x4a4*>(p);
+    return make_uint4(t.x, t.y, t.z, t.w);
+}
+
+__device__ __forceinline__ uint32_t nonfinite(float v) { return (__float_as_uint(v) & 0x7f800000u) == 0x7f800000u; }
+
+// ============================================================================ range
+
+// Per-workgroup min / max / non-finite flag; the last workgroup to finish (atomic count)
+// reduces the partials and writes the eb parameters into the result record:
+//   eb_abs = eb_rel * (double)(max - min);  ebx2 = eb_abs * 2;  ebx2_r = 1 / ebx2  (double),
+//   both narrowed to float.  The workgroups also clear the look-back state of the kernel
+//   that follows (zero_states / zero_ticket).
+__global__ __launch_bounds__(kWG) void k_range(const float* __restrict__ in, Geom g, double eb_rel,
+                                               uint64_t* part, Ctr* ctr, fz_result* res, uint64_t* zero_states,
+                                               uint32_t nzero, uint32_t* zero_ticket)
+{
+    __shared__ float    s_lo[4], s_hi[4];
+    __shared__ uint32_t s_bad[4];
+    __shared__ uint32_t s_last;
+    const uint32_t tid = threadIdx.x, gtid = blockIdx.x * kWG + tid, gstride = gridDim.x * kWG;
+    for (uint32_t i = gtid; i < nzero; i += gstride) zero_states[i] = 0;
+    if (gtid == 0 && zero_ticket) *zero_ticket = 0;
+
+    float    lo = __builtin_huge_valf(), hi = -__builtin_huge_valf();
+    uint32_t bad = 0;
+    const uint32_t n4 = g.len / 4;
+    const float4*  in4 = reinterpret_cast<const float4*>(in);
+    for (uint32_t i = gtid; i < n4; i += gstride) {
+        const float4 v = in4[i];
+        lo = fminf(lo, fminf(fminf(v.x, v.y), fminf(v.z, v.w)));
+        hi = fmaxf(hi, fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w)));
+        bad |= nonfinite(v.x) | nonfinite(v.y) | nonfinite(v.z) | nonfinite(v.w);
+    }
+    for (uint32_t i = 4 * n4 + gtid; i < g.len; i += gstride) {
+        const float v = in[i];
+        lo = fminf(lo, v);
+        hi = fmaxf(hi, v);
+        bad |= nonfinite(v);
+    }
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    bad = ballot64(bad != 0) != 0;
+    const uint32_t wv = tid >> 6;
+    if ((tid & 63) == 0) {
+        s_lo[wv] = lo;
+        s_hi[wv] = hi;
+        s_bad[wv] = bad;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        lo = fminf(fminf(s_lo[0], s_lo[1]), fminf(s_lo[2], s_lo[3]));
+        hi = fmaxf(fmaxf(s_hi[0], s_hi[1]), fmaxf(s_hi[2], s_hi[3]));
+        bad = s_bad[0] | s_bad[1] | s_bad[2] | s_bad[3];
+        const uint64_t packed = (uint64_t)__float_as_uint(lo) | ((uint64_t)__float_as_uint(hi) << 32);
+        __hip_atomic_store(&part[2 * blockIdx.x], packed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&part[2 * blockIdx.x + 1], (uint64_t)bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        s_last = (atomicAdd(&ctr->range_done, 1u) == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    lo = __builtin_huge_valf();
+    hi = -__builtin_huge_valf();
+    bad = 0;
+    for (uint32_t i = tid; i < gridDim.x; i += kWG) {
+        const uint64_t p = __hip_atomic_load(&part[2 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t b = __hip_atomic_load(&part[2 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        lo = fminf(lo, __uint_as_float((uint32_t)p));
+        hi = fmaxf(hi, __uint_as_float((uint32_t)(p >> 32)));
+        bad |= (uint32_t)b;
+    }
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    bad = ballot64(bad != 0) != 0;
+    __syncthreads();
+    if ((tid & 63) == 0) {
+        s_lo[wv] = lo;
+        s_hi[wv] = hi;
+        s_bad[wv] = bad;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        lo = fminf(fminf(s_lo[0], s_lo[1]), fminf(s_lo[2], s_lo[3]));
+        hi = fmaxf(fmaxf(s_hi[0], s_hi[1]), fmaxf(s_hi[2], s_hi[3]));
+        bad = s_bad[0] | s_bad[1] | s_bad[2] | s_bad[3];
+        const float  range = hi - lo;
+        const double eb = eb_rel * (double)range;
+        const double e2 = eb * 2.0;
+        const double e2r = 1.0 / e2;
+        res->status = bad ? ST_NONFINITE : (range == 0.0f ? ST_CONSTANT : ST_SUCCESS);
+        res->ndim = (uint32_t)g.ndim;
+        res->vmin = lo;
+        res->vmax = hi;
+        res->ebx2_r = (float)e2r;
+        res->ebx2 = (float)e2;
+        res->eb_abs = eb;
+        res->nblocks = 0;
+        res->payload_bytes = 0;
+        res->tail_len = g.mode == 0 ? (uint64_t)(g.len - (uint32_t)kTileCodes * g.T) : 0;
+        res->overflow = 0;
+        ctr->range_done = 0;
+    }
+}
+
+hipError_t launch_range(const float* in, const Geom& g, double eb_rel, uint64_t* part, Ctr* ctr, fz_result* res,
+                        uint64_t* zero_states, uint32_t nzero, uint32_t* zero_ticket, hipStream_t s)
+{
+    FZ_LAUNCH(k_range<<<range_grid(g.len), kWG, 0, s>>>(in, g, eb_rel, part, ctr, res, zero_states, nzero, zero_ticket));
+    return hipGetLastError();
+}
+
+// ============================================================================ fused compress
+//
+// Two kernels after the range kernel, no cross-workgroup waiting in the heavy one:
+//   k_tile_compress  one 2048-code tile per workgroup: dual-quant, 32x32 bit transpose, block
+//                    flags (bitflag words go straight to the payload), the tile's nonzero
+//                    blocks compacted in order into its own 4 KB staging slot, its block count;
+//   k_scan_gather    exclusive scan of the tile counts (256 tiles per workgroup, decoupled
+//                    look-back across workgroups) and the copy of every staged block to
+//                    payload + 4 W + 16 * (global rank)  (encodeDebug's placement, fz.cu:56-57).
+
+// Codes of the len - 2048 T positions no tile covers: returned in `tail` in FIXED mode
+// (the reference never encodes them, fz.cu:173).  Also clears the bitflag words of the
+// partial tile (fz.cu:167 memsets them) and finishes the record when there is no tile.
+template <int NDIM>
+__device__ void tail_work(const float* __restrict__ in, const Geom& g, float r, fz_result* res, uint32_t* bwords,
+                          uint16_t* tail, uint32_t& ovf)
+{
+    const uint32_t tid = threadIdx.x;
+    const uint32_t t0 = (uint32_t)kTileCodes * g.T;
+    const uint32_t tail_len = g.len - t0;
+    if (8 * tid < tail_len) {
+        uint32_t w[4];
+        dq8<NDIM>(in, t0 + 8 * tid, g, r, w, ovf);
+        if (g.mode == 0 && tail) {
+#pragma unroll
+            for (int i = 0; i < 8; i++)
+                if (8 * tid + i < tail_len) tail[8 * tid + i] = (uint16_t)(w[i >> 1] >> (16 * (i & 1)));
+        }
+    }
+    for (uint32_t i = 8 * g.T + tid; i < g.W; i += kWG) bwords[i] = 0;
+    if (g.T == 0 && tid == 0) {
+        res->nblocks = 0;
+        res->payload_bytes = 4ull * g.W;
+    }
+}
+
+// Which dual-quant path a tile takes (uniform over the workgroup).
+template <int NDIM>
+__device__ __forceinline__ int tile_path(const Geom& g, uint32_t t0, bool safe)
+{
+    if (!safe || t0 + (uint32_t)kTileCodes > g.xyz) return 0;  // guarded generic
+    if constexpr (NDIM == 1) return 2;
+    const uint32_t halo = (NDIM == 2) ? g.X + 1 : g.XY + g.X + 1;
+    if (t0 < halo) return 0;
+    return (g.X & 3u) == 0 ? 2 : 1;  // aligned fast path / unguarded generic
+}
+
+template <int NDIM>
+__device__ __forceinline__ void dq8_tile(const float* __restrict__ in, uint32_t p0, const Geom& g, float r, int path,
+                                         uint32_t w[4], uint32_t& ovf)
+{
+    if (path == 2) {
+        if constexpr (NDIM == 1) dq8_fast1(in, p0, g, r, w, ovf);
+        else dq8_a4<NDIM>(in, p0, g, r, w, ovf);
+    } else if (path == 1) {
+        dq8<NDIM, false>(in, p0, g, r, w, ovf);
+    } else {
+        dq8<NDIM, true>(in, p0, g, r, w, ovf);
+    }
+}
+
+template <int NDIM>
+__global__ __launch_bounds__(kWG) void k_tile_compress(const float* __restrict__ in, Geom g, fz_result* res,
+                                                       uint8_t* __restrict__ payload, uint8_t* __restrict__ staging,
+                                                       uint32_t* __restrict__ counts, uint16_t* __restrict__ tail)
+{
+    __shared__ uint32_t sh[32 * kLdsPitch];
+    __shared__ uint32_t sh_wcnt[4];
+    __shared__ uint32_t sh_ovf;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (res->status != ST_SUCCESS) return;  // the range kernel rejected the field (uniform)
+    const uint32_t tile = blockIdx.x;
+    const float    r = res->ebx2_r;
+    uint32_t*      bwords = reinterpret_cast<uint32_t*>(payload);
+    uint32_t       ovf = 0;
+    if (tid == 0) sh_ovf = 0;
+
+    if (tile >= g.T) {
+        tail_work<NDIM>(in, g, r, res, bwords, tail, ovf);
+        ovf = wave_sum(ovf);
+        if (lane == 0 && ovf) atomicAdd(reinterpret_cast<unsigned long long*>(&res->overflow), (unsigned long long)ovf);
+        return;
+    }
+
+    // |s| < 2^23 everywhere: the fast paths' integer conversions are exact
+    const float amax = fmaxf(__builtin_fabsf(res->vmin), __builtin_fabsf(res->vmax));
+    const bool  safe = amax * r < 8388608.0f;
+    const uint32_t t0 = tile * (uint32_t)kTileCodes;
+    const int      path = tile_path<NDIM>(g, t0, safe);
+
+    // 1. dual-quant: words 4t..4t+3 of the tile
+    uint32_t w[4];
+    dq8_tile<NDIM>(in, t0 + 8 * tid, g, r, path, w, ovf);
+
+    // 2. bit transpose of group k = t/8 (skipped for an all-zero wave); register q then holds
+    //    o[32(4L+q) + k]
+    const uint32_t L = tid & 7, k = tid >> 3;
+    const bool     wave_nz = ballot64((w[0] | w[1] | w[2] | w[3]) != 0) != 0;
+    if (wave_nz) transpose4(w, L);
+#pragma unroll
+    for (int q = 0; q < 4; q++) sh[kLdsPitch * (4 * L + q) + k] = w[q];
+    ovf = wave_sum(ovf);
+    if (lane == 0 && ovf) atomicAdd(&sh_ovf, ovf);
+    __syncthreads();
+
+    // 3. thread i owns 16-byte block i = o[4i .. 4i+3]; flag = block != 0 (fz.cu:84-91)
+    const uint32_t c = tid >> 3, k0 = 4 * (tid & 7);
+    const uint4    blk = make_uint4(sh[kLdsPitch * c + k0], sh[kLdsPitch * c + k0 + 1], sh[kLdsPitch * c + k0 + 2],
+                                    sh[kLdsPitch * c + k0 + 3]);
+    const bool     f = (blk.x | blk.y | blk.z | blk.w) != 0;
+    const uint64_t mask = ballot64(f);
+    // bitflag word 8*tile + y holds flags 32y..32y+31 (LSB first, fz.cu:95-99): wave wv covers
+    // words 2wv (lanes 0-31) and 2wv+1 (lanes 32-63)
+    if (lane < 2) bwords[8 * tile + 2 * wv + lane] = lane ? (uint32_t)(mask >> 32) : (uint32_t)mask;
+    const uint32_t rank = rank_below(mask);
+    if (lane == 0) sh_wcnt[wv] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    const uint32_t c0 = sh_wcnt[0], c1 = sh_wcnt[1], c2 = sh_wcnt[2], c3 = sh_wcnt[3];
+    const uint32_t woff = (wv > 0 ? c0 : 0u) + (wv > 1 ? c1 : 0u) + (wv > 2 ? c2 : 0u);
+    if (tid == 0) {
+        counts[tile] = c0 + c1 + c2 + c3;
+        if (sh_ovf) atomicAdd(reinterpret_cast<unsigned long long*>(&res->overflow), (unsigned long long)sh_ovf);
+    }
+    // 4. in-tile compaction into the tile's staging slot
+    if (f) *reinterpret_cast<uint4*>(staging + (size_t)tile * 4096 + 16 * (woff + rank)) = blk;
+}
+
+constexpr uint32_t kGatherTiles = kGatherChunk;
+
+__global__ __launch_bounds__(kWG) void k_scan_gather(const uint32_t* __restrict__ counts, uint32_t T,
+                                                     const uint8_t* __restrict__ staging, uint8_t* __restrict__ payload,
+                                                     uint64_t* states, uint32_t* ticket, fz_result* res, uint32_t W)
+{
+    __shared__ uint32_t sh_off[kGatherTiles];
+    __shared__ uint32_t sh_w[4];
+    __shared__ uint32_t sh_chunk, sh_excl;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (res->status != ST_SUCCESS) return;
+    if (tid == 0) sh_chunk = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t chunk = sh_chunk, t = chunk * kGatherTiles + tid;
+    const uint32_t cnt = t < T ? counts[t] : 0u;
+    const uint32_t incl = wave_incl_scan(cnt);
+    if (lane == 63) sh_w[wv] = incl;
+    __syncthreads();
+    const uint32_t w0 = sh_w[0], w1 = sh_w[1], w2 = sh_w[2], w3 = sh_w[3];
+    const uint32_t agg = w0 + w1 + w2 + w3;
+    const uint32_t woff = (wv > 0 ? w0 : 0u) + (wv > 1 ? w1 : 0u) + (wv > 2 ? w2 : 0u);
+    sh_off[tid] = woff + incl - cnt;
+    if (wv == 0) {
+        const uint32_t excl = lookback(states, chunk, agg);
+        if (lane == 0) {
+            sh_excl = excl;
+            if (chunk == (T - 1) / kGatherTiles) {
+                res->nblocks = (uint64_t)excl + agg;
+                res->payload_bytes = 4ull * W + 16ull * ((uint64_t)excl + agg);
+            }
+        }
+    }
+    __syncthreads();
+    uint8_t* out = payload + 4ull * W + 16ull * sh_excl;
+    for (uint32_t j = tid; j < agg; j += kWG) {
+        // last tile k of the chunk whose first block index is <= j
+        uint32_t lo = 0, hi = kGatherTiles - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (sh_off[mid] <= j) lo = mid;
+            else hi = mid - 1;
+        }
+        const uint4 v = *reinterpret_cast<const uint4*>(staging + (size_t)(chunk * kGatherTiles + lo) * 4096 +
+                                                       16 * (j - sh_off[lo]));
+        st16(out + 16ull * j, v);
+    }
+}
+
+hipError_t launch_fused_compress(const float* in, const Geom& g, fz_result* res, Ctr* ctr, uint64_t* states,
+                                 uint8_t* payload, uint16_t* tail, uint8_t* staging, uint32_t* counts, hipStream_t s)
+{
+    const dim3 grid(g.T + 1);
+    switch (g.ndim) {
+        case 1: FZ_LAUNCH(k_tile_compress<1><<<grid, kWG, 0, s>>>(in, g, res, payload, staging, counts, tail)); break;
+        case 2: FZ_LAUNCH(k_tile_compress<2><<<grid, kWG, 0, s>>>(in, g, res, payload, staging, counts, tail)); break;
+        default: FZ_LAUNCH(k_tile_compress<3><<<grid, kWG, 0, s>>>(in, g, res, payload, staging, counts, tail)); break;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || g.T == 0) return e;
+    const uint32_t nchunks = (g.T + kGatherTiles - 1) / kGatherTiles;
+    FZ_LAUNCH(k_scan_gather<<<nchunks, kWG, 0, s>>>(counts, g.T, staging, payload, states, &ctr->fused_ticket, res,
+                                                   g.W));
+    return hipGetLastError();
+}
+
+// ============================================================================ plain stages
+
+// Lorenzo stage: 8 codes per thread for every position < len (codes beyond x*y*z are 0).
+template <int NDIM>
+__global__ __launch_bounds__(kWG) void k_dualquant(const float* __restrict__ in, Geom g, float r_host,
+                                                   const fz_result* res, uint16_t* __restrict__ codes,
+                                                   unsigned long long* ovf_out)
+{
+    if (res && res->status != ST_SUCCESS) return;
+    const float    r = res ? res->ebx2_r : r_host;
+    const uint32_t p0 = 8 * (blockIdx.x * kWG + threadIdx.x);
+    uint32_t       ovf = 0;
+    if (p0 < g.len) {
+        uint32_t w[4];
+        dq8<NDIM>(in, p0, g, r, w, ovf);
+        if (p0 + 8 <= g.len) {
+            *reinterpret_cast<uint4*>(codes + p0) = make_uint4(w[0], w[1], w[2], w[3]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; i++)
+                if (p0 + i < g.len) codes[p0 + i] = (uint16_t)(w[i >> 1] >> (16 * (i & 1)));
+        }
+    }
+    if (ovf_out) {
+        ovf = wave_sum(ovf);
+        if ((threadIdx.x & 63) == 0 && ovf) atomicAdd(ovf_out, (unsigned long long)ovf);
+    }
+}
+
+hipError_t launch_dualquant(const float* in, const Geom& g, float r_host, const fz_result* res, uint16_t* codes,
+                            unsigned long long* ovf, hipStream_t s)
+{
+    const uint32_t groups = (g.len + 7) / 8;
+    const dim3     grid((groups + kWG - 1) / kWG);
+    switch (g.ndim) {
+        case 1: FZ_LAUNCH(k_dualquant<1><<<grid, kWG, 0, s>>>(in, g, r_host, res, codes, ovf)); break;
+        case 2: FZ_LAUNCH(k_dualquant<2><<<grid, kWG, 0, s>>>(in, g, r_host, res, codes, ovf)); break;
+        default: FZ_LAUNCH(k_dualquant<3><<<grid, kWG, 0, s>>>(in, g, r_host, res, codes, ovf)); break;
+    }
+    return hipGetLastError();
+}
+
+// bitshuffleAndBitflag: one tile per workgroup.  With the extra workgroup (blockIdx == T)
+// the words past the last full tile are copied unshuffled (fz.cu:154-155) and the flags
+// past it cleared (fz.cu:167-168), so the stage output is complete without memsets.
+__global__ __launch_bounds__(kWG) void k_bitshuffle(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                    Geom g, uint32_t* __restrict__ bitflags,
+                                                    uint32_t* __restrict__ byteflags)
+{
+    __shared__ uint32_t sh[32 * kLdsPitch];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, tile = blockIdx.x;
+    if (tile >= g.T) {
+        const uint32_t w0 = kTileWords * g.T, nw = (g.len + 1) / 2;
+        for (uint32_t i = w0 + tid; i < nw; i += kWG) out[i] = in[i];
+        for (uint32_t i = 8 * g.T + tid; i < g.W; i += kWG) bitflags[i] = 0;
+        for (uint32_t i = 256 * g.T + tid; i < 32 * g.W; i += kWG) byteflags[i] = 0;
+        return;
+    }
+    const uint4 v = *reinterpret_cast<const uint4*>(in + (size_t)tile * kTileWords + 4 * tid);
+    uint32_t    w[4] = {v.x, v.y, v.z, v.w};
+    const uint32_t L = tid & 7, k = tid >> 3;
+    transpose4(w, L);
+#pragma unroll
+    for (int q = 0; q < 4; q++) sh[kLdsPitch * (4 * L + q) + k] = w[q];
+    __syncthreads();
+    const uint32_t c = tid >> 3, k0 = 4 * (tid & 7);
+    const uint4    blk = make_uint4(sh[kLdsPitch * c + k0], sh[kLdsPitch * c + k0 + 1], sh[kLdsPitch * c + k0 + 2],
+                                    sh[kLdsPitch * c + k0 + 3]);
+    *reinterpret_cast<uint4*>(out + (size_t)tile * kTileWords + 4 * tid) = blk;
+    const bool f = (blk.x | blk.y | blk.z | blk.w) != 0;
+    byteflags[(size_t)tile * kTileBlocks + tid] = f ? 1u : 0u;
+    const uint64_t mask = ballot64(f);
+    if (lane < 2) bitflags[8 * tile + 2 * wv + lane] = lane ? (uint32_t)(mask >> 32) : (uint32_t)mask;
+}
+
+hipError_t launch_bitshuffle(const uint32_t* in, uint32_t* out, const Geom& g, uint32_t* bitflags,
+                             uint32_t* byteflags, bool with_tail_wg, hipStream_t s)
+{
+    const uint32_t nwg = g.T + (with_tail_wg ? 1u : 0u);
+    if (nwg == 0) return hipSuccess;
+    FZ_LAUNCH(k_bitshuffle<<<nwg, kWG, 0, s>>>(in, out, g, bitflags, byteflags));
+    return hipGetLastError();
+}
+
+// Inverse of k_bitshuffle on full tiles (the transpose is its own inverse: gather the
+// strided column words of group k, transpose, store rows 32k..32k+31).
+__global__ __launch_bounds__(kWG) void k_unshuffle(const uint32_t* __restrict__ in, uint32_t* __restrict__ out)
+{
+    __shared__ uint32_t sh[32 * kLdsPitch];
+    const uint32_t tid = threadIdx.x, tile = blockIdx.x;
+    const uint4    v = *reinterpret_cast<const uint4*>(in + (size_t)tile * kTileWords + 4 * tid);
+    const uint32_t c = tid >> 3, k0 = 4 * (tid & 7);
+    sh[kLdsPitch * c + k0] = v.x;
+    sh[kLdsPitch * c + k0 + 1] = v.y;
+    sh[kLdsPitch * c + k0 + 2] = v.z;
+    sh[kLdsPitch * c + k0 + 3] = v.w;
+    __syncthreads();
+    const uint32_t L = tid & 7, k = tid >> 3;
+    uint32_t       w[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) w[q] = sh[kLdsPitch * (4 * L + q) + k];
+    transpose4(w, L);
+    *reinterpret_cast<uint4*>(out + (size_t)tile * kTileWords + 4 * tid) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+hipError_t launch_unshuffle(const uint32_t* in, uint32_t* out, uint32_t ntiles, hipStream_t s)
+{
+    if (ntiles == 0) return hipSuccess;
+    FZ_LAUNCH(k_unshuffle<<<ntiles, kWG, 0, s>>>(in, out));
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------- scan
+
+__global__ void k_fill_u64(uint64_t* p, uint32_t n, uint32_t* ticket)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+    if (i == 0 && ticket) *ticket = 0;
+}
+
+hipError_t launch_scan_init(uint64_t* states, uint32_t nstates, uint32_t* ticket, hipStream_t s)
+{
+    const uint32_t n = nstates > 0 ? nstates : 1;
+    FZ_LAUNCH(k_fill_u64<<<(n + kWG - 1) / kWG, kWG, 0, s>>>(states, nstates, ticket));
+    return hipGetLastError();
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    FZ_LAUNCH(k_fill_u32<<<(unsigned)((n + kWG - 1) / kWG), kWG, 0, s>>>(p, n, v));
+    return hipGetLastError();
+}
+
+// Exclusive sum of n u32 (wrapping), 4096 items per workgroup, 16 consecutive per thread.
+// Optionally out[n] = total and the payload size in the result record.
+__global__ __launch_bounds__(kWG) void k_scan(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n,
+                                              uint64_t* states, uint32_t* ticket, uint32_t write_total,
+                                              fz_result* res, uint32_t W)
+{
+    __shared__ uint32_t sh_tile, sh_excl, sh_w[4];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (tid == 0) sh_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t chunk = sh_tile;
+    const uint32_t base = chunk * kScanChunk + 16 * tid;
+    uint32_t       v[16];
+    if (base + 16 <= n) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint4 t = *reinterpret_cast<const uint4*>(in + base + 4 * j);
+            v[4 * j] = t.x;
+            v[4 * j + 1] = t.y;
+            v[4 * j + 2] = t.z;
+            v[4 * j + 3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 16; j++) v[j] = (base + j < n) ? in[base + j] : 0u;
+    }
+    uint32_t tsum = 0;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const uint32_t x = v[j];
+        v[j] = tsum;
+        tsum += x;
+    }
+    const uint32_t incl = wave_incl_scan(tsum);
+    if (lane == 63) sh_w[wv] = incl;
+    __syncthreads();
+    const uint32_t w0 = sh_w[0], w1 = sh_w[1], w2 = sh_w[2], w3 = sh_w[3];
+    const uint32_t agg = w0 + w1 + w2 + w3;
+    const uint32_t woff = (wv > 0 ? w0 : 0u) + (wv > 1 ? w1 : 0u) + (wv > 2 ? w2 : 0u);
+    if (wv == 0) {
+        const uint32_t excl = lookback(states, chunk, agg);
+        if (lane == 0) {
+            sh_excl = excl;
+            const uint32_t nchunks = (n + kScanChunk - 1) / kScanChunk;
+            if (chunk == nchunks - 1) {
+                if (write_total) out[n] = excl + agg;
+                if (res) {
+                    res->nblocks = (uint64_t)(excl + agg);
+                    res->payload_bytes = 4ull * W + 16ull * (uint64_t)(excl + agg);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t off = sh_excl + woff + incl - tsum;
+    if (base + 16 <= n) {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            *reinterpret_cast<uint4*>(out + base + 4 * j) =
+                make_uint4(off + v[4 * j], off + v[4 * j + 1], off + v[4 * j + 2], off + v[4 * j + 3]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+            if (base + j < n) out[base + j] = off + v[j];
+    }
+}
+
+hipError_t launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* states, uint32_t* ticket,
+                       bool write_total, fz_result* res, uint32_t W, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    const uint32_t nchunks = (n + kScanChunk - 1) / kScanChunk;
+    FZ_LAUNCH(k_scan<<<nchunks, kWG, 0, s>>>(in, out, n, states, ticket, write_total ? 1u : 0u, res, W));
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------- encode
+
+__global__ __launch_bounds__(kWG) void k_encode(const uint8_t* __restrict__ in, uint8_t* __restrict__ out,
+                                                const uint32_t* __restrict__ presum, uint32_t nblocks)
+{
+    const uint32_t i = blockIdx.x * kWG + threadIdx.x;
+    if (i >= nblocks) return;
+    const uint32_t a = presum[i], b = presum[i + 1];
+    if (a != b) st16(out + 16ull * a, *reinterpret_cast<const uint4*>(in + 16ull * i));
+}
+
+hipError_t launch_encode(const uint8_t* in, uint8_t* out, const uint32_t* presum, uint32_t nblocks, hipStream_t s)
+{
+    if (nblocks == 0) return hipSuccess;
+    FZ_LAUNCH(k_encode<<<(nblocks + kWG - 1) / kWG, kWG, 0, s>>>(in, out, presum, nblocks));
+    return hipGetLastError();
+}
+
+// ============================================================================ inverse
+
+// tile_excl[t] = number of set flags in tiles < t (popcount of bitflag words 8t..8t+7);
+// one thread per tile, 256 tiles per workgroup, look-back across workgroups.
+__global__ __launch_bounds__(kWG) void k_tile_scan(const uint32_t* __restrict__ bitflags, uint32_t T,
+                                                   uint32_t* __restrict__ tile_excl, uint64_t* states,
+                                                   uint32_t* ticket)
+{
+    __shared__ uint32_t sh_chunk, sh_excl, sh_w[4];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (tid == 0) sh_chunk = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t chunk = sh_chunk, t = chunk * kWG + tid;
+    uint32_t       cnt = 0;
+    if (t < T) {
+        const uint4 a = *reinterpret_cast<const uint4*>(bitflags + 8 * (size_t)t);
+        const uint4 b = *reinterpret_cast<const uint4*>(bitflags + 8 * (size_t)t + 4);
+        cnt = __popc(a.x) + __popc(a.y) + __popc(a.z) + __popc(a.w) + __popc(b.x) + __popc(b.y) + __popc(b.z) +
+              __popc(b.w);
+    }
+    const uint32_t incl = wave_incl_scan(cnt);
+    if (lane == 63) sh_w[wv] = incl;
+    __syncthreads();
+    const uint32_t w0 = sh_w[0], w1 = sh_w[1], w2 = sh_w[2], w3 = sh_w[3];
+    const uint32_t agg = w0 + w1 + w2 + w3;
+    const uint32_t woff = (wv > 0 ? w0 : 0u) + (wv > 1 ? w1 : 0u) + (wv > 2 ? w2 : 0u);
+    if (wv == 0) {
+        const uint32_t excl = lookback(states, chunk, agg);
+        if (lane == 0) sh_excl = excl;
+    }
+    __syncthreads();
+    if (t < T) tile_excl[t] = sh_excl + woff + incl - cnt;
+}
+
+hipError_t launch_tile_scan(const uint32_t* bitflags, uint32_t T, uint32_t* tile_excl, uint64_t* states,
+                            uint32_t* ticket, hipStream_t s)
+{
+    if (T == 0) return hipSuccess;
+    FZ_LAUNCH(k_tile_scan<<<(T + kWG - 1) / kWG, kWG, 0, s>>>(bitflags, T, tile_excl, states, ticket));
+    return hipGetLastError();
+}
+
+// Scatter the tile's nonzero blocks back (block i: payload + 4W + 16 * (tile_excl + rank)),
+// unshuffle, store the 1024 code words.  The extra workgroup writes the tail codes (or
+// zeros when there is no tail buffer).
+__global__ __launch_bounds__(kWG) void k_decode_unshuffle(const uint8_t* __restrict__ payload,
+                                                          const uint32_t* __restrict__ tile_excl,
+                                                          const uint16_t* __restrict__ tail, Geom g,
+                                                          uint32_t* __restrict__ codes)
+{
+    __shared__ uint32_t sh[32 * kLdsPitch];
+    __shared__ uint32_t sh_w[4];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, tile = blockIdx.x;
+    if (tile >= g.T) {
+        uint16_t*      c16 = reinterpret_cast<uint16_t*>(codes);
+        const uint32_t t0 = kTileCodes * g.T;
+        for (uint32_t i = tid; t0 + i < g.len; i += kWG) c16[t0 + i] = tail ? tail[i] : (uint16_t)0;
+        return;
+    }
+    const uint32_t* bwords = reinterpret_cast<const uint32_t*>(payload);
+    const uint32_t  word = bwords[8 * tile + (tid >> 5)];
+    const bool      f = (word >> (tid & 31)) & 1u;
+    const uint64_t  mask = ballot64(f);
+    if (lane == 0) sh_w[wv] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    const uint32_t woff = (wv > 0 ? sh_w[0] : 0u) + (wv > 1 ? sh_w[1] : 0u) + (wv > 2 ? sh_w[2] : 0u);
+    uint4          blk = make_uint4(0, 0, 0, 0);
+    if (f) blk = ld16(payload + 4ull * g.W + 16ull * (tile_excl[tile] + woff + rank_below(mask)));
+    const uint32_t c = tid >> 3, k0 = 4 * (tid & 7);
+    sh[kLdsPitch * c + k0] = blk.x;
+    sh[kLdsPitch * c + k0 + 1] = blk.y;
+    sh[kLdsPitch * c + k0 + 2] = blk.z;
+    sh[kLdsPitch * c + k0 + 3] = blk.w;
+    __syncthreads();
+    const uint32_t L = tid & 7, k = tid >> 3;
+    uint32_t       w[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) w[q] = sh[kLdsPitch * (4 * L + q) + k];
+    transpose4(w, L);
+    *reinterpret_cast<uint4*>(codes + (size_t)tile * kTileWords + 4 * tid) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+hipError_t launch_decode_unshuffle(const uint8_t* payload, const uint32_t* tile_excl, const uint16_t* tail,
+                                   const Geom& g, uint32_t* codes, hipStream_t s)
+{
+    FZ_LAUNCH(k_decode_unshuffle<<<g.T + 1, kWG, 0, s>>>(payload, tile_excl, tail, g, codes));
+    return hipGetLastError();
+}
+
+__device__ __forceinline__ int32_t decode(uint32_t c)
+{
+    const int32_t m = (int32_t)(c & 0x7fffu);
+    return (c & 0x8000u) ? -m : m;
+}
+
+// 1D: 16 consecutive codes per thread, 16 threads per 256-block (block-local inclusive
+// prefix sum), x = (float)s * ebx2.
+__global__ __launch_bounds__(kWG) void k_reconstruct_1d(const uint16_t* __restrict__ codes, float* __restrict__ out,
+                                                        Geom g, float ebx2)
+{
+    const uint32_t p0 = (blockIdx.x * kWG + threadIdx.x) * 16;
+    int32_t        s[16];
+    if (p0 + 16 <= g.xyz) {
+        const uint4 a = *reinterpret_cast<const uint4*>(codes + p0);
+        const uint4 b = *reinterpret_cast<const uint4*>(codes + p0 + 8);
+        const uint32_t wd[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            s[2 * j] = decode(wd[j] & 0xffffu);
+            s[2 * j + 1] = decode(wd[j] >> 16);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 16; j++) s[j] = (p0 + j < g.xyz) ? decode(codes[p0 + j]) : 0;
+    }
+#pragma unroll
+    for (int j = 1; j < 16; j++) s[j] += s[j - 1];
+    // exclusive prefix of the 16 thread totals of this 256-block (lanes 16-aligned)
+    int32_t t = s[15];
+#pragma unroll
+    for (int d = 1; d < 16; d <<= 1) {
+        const int32_t o = __shfl_up(t, d, 16);
+        if ((threadIdx.x & 15) >= (uint32_t)d) t += o;
+    }
+    const int32_t ex = t - s[15];
+    if (p0 + 16 <= g.xyz) {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            *reinterpret_cast<float4*>(out + p0 + 4 * j) =
+                make_float4((float)(ex + s[4 * j]) * ebx2, (float)(ex + s[4 * j + 1]) * ebx2,
+                            (float)(ex + s[4 * j + 2]) * ebx2, (float)(ex + s[4 * j + 3]) * ebx2);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+            if (p0 + j < g.xyz) out[p0 + j] = (float)(ex + s[j]) * ebx2;
+    }
+}
+
+// 2D: workgroup = 16 rows x 256 columns (16 Lorenzo blocks); thread = one column of a block.
+// Prefix over y in registers, then over x across the 16 lanes of the block.
+__global__ __launch_bounds__(kWG) void k_reconstruct_2d(const uint16_t* __restrict__ codes, float* __restrict__ out,
+                                                        Geom g, float ebx2)
+{
+    const uint32_t x = blockIdx.x * kWG + threadIdx.x, y0 = blockIdx.y * 16;
+    const bool     xin = x < g.X;
+    int32_t        s[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++)
+        s[r] = (xin && y0 + r < g.Y) ? decode(codes[(size_t)(y0 + r) * g.X + x]) : 0;
+#pragma unroll
+    for (int r = 1; r < 16; r++) s[r] += s[r - 1];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+#pragma unroll
+        for (int d = 1; d < 16; d <<= 1) {
+            const int32_t o = __shfl_up(s[r], d, 16);
+            if ((threadIdx.x & 15) >= (uint32_t)d) s[r] += o;
+        }
+    }
+    if (xin) {
+#pragma unroll
+        for (int r = 0; r < 16; r++)
+            if (y0 + r < g.Y) out[(size_t)(y0 + r) * g.X + x] = (float)s[r] * ebx2;
+    }
+}
+
+// 3D: workgroup = 32 (x) x 8 (y) x 8 (z) like the reference's (32,1,8) block.  Thread (tx, ty)
+// holds the z column of (x, y): prefix over z in registers, over x across 8-lane groups; then
+// an LDS transpose puts the y column in registers for the last prefix.
+__global__ __launch_bounds__(kWG) void k_reconstruct_3d(const uint16_t* __restrict__ codes, float* __restrict__ out,
+                                                        Geom g, float ebx2)
+{
+    __shared__ int32_t sh[8][8][33];
+    const uint32_t tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const uint32_t x = blockIdx.x * 32 + tx, y0 = blockIdx.y * 8, z0 = blockIdx.z * 8;
+    const bool     xin = x < g.X;
+    int32_t        s[8];
+    {
+        const uint32_t y = y0 + ty;
+#pragma unroll
+        for (int z = 0; z < 8; z++)
+            s[z] = (xin && y < g.Y && z0 + z < g.Z) ? decode(codes[(size_t)(z0 + z) * g.XY + (size_t)y * g.X + x]) : 0;
+    }
+#pragma unroll
+    for (int z = 1; z < 8; z++) s[z] += s[z - 1];
+#pragma unroll
+    for (int z = 0; z < 8; z++) {
+#pragma unroll
+        for (int d = 1; d < 8; d <<= 1) {
+            const int32_t o = __shfl_up(s[z], d, 8);
+            if ((tx & 7) >= (uint32_t)d) s[z] += o;
+        }
+        sh[z][ty][tx] = s[z];
+    }
+    __syncthreads();
+    const uint32_t tz = ty;
+    int32_t        v[8];
+#pragma unroll
+    for (int y = 0; y < 8; y++) v[y] = sh[tz][y][tx];
+#pragma unroll
+    for (int y = 1; y < 8; y++) v[y] += v[y - 1];
+    if (xin && z0 + tz < g.Z) {
+#pragma unroll
+        for (int y = 0; y < 8; y++)
+            if (y0 + y < g.Y) out[(size_t)(z0 + tz) * g.XY + (size_t)(y0 + y) * g.X + x] = (float)v[y] * ebx2;
+    }
+}
+
+hipError_t launch_reconstruct(const uint16_t* codes, float* out, const Geom& g, float ebx2, hipStream_t s)
+{
+    if (g.ndim == 1) {
+        const uint32_t thr = (g.xyz + 15) / 16;
+        FZ_LAUNCH(k_reconstruct_1d<<<(thr + kWG - 1) / kWG, kWG, 0, s>>>(codes, out, g, ebx2));
+    } else if (g.ndim == 2) {
+        const dim3 grid((g.X + kWG - 1) / kWG, (g.Y + 15) / 16);
+        FZ_LAUNCH(k_reconstruct_2d<<<grid, kWG, 0, s>>>(codes, out, g, ebx2));
+    } else {
+        const dim3 grid((g.X + 31) / 32, (g.Y + 7) / 8, (g.Z + 7) / 8);
+        FZ_LAUNCH(k_reconstruct_3d<<<grid, kWG, 0, s>>>(codes, out, g, ebx2));
+    }
+    return hipGetLastError();
+}
+
+}  // namespace fz

@@ -1,0 +1,293 @@
+"""Python host binding of the gfx950 FZ library (lib/libfzgpu.so, C ABI in include/fz_capi.h).
+
+PyTorch is used only for device memory and streams; every byte of the pipeline is computed
+by the HIP kernels behind the C ABI.  There is no CPU fallback: when the library is missing
+every call raises.
+
+Mirrors of the reference's surface:
+    run_szs(file, x, y, z, eb)          -> fz.cu:113 runSzs (prints fz.cu:207-209)
+    lorenzo_construct(...)              -> lorenzo_var.cuh:520 launch_construct_LorenzoI_var
+    bitshuffle_bitflag(...)             -> fz.cu:61 bitshuffleAndBitflag
+    exclusive_sum(...)                  -> cub::DeviceScan::ExclusiveSum (fz.cu:188-192)
+    encode_blocks(...)                  -> fz.cu:49 encodeDebug
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "lib", "libfzgpu.so")
+
+MODE_FIXED = 0
+MODE_LITERAL = 1
+
+STATUS = {0: "success", 1: "invalid argument", 2: "workspace too small", 3: "HIP runtime error",
+          4: "constant field", 5: "non-finite value", 6: "corrupt payload", 7: "capacity exceeded",
+          8: "not ready"}
+
+
+class FzError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        super().__init__(f"{what}: {STATUS.get(status, status)} ({status})")
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_uint32), ("ndim", ctypes.c_uint32),
+                ("vmin", ctypes.c_float), ("vmax", ctypes.c_float),
+                ("ebx2_r", ctypes.c_float), ("ebx2", ctypes.c_float),
+                ("eb_abs", ctypes.c_double),
+                ("nblocks", ctypes.c_uint64), ("payload_bytes", ctypes.c_uint64),
+                ("tail_len", ctypes.c_uint64), ("overflow", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+assert ctypes.sizeof(Result) == 64
+
+
+class Sizes(ctypes.Structure):
+    _fields_ = [("len", ctypes.c_uint64), ("tiles", ctypes.c_uint64), ("flag_words", ctypes.c_uint64),
+                ("byte_flags", ctypes.c_uint64), ("payload_max_bytes", ctypes.c_uint64),
+                ("tail_max_codes", ctypes.c_uint64), ("workspace_bytes", ctypes.c_uint64)]
+
+
+# every symbol declared in include/fz_capi.h
+EXPORTS = ["fz_version", "fz_status_string", "fz_last_error", "fz_plan_create", "fz_plan_destroy", "fz_plan_sizes",
+           "fz_workspace_init", "fz_compress_f32", "fz_compress_plain_f32", "fz_decompress_f32",
+           "fz_compress_f32_sync", "fz_lorenzo_construct_f32", "fz_lorenzo_reconstruct_f32",
+           "fz_bitshuffle_bitflag", "fz_bitunshuffle", "fz_exclusive_sum_u32", "fz_encode_blocks",
+           "fz_plan_profile", "fz_plan_kernel_times"]
+
+_lib = None
+
+
+def lib():
+    """Load libfzgpu.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"FZ HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u32, u64, i32, f32, f64 = (ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int,
+                                   ctypes.c_float, ctypes.c_double)
+    L.fz_version.restype = ctypes.c_char_p
+    L.fz_last_error.restype = ctypes.c_char_p
+    L.fz_status_string.argtypes = [i32]
+    L.fz_status_string.restype = ctypes.c_char_p
+    L.fz_plan_create.argtypes = [ctypes.POINTER(vp), u32, u32, u32, u64, i32]
+    L.fz_plan_destroy.argtypes = [vp]
+    L.fz_plan_sizes.argtypes = [vp, ctypes.POINTER(Sizes)]
+    L.fz_workspace_init.argtypes = [vp, vp, ctypes.c_size_t, vp]
+    L.fz_compress_f32.argtypes = [vp, vp, f64, vp, ctypes.c_size_t, vp, vp, vp, vp]
+    L.fz_compress_plain_f32.argtypes = [vp, vp, f64, vp, ctypes.c_size_t, vp, vp, vp, vp]
+    L.fz_compress_f32_sync.argtypes = [vp, vp, f64, vp, ctypes.c_size_t, vp, vp, vp, ctypes.POINTER(Result), vp]
+    L.fz_decompress_f32.argtypes = [vp, vp, vp, f32, vp, ctypes.c_size_t, vp, vp]
+    L.fz_lorenzo_construct_f32.argtypes = [vp, vp, u32, u32, u32, u64, f64, i32, ctypes.POINTER(f32), vp]
+    L.fz_lorenzo_reconstruct_f32.argtypes = [vp, vp, u32, u32, u32, f64, ctypes.POINTER(f32), vp]
+    L.fz_bitshuffle_bitflag.argtypes = [vp, vp, u64, vp, vp, vp]
+    L.fz_bitunshuffle.argtypes = [vp, vp, u64, vp]
+    L.fz_exclusive_sum_u32.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t), vp, vp, u64, vp]
+    L.fz_encode_blocks.argtypes = [vp, vp, vp, u64, vp]
+    L.fz_plan_profile.argtypes = [vp, i32]
+    L.fz_plan_kernel_times.argtypes = [vp, ctypes.POINTER(f32), i32, ctypes.POINTER(i32)]
+    for name in EXPORTS:
+        if name not in ("fz_version", "fz_status_string", "fz_last_error"):
+            getattr(L, name).restype = i32
+    _lib = L
+    return L
+
+
+def version():
+    return lib().fz_version().decode()
+
+
+def _check(st, what):
+    if st != 0:
+        if st == 3:
+            what = f"{what} [{lib().fz_last_error().decode()}]"
+        raise FzError(st, what)
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream=None):
+    import torch
+    s = torch.cuda.current_stream() if stream is None else stream
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _empty(nbytes, device="cuda"):
+    import torch
+    return torch.empty(max(int(nbytes), 16), dtype=torch.uint8, device=device)
+
+
+class Plan:
+    """Geometry + mode of one field and its device workspace (allocated once, reused)."""
+
+    def __init__(self, x, y, z, length=0, mode=MODE_FIXED, device="cuda", workspace=True):
+        self.x, self.y, self.z = int(x), int(y), int(z)
+        self.mode = mode
+        self.device = device
+        h = ctypes.c_void_p()
+        _check(lib().fz_plan_create(ctypes.byref(h), self.x, self.y, self.z, int(length), mode), "fz_plan_create")
+        self.handle = h
+        s = Sizes()
+        _check(lib().fz_plan_sizes(h, ctypes.byref(s)), "fz_plan_sizes")
+        self.sizes = s
+        self.len = s.len
+        self.ws = None
+        if workspace:
+            self.ws = _empty(s.workspace_bytes, device)
+            _check(lib().fz_workspace_init(h, _ptr(self.ws), self.ws.numel(), _stream()), "fz_workspace_init")
+
+    @property
+    def dims(self):
+        return (self.x, self.y, self.z)
+
+    def close(self):
+        if self.handle is not None and _lib is not None:
+            _lib.fz_plan_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def profile(self, enable=True):
+        _check(lib().fz_plan_profile(self.handle, 1 if enable else 0), "fz_plan_profile")
+
+    def kernel_times(self):
+        """Per-kernel milliseconds of the last compress/decompress call (waits for it)."""
+        buf = (ctypes.c_float * 8)()
+        cnt = ctypes.c_int(0)
+        _check(lib().fz_plan_kernel_times(self.handle, buf, 8, ctypes.byref(cnt)), "fz_plan_kernel_times")
+        return [buf[i] for i in range(cnt.value)]
+
+    def alloc_outputs(self):
+        import torch
+        payload = _empty(self.sizes.payload_max_bytes + 16, self.device)
+        tail = torch.empty(2048, dtype=torch.int16, device=self.device)
+        res = torch.zeros(64, dtype=torch.uint8, device=self.device)
+        return payload, tail, res
+
+    def compress_async(self, d_in, eb_rel, payload, tail, res, fused=True, stream=None):
+        """Launch only (graph-capturable); result record lands in `res` (64 device bytes)."""
+        fn = lib().fz_compress_f32 if fused else lib().fz_compress_plain_f32
+        _check(fn(self.handle, _ptr(d_in), float(eb_rel), _ptr(self.ws), self.ws.numel(), _ptr(payload), _ptr(tail),
+                  _ptr(res), _stream(stream)), "fz_compress")
+
+    def compress(self, d_in, eb_rel, fused=True, stream=None):
+        """Returns (payload uint8 cuda tensor [payload_bytes], tail uint16 cuda tensor, result dict)."""
+        import torch
+        payload, tail, res = self.alloc_outputs()
+        self.compress_async(d_in, eb_rel, payload, tail, res, fused, stream)
+        r = read_result(res)
+        if r["status"] != 0:
+            raise FzError(r["status"], "compress")
+        return payload[: r["payload_bytes"]], tail[: r["tail_len"]], r
+
+    def decompress(self, payload, tail, ebx2, out=None, stream=None):
+        import torch
+        n = self.x * self.y * self.z
+        if out is None:
+            out = torch.empty(n, dtype=torch.float32, device=self.device)
+        src = payload
+        if payload.data_ptr() % 16:
+            src = payload.clone()
+        t = tail if (tail is not None and tail.numel() > 0) else None
+        _check(lib().fz_decompress_f32(self.handle, _ptr(src), _ptr(t), float(ebx2), _ptr(self.ws), self.ws.numel(),
+                                       _ptr(out), _stream(stream)), "fz_decompress")
+        return out
+
+
+def read_result(res_tensor):
+    """Copy the 64-byte device result record to the host (synchronises)."""
+    raw = bytes(res_tensor[:64].cpu().numpy().tobytes())
+    return Result.from_buffer_copy(raw).as_dict()
+
+
+# ---------------------------------------------------------------- per-stage mirrors
+
+def lorenzo_construct(d_data, dims, eb_abs, mode=MODE_FIXED, length=0, stream=None):
+    import torch
+    n = int(length) if length else dims[0] * dims[1] * dims[2]
+    codes = torch.empty(n + 8, dtype=torch.int16, device=d_data.device)
+    t = ctypes.c_float()
+    _check(lib().fz_lorenzo_construct_f32(_ptr(d_data), _ptr(codes), dims[0], dims[1], dims[2], n, float(eb_abs),
+                                          mode, ctypes.byref(t), _stream(stream)), "fz_lorenzo_construct_f32")
+    return codes[:n], t.value
+
+
+def lorenzo_reconstruct(d_codes, dims, eb_abs, stream=None):
+    import torch
+    n = dims[0] * dims[1] * dims[2]
+    out = torch.empty(n, dtype=torch.float32, device=d_codes.device)
+    t = ctypes.c_float()
+    _check(lib().fz_lorenzo_reconstruct_f32(_ptr(d_codes), _ptr(out), dims[0], dims[1], dims[2], float(eb_abs),
+                                            ctypes.byref(t), _stream(stream)), "fz_lorenzo_reconstruct_f32")
+    return out, t.value
+
+
+def bitshuffle_bitflag(d_words, length, stream=None):
+    """d_words: int32 cuda tensor of >= 1024*floor(len/2048) words.  Returns (out, bitflags, byteflags)."""
+    import torch
+    T = int(length) // 2048
+    out = d_words.clone()
+    bf = torch.zeros(max(8 * T, 1), dtype=torch.int32, device=d_words.device)
+    F = torch.zeros(max(256 * T, 1), dtype=torch.int32, device=d_words.device)
+    _check(lib().fz_bitshuffle_bitflag(_ptr(d_words), _ptr(out), int(length), _ptr(bf), _ptr(F), _stream(stream)),
+           "fz_bitshuffle_bitflag")
+    return out, bf[: 8 * T], F[: 256 * T]
+
+
+def bitunshuffle(d_words, length, stream=None):
+    out = d_words.clone()
+    _check(lib().fz_bitunshuffle(_ptr(d_words), _ptr(out), int(length), _stream(stream)), "fz_bitunshuffle")
+    return out
+
+
+def exclusive_sum(d_in, stream=None):
+    """cub::DeviceScan::ExclusiveSum idiom: size query, temp allocation, run."""
+    import torch
+    n = d_in.numel()
+    nbytes = ctypes.c_size_t(0)
+    _check(lib().fz_exclusive_sum_u32(None, ctypes.byref(nbytes), _ptr(d_in), None, n, _stream(stream)),
+           "fz_exclusive_sum_u32(query)")
+    temp = _empty(nbytes.value, d_in.device)
+    out = torch.empty_like(d_in)
+    _check(lib().fz_exclusive_sum_u32(_ptr(temp), ctypes.byref(nbytes), _ptr(d_in), _ptr(out), n, _stream(stream)),
+           "fz_exclusive_sum_u32")
+    return out
+
+
+def encode_blocks(d_in_bytes, d_presum, num_blocks, out_bytes, stream=None):
+    import torch
+    out = torch.zeros(max(int(out_bytes), 16), dtype=torch.uint8, device=d_in_bytes.device)
+    _check(lib().fz_encode_blocks(_ptr(d_in_bytes), _ptr(out), _ptr(d_presum), int(num_blocks), _stream(stream)),
+           "fz_encode_blocks")
+    return out[: int(out_bytes)]
+
+
+def run_szs(file_name, x, y, z, eb, mode=MODE_FIXED, fused=True):
+    """fz.cu:113-209: read the file (float count from its size), compress on the GPU, print
+    the three lines of fz.cu:207-209.  Returns (payload, tail, result)."""
+    import numpy as np
+    import torch
+    file_size = os.path.getsize(file_name)
+    n = file_size // 4
+    host = np.fromfile(file_name, dtype=np.float32, count=n)
+    d = torch.from_numpy(host).to("cuda")
+    plan = Plan(x, y, z, n, mode)
+    payload, tail, r = plan.compress(d, eb, fused=fused)
+    csize = r["payload_bytes"]
+    print(f"original size: {file_size}")
+    print(f"compressed size: {csize}")
+    print("compression ratio: %f" % (np.float32(file_size) / np.float32(csize)))
+    return payload, tail, r

@@ -1,0 +1,170 @@
+/*
+ * fz_capi.h -- C ABI of the MI355X (gfx950) FZ compression path.
+ *
+ * Path: float32 field + value-range-relative error bound
+ *       -> Lorenzo dual-quant (sign-magnitude u16 codes)
+ *       -> bitshuffle of 2048-code tiles + zero-block bit flags
+ *       -> exclusive prefix sum over the flags
+ *       -> sparse encode of the nonzero 16-byte blocks
+ *       => payload = LE u32 bitflags B[0..W) || nonzero blocks (16 K bytes)
+ * and its inverse.  The payload bytes equal the reference's d_out layout and the size
+ * formula of fz.cu:208 (4 W + 16 K).
+ *
+ * Conventions (mirroring the reference's C launch layer, claunch_cuda.h:17-29):
+ *   - extern "C", every call returns fz_status (0 = success), never exits;
+ *   - device pointers only, the caller owns every buffer; stream argument last;
+ *   - the compress/decompress entry points allocate nothing, synchronise nothing and
+ *     launch kernels only, so they can be captured into a hipGraph;
+ *   - workspace sizes are queried first (the CUB two-phase idiom of fz.cu:188-192).
+ *
+ * Geometry: x is the fastest axis (fz.cu:231-233, len3 = dim3(x, y, z) at fz.cu:116);
+ * ndim = 1 if y == z == 1, 2 if z == 1, else 3 (lorenzo_var.cuh:537-544).  `len` is the
+ * number of floats in the input (the reference takes it from the file size, fz.cu:117-118)
+ * and must be >= x*y*z; positions >= x*y*z get code 0.
+ */
+#ifndef FZ_CAPI_H
+#define FZ_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FZ_CAPI_VERSION 1
+
+/* Status codes.  The reference prints and exit()s on CUDA errors (cuda_err.cuh:169) and
+ * throws for ndim > 3 (lorenzo_var.cuh:587); here every failure is a return code. */
+typedef enum fz_status {
+    FZ_SUCCESS = 0,
+    FZ_ERR_INVALID_ARG = 1,         /* bad dims / eb / mode / alignment / len < x*y*z   */
+    FZ_ERR_WORKSPACE_TOO_SMALL = 2, /* workspace smaller than fz_workspace_bytes()      */
+    FZ_ERR_HIP = 3,                 /* a HIP runtime call failed                        */
+    FZ_ERR_CONSTANT_FIELD = 4,      /* max == min: the reference divides by zero (D5)   */
+    FZ_ERR_NONFINITE = 5,           /* NaN or Inf in the input                          */
+    FZ_ERR_CORRUPT = 6,             /* payload inconsistent with its bitflags           */
+    FZ_ERR_CAPACITY = 7,            /* request exceeds a fixed capacity                 */
+    FZ_ERR_NOT_READY = 8            /* result record not yet written                    */
+} fz_status;
+
+/* FZ_MODE_FIXED: 1D codes keep the sign of every delta (the reference drops the sign of
+ * 3 of 4 deltas, lorenzo_var.cuh:59-63) and the len mod 2048 codes that no tile covers
+ * (never encoded by the reference, fz.cu:173) are returned in the tail buffer.
+ * FZ_MODE_LITERAL: byte-identical to the reference, including both losses.
+ * 2D and 3D payloads are identical in the two modes. */
+typedef enum fz_mode { FZ_MODE_FIXED = 0, FZ_MODE_LITERAL = 1 } fz_mode;
+
+/* Device-resident record written by the compress kernels (64 bytes). */
+typedef struct fz_result {
+    uint32_t status;        /* fz_status of the device-side checks (range, finiteness)  */
+    uint32_t ndim;
+    float    vmin, vmax;    /* value range (fz.cu:137)                                   */
+    float    ebx2_r, ebx2;  /* float kernel parameters (lorenzo_var.cuh:562-563)         */
+    double   eb_abs;        /* eb_rel * (vmax - vmin)  (fz.cu:149)                        */
+    uint64_t nblocks;       /* K: nonzero 16-byte blocks                                 */
+    uint64_t payload_bytes; /* 4 W + 16 K  (fz.cu:208)                                   */
+    uint64_t tail_len;      /* codes returned in the tail buffer (0 in LITERAL mode)     */
+    uint64_t overflow;      /* deltas with |d| >= 32768 (their code aliases the sign)    */
+} fz_result;
+
+/* Sizes of one field's buffers. */
+typedef struct fz_sizes {
+    uint64_t len;               /* floats in the input                                   */
+    uint64_t tiles;             /* T = floor(len / 2048)            (fz.cu:173)           */
+    uint64_t flag_words;        /* W = ceil(len / 256)              (fz.cu:177-178)       */
+    uint64_t byte_flags;        /* N = 32 W                                               */
+    uint64_t payload_max_bytes; /* 4 W + 4096 T: every block nonzero                      */
+    uint64_t tail_max_codes;    /* len - 2048 T (< 2048)                                  */
+    uint64_t workspace_bytes;   /* device scratch for every entry point of this plan      */
+} fz_sizes;
+
+typedef struct fz_plan_s* fz_plan;
+
+const char* fz_version(void);
+const char* fz_status_string(fz_status s);
+/* Text of the HIP error behind the calling thread's most recent FZ_ERR_HIP. */
+const char* fz_last_error(void);
+
+/* Plan = geometry + mode; host memory only.  Replaces the per-call set-up of runSzs
+ * (fz.cu:113-169: sizes, cudaMalloc/cudaMemset).  len = 0 means x*y*z. */
+fz_status fz_plan_create(fz_plan* plan, uint32_t x, uint32_t y, uint32_t z, uint64_t len, int mode);
+fz_status fz_plan_destroy(fz_plan plan);
+fz_status fz_plan_sizes(const fz_plan plan, fz_sizes* sizes);
+
+/* Puts a freshly allocated workspace into its reset state (one fill kernel).  Call once
+ * after allocating it; every entry point leaves it reset again when its kernels finish.
+ * One workspace serves one stream at a time. */
+fz_status fz_workspace_init(const fz_plan plan, void* d_ws, size_t ws_bytes, hipStream_t stream);
+
+/* Fused single-field compress (replaces runSzs, fz.cu:113-209, without its file I/O and
+ * host round trips): device value range -> eb -> one fused dual-quant/bitshuffle/flag/
+ * look-back-scan/encode kernel.  Writes payload (capacity payload_max_bytes), the tail
+ * codes (FIXED mode; capacity tail_max_codes) and *d_result.  d_in must be 16-byte aligned. */
+fz_status fz_compress_f32(const fz_plan plan, const float* d_in, double eb_rel, void* d_ws, size_t ws_bytes,
+                          uint8_t* d_payload, uint16_t* d_tail, fz_result* d_result, hipStream_t stream);
+
+/* Same output through the plain per-stage pipeline (one kernel per reference stage:
+ * range, Lorenzo, bitshuffle+flags, scan init + scan, encode). */
+fz_status fz_compress_plain_f32(const fz_plan plan, const float* d_in, double eb_rel, void* d_ws, size_t ws_bytes,
+                                uint8_t* d_payload, uint16_t* d_tail, fz_result* d_result, hipStream_t stream);
+
+/* Inverse: payload (+ tail codes in FIXED mode, NULL otherwise) -> x*y*z floats.
+ * ebx2 = fz_result.ebx2 of the compress call.  The inverse of the reference's Lorenzo
+ * (x_lorenzo_*, lorenzo_var.cuh:297-518) cannot decode sign-magnitude codes, so its block
+ * structure is kept with exact int32 prefix sums and one float multiply by ebx2. */
+fz_status fz_decompress_f32(const fz_plan plan, const uint8_t* d_payload, const uint16_t* d_tail, float ebx2,
+                            void* d_ws, size_t ws_bytes, float* d_out, hipStream_t stream);
+
+/* Convenience: runs fz_compress_f32, waits for the stream and copies *d_result to *h_result. */
+fz_status fz_compress_f32_sync(const fz_plan plan, const float* d_in, double eb_rel, void* d_ws, size_t ws_bytes,
+                               uint8_t* d_payload, uint16_t* d_tail, fz_result* d_result, fz_result* h_result,
+                               hipStream_t stream);
+
+/* Per-kernel timing (the reference's cuda_timer_t around its Lorenzo launch, timer.hh:45-85,
+ * extended to every kernel): when enabled, the compress/decompress calls on this plan record
+ * an event before their first kernel and after each kernel; fz_plan_kernel_times waits for the
+ * last one and returns the per-kernel milliseconds of the most recent call.  Off by default;
+ * enabling it creates the events (host set-up, outside any hot path). */
+fz_status fz_plan_profile(fz_plan plan, int enable);
+fz_status fz_plan_kernel_times(const fz_plan plan, float* ms, int capacity, int* count);
+
+/* ---- per-stage entry points (the reference's kernel boundaries) ---- */
+
+/* launch_construct_LorenzoI_var<float, uint16_t, float> (lorenzo_var.cuh:520-597).
+ * eb_abs = eb_rel * range (fz.cu:149).  Writes codes for all len positions (0 beyond
+ * x*y*z), so d_delta needs no pre-zeroing.  time_elapsed (ms) may be NULL; when given,
+ * the call times itself with events and waits for the stream, as the reference does. */
+fz_status fz_lorenzo_construct_f32(const float* d_data, uint16_t* d_delta, uint32_t x, uint32_t y, uint32_t z,
+                                   uint64_t len, double eb_abs, int mode, float* time_elapsed, hipStream_t stream);
+
+/* launch_reconstruct_LorenzoI_var (lorenzo_var.cuh:599-673) for sign-magnitude codes. */
+fz_status fz_lorenzo_reconstruct_f32(const uint16_t* d_delta, float* d_xdata, uint32_t x, uint32_t y, uint32_t z,
+                                     double eb_abs, float* time_elapsed, hipStream_t stream);
+
+/* bitshuffleAndBitflag (fz.cu:61-111), grid floor(len/2048): tiles of 1024 u32 words.
+ * Writes the shuffled words, byte flags [256 T] and bit flags [8 T] of the full tiles only,
+ * like the reference kernel. */
+fz_status fz_bitshuffle_bitflag(const uint32_t* d_in, uint32_t* d_out, uint64_t len, uint32_t* d_bitflags,
+                                uint32_t* d_byteflags, hipStream_t stream);
+
+/* Inverse of fz_bitshuffle_bitflag over the same tiles. */
+fz_status fz_bitunshuffle(const uint32_t* d_in, uint32_t* d_out, uint64_t len, hipStream_t stream);
+
+/* cub::DeviceScan::ExclusiveSum (fz.cu:188-192): call with d_temp == NULL to get
+ * *temp_bytes, then again to run.  Single pass, decoupled look-back. */
+fz_status fz_exclusive_sum_u32(void* d_temp, size_t* temp_bytes, const uint32_t* d_in, uint32_t* d_out,
+                               uint64_t num_items, hipStream_t stream);
+
+/* encodeDebug (fz.cu:49-59) with its intended semantics: block i (16 bytes) of d_in is
+ * copied to d_out + 16*presum[i] iff presum[i+1] != presum[i]; presum has num_blocks+1
+ * entries. */
+fz_status fz_encode_blocks(const uint8_t* d_in, uint8_t* d_out, const uint32_t* d_presum, uint64_t num_blocks,
+                           hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FZ_CAPI_H */

@@ -1,0 +1,195 @@
+"""GPU parity: every byte of the HIP path against the CPU oracle on the same inputs.
+
+Bit-exact bar for codes, flags, scan offsets, payload, tail and decompressed floats; the
+error bound max|x - x'| <= eb*(max-min) + 4*2^-24*max|x| (the last term is the binary32
+rounding of the prequant multiply, of the dequant multiply and of the two double->float
+narrowings of ebx2 and 1/ebx2).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import fzgpu
+import oracle as O
+from fzgpu import datagen as D
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _field(dims, seed, zero=0.3, scale=1.0):
+    rng = np.random.default_rng(seed)
+    n = dims[0] * dims[1] * dims[2]
+    x = (np.cumsum(rng.normal(size=n)) * scale).astype(np.float32)
+    x[rng.random(n) < zero] = 0
+    x[0] = 1.5  # never a constant field
+    return x
+
+
+def _to_dev(x):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x)).to("cuda")
+
+
+def _np(t, dtype=None):
+    a = t.cpu().numpy()
+    return a.view(dtype) if dtype is not None else a
+
+
+SHAPES = [(5000, 1, 1), (4096, 1, 1), (2047, 1, 1), (2048, 1, 1), (9, 1, 1), (100, 77, 1), (16, 16, 1),
+          (17, 33, 1), (3600, 20, 1), (1, 3000, 1), (33, 20, 17), (64, 64, 64), (500, 50, 10), (1, 1, 5000),
+          (7, 1, 900)]
+
+
+def _check_against_oracle(x, dims, eb, mode, fused, length=None):
+    n = x.size
+    say(f"case dims={dims} len={n} eb={eb} mode={mode} fused={fused}")
+    ref = O.compress(x, dims, eb, mode)
+    assert ref["status"] == 0
+    plan = fzgpu.Plan(*dims, length=n, mode=mode)
+    payload, tail, r = plan.compress(_to_dev(x), eb, fused=fused)
+    info = ref["info"]
+    for k_gpu, k_ref in [("nblocks", "K"), ("payload_bytes", "payload_bytes"), ("overflow", "overflow"),
+                         ("vmin", "vmin"), ("vmax", "vmax"), ("ebx2", "ebx2"), ("ebx2_r", "ebx2_r"),
+                         ("eb_abs", "eb_abs")]:
+        assert r[k_gpu] == info[k_ref], (k_gpu, r[k_gpu], info[k_ref])
+    assert np.array_equal(_np(payload), ref["payload"])
+    if mode == O.MODE_FIXED:
+        assert r["tail_len"] == info["tail_len"]
+        assert np.array_equal(_np(tail, np.uint16), ref["tail"])
+    return plan, payload, tail, r, ref
+
+
+@pytest.mark.parametrize("dims", SHAPES)
+@pytest.mark.parametrize("mode", [O.MODE_FIXED, O.MODE_LITERAL])
+def test_fused_compress_bit_exact(dims, mode):
+    x = _field(dims, sum(dims) + mode)
+    _check_against_oracle(x, dims, 1e-3, mode, fused=True)
+
+
+@pytest.mark.parametrize("dims", SHAPES)
+def test_plain_compress_bit_exact(dims):
+    x = _field(dims, sum(dims) + 7)
+    _check_against_oracle(x, dims, 1e-4, O.MODE_FIXED, fused=False)
+
+
+@pytest.mark.parametrize("dims", SHAPES)
+def test_decompress_bit_exact_and_error_bound(dims):
+    x = _field(dims, sum(dims) + 3)
+    plan, payload, tail, r, ref = _check_against_oracle(x, dims, 1e-3, O.MODE_FIXED, fused=True)
+    say(f"decompress dims={dims}")
+    xr = _np(plan.decompress(payload, tail, r["ebx2"]))
+    rc, xo = O.decompress(ref["payload"], ref["tail"], x.size, dims, ref["info"]["ebx2"])
+    assert rc == 0
+    assert np.array_equal(xr.view(np.uint32), xo.view(np.uint32))
+    tol = r["eb_abs"] + 4 * 2.0**-24 * float(np.abs(x).max())
+    assert float(np.abs(xr - x).max()) <= tol
+
+
+def test_len_longer_than_dims():
+    x = _field((6000, 1, 1), 21)
+    _check_against_oracle(x, (1000, 3, 1), 1e-3, O.MODE_FIXED, fused=True)
+    _check_against_oracle(x, (1000, 3, 1), 1e-3, O.MODE_FIXED, fused=False)
+
+
+def test_overflow_counted_like_the_oracle():
+    x = np.zeros(9000, np.float32)
+    x[100] = 1.0
+    x[5000:5100] = np.linspace(0, 1, 100, dtype=np.float32)
+    _check_against_oracle(x, (9000, 1, 1), 1e-6, O.MODE_FIXED, fused=True)
+
+
+def test_deterministic_repeats():
+    x = _field((3600, 90, 1), 5)
+    plan = fzgpu.Plan(3600, 90, 1)
+    d = _to_dev(x)
+    outs = [_np(plan.compress(d, 1e-3)[0]) for _ in range(4)]
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+
+
+def test_error_status_constant_and_nonfinite():
+    plan = fzgpu.Plan(5000, 1, 1)
+    with pytest.raises(fzgpu.FzError) as e:
+        plan.compress(_to_dev(np.full(5000, 2.5, np.float32)), 1e-3)
+    assert e.value.status == 4
+    x = _field((5000, 1, 1), 2)
+    x[4321] = np.inf
+    with pytest.raises(fzgpu.FzError) as e:
+        plan.compress(_to_dev(x), 1e-3)
+    assert e.value.status == 5
+    # the plan is still usable afterwards
+    _check_against_oracle(_field((5000, 1, 1), 3), (5000, 1, 1), 1e-3, O.MODE_FIXED, True)
+
+
+# ---------------------------------------------------------------- per-stage entry points
+
+@pytest.mark.parametrize("dims", [(5000, 1, 1), (100, 77, 1), (33, 20, 17)])
+@pytest.mark.parametrize("mode", [O.MODE_FIXED, O.MODE_LITERAL])
+def test_stage_lorenzo_construct(dims, mode):
+    x = _field(dims, 31)
+    say(f"stage lorenzo dims={dims} mode={mode}")
+    eb_abs = 1e-3 * float(x.max() - x.min())
+    codes, _ = fzgpu.lorenzo_construct(_to_dev(x), dims, eb_abs, mode)
+    ref, _ = O.dualquant(x, dims, np.float32(1.0 / (eb_abs * 2)), mode)
+    assert np.array_equal(_np(codes, np.uint16), ref)
+    back, _ = fzgpu.lorenzo_reconstruct(codes, dims, eb_abs)
+    assert np.array_equal(_np(back).view(np.uint32), O.reconstruct(ref, dims, np.float32(eb_abs * 2)).view(np.uint32))
+
+
+def test_stage_bitshuffle_scan_encode():
+    import torch
+    rng = np.random.default_rng(4)
+    n = 2048 * 7 + 300
+    codes = rng.integers(0, 65536, size=n, dtype=np.uint32).astype(np.uint16)
+    codes[rng.random(n) < 0.8] = 0
+    words = np.zeros((n + 1) // 2 + 8, np.uint32)
+    words.view(np.uint16)[:n] = codes
+    say("stage bitshuffle")
+    out, bf, F = fzgpu.bitshuffle_bitflag(_to_dev(words.view(np.int32)), n)
+    T = n // 2048
+    o_ref, bf_ref, F_ref = O.bitshuffle_flags(words, T)
+    assert np.array_equal(_np(out, np.uint32)[: 1024 * T], o_ref[: 1024 * T])
+    assert np.array_equal(_np(bf, np.uint32), bf_ref)
+    assert np.array_equal(_np(F, np.uint32), F_ref)
+    back = fzgpu.bitunshuffle(out, n)
+    assert np.array_equal(_np(back, np.uint32)[: 1024 * T], words[: 1024 * T])
+    say("stage scan")
+    for m in (1, 4095, 4096, 4097, 1_000_003):
+        a = rng.integers(0, 3, size=m, dtype=np.uint32)
+        ex = fzgpu.exclusive_sum(_to_dev(a.view(np.int32)))
+        ref = np.concatenate([[0], np.cumsum(a, dtype=np.uint64)[:-1]]).astype(np.uint32)
+        assert np.array_equal(_np(ex, np.uint32), ref), m
+    say("stage encode")
+    nb = 256 * T
+    P = np.concatenate([[0], np.cumsum(F_ref)]).astype(np.uint32)
+    enc = fzgpu.encode_blocks(out.view(torch.uint8), _to_dev(P.view(np.int32)), nb, 16 * int(P[nb]))
+    assert np.array_equal(_np(enc), O.encode(o_ref.view(np.uint8), P, nb))
+
+
+# ---------------------------------------------------------------- full-size workloads
+
+@pytest.mark.parametrize("name", list(D.CONFIGS))
+def test_full_size_configs_bit_exact(name):
+    x, dims, eb = D.config_field(name)
+    x = x.ravel()
+    say(f"full config {name}")
+    ref = O.compress(x, dims, eb, O.MODE_FIXED, nthreads=16)
+    assert ref["status"] == 0
+    plan = fzgpu.Plan(*dims)
+    d = _to_dev(x)
+    for fused in (True, False):
+        payload, tail, r = plan.compress(d, eb, fused=fused)
+        assert r["payload_bytes"] == ref["info"]["payload_bytes"]
+        assert np.array_equal(_np(payload), ref["payload"]), fused
+        assert np.array_equal(_np(tail, np.uint16), ref["tail"])
+    xr = _np(plan.decompress(payload, tail, r["ebx2"]))
+    tol = r["eb_abs"] + 4 * 2.0**-24 * float(np.abs(x).max())
+    err = float(np.abs(xr - x).max())
+    say(f"  CR {4 * x.size / r['payload_bytes']:.3f} maxerr/eb {err / r['eb_abs']:.6f}")
+    assert err <= tol
