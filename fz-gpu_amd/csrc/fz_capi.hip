@@ -25,7 +25,7 @@ Layout make_layout(const Geom& g)
 {
     Layout L{};
     const uint64_t N = 32ull * g.W;
-    L.n_tstate = (g.T + kGatherChunk - 1) / kGatherChunk + 1;
+    L.n_tstate = gather_grid(g.T) + 1;
     L.n_sstate = (uint32_t)((N + kScanChunk - 1) / kScanChunk) + 1;
     L.n_dstate = (g.T + kWG - 1) / kWG + 1;
     size_t o = 0;
@@ -226,10 +226,12 @@ fz_status fz_compress_f32(const fz_plan plan, const float* d_in, double eb_rel, 
     uint64_t*     tstate = reinterpret_cast<uint64_t*>(ws + L.tstate);
     FZ_TRY(prof_mark(plan, stream, true));
     FZ_TRY(launch_range(d_in, g, eb_rel, reinterpret_cast<uint64_t*>(ws + L.part), ctr, d_result, tstate, L.n_tstate,
-                        &ctr->fused_ticket, stream));
+                        nullptr, stream));
     FZ_TRY(prof_mark(plan, stream, false));
-    FZ_TRY(launch_fused_compress(d_in, g, d_result, ctr, tstate, d_payload, d_tail, ws + L.stage,
-                                 reinterpret_cast<uint32_t*>(ws + L.counts), stream));
+    uint32_t* counts = reinterpret_cast<uint32_t*>(ws + L.counts);
+    FZ_TRY(launch_tile_compress(d_in, g, d_result, d_payload, d_tail, ws + L.stage, counts, stream));
+    FZ_TRY(prof_mark(plan, stream, false));
+    FZ_TRY(launch_scan_gather(g, d_result, tstate, d_payload, ws + L.stage, counts, stream));
     FZ_TRY(prof_mark(plan, stream, false));
     return FZ_SUCCESS;
 }
@@ -253,7 +255,7 @@ fz_status fz_compress_plain_f32(const fz_plan plan, const float* d_in, double eb
     FZ_TRY(launch_range(d_in, g, eb_rel, reinterpret_cast<uint64_t*>(ws + L.part), ctr, d_result, sstate, L.n_sstate,
                         &ctr->scan_ticket, stream));
     FZ_TRY(prof_mark(plan, stream, false));
-    FZ_TRY(launch_dualquant(d_in, g, 0.0f, d_result, codes,
+    FZ_TRY(launch_dualquant(d_in, g, 0.0f, 0.0f, d_result, codes,
                             reinterpret_cast<unsigned long long*>(&d_result->overflow), stream));
     FZ_TRY(prof_mark(plan, stream, false));
     FZ_TRY(launch_bitshuffle(reinterpret_cast<const uint32_t*>(codes), shuf, g, reinterpret_cast<uint32_t*>(d_payload),
@@ -354,7 +356,8 @@ fz_status fz_lorenzo_construct_f32(const float* d_data, uint16_t* d_delta, uint3
     const double ebx2 = eb_abs * 2;        // lorenzo_var.cuh:562
     const float  ebx2_r = (float)(1 / ebx2);  // :563, narrowed at the kernel parameter
     EventTimer  tm(time_elapsed, stream);
-    FZ_TRY(launch_dualquant(d_data, g, ebx2_r, nullptr, d_delta, nullptr, stream));
+    // the fast paths need max|x| * ebx2_r < 2^23; without the range at hand take the guarded path
+    FZ_TRY(launch_dualquant(d_data, g, ebx2_r, __builtin_huge_valf(), nullptr, d_delta, nullptr, stream));
     FZ_TRY(tm.finish());
     return FZ_SUCCESS;
 }

@@ -20,13 +20,13 @@ struct Ctr {
     uint32_t pad[59];
 };
 
-constexpr uint32_t kRangeMaxWG = 1024;
+constexpr uint32_t kRangeMaxWG = 256;
 constexpr uint32_t kScanChunk = 4096;  // items per workgroup of the u32 scan
-constexpr uint32_t kGatherChunk = 256;  // tiles per workgroup of the compress scan + gather
+constexpr uint32_t kGatherWG = 256;  // max workgroups of the compress scan + gather
 
 inline uint32_t range_grid(uint32_t len)
 {
-    uint64_t g = ((uint64_t)len + 4095) / 4096;
+    uint64_t g = ((uint64_t)len + 8191) / 8192;
     if (g < 1) g = 1;
     if (g > kRangeMaxWG) g = kRangeMaxWG;
     return (uint32_t)g;
@@ -35,10 +35,14 @@ inline uint32_t range_grid(uint32_t len)
 // -------- launchers (return hipError_t of the launch)
 hipError_t launch_range(const float* in, const Geom& g, double eb_rel, uint64_t* part, Ctr* ctr, fz_result* res,
                         uint64_t* zero_states, uint32_t nzero, uint32_t* zero_ticket, hipStream_t s);
-hipError_t launch_fused_compress(const float* in, const Geom& g, fz_result* res, Ctr* ctr, uint64_t* states,
-                                 uint8_t* payload, uint16_t* tail, uint8_t* staging, uint32_t* counts, hipStream_t s);
-hipError_t launch_dualquant(const float* in, const Geom& g, float r_host, const fz_result* res, uint16_t* codes,
-                            unsigned long long* ovf, hipStream_t s);
+hipError_t launch_tile_compress(const float* in, const Geom& g, fz_result* res, uint8_t* payload, uint16_t* tail,
+                                uint8_t* staging, uint32_t* counts, hipStream_t s);
+hipError_t launch_scan_gather(const Geom& g, fz_result* res, uint64_t* states, uint8_t* payload,
+                              const uint8_t* staging, const uint32_t* counts, hipStream_t s);
+uint32_t   gather_grid(uint32_t T);
+// r_host / r_amax: ebx2_r and max|x| when res is null (the per-stage entry point)
+hipError_t launch_dualquant(const float* in, const Geom& g, float r_host, float r_amax, const fz_result* res,
+                            uint16_t* codes, unsigned long long* ovf, hipStream_t s);
 hipError_t launch_bitshuffle(const uint32_t* in, uint32_t* out, const Geom& g, uint32_t* bitflags,
                              uint32_t* byteflags, bool with_tail_wg, hipStream_t s);
 hipError_t launch_unshuffle(const uint32_t* in, uint32_t* out, uint32_t ntiles, hipStream_t s);

@@ -60,11 +60,19 @@ __global__ __launch_bounds__(kWG) void k_range(const float* __restrict__ in, Geo
     uint32_t bad = 0;
     const uint32_t n4 = g.len / 4;
     const float4*  in4 = reinterpret_cast<const float4*>(in);
-    for (uint32_t i = gtid; i < n4; i += gstride) {
-        const float4 v = in4[i];
-        lo = fminf(lo, fminf(fminf(v.x, v.y), fminf(v.z, v.w)));
-        hi = fmaxf(hi, fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w)));
-        bad |= nonfinite(v.x) | nonfinite(v.y) | nonfinite(v.z) | nonfinite(v.w);
+    // 8 independent 16-byte loads in flight per thread; indices past the end re-read the last
+    // vector, which leaves min / max / finiteness unchanged
+    constexpr uint32_t U = 8;
+    for (uint32_t base = blockIdx.x * (kWG * U) + tid; base < n4; base += gridDim.x * kWG * U) {
+        float4 v[U];
+#pragma unroll
+        for (uint32_t j = 0; j < U; j++) v[j] = in4[min(base + j * kWG, n4 - 1)];
+#pragma unroll
+        for (uint32_t j = 0; j < U; j++) {
+            lo = fminf(lo, fminf(fminf(v[j].x, v[j].y), fminf(v[j].z, v[j].w)));
+            hi = fmaxf(hi, fmaxf(fmaxf(v[j].x, v[j].y), fmaxf(v[j].z, v[j].w)));
+            bad |= nonfinite(v[j].x) | nonfinite(v[j].y) | nonfinite(v[j].z) | nonfinite(v[j].w);
+        }
     }
     for (uint32_t i = 4 * n4 + gtid; i < g.len; i += gstride) {
         const float v = in[i];
@@ -272,33 +280,35 @@ __global__ __launch_bounds__(kWG) void k_tile_compress(const float* __restrict__
     if (f) *reinterpret_cast<uint4*>(staging + (size_t)tile * 4096 + 16 * (woff + rank)) = blk;
 }
 
-constexpr uint32_t kGatherTiles = kGatherChunk;
-
-__global__ __launch_bounds__(kWG) void k_scan_gather(const uint32_t* __restrict__ counts, uint32_t T,
+// Scan of the tile counts + placement.  The grid is small (<= kGatherWG workgroups, each a
+// contiguous run of tiles) and uses blockIdx order for the look-back: at most
+// kGatherWG - 1 workgroups can be waiting, far fewer than the device holds, so a workgroup
+// that has not started yet always finds a free slot and the chain completes.
+__global__ __launch_bounds__(kWG) void k_scan_gather(const uint32_t* __restrict__ counts, uint32_t T, uint32_t tpw,
                                                      const uint8_t* __restrict__ staging, uint8_t* __restrict__ payload,
-                                                     uint64_t* states, uint32_t* ticket, fz_result* res, uint32_t W)
+                                                     uint64_t* states, fz_result* res, uint32_t W)
 {
-    __shared__ uint32_t sh_off[kGatherTiles];
+    __shared__ uint32_t sh_off[kWG];
     __shared__ uint32_t sh_w[4];
-    __shared__ uint32_t sh_chunk, sh_excl;
+    __shared__ uint32_t sh_excl, sh_base;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     if (res->status != ST_SUCCESS) return;
-    if (tid == 0) sh_chunk = atomicAdd(ticket, 1u);
+    const uint32_t wg = blockIdx.x;
+    const uint32_t t_lo = wg * tpw, t_hi = min(T, t_lo + tpw);
+    // 1. this workgroup's block count
+    uint32_t part = 0;
+    for (uint32_t t = t_lo + tid; t < t_hi; t += kWG) part += counts[t];
+    part = wave_sum(part);
+    if (lane == 0) sh_w[wv] = part;
     __syncthreads();
-    const uint32_t chunk = sh_chunk, t = chunk * kGatherTiles + tid;
-    const uint32_t cnt = t < T ? counts[t] : 0u;
-    const uint32_t incl = wave_incl_scan(cnt);
-    if (lane == 63) sh_w[wv] = incl;
-    __syncthreads();
-    const uint32_t w0 = sh_w[0], w1 = sh_w[1], w2 = sh_w[2], w3 = sh_w[3];
-    const uint32_t agg = w0 + w1 + w2 + w3;
-    const uint32_t woff = (wv > 0 ? w0 : 0u) + (wv > 1 ? w1 : 0u) + (wv > 2 ? w2 : 0u);
-    sh_off[tid] = woff + incl - cnt;
+    const uint32_t agg = sh_w[0] + sh_w[1] + sh_w[2] + sh_w[3];
+    // 2. look-back in blockIdx order
     if (wv == 0) {
-        const uint32_t excl = lookback(states, chunk, agg);
+        const uint32_t excl = lookback(states, wg, agg);
         if (lane == 0) {
             sh_excl = excl;
-            if (chunk == (T - 1) / kGatherTiles) {
+            sh_base = 0;
+            if (wg == gridDim.x - 1) {
                 res->nblocks = (uint64_t)excl + agg;
                 res->payload_bytes = 4ull * W + 16ull * ((uint64_t)excl + agg);
             }
@@ -306,22 +316,38 @@ __global__ __launch_bounds__(kWG) void k_scan_gather(const uint32_t* __restrict_
     }
     __syncthreads();
     uint8_t* out = payload + 4ull * W + 16ull * sh_excl;
-    for (uint32_t j = tid; j < agg; j += kWG) {
-        // last tile k of the chunk whose first block index is <= j
-        uint32_t lo = 0, hi = kGatherTiles - 1;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1) >> 1;
-            if (sh_off[mid] <= j) lo = mid;
-            else hi = mid - 1;
+    // 3. runs of 256 tiles: offsets, then one 16-byte block per thread
+    for (uint32_t r0 = t_lo; r0 < t_hi; r0 += kWG) {
+        const uint32_t t = r0 + tid;
+        const uint32_t cnt = t < t_hi ? counts[t] : 0u;
+        const uint32_t incl = wave_incl_scan(cnt);
+        __syncthreads();
+        if (lane == 63) sh_w[wv] = incl;
+        __syncthreads();
+        const uint32_t w0 = sh_w[0], w1 = sh_w[1], w2 = sh_w[2], w3 = sh_w[3];
+        const uint32_t run = w0 + w1 + w2 + w3;
+        const uint32_t base = sh_base;
+        sh_off[tid] = (wv > 0 ? w0 : 0u) + (wv > 1 ? w1 : 0u) + (wv > 2 ? w2 : 0u) + incl - cnt;
+        __syncthreads();
+        for (uint32_t j = tid; j < run; j += kWG) {
+            uint32_t lo = 0, hi = kWG - 1;  // last tile of the run whose first block is <= j
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (sh_off[mid] <= j) lo = mid;
+                else hi = mid - 1;
+            }
+            const uint4 v = *reinterpret_cast<const uint4*>(staging + (size_t)(r0 + lo) * 4096 + 16 * (j - sh_off[lo]));
+            st16(out + 16ull * (base + j), v);
         }
-        const uint4 v = *reinterpret_cast<const uint4*>(staging + (size_t)(chunk * kGatherTiles + lo) * 4096 +
-                                                       16 * (j - sh_off[lo]));
-        st16(out + 16ull * j, v);
+        __syncthreads();
+        if (tid == 0) sh_base = base + run;
     }
 }
 
-hipError_t launch_fused_compress(const float* in, const Geom& g, fz_result* res, Ctr* ctr, uint64_t* states,
-                                 uint8_t* payload, uint16_t* tail, uint8_t* staging, uint32_t* counts, hipStream_t s)
+inline uint32_t gather_tpw(uint32_t T) { return (T + kGatherWG - 1) / kGatherWG; }
+
+hipError_t launch_tile_compress(const float* in, const Geom& g, fz_result* res, uint8_t* payload, uint16_t* tail,
+                                uint8_t* staging, uint32_t* counts, hipStream_t s)
 {
     const dim3 grid(g.T + 1);
     switch (g.ndim) {
@@ -329,11 +355,17 @@ hipError_t launch_fused_compress(const float* in, const Geom& g, fz_result* res,
         case 2: FZ_LAUNCH(k_tile_compress<2><<<grid, kWG, 0, s>>>(in, g, res, payload, staging, counts, tail)); break;
         default: FZ_LAUNCH(k_tile_compress<3><<<grid, kWG, 0, s>>>(in, g, res, payload, staging, counts, tail)); break;
     }
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess || g.T == 0) return e;
-    const uint32_t nchunks = (g.T + kGatherTiles - 1) / kGatherTiles;
-    FZ_LAUNCH(k_scan_gather<<<nchunks, kWG, 0, s>>>(counts, g.T, staging, payload, states, &ctr->fused_ticket, res,
-                                                   g.W));
+    return hipGetLastError();
+}
+
+uint32_t gather_grid(uint32_t T) { return T == 0 ? 0u : (T + gather_tpw(T) - 1) / gather_tpw(T); }
+
+hipError_t launch_scan_gather(const Geom& g, fz_result* res, uint64_t* states, uint8_t* payload,
+                              const uint8_t* staging, const uint32_t* counts, hipStream_t s)
+{
+    if (g.T == 0) return hipSuccess;
+    const uint32_t tpw = gather_tpw(g.T);
+    FZ_LAUNCH(k_scan_gather<<<gather_grid(g.T), kWG, 0, s>>>(counts, g.T, tpw, staging, payload, states, res, g.W));
     return hipGetLastError();
 }
 
@@ -341,7 +373,7 @@ hipError_t launch_fused_compress(const float* in, const Geom& g, fz_result* res,
 
 // Lorenzo stage: 8 codes per thread for every position < len (codes beyond x*y*z are 0).
 template <int NDIM>
-__global__ __launch_bounds__(kWG) void k_dualquant(const float* __restrict__ in, Geom g, float r_host,
+__global__ __launch_bounds__(kWG) void k_dualquant(const float* __restrict__ in, Geom g, float r_host, float r_amax,
                                                    const fz_result* res, uint16_t* __restrict__ codes,
                                                    unsigned long long* ovf_out)
 {
@@ -349,9 +381,12 @@ __global__ __launch_bounds__(kWG) void k_dualquant(const float* __restrict__ in,
     const float    r = res ? res->ebx2_r : r_host;
     const uint32_t p0 = 8 * (blockIdx.x * kWG + threadIdx.x);
     uint32_t       ovf = 0;
-    if (p0 < g.len) {
+    // one workgroup = one 2048-position tile: same path choice as the fused kernel
+    const float amax = res ? fmaxf(__builtin_fabsf(res->vmin), __builtin_fabsf(res->vmax)) : r_amax;
+    const int   path = tile_path<NDIM>(g, blockIdx.x * (uint32_t)kTileCodes, amax * r < 8388608.0f);
+    if (path != 0 || p0 < g.len) {
         uint32_t w[4];
-        dq8<NDIM>(in, p0, g, r, w, ovf);
+        dq8_tile<NDIM>(in, p0, g, r, path, w, ovf);
         if (p0 + 8 <= g.len) {
             *reinterpret_cast<uint4*>(codes + p0) = make_uint4(w[0], w[1], w[2], w[3]);
         } else {
@@ -366,15 +401,15 @@ __global__ __launch_bounds__(kWG) void k_dualquant(const float* __restrict__ in,
     }
 }
 
-hipError_t launch_dualquant(const float* in, const Geom& g, float r_host, const fz_result* res, uint16_t* codes,
-                            unsigned long long* ovf, hipStream_t s)
+hipError_t launch_dualquant(const float* in, const Geom& g, float r_host, float r_amax, const fz_result* res,
+                            uint16_t* codes, unsigned long long* ovf, hipStream_t s)
 {
     const uint32_t groups = (g.len + 7) / 8;
     const dim3     grid((groups + kWG - 1) / kWG);
     switch (g.ndim) {
-        case 1: FZ_LAUNCH(k_dualquant<1><<<grid, kWG, 0, s>>>(in, g, r_host, res, codes, ovf)); break;
-        case 2: FZ_LAUNCH(k_dualquant<2><<<grid, kWG, 0, s>>>(in, g, r_host, res, codes, ovf)); break;
-        default: FZ_LAUNCH(k_dualquant<3><<<grid, kWG, 0, s>>>(in, g, r_host, res, codes, ovf)); break;
+        case 1: FZ_LAUNCH(k_dualquant<1><<<grid, kWG, 0, s>>>(in, g, r_host, r_amax, res, codes, ovf)); break;
+        case 2: FZ_LAUNCH(k_dualquant<2><<<grid, kWG, 0, s>>>(in, g, r_host, r_amax, res, codes, ovf)); break;
+        default: FZ_LAUNCH(k_dualquant<3><<<grid, kWG, 0, s>>>(in, g, r_host, r_amax, res, codes, ovf)); break;
     }
     return hipGetLastError();
 }

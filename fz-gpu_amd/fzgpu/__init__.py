@@ -70,6 +70,12 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"FZ HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+    # torch ships its own libamdhip64.so.7; load it first so that this process has a single
+    # HIP runtime (the library's NEEDED entry then resolves to the copy already loaded)
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp, u32, u64, i32, f32, f64 = (ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int,
                                    ctypes.c_float, ctypes.c_double)
