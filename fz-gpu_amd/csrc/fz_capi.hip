@@ -222,16 +222,16 @@ fz_status fz_compress_f32(const fz_plan plan, const float* d_in, double eb_rel, 
     const Geom&   g = plan->g;
     const Layout& L = plan->L;
     uint8_t*      ws = static_cast<uint8_t*>(d_ws);
-    Ctr*          ctr = reinterpret_cast<Ctr*>(ws + L.ctr);
     uint64_t*     tstate = reinterpret_cast<uint64_t*>(ws + L.tstate);
     FZ_TRY(prof_mark(plan, stream, true));
-    FZ_TRY(launch_range(d_in, g, eb_rel, reinterpret_cast<uint64_t*>(ws + L.part), ctr, d_result, tstate, L.n_tstate,
+    FZ_TRY(launch_range(d_in, g, eb_rel, reinterpret_cast<float4*>(ws + L.part), d_result, tstate, L.n_tstate,
                         nullptr, stream));
     FZ_TRY(prof_mark(plan, stream, false));
     uint32_t* counts = reinterpret_cast<uint32_t*>(ws + L.counts);
     FZ_TRY(launch_tile_compress(d_in, g, d_result, d_payload, d_tail, ws + L.stage, counts, stream));
     FZ_TRY(prof_mark(plan, stream, false));
-    FZ_TRY(launch_scan_gather(g, d_result, tstate, d_payload, ws + L.stage, counts, stream));
+    FZ_TRY(launch_scan_gather(g, d_result, tstate, d_payload, ws + L.stage, counts,
+                              reinterpret_cast<uint32_t*>(ws + L.texcl), stream));
     FZ_TRY(prof_mark(plan, stream, false));
     return FZ_SUCCESS;
 }
@@ -252,7 +252,7 @@ fz_status fz_compress_plain_f32(const fz_plan plan, const float* d_in, double eb
     uint32_t*     P = reinterpret_cast<uint32_t*>(ws + L.P);
     const uint32_t N = 32u * g.W;
     FZ_TRY(prof_mark(plan, stream, true));
-    FZ_TRY(launch_range(d_in, g, eb_rel, reinterpret_cast<uint64_t*>(ws + L.part), ctr, d_result, sstate, L.n_sstate,
+    FZ_TRY(launch_range(d_in, g, eb_rel, reinterpret_cast<float4*>(ws + L.part), d_result, sstate, L.n_sstate,
                         &ctr->scan_ticket, stream));
     FZ_TRY(prof_mark(plan, stream, false));
     FZ_TRY(launch_dualquant(d_in, g, 0.0f, 0.0f, d_result, codes,

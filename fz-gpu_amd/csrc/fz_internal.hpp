@@ -10,35 +10,37 @@
 
 namespace fz {
 
-// Counters at the start of every workspace (one 256-byte line).
+constexpr uint32_t kRangeMaxWG = 1024;  // partial-range workgroups (a multiple of 8)
+
+// Counters at the start of every workspace.
 struct Ctr {
-    uint32_t range_done;    // last-workgroup detection of the range kernel (self-resetting)
-    uint32_t fused_ticket;  // tile tickets of the fused compress kernel (reset by the range kernel)
+    uint32_t unused0;
+    uint32_t unused1;
     uint32_t scan_ticket;   // chunk tickets of the flag scan (reset by the range kernel / scan init)
     uint32_t dscan_ticket;  // tickets of the decompress tile-count scan (reset by its init)
-    uint32_t dec_ticket;    // unused slot
-    uint32_t pad[59];
+    uint32_t pad[60];
 };
 
-constexpr uint32_t kRangeMaxWG = 256;
 constexpr uint32_t kScanChunk = 4096;  // items per workgroup of the u32 scan
 constexpr uint32_t kGatherWG = 256;  // max workgroups of the compress scan + gather
 
+// partial-range workgroups: a multiple of 8 (one slice per XCD region), ~16K floats each
 inline uint32_t range_grid(uint32_t len)
 {
-    uint64_t g = ((uint64_t)len + 8191) / 8192;
-    if (g < 1) g = 1;
+    uint64_t g = ((uint64_t)len + 16383) / 16384;
+    g = (g + 7) & ~7ull;
+    if (g < 8) g = 8;
     if (g > kRangeMaxWG) g = kRangeMaxWG;
     return (uint32_t)g;
 }
 
 // -------- launchers (return hipError_t of the launch)
-hipError_t launch_range(const float* in, const Geom& g, double eb_rel, uint64_t* part, Ctr* ctr, fz_result* res,
+hipError_t launch_range(const float* in, const Geom& g, double eb_rel, float4* part, fz_result* res,
                         uint64_t* zero_states, uint32_t nzero, uint32_t* zero_ticket, hipStream_t s);
 hipError_t launch_tile_compress(const float* in, const Geom& g, fz_result* res, uint8_t* payload, uint16_t* tail,
                                 uint8_t* staging, uint32_t* counts, hipStream_t s);
 hipError_t launch_scan_gather(const Geom& g, fz_result* res, uint64_t* states, uint8_t* payload,
-                              const uint8_t* staging, const uint32_t* counts, hipStream_t s);
+                              const uint8_t* staging, const uint32_t* counts, uint32_t* tile_excl, hipStream_t s);
 uint32_t   gather_grid(uint32_t T);
 // r_host / r_amax: ebx2_r and max|x| when res is null (the per-stage entry point)
 hipError_t launch_dualquant(const float* in, const Geom& g, float r_host, float r_amax, const fz_result* res,

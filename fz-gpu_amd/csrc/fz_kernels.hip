@@ -40,45 +40,68 @@ __device__ __forceinline__ uint32_t nonfinite(float v) { return (__float_as_uint
 
 // ============================================================================ range
 
-// Per-workgroup min / max / non-finite flag; the last workgroup to finish (atomic count)
-// reduces the partials and writes the eb parameters into the result record:
+// Value range in two kernels: k_range_part (per-workgroup min / max / non-finite flag, plain
+// stores, no fences) and k_range_final (one workgroup: reduce the partials, derive the eb
+// parameters, clear the look-back state of the kernels that follow).  The kernel boundary
+// orders the partial stores before the final reads; an in-kernel last-workgroup scheme would
+// need a device-scope release per workgroup, which writes back L2 on this part.
+//
 //   eb_abs = eb_rel * (double)(max - min);  ebx2 = eb_abs * 2;  ebx2_r = 1 / ebx2  (double),
-//   both narrowed to float.  The workgroups also clear the look-back state of the kernel
-//   that follows (zero_states / zero_ticket).
-__global__ __launch_bounds__(kWG) void k_range(const float* __restrict__ in, Geom g, double eb_rel,
-                                               uint64_t* part, Ctr* ctr, fz_result* res, uint64_t* zero_states,
-                                               uint32_t nzero, uint32_t* zero_ticket)
+//   both narrowed to float (fz.cu:137, 149; lorenzo_var.cuh:562-563).
+//
+// Workgroup b covers the b/8-th of the 128 slices of region b%8, where region x is the
+// x-th eighth of the field in the XCD-aware tile order of k_tile_compress: workgroups are
+// placed round-robin over the 8 XCDs, so the slices an XCD reduces are the tiles it then
+// compresses (a placement hint only; any placement gives the same result).
+
+__device__ __forceinline__ uint32_t region_start(uint32_t x, uint32_t n)
+{
+    return x * (n >> 3) + min(x, n & 7u);
+}
+
+// tile of workgroup b < n in the XCD-aware order (bijective on [0, n))
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n)
+{
+    return region_start(b & 7u, n) + (b >> 3);
+}
+
+__global__ __launch_bounds__(kWG) void k_range_part(const float* __restrict__ in, Geom g, float4* __restrict__ part)
 {
     __shared__ float    s_lo[4], s_hi[4];
     __shared__ uint32_t s_bad[4];
-    __shared__ uint32_t s_last;
-    const uint32_t tid = threadIdx.x, gtid = blockIdx.x * kWG + tid, gstride = gridDim.x * kWG;
-    for (uint32_t i = gtid; i < nzero; i += gstride) zero_states[i] = 0;
-    if (gtid == 0 && zero_ticket) *zero_ticket = 0;
-
+    const uint32_t tid = threadIdx.x;
+    // this workgroup's span of float4s: slice (b >> 3) of region (b & 7), regions in tile units
+    const uint32_t nunits = g.T + 1;  // the last unit is the partial tile
+    const uint32_t x = blockIdx.x & 7u, i = blockIdx.x >> 3, per = gridDim.x >> 3;
+    const uint32_t r_lo = region_start(x, nunits), r_hi = region_start(x + 1, nunits);
+    const uint32_t u_lo = r_lo + (uint32_t)(((uint64_t)(r_hi - r_lo) * i) / per);
+    const uint32_t u_hi = r_lo + (uint32_t)(((uint64_t)(r_hi - r_lo) * (i + 1)) / per);
+    const uint32_t f_lo = min(u_lo * (uint32_t)kTileCodes, g.len), f_hi = min(u_hi * (uint32_t)kTileCodes, g.len);
     float    lo = __builtin_huge_valf(), hi = -__builtin_huge_valf();
     uint32_t bad = 0;
-    const uint32_t n4 = g.len / 4;
+    const uint32_t v_lo = (f_lo + 3) / 4, v_hi = f_hi / 4;  // whole float4s inside [f_lo, f_hi)
     const float4*  in4 = reinterpret_cast<const float4*>(in);
-    // 8 independent 16-byte loads in flight per thread; indices past the end re-read the last
-    // vector, which leaves min / max / finiteness unchanged
-    constexpr uint32_t U = 8;
-    for (uint32_t base = blockIdx.x * (kWG * U) + tid; base < n4; base += gridDim.x * kWG * U) {
-        float4 v[U];
+    constexpr uint32_t U = 4;
+    if (v_lo < v_hi) {
+        for (uint32_t base = v_lo + tid; base < v_hi; base += kWG * U) {
+            float4 v[U];
 #pragma unroll
-        for (uint32_t j = 0; j < U; j++) v[j] = in4[min(base + j * kWG, n4 - 1)];
+            for (uint32_t j = 0; j < U; j++) v[j] = in4[min(base + j * kWG, v_hi - 1)];  // re-reads are harmless
 #pragma unroll
-        for (uint32_t j = 0; j < U; j++) {
-            lo = fminf(lo, fminf(fminf(v[j].x, v[j].y), fminf(v[j].z, v[j].w)));
-            hi = fmaxf(hi, fmaxf(fmaxf(v[j].x, v[j].y), fmaxf(v[j].z, v[j].w)));
-            bad |= nonfinite(v[j].x) | nonfinite(v[j].y) | nonfinite(v[j].z) | nonfinite(v[j].w);
+            for (uint32_t j = 0; j < U; j++) {
+                lo = fminf(lo, fminf(fminf(v[j].x, v[j].y), fminf(v[j].z, v[j].w)));
+                hi = fmaxf(hi, fmaxf(fmaxf(v[j].x, v[j].y), fmaxf(v[j].z, v[j].w)));
+                bad |= nonfinite(v[j].x) | nonfinite(v[j].y) | nonfinite(v[j].z) | nonfinite(v[j].w);
+            }
         }
     }
-    for (uint32_t i = 4 * n4 + gtid; i < g.len; i += gstride) {
-        const float v = in[i];
-        lo = fminf(lo, v);
-        hi = fmaxf(hi, v);
-        bad |= nonfinite(v);
+    // scalars at the span's edges (fewer than 4 each side)
+    const uint32_t e0 = min(4 * v_lo, f_hi), e1 = max(4 * v_hi, e0);
+    for (uint32_t k = f_lo + tid; k < e0; k += kWG) {
+        lo = fminf(lo, in[k]); hi = fmaxf(hi, in[k]); bad |= nonfinite(in[k]);
+    }
+    for (uint32_t k = e1 + tid; k < f_hi; k += kWG) {
+        lo = fminf(lo, in[k]); hi = fmaxf(hi, in[k]); bad |= nonfinite(in[k]);
     }
     lo = wave_min(lo);
     hi = wave_max(hi);
@@ -90,33 +113,33 @@ __global__ __launch_bounds__(kWG) void k_range(const float* __restrict__ in, Geo
         s_bad[wv] = bad;
     }
     __syncthreads();
-    if (tid == 0) {
-        lo = fminf(fminf(s_lo[0], s_lo[1]), fminf(s_lo[2], s_lo[3]));
-        hi = fmaxf(fmaxf(s_hi[0], s_hi[1]), fmaxf(s_hi[2], s_hi[3]));
-        bad = s_bad[0] | s_bad[1] | s_bad[2] | s_bad[3];
-        const uint64_t packed = (uint64_t)__float_as_uint(lo) | ((uint64_t)__float_as_uint(hi) << 32);
-        __hip_atomic_store(&part[2 * blockIdx.x], packed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&part[2 * blockIdx.x + 1], (uint64_t)bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        s_last = (atomicAdd(&ctr->range_done, 1u) == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    lo = __builtin_huge_valf();
-    hi = -__builtin_huge_valf();
-    bad = 0;
-    for (uint32_t i = tid; i < gridDim.x; i += kWG) {
-        const uint64_t p = __hip_atomic_load(&part[2 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint64_t b = __hip_atomic_load(&part[2 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        lo = fminf(lo, __uint_as_float((uint32_t)p));
-        hi = fmaxf(hi, __uint_as_float((uint32_t)(p >> 32)));
-        bad |= (uint32_t)b;
+    if (tid == 0)
+        part[blockIdx.x] = make_float4(fminf(fminf(s_lo[0], s_lo[1]), fminf(s_lo[2], s_lo[3])),
+                                       fmaxf(fmaxf(s_hi[0], s_hi[1]), fmaxf(s_hi[2], s_hi[3])),
+                                       __uint_as_float(s_bad[0] | s_bad[1] | s_bad[2] | s_bad[3]), 0.0f);
+}
+
+__global__ __launch_bounds__(kWG) void k_range_final(const float4* __restrict__ part, uint32_t nparts, Geom g,
+                                                     double eb_rel, fz_result* res, uint64_t* zero_states,
+                                                     uint32_t nzero, uint32_t* zero_ticket)
+{
+    __shared__ float    s_lo[4], s_hi[4];
+    __shared__ uint32_t s_bad[4];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = tid; i < nzero; i += kWG) zero_states[i] = 0;
+    if (tid == 0 && zero_ticket) *zero_ticket = 0;
+    float    lo = __builtin_huge_valf(), hi = -__builtin_huge_valf();
+    uint32_t bad = 0;
+    for (uint32_t i = tid; i < nparts; i += kWG) {
+        const float4 p = part[i];
+        lo = fminf(lo, p.x);
+        hi = fmaxf(hi, p.y);
+        bad |= __float_as_uint(p.z);
     }
     lo = wave_min(lo);
     hi = wave_max(hi);
     bad = ballot64(bad != 0) != 0;
-    __syncthreads();
+    const uint32_t wv = tid >> 6;
     if ((tid & 63) == 0) {
         s_lo[wv] = lo;
         s_hi[wv] = hi;
@@ -142,14 +165,17 @@ __global__ __launch_bounds__(kWG) void k_range(const float* __restrict__ in, Geo
         res->payload_bytes = 0;
         res->tail_len = g.mode == 0 ? (uint64_t)(g.len - (uint32_t)kTileCodes * g.T) : 0;
         res->overflow = 0;
-        ctr->range_done = 0;
     }
 }
 
-hipError_t launch_range(const float* in, const Geom& g, double eb_rel, uint64_t* part, Ctr* ctr, fz_result* res,
+hipError_t launch_range(const float* in, const Geom& g, double eb_rel, float4* part, fz_result* res,
                         uint64_t* zero_states, uint32_t nzero, uint32_t* zero_ticket, hipStream_t s)
 {
-    FZ_LAUNCH(k_range<<<range_grid(g.len), kWG, 0, s>>>(in, g, eb_rel, part, ctr, res, zero_states, nzero, zero_ticket));
+    const uint32_t nparts = range_grid(g.len);
+    FZ_LAUNCH(k_range_part<<<nparts, kWG, 0, s>>>(in, g, part));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    FZ_LAUNCH(k_range_final<<<1, kWG, 0, s>>>(part, nparts, g, eb_rel, res, zero_states, nzero, zero_ticket));
     return hipGetLastError();
 }
 
@@ -224,19 +250,21 @@ __global__ __launch_bounds__(kWG) void k_tile_compress(const float* __restrict__
     __shared__ uint32_t sh_ovf;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     if (res->status != ST_SUCCESS) return;  // the range kernel rejected the field (uniform)
-    const uint32_t tile = blockIdx.x;
-    const float    r = res->ebx2_r;
-    uint32_t*      bwords = reinterpret_cast<uint32_t*>(payload);
-    uint32_t       ovf = 0;
+    const float r = res->ebx2_r;
+    uint32_t*   bwords = reinterpret_cast<uint32_t*>(payload);
+    uint32_t    ovf = 0;
     if (tid == 0) sh_ovf = 0;
 
-    if (tile >= g.T) {
+    if (blockIdx.x >= g.T) {
         tail_work<NDIM>(in, g, r, res, bwords, tail, ovf);
         ovf = wave_sum(ovf);
         if (lane == 0 && ovf) atomicAdd(reinterpret_cast<unsigned long long*>(&res->overflow), (unsigned long long)ovf);
         return;
     }
 
+    // XCD-aware order: an XCD walks its eighth of the field, so the neighbour rows a tile
+    // re-reads were fetched into the same L2 moments before
+    const uint32_t tile = xcd_tile(blockIdx.x, g.T);
     // |s| < 2^23 everywhere: the fast paths' integer conversions are exact
     const float amax = fmaxf(__builtin_fabsf(res->vmin), __builtin_fabsf(res->vmax));
     const bool  safe = amax * r < 8388608.0f;
@@ -280,34 +308,30 @@ __global__ __launch_bounds__(kWG) void k_tile_compress(const float* __restrict__
     if (f) *reinterpret_cast<uint4*>(staging + (size_t)tile * 4096 + 16 * (woff + rank)) = blk;
 }
 
-// Scan of the tile counts + placement.  The grid is small (<= kGatherWG workgroups, each a
-// contiguous run of tiles) and uses blockIdx order for the look-back: at most
-// kGatherWG - 1 workgroups can be waiting, far fewer than the device holds, so a workgroup
-// that has not started yet always finds a free slot and the chain completes.
-__global__ __launch_bounds__(kWG) void k_scan_gather(const uint32_t* __restrict__ counts, uint32_t T, uint32_t tpw,
-                                                     const uint8_t* __restrict__ staging, uint8_t* __restrict__ payload,
-                                                     uint64_t* states, fz_result* res, uint32_t W)
+// Exclusive scan of the tile counts -> tile_excl[].  The grid is small (<= kGatherWG
+// workgroups, each a contiguous run of tiles) and uses blockIdx order for the look-back: at
+// most kGatherWG - 1 workgroups can be waiting, far fewer than the device holds, so a
+// workgroup that has not started yet always finds a free slot and the chain completes.
+__global__ __launch_bounds__(kWG) void k_tile_offsets(const uint32_t* __restrict__ counts, uint32_t T, uint32_t tpw,
+                                                      uint32_t* __restrict__ tile_excl, uint64_t* states,
+                                                      fz_result* res, uint32_t W)
 {
-    __shared__ uint32_t sh_off[kWG];
     __shared__ uint32_t sh_w[4];
-    __shared__ uint32_t sh_excl, sh_base;
+    __shared__ uint32_t sh_excl;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     if (res->status != ST_SUCCESS) return;
     const uint32_t wg = blockIdx.x;
     const uint32_t t_lo = wg * tpw, t_hi = min(T, t_lo + tpw);
-    // 1. this workgroup's block count
     uint32_t part = 0;
     for (uint32_t t = t_lo + tid; t < t_hi; t += kWG) part += counts[t];
     part = wave_sum(part);
     if (lane == 0) sh_w[wv] = part;
     __syncthreads();
     const uint32_t agg = sh_w[0] + sh_w[1] + sh_w[2] + sh_w[3];
-    // 2. look-back in blockIdx order
     if (wv == 0) {
         const uint32_t excl = lookback(states, wg, agg);
         if (lane == 0) {
             sh_excl = excl;
-            sh_base = 0;
             if (wg == gridDim.x - 1) {
                 res->nblocks = (uint64_t)excl + agg;
                 res->payload_bytes = 4ull * W + 16ull * ((uint64_t)excl + agg);
@@ -315,8 +339,7 @@ __global__ __launch_bounds__(kWG) void k_scan_gather(const uint32_t* __restrict_
         }
     }
     __syncthreads();
-    uint8_t* out = payload + 4ull * W + 16ull * sh_excl;
-    // 3. runs of 256 tiles: offsets, then one 16-byte block per thread
+    uint32_t base = sh_excl;
     for (uint32_t r0 = t_lo; r0 < t_hi; r0 += kWG) {
         const uint32_t t = r0 + tid;
         const uint32_t cnt = t < t_hi ? counts[t] : 0u;
@@ -325,23 +348,22 @@ __global__ __launch_bounds__(kWG) void k_scan_gather(const uint32_t* __restrict_
         if (lane == 63) sh_w[wv] = incl;
         __syncthreads();
         const uint32_t w0 = sh_w[0], w1 = sh_w[1], w2 = sh_w[2], w3 = sh_w[3];
-        const uint32_t run = w0 + w1 + w2 + w3;
-        const uint32_t base = sh_base;
-        sh_off[tid] = (wv > 0 ? w0 : 0u) + (wv > 1 ? w1 : 0u) + (wv > 2 ? w2 : 0u) + incl - cnt;
-        __syncthreads();
-        for (uint32_t j = tid; j < run; j += kWG) {
-            uint32_t lo = 0, hi = kWG - 1;  // last tile of the run whose first block is <= j
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi + 1) >> 1;
-                if (sh_off[mid] <= j) lo = mid;
-                else hi = mid - 1;
-            }
-            const uint4 v = *reinterpret_cast<const uint4*>(staging + (size_t)(r0 + lo) * 4096 + 16 * (j - sh_off[lo]));
-            st16(out + 16ull * (base + j), v);
-        }
-        __syncthreads();
-        if (tid == 0) sh_base = base + run;
+        if (t < t_hi) tile_excl[t] = base + (wv > 0 ? w0 : 0u) + (wv > 1 ? w1 : 0u) + (wv > 2 ? w2 : 0u) + incl - cnt;
+        base += w0 + w1 + w2 + w3;
     }
+}
+
+// One workgroup per tile: its staged blocks -> payload + 4 W + 16 * (tile_excl + i).
+__global__ __launch_bounds__(kWG) void k_gather(const uint32_t* __restrict__ counts,
+                                                const uint32_t* __restrict__ tile_excl,
+                                                const uint8_t* __restrict__ staging, uint8_t* __restrict__ payload,
+                                                const fz_result* res, uint32_t W)
+{
+    if (res->status != ST_SUCCESS) return;
+    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x), i = threadIdx.x;  // same XCD as the tile's writer
+    if (i < counts[t])
+        st16(payload + 4ull * W + 16ull * (tile_excl[t] + i),
+             *reinterpret_cast<const uint4*>(staging + (size_t)t * 4096 + 16 * i));
 }
 
 inline uint32_t gather_tpw(uint32_t T) { return (T + kGatherWG - 1) / kGatherWG; }
@@ -361,11 +383,14 @@ hipError_t launch_tile_compress(const float* in, const Geom& g, fz_result* res, 
 uint32_t gather_grid(uint32_t T) { return T == 0 ? 0u : (T + gather_tpw(T) - 1) / gather_tpw(T); }
 
 hipError_t launch_scan_gather(const Geom& g, fz_result* res, uint64_t* states, uint8_t* payload,
-                              const uint8_t* staging, const uint32_t* counts, hipStream_t s)
+                              const uint8_t* staging, const uint32_t* counts, uint32_t* tile_excl, hipStream_t s)
 {
     if (g.T == 0) return hipSuccess;
     const uint32_t tpw = gather_tpw(g.T);
-    FZ_LAUNCH(k_scan_gather<<<gather_grid(g.T), kWG, 0, s>>>(counts, g.T, tpw, staging, payload, states, res, g.W));
+    FZ_LAUNCH(k_tile_offsets<<<gather_grid(g.T), kWG, 0, s>>>(counts, g.T, tpw, tile_excl, states, res, g.W));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    FZ_LAUNCH(k_gather<<<g.T, kWG, 0, s>>>(counts, tile_excl, staging, payload, res, g.W));
     return hipGetLastError();
 }
 
