@@ -25,7 +25,7 @@ Layout make_layout(const Geom& g)
 {
     Layout L{};
     const uint64_t N = 32ull * g.W;
-    L.n_tstate = gather_grid(g.T) + 1;
+    L.n_tstate = gather_grid(g.T) + 2;
     L.n_sstate = (uint32_t)((N + kScanChunk - 1) / kScanChunk) + 1;
     L.n_dstate = (g.T + kWG - 1) / kWG + 1;
     size_t o = 0;
@@ -224,14 +224,10 @@ fz_status fz_compress_f32(const fz_plan plan, const float* d_in, double eb_rel, 
     uint8_t*      ws = static_cast<uint8_t*>(d_ws);
     uint64_t*     tstate = reinterpret_cast<uint64_t*>(ws + L.tstate);
     FZ_TRY(prof_mark(plan, stream, true));
-    FZ_TRY(launch_range(d_in, g, eb_rel, reinterpret_cast<float4*>(ws + L.part), d_result, tstate, L.n_tstate,
-                        nullptr, stream));
-    FZ_TRY(prof_mark(plan, stream, false));
-    uint32_t* counts = reinterpret_cast<uint32_t*>(ws + L.counts);
-    FZ_TRY(launch_tile_compress(d_in, g, d_result, d_payload, d_tail, ws + L.stage, counts, stream));
-    FZ_TRY(prof_mark(plan, stream, false));
-    FZ_TRY(launch_scan_gather(g, d_result, tstate, d_payload, ws + L.stage, counts,
-                              reinterpret_cast<uint32_t*>(ws + L.texcl), stream));
+    hipEvent_t marks[2] = {nullptr, nullptr};
+    for (int i = 0; i < 2 && plan->prof && plan->nev < kMaxProfEvents; i++) marks[i] = plan->ev[plan->nev++];
+    FZ_TRY(launch_compress(d_in, g, eb_rel, reinterpret_cast<float4*>(ws + L.part), d_result, d_payload, d_tail,
+                           ws + L.stage, reinterpret_cast<uint32_t*>(ws + L.counts), tstate, marks, stream));
     FZ_TRY(prof_mark(plan, stream, false));
     return FZ_SUCCESS;
 }

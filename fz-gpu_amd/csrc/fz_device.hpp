@@ -275,115 +275,155 @@ __device__ __forceinline__ void pack8(const uint32_t c[8], uint32_t w[4])
     for (int k = 0; k < 4; k++) w[k] = c[2 * k] | (c[2 * k + 1] << 16);
 }
 
-// 1D, any full tile: the 256-element blocks start at multiples of 8.
-__device__ __forceinline__ void dq8_fast1(const float* __restrict__ in, uint32_t p0, const Geom& g, float r,
-                                          uint32_t w[4], uint32_t& ovf)
-{
+// Raw input of one thread's 8 positions and the neighbour windows the fast paths need,
+// loaded separately from the arithmetic so a workgroup can prefetch its next tile.
+template <int NDIM>
+struct RawTile;
+template <>
+struct RawTile<1> {
     float v[8];
-    load8(in + p0, v);
-    const float vw = in[(p0 > 0 ? p0 : 1u) - 1];
-    const bool  first = (p0 & 255u) == 0;
-    if (ballot64(((bits_or(v) | __float_as_uint(vw)) & 0x7fffffffu) != 0) == 0) {
-        w[0] = w[1] = w[2] = w[3] = 0;
-        return;
+    float vw;
+};
+template <>
+struct RawTile<2> {
+    float v[8], vn[8];
+    float vw, vnw;
+};
+template <>
+struct RawTile<3> {
+    float v[8], vn[8], vu[8], vun[8];
+    float vw, vnw, vuw, vunw;
+};
+
+// Requires the fast-path geometry (tile_path == 2 ignoring the value range).
+template <int NDIM>
+__device__ __forceinline__ void raw_load(const float* __restrict__ in, uint32_t p0, const Geom& g, RawTile<NDIM>& R)
+{
+    load8(in + p0, R.v);
+    if constexpr (NDIM == 1) {
+        R.vw = in[(p0 > 0 ? p0 : 1u) - 1];
+    } else {
+        load8(in + (p0 - g.X), R.vn);
+        R.vw = in[p0 - 1];
+        R.vnw = in[p0 - g.X - 1];
+        if constexpr (NDIM == 3) {
+            load8(in + (p0 - g.XY), R.vu);
+            load8(in + (p0 - g.XY - g.X), R.vun);
+            R.vuw = in[p0 - g.XY - 1];
+            R.vunw = in[p0 - g.XY - g.X - 1];
+        }
     }
-    float s[8];
+}
+
+// 1D: the 256-element blocks start at multiples of 8, so only position 0 of a thread can
+// start a block.  2D / 3D when X % 4 == 0: each half (4 positions) of the thread's 8 lies in
+// one row, its neighbour rows are 16-byte aligned, and only a half's first position can sit
+// on an x-edge of the Lorenzo block (16 for 2D, 8 for 3D).
+template <int NDIM>
+__device__ __forceinline__ void raw_compute(const RawTile<NDIM>& R, uint32_t p0, const Geom& g, float r, uint32_t w[4],
+                                            uint32_t& ovf)
+{
+    uint32_t c[8];
+    if constexpr (NDIM == 1) {
+        if (ballot64(((bits_or(R.v) | __float_as_uint(R.vw)) & 0x7fffffffu) != 0) == 0) {
+            w[0] = w[1] = w[2] = w[3] = 0;
+            return;
+        }
+        float s[8];
 #pragma unroll
-    for (int i = 0; i < 8; i++) s[i] = prequant(v[i], r);
-    const float sw = first ? 0.0f : prequant(vw, r);
-    uint32_t    c[8];
-    c[0] = code_fast(s[0] - sw, true, ovf);
+        for (int i = 0; i < 8; i++) s[i] = prequant(R.v[i], r);
+        const float sw = (p0 & 255u) == 0 ? 0.0f : prequant(R.vw, r);
+        c[0] = code_fast(s[0] - sw, true, ovf);
 #pragma unroll
-    for (int i = 1; i < 8; i++) c[i] = code_fast(s[i] - s[i - 1], g.mode == 0 || (i & 3) == 0, ovf);
+        for (int i = 1; i < 8; i++) c[i] = code_fast(s[i] - s[i - 1], g.mode == 0 || (i & 3) == 0, ovf);
+    } else {
+        constexpr uint32_t BM = NDIM == 2 ? 15u : 7u;
+        bool bx[2], by[2], bz[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const uint32_t p = p0 + 4 * h;
+            const uint32_t q = fdiv(p, g.divX);
+            const uint32_t x = p - q * g.X;
+            uint32_t y = q, z = 0;
+            if constexpr (NDIM == 3) {
+                z = fdiv(q, g.divY);
+                y = q - z * g.Y;
+            }
+            bx[h] = (x & BM) != 0;
+            by[h] = (y & BM) != 0;
+            bz[h] = (z & BM) != 0;
+        }
+        uint32_t nz = bits_or(R.v) | bits_or(R.vn) | ((__float_as_uint(R.vw) | __float_as_uint(R.vnw)) & 0x7fffffffu);
+        if constexpr (NDIM == 3)
+            nz |= bits_or(R.vu) | bits_or(R.vun) | ((__float_as_uint(R.vuw) | __float_as_uint(R.vunw)) & 0x7fffffffu);
+        if (ballot64(nz != 0) == 0) {
+            w[0] = w[1] = w[2] = w[3] = 0;
+            return;
+        }
+        // S[j], N[j], U[j], UN[j] for positions p0-1+j (j = 0 is the west column); rows that lie
+        // on a y / z block edge contribute zero
+        float S[9], N[9];
+        S[0] = bx[0] ? prequant(R.vw, r) : 0.0f;
+        N[0] = (bx[0] && by[0]) ? prequant(R.vnw, r) : 0.0f;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            S[i + 1] = prequant(R.v[i], r);
+            N[i + 1] = by[i >> 2] ? prequant(R.vn[i], r) : 0.0f;
+        }
+        if constexpr (NDIM == 2) {
+            float c1[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) c1[i] = S[i + 1] - N[i + 1];
+            const float w10 = S[0] - N[0];  // west lane's c1 (0 - 0 on an x-edge)
+            c[0] = code_fast(c1[0] - w10, true, ovf);
+#pragma unroll
+            for (int i = 1; i < 8; i++) {
+                const float w1 = (i == 4) ? (bx[1] ? c1[3] : 0.0f) : c1[i - 1];
+                c[i] = code_fast(c1[i] - w1, true, ovf);
+            }
+        } else {
+            float U[9], UN[9];
+            U[0] = (bx[0] && bz[0]) ? prequant(R.vuw, r) : 0.0f;
+            UN[0] = (bx[0] && by[0] && bz[0]) ? prequant(R.vunw, r) : 0.0f;
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                U[i + 1] = bz[i >> 2] ? prequant(R.vu[i], r) : 0.0f;
+                UN[i + 1] = (by[i >> 2] && bz[i >> 2]) ? prequant(R.vun[i], r) : 0.0f;
+            }
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                // west-side terms exist unless this is a half's first position on an x-edge
+                const bool wx = (i & 3) != 0 || bx[i >> 2];
+                float      p = wx ? UN[i] : 0.0f;  // NW3
+                p = p - (wx ? N[i] : 0.0f);        // A
+                p = p - (wx ? U[i] : 0.0f);        // B
+                p = p - UN[i + 1];                 // C
+                p = p + (wx ? S[i] : 0.0f);        // D
+                p = p + N[i + 1];                  // E
+                p = p + U[i + 1];                  // F
+                c[i] = code_fast(S[i + 1] - p, true, ovf);
+            }
+        }
+    }
     pack8(c, w);
 }
 
-// 2D / 3D when X % 4 == 0: each half (4 positions) of the thread's 8 lies in one row, its
-// neighbour rows are 16-byte aligned, and only a half's first position can sit on an x-edge
-// of the Lorenzo block (16 for 2D, 8 for 3D).  Requires p0 >= X + 1 (2D) / XY + X + 1 (3D).
+__device__ __forceinline__ void dq8_fast1(const float* __restrict__ in, uint32_t p0, const Geom& g, float r,
+                                          uint32_t w[4], uint32_t& ovf)
+{
+    RawTile<1> R;
+    raw_load<1>(in, p0, g, R);
+    raw_compute<1>(R, p0, g, r, w, ovf);
+}
+
+// Requires p0 >= X + 1 (2D) / XY + X + 1 (3D) and X % 4 == 0.
 template <int NDIM>
 __device__ __forceinline__ void dq8_a4(const float* __restrict__ in, uint32_t p0, const Geom& g, float r,
                                        uint32_t w[4], uint32_t& ovf)
 {
-    constexpr uint32_t BM = NDIM == 2 ? 15u : 7u;
-    bool bx[2], by[2], bz[2];
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-        const uint32_t p = p0 + 4 * h;
-        const uint32_t q = fdiv(p, g.divX);
-        const uint32_t x = p - q * g.X;
-        uint32_t y = q, z = 0;
-        if constexpr (NDIM == 3) {
-            z = fdiv(q, g.divY);
-            y = q - z * g.Y;
-        }
-        bx[h] = (x & BM) != 0;
-        by[h] = (y & BM) != 0;
-        bz[h] = (z & BM) != 0;
-    }
-    float v[8], vn[8], vw, vnw;
-    load8(in + p0, v);
-    load8(in + (p0 - g.X), vn);
-    vw = in[p0 - 1];
-    vnw = in[p0 - g.X - 1];
-    uint32_t nz = bits_or(v) | bits_or(vn) | ((__float_as_uint(vw) | __float_as_uint(vnw)) & 0x7fffffffu);
-    float vu[8], vun[8], vuw = 0.0f, vunw = 0.0f;
-    if constexpr (NDIM == 3) {
-        load8(in + (p0 - g.XY), vu);
-        load8(in + (p0 - g.XY - g.X), vun);
-        vuw = in[p0 - g.XY - 1];
-        vunw = in[p0 - g.XY - g.X - 1];
-        nz |= bits_or(vu) | bits_or(vun) | ((__float_as_uint(vuw) | __float_as_uint(vunw)) & 0x7fffffffu);
-    }
-    if (ballot64(nz != 0) == 0) {
-        w[0] = w[1] = w[2] = w[3] = 0;
-        return;
-    }
-    // S[j], N[j], U[j], UN[j] for positions p0-1+j (j = 0 is the west column); rows that lie
-    // on a y / z block edge contribute zero
-    float S[9], N[9], U[9], UN[9];
-    S[0] = bx[0] ? prequant(vw, r) : 0.0f;
-    N[0] = (bx[0] && by[0]) ? prequant(vnw, r) : 0.0f;
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        S[i + 1] = prequant(v[i], r);
-        N[i + 1] = by[i >> 2] ? prequant(vn[i], r) : 0.0f;
-    }
-    uint32_t c[8];
-    if constexpr (NDIM == 2) {
-        float c1[8];
-#pragma unroll
-        for (int i = 0; i < 8; i++) c1[i] = S[i + 1] - N[i + 1];
-        const float w10 = S[0] - N[0];                   // west lane's c1 (0 - 0 on an x-edge)
-        c[0] = code_fast(c1[0] - w10, true, ovf);
-#pragma unroll
-        for (int i = 1; i < 8; i++) {
-            const float w1 = (i == 4) ? (bx[1] ? c1[3] : 0.0f) : c1[i - 1];
-            c[i] = code_fast(c1[i] - w1, true, ovf);
-        }
-    } else {
-        U[0] = (bx[0] && bz[0]) ? prequant(vuw, r) : 0.0f;
-        UN[0] = (bx[0] && by[0] && bz[0]) ? prequant(vunw, r) : 0.0f;
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-            U[i + 1] = bz[i >> 2] ? prequant(vu[i], r) : 0.0f;
-            UN[i + 1] = (by[i >> 2] && bz[i >> 2]) ? prequant(vun[i], r) : 0.0f;
-        }
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-            // west-side terms exist unless this is a half's first position on an x-edge
-            const bool  wx = (i & 3) != 0 || bx[i >> 2];
-            float p = wx ? UN[i] : 0.0f;              // NW3
-            p = p - (wx ? N[i] : 0.0f);               // A
-            p = p - (wx ? U[i] : 0.0f);               // B
-            p = p - UN[i + 1];                        // C
-            p = p + (wx ? S[i] : 0.0f);               // D
-            p = p + N[i + 1];                         // E
-            p = p + U[i + 1];                         // F
-            c[i] = code_fast(S[i + 1] - p, true, ovf);
-        }
-    }
-    pack8(c, w);
+    RawTile<NDIM> R;
+    raw_load<NDIM>(in, p0, g, R);
+    raw_compute<NDIM>(R, p0, g, r, w, ovf);
 }
 
 // ---------------------------------------------------------------- 32x32 bit transpose
@@ -496,6 +536,56 @@ __device__ __forceinline__ uint32_t lookback(uint64_t* states, uint32_t tile, ui
         base -= 64;
     }
     if (l == 0) state_store(&states[tile], kFlagInc | (excl + aggregate));
+    return excl;
+}
+
+// Look-back over K windows of 64 predecessors per round trip (all loads issued together), for
+// grids whose workgroups finish at about the same time.  Called by all 64 lanes of one wave.
+template <int K>
+__device__ __forceinline__ uint32_t lookback_wide(uint64_t* states, uint32_t pos, uint32_t aggregate)
+{
+    const uint32_t l = lane_id();
+    if (pos == 0) {
+        if (l == 0) state_store(&states[0], kFlagInc | aggregate);
+        return 0;
+    }
+    if (l == 0) state_store(&states[pos], kFlagAgg | aggregate);
+    uint32_t excl = 0;
+    int64_t  base = (int64_t)pos - 1;
+    while (true) {
+        uint64_t s[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int64_t idx = base - (int64_t)l - 64 * k;
+            s[k] = idx >= 0 ? state_load(&states[idx]) : kFlagInc;
+        }
+        bool     retry = false, done = false;
+        uint32_t add = 0;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            if (!done && !retry) {
+                const uint32_t flag = (uint32_t)(s[k] >> 32);
+                const uint64_t inc = ballot64(flag == 2u);
+                const uint64_t bad = ballot64(flag == 0u);
+                const uint32_t lp = inc ? (uint32_t)__builtin_ctzll(inc) : 64u;
+                const uint64_t need = lp >= 63u ? ~0ull : ((2ull << lp) - 1ull);
+                if (bad & need) {
+                    retry = true;
+                } else {
+                    add += wave_sum(l <= lp ? (uint32_t)s[k] : 0u);
+                    if (lp < 64u) done = true;
+                }
+            }
+        }
+        if (retry) {
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        excl += add;
+        if (done) break;
+        base -= 64 * K;
+    }
+    if (l == 0) state_store(&states[pos], kFlagInc | (excl + aggregate));
     return excl;
 }
 

@@ -10,7 +10,7 @@
 
 namespace fz {
 
-constexpr uint32_t kRangeMaxWG = 1024;  // partial-range workgroups (a multiple of 8)
+constexpr uint32_t kRangeMaxWG = 256;  // partial-range workgroups (a multiple of 8, <= 256)
 
 // Counters at the start of every workspace.
 struct Ctr {
@@ -25,9 +25,11 @@ constexpr uint32_t kScanChunk = 4096;  // items per workgroup of the u32 scan
 constexpr uint32_t kGatherWG = 256;  // max workgroups of the compress scan + gather
 
 // partial-range workgroups: a multiple of 8 (one slice per XCD region), ~16K floats each
+// partial-range workgroups (1024 threads each): a multiple of 8 (one slice per XCD region),
+// ~64K floats each, at most 256 (the tile kernel reduces one partial per thread)
 inline uint32_t range_grid(uint32_t len)
 {
-    uint64_t g = ((uint64_t)len + 16383) / 16384;
+    uint64_t g = ((uint64_t)len + 65535) / 65536;
     g = (g + 7) & ~7ull;
     if (g < 8) g = 8;
     if (g > kRangeMaxWG) g = kRangeMaxWG;
@@ -37,10 +39,11 @@ inline uint32_t range_grid(uint32_t len)
 // -------- launchers (return hipError_t of the launch)
 hipError_t launch_range(const float* in, const Geom& g, double eb_rel, float4* part, fz_result* res,
                         uint64_t* zero_states, uint32_t nzero, uint32_t* zero_ticket, hipStream_t s);
-hipError_t launch_tile_compress(const float* in, const Geom& g, fz_result* res, uint8_t* payload, uint16_t* tail,
-                                uint8_t* staging, uint32_t* counts, hipStream_t s);
-hipError_t launch_scan_gather(const Geom& g, fz_result* res, uint64_t* states, uint8_t* payload,
-                              const uint8_t* staging, const uint32_t* counts, uint32_t* tile_excl, hipStream_t s);
+// Fused compress (3 kernels).  marks[0] / marks[1] (may be null) are recorded after the first
+// and second kernel for per-kernel timing.
+hipError_t launch_compress(const float* in, const Geom& g, double eb_rel, float4* part, fz_result* res,
+                           uint8_t* payload, uint16_t* tail, uint8_t* staging, uint32_t* counts, uint64_t* states,
+                           hipEvent_t* marks, hipStream_t s);
 uint32_t   gather_grid(uint32_t T);
 // r_host / r_amax: ebx2_r and max|x| when res is null (the per-stage entry point)
 hipError_t launch_dualquant(const float* in, const Geom& g, float r_host, float r_amax, const fz_result* res,

@@ -65,11 +65,19 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n)
     return region_start(b & 7u, n) + (b >> 3);
 }
 
-__global__ __launch_bounds__(kWG) void k_range_part(const float* __restrict__ in, Geom g, float4* __restrict__ part)
+constexpr int kRangeWG = 1024;  // threads per partial-range workgroup
+
+__global__ __launch_bounds__(kRangeWG) void k_range_part(const float* __restrict__ in, Geom g,
+                                                         float4* __restrict__ part, uint64_t* zero_states,
+                                                         uint32_t nzero, fz_result* zero_ovf)
 {
-    __shared__ float    s_lo[4], s_hi[4];
-    __shared__ uint32_t s_bad[4];
+    __shared__ float    s_lo[kRangeWG / 64], s_hi[kRangeWG / 64];
+    __shared__ uint32_t s_bad[kRangeWG / 64];
     const uint32_t tid = threadIdx.x;
+    if (blockIdx.x == 0) {  // clear the look-back state and overflow count of the kernels that follow
+        for (uint32_t i = tid; i < nzero; i += kRangeWG) zero_states[i] = 0;
+        if (tid == 0 && zero_ovf) zero_ovf->overflow = 0;
+    }
     // this workgroup's span of float4s: slice (b >> 3) of region (b & 7), regions in tile units
     const uint32_t nunits = g.T + 1;  // the last unit is the partial tile
     const uint32_t x = blockIdx.x & 7u, i = blockIdx.x >> 3, per = gridDim.x >> 3;
@@ -83,10 +91,10 @@ __global__ __launch_bounds__(kWG) void k_range_part(const float* __restrict__ in
     const float4*  in4 = reinterpret_cast<const float4*>(in);
     constexpr uint32_t U = 4;
     if (v_lo < v_hi) {
-        for (uint32_t base = v_lo + tid; base < v_hi; base += kWG * U) {
+        for (uint32_t base = v_lo + tid; base < v_hi; base += kRangeWG * U) {
             float4 v[U];
 #pragma unroll
-            for (uint32_t j = 0; j < U; j++) v[j] = in4[min(base + j * kWG, v_hi - 1)];  // re-reads are harmless
+            for (uint32_t j = 0; j < U; j++) v[j] = in4[min(base + j * kRangeWG, v_hi - 1)];  // re-reads are harmless
 #pragma unroll
             for (uint32_t j = 0; j < U; j++) {
                 lo = fminf(lo, fminf(fminf(v[j].x, v[j].y), fminf(v[j].z, v[j].w)));
@@ -97,10 +105,10 @@ __global__ __launch_bounds__(kWG) void k_range_part(const float* __restrict__ in
     }
     // scalars at the span's edges (fewer than 4 each side)
     const uint32_t e0 = min(4 * v_lo, f_hi), e1 = max(4 * v_hi, e0);
-    for (uint32_t k = f_lo + tid; k < e0; k += kWG) {
+    for (uint32_t k = f_lo + tid; k < e0; k += kRangeWG) {
         lo = fminf(lo, in[k]); hi = fmaxf(hi, in[k]); bad |= nonfinite(in[k]);
     }
-    for (uint32_t k = e1 + tid; k < f_hi; k += kWG) {
+    for (uint32_t k = e1 + tid; k < f_hi; k += kRangeWG) {
         lo = fminf(lo, in[k]); hi = fmaxf(hi, in[k]); bad |= nonfinite(in[k]);
     }
     lo = wave_min(lo);
@@ -113,10 +121,64 @@ __global__ __launch_bounds__(kWG) void k_range_part(const float* __restrict__ in
         s_bad[wv] = bad;
     }
     __syncthreads();
-    if (tid == 0)
-        part[blockIdx.x] = make_float4(fminf(fminf(s_lo[0], s_lo[1]), fminf(s_lo[2], s_lo[3])),
-                                       fmaxf(fmaxf(s_hi[0], s_hi[1]), fmaxf(s_hi[2], s_hi[3])),
-                                       __uint_as_float(s_bad[0] | s_bad[1] | s_bad[2] | s_bad[3]), 0.0f);
+    if (tid < 64) {
+        const bool live = tid < kRangeWG / 64;
+        lo = wave_min(live ? s_lo[tid] : __builtin_huge_valf());
+        hi = wave_max(live ? s_hi[tid] : -__builtin_huge_valf());
+        bad = ballot64(live && s_bad[tid] != 0) != 0;
+        if (tid == 0) part[blockIdx.x] = make_float4(lo, hi, __uint_as_float(bad), 0.0f);
+    }
+}
+
+// Reduces the partials of k_range_part inside any 256-thread workgroup (nparts <= 256) and
+// derives the eb parameters in thread 0.  Returns them to every thread.
+struct EbParams {
+    float    lo, hi, r, ebx2;
+    double   eb;
+    uint32_t status;
+    bool     safe;
+};
+
+__device__ __forceinline__ EbParams reduce_partials(const float4* __restrict__ part, uint32_t nparts, double eb_rel)
+{
+    __shared__ float    s_lo[4], s_hi[4];
+    __shared__ uint32_t s_bad[4];
+    __shared__ EbParams s_eb;
+    const uint32_t tid = threadIdx.x, wv = tid >> 6;
+    float    lo = __builtin_huge_valf(), hi = -__builtin_huge_valf();
+    uint32_t bad = 0;
+    if (tid < nparts) {
+        const float4 p = part[tid];
+        lo = p.x;
+        hi = p.y;
+        bad = __float_as_uint(p.z);
+    }
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    bad = ballot64(bad != 0) != 0;
+    if ((tid & 63) == 0) {
+        s_lo[wv] = lo;
+        s_hi[wv] = hi;
+        s_bad[wv] = bad;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        EbParams e;
+        e.lo = fminf(fminf(s_lo[0], s_lo[1]), fminf(s_lo[2], s_lo[3]));
+        e.hi = fmaxf(fmaxf(s_hi[0], s_hi[1]), fmaxf(s_hi[2], s_hi[3]));
+        const uint32_t b = s_bad[0] | s_bad[1] | s_bad[2] | s_bad[3];
+        const float    range = e.hi - e.lo;
+        e.eb = eb_rel * (double)range;
+        const double e2 = e.eb * 2.0;
+        const double e2r = 1.0 / e2;
+        e.r = (float)e2r;
+        e.ebx2 = (float)e2;
+        e.status = b ? ST_NONFINITE : (range == 0.0f ? ST_CONSTANT : ST_SUCCESS);
+        e.safe = fmaxf(__builtin_fabsf(e.lo), __builtin_fabsf(e.hi)) * e.r < 8388608.0f;
+        s_eb = e;
+    }
+    __syncthreads();
+    return s_eb;
 }
 
 __global__ __launch_bounds__(kWG) void k_range_final(const float4* __restrict__ part, uint32_t nparts, Geom g,
@@ -172,7 +234,7 @@ hipError_t launch_range(const float* in, const Geom& g, double eb_rel, float4* p
                         uint64_t* zero_states, uint32_t nzero, uint32_t* zero_ticket, hipStream_t s)
 {
     const uint32_t nparts = range_grid(g.len);
-    FZ_LAUNCH(k_range_part<<<nparts, kWG, 0, s>>>(in, g, part));
+    FZ_LAUNCH(k_range_part<<<nparts, kRangeWG, 0, s>>>(in, g, part, nullptr, 0, nullptr));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     FZ_LAUNCH(k_range_final<<<1, kWG, 0, s>>>(part, nparts, g, eb_rel, res, zero_states, nzero, zero_ticket));
@@ -209,21 +271,29 @@ __device__ void tail_work(const float* __restrict__ in, const Geom& g, float r, 
         }
     }
     for (uint32_t i = 8 * g.T + tid; i < g.W; i += kWG) bwords[i] = 0;
-    if (g.T == 0 && tid == 0) {
+    if (res && g.T == 0 && tid == 0) {
         res->nblocks = 0;
         res->payload_bytes = 4ull * g.W;
     }
 }
 
-// Which dual-quant path a tile takes (uniform over the workgroup).
+// Which dual-quant path a tile takes (uniform over the workgroup): 2 = fast (aligned float4
+// neighbour windows), 1 = unguarded generic, 0 = guarded generic.  geom_path ignores the
+// value range; the fast and unguarded paths also need every |s| < 2^23 (`safe`).
 template <int NDIM>
-__device__ __forceinline__ int tile_path(const Geom& g, uint32_t t0, bool safe)
+__device__ __forceinline__ int geom_path(const Geom& g, uint32_t t0)
 {
-    if (!safe || t0 + (uint32_t)kTileCodes > g.xyz) return 0;  // guarded generic
+    if (t0 + (uint32_t)kTileCodes > g.xyz) return 0;
     if constexpr (NDIM == 1) return 2;
     const uint32_t halo = (NDIM == 2) ? g.X + 1 : g.XY + g.X + 1;
     if (t0 < halo) return 0;
-    return (g.X & 3u) == 0 ? 2 : 1;  // aligned fast path / unguarded generic
+    return (g.X & 3u) == 0 ? 2 : 1;
+}
+
+template <int NDIM>
+__device__ __forceinline__ int tile_path(const Geom& g, uint32_t t0, bool safe)
+{
+    return safe ? geom_path<NDIM>(g, t0) : 0;
 }
 
 template <int NDIM>
@@ -241,45 +311,60 @@ __device__ __forceinline__ void dq8_tile(const float* __restrict__ in, uint32_t 
 }
 
 template <int NDIM>
-__global__ __launch_bounds__(kWG) void k_tile_compress(const float* __restrict__ in, Geom g, fz_result* res,
-                                                       uint8_t* __restrict__ payload, uint8_t* __restrict__ staging,
-                                                       uint32_t* __restrict__ counts, uint16_t* __restrict__ tail)
+__global__ __launch_bounds__(kWG) void k_tile_compress(const float* __restrict__ in, Geom g, double eb_rel,
+                                                       const float4* __restrict__ part, uint32_t nparts,
+                                                       fz_result* res, uint8_t* __restrict__ payload,
+                                                       uint8_t* __restrict__ staging, uint32_t* __restrict__ counts,
+                                                       uint16_t* __restrict__ tail)
 {
     __shared__ uint32_t sh[32 * kLdsPitch];
     __shared__ uint32_t sh_wcnt[4];
     __shared__ uint32_t sh_ovf;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    if (res->status != ST_SUCCESS) return;  // the range kernel rejected the field (uniform)
-    const float r = res->ebx2_r;
+    const bool     is_tail = blockIdx.x >= g.T;
+    // XCD-aware order: an XCD walks its eighth of the field, so the neighbour rows a tile
+    // re-reads were fetched into the same L2 moments before
+    const uint32_t tile = is_tail ? g.T : xcd_tile(blockIdx.x, g.T);
+    const uint32_t t0 = tile * (uint32_t)kTileCodes, p0 = t0 + 8 * tid;
+    const int      gp = is_tail ? 0 : geom_path<NDIM>(g, t0);
+    // the tile's loads need no eb: issue them before reducing the range partials
+    RawTile<NDIM> R;
+    if (gp == 2) raw_load<NDIM>(in, p0, g, R);
+    const EbParams e = reduce_partials(part, nparts, eb_rel);
+    if (blockIdx.x == 0 && tid == 0) {
+        res->status = e.status;
+        res->ndim = (uint32_t)g.ndim;
+        res->vmin = e.lo;
+        res->vmax = e.hi;
+        res->ebx2_r = e.r;
+        res->ebx2 = e.ebx2;
+        res->eb_abs = e.eb;
+        res->nblocks = 0;
+        res->payload_bytes = 0;
+        res->tail_len = g.mode == 0 ? (uint64_t)(g.len - (uint32_t)kTileCodes * g.T) : 0;
+    }
+    if (e.status != ST_SUCCESS) return;  // uniform
+    const float r = e.r;
     uint32_t*   bwords = reinterpret_cast<uint32_t*>(payload);
     uint32_t    ovf = 0;
     if (tid == 0) sh_ovf = 0;
 
-    if (blockIdx.x >= g.T) {
+    if (is_tail) {
         tail_work<NDIM>(in, g, r, res, bwords, tail, ovf);
         ovf = wave_sum(ovf);
         if (lane == 0 && ovf) atomicAdd(reinterpret_cast<unsigned long long*>(&res->overflow), (unsigned long long)ovf);
         return;
     }
 
-    // XCD-aware order: an XCD walks its eighth of the field, so the neighbour rows a tile
-    // re-reads were fetched into the same L2 moments before
-    const uint32_t tile = xcd_tile(blockIdx.x, g.T);
-    // |s| < 2^23 everywhere: the fast paths' integer conversions are exact
-    const float amax = fmaxf(__builtin_fabsf(res->vmin), __builtin_fabsf(res->vmax));
-    const bool  safe = amax * r < 8388608.0f;
-    const uint32_t t0 = tile * (uint32_t)kTileCodes;
-    const int      path = tile_path<NDIM>(g, t0, safe);
-
     // 1. dual-quant: words 4t..4t+3 of the tile
     uint32_t w[4];
-    dq8_tile<NDIM>(in, t0 + 8 * tid, g, r, path, w, ovf);
+    if (gp == 2 && e.safe) raw_compute<NDIM>(R, p0, g, r, w, ovf);
+    else dq8_tile<NDIM>(in, p0, g, r, e.safe ? gp : 0, w, ovf);
 
     // 2. bit transpose of group k = t/8 (skipped for an all-zero wave); register q then holds
     //    o[32(4L+q) + k]
     const uint32_t L = tid & 7, k = tid >> 3;
-    const bool     wave_nz = ballot64((w[0] | w[1] | w[2] | w[3]) != 0) != 0;
-    if (wave_nz) transpose4(w, L);
+    if (ballot64((w[0] | w[1] | w[2] | w[3]) != 0) != 0) transpose4(w, L);
 #pragma unroll
     for (int q = 0; q < 4; q++) sh[kLdsPitch * (4 * L + q) + k] = w[q];
     ovf = wave_sum(ovf);
@@ -308,28 +393,46 @@ __global__ __launch_bounds__(kWG) void k_tile_compress(const float* __restrict__
     if (f) *reinterpret_cast<uint4*>(staging + (size_t)tile * 4096 + 16 * (woff + rank)) = blk;
 }
 
-// Exclusive scan of the tile counts -> tile_excl[].  The grid is small (<= kGatherWG
-// workgroups, each a contiguous run of tiles) and uses blockIdx order for the look-back: at
+// Scan of the tile counts + placement of the staged blocks.  The grid is small (<= kGatherWG
+// workgroups, each a contiguous run of tiles) and chains its look-back in blockIdx order: at
 // most kGatherWG - 1 workgroups can be waiting, far fewer than the device holds, so a
 // workgroup that has not started yet always finds a free slot and the chain completes.
-__global__ __launch_bounds__(kWG) void k_tile_offsets(const uint32_t* __restrict__ counts, uint32_t T, uint32_t tpw,
-                                                      uint32_t* __restrict__ tile_excl, uint64_t* states,
-                                                      fz_result* res, uint32_t W)
+constexpr int kGatherThreads = 1024;  // threads per scan + gather workgroup
+constexpr int kGW = kGatherThreads / 64;
+
+__global__ __launch_bounds__(kGatherThreads) void k_scan_gather(const uint32_t* __restrict__ counts, uint32_t T,
+                                                                uint32_t tpw, const uint8_t* __restrict__ staging,
+                                                                uint8_t* __restrict__ payload, uint64_t* states,
+                                                                fz_result* res, uint32_t W)
 {
-    __shared__ uint32_t sh_w[4];
+    __shared__ uint32_t sh_off[kGatherThreads];
+    __shared__ uint32_t sh_w[kGW];
     __shared__ uint32_t sh_excl;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     if (res->status != ST_SUCCESS) return;
     const uint32_t wg = blockIdx.x;
     const uint32_t t_lo = wg * tpw, t_hi = min(T, t_lo + tpw);
+    // exclusive prefix over the waves' partial sums (in every thread), and their total
+    auto wave_prefix = [&](uint32_t& total) {
+        uint32_t pre = 0, tot = 0;
+#pragma unroll
+        for (int i = 0; i < kGW; i++) {
+            const uint32_t v = sh_w[i];
+            pre += (uint32_t)i < wv ? v : 0u;
+            tot += v;
+        }
+        total = tot;
+        return pre;
+    };
     uint32_t part = 0;
-    for (uint32_t t = t_lo + tid; t < t_hi; t += kWG) part += counts[t];
+    for (uint32_t t = t_lo + tid; t < t_hi; t += kGatherThreads) part += counts[t];
     part = wave_sum(part);
     if (lane == 0) sh_w[wv] = part;
     __syncthreads();
-    const uint32_t agg = sh_w[0] + sh_w[1] + sh_w[2] + sh_w[3];
+    uint32_t agg;
+    (void)wave_prefix(agg);
     if (wv == 0) {
-        const uint32_t excl = lookback(states, wg, agg);
+        const uint32_t excl = lookback_wide<4>(states, wg, agg);
         if (lane == 0) {
             sh_excl = excl;
             if (wg == gridDim.x - 1) {
@@ -339,58 +442,73 @@ __global__ __launch_bounds__(kWG) void k_tile_offsets(const uint32_t* __restrict
         }
     }
     __syncthreads();
-    uint32_t base = sh_excl;
-    for (uint32_t r0 = t_lo; r0 < t_hi; r0 += kWG) {
+    uint8_t* out = payload + 4ull * W + 16ull * sh_excl;
+    uint32_t base = 0;
+    // runs of 1024 tiles: offsets in LDS, then 4 independent blocks per thread per step
+    for (uint32_t r0 = t_lo; r0 < t_hi; r0 += kGatherThreads) {
         const uint32_t t = r0 + tid;
         const uint32_t cnt = t < t_hi ? counts[t] : 0u;
         const uint32_t incl = wave_incl_scan(cnt);
         __syncthreads();
         if (lane == 63) sh_w[wv] = incl;
         __syncthreads();
-        const uint32_t w0 = sh_w[0], w1 = sh_w[1], w2 = sh_w[2], w3 = sh_w[3];
-        if (t < t_hi) tile_excl[t] = base + (wv > 0 ? w0 : 0u) + (wv > 1 ? w1 : 0u) + (wv > 2 ? w2 : 0u) + incl - cnt;
-        base += w0 + w1 + w2 + w3;
+        uint32_t run;
+        sh_off[tid] = wave_prefix(run) + incl - cnt;
+        __syncthreads();
+        for (uint32_t j0 = 0; j0 < run; j0 += 4 * kGatherThreads) {
+            uint32_t src[4];
+            bool     ok[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t j = j0 + tid + u * kGatherThreads;
+                ok[u] = j < run;
+                uint32_t lo = 0, hi = kGatherThreads - 1;  // last tile of the run whose first block is <= j
+#pragma unroll
+                for (int it = 0; it < 10; it++) {
+                    const uint32_t mid = (lo + hi + 1) >> 1;
+                    if (sh_off[mid] <= j) lo = mid;
+                    else hi = mid - 1;
+                }
+                src[u] = (r0 + lo) * 4096u + 16u * (j - sh_off[lo]);
+            }
+            uint4 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (ok[u]) v[u] = *reinterpret_cast<const uint4*>(staging + src[u]);
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (ok[u]) st16(out + 16ull * (base + j0 + tid + u * kGatherThreads), v[u]);
+        }
+        base += run;
+        __syncthreads();
     }
-}
-
-// One workgroup per tile: its staged blocks -> payload + 4 W + 16 * (tile_excl + i).
-__global__ __launch_bounds__(kWG) void k_gather(const uint32_t* __restrict__ counts,
-                                                const uint32_t* __restrict__ tile_excl,
-                                                const uint8_t* __restrict__ staging, uint8_t* __restrict__ payload,
-                                                const fz_result* res, uint32_t W)
-{
-    if (res->status != ST_SUCCESS) return;
-    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x), i = threadIdx.x;  // same XCD as the tile's writer
-    if (i < counts[t])
-        st16(payload + 4ull * W + 16ull * (tile_excl[t] + i),
-             *reinterpret_cast<const uint4*>(staging + (size_t)t * 4096 + 16 * i));
 }
 
 inline uint32_t gather_tpw(uint32_t T) { return (T + kGatherWG - 1) / kGatherWG; }
 
-hipError_t launch_tile_compress(const float* in, const Geom& g, fz_result* res, uint8_t* payload, uint16_t* tail,
-                                uint8_t* staging, uint32_t* counts, hipStream_t s)
-{
-    const dim3 grid(g.T + 1);
-    switch (g.ndim) {
-        case 1: FZ_LAUNCH(k_tile_compress<1><<<grid, kWG, 0, s>>>(in, g, res, payload, staging, counts, tail)); break;
-        case 2: FZ_LAUNCH(k_tile_compress<2><<<grid, kWG, 0, s>>>(in, g, res, payload, staging, counts, tail)); break;
-        default: FZ_LAUNCH(k_tile_compress<3><<<grid, kWG, 0, s>>>(in, g, res, payload, staging, counts, tail)); break;
-    }
-    return hipGetLastError();
-}
-
 uint32_t gather_grid(uint32_t T) { return T == 0 ? 0u : (T + gather_tpw(T) - 1) / gather_tpw(T); }
 
-hipError_t launch_scan_gather(const Geom& g, fz_result* res, uint64_t* states, uint8_t* payload,
-                              const uint8_t* staging, const uint32_t* counts, uint32_t* tile_excl, hipStream_t s)
+// Fused compress: range partials, tiles, scan + placement.
+hipError_t launch_compress(const float* in, const Geom& g, double eb_rel, float4* part, fz_result* res,
+                           uint8_t* payload, uint16_t* tail, uint8_t* staging, uint32_t* counts, uint64_t* states,
+                           hipEvent_t* marks, hipStream_t s)
 {
-    if (g.T == 0) return hipSuccess;
-    const uint32_t tpw = gather_tpw(g.T);
-    FZ_LAUNCH(k_tile_offsets<<<gather_grid(g.T), kWG, 0, s>>>(counts, g.T, tpw, tile_excl, states, res, g.W));
+    const uint32_t nparts = range_grid(g.len);
+    const uint32_t G = gather_grid(g.T);
+    FZ_LAUNCH(k_range_part<<<nparts, kRangeWG, 0, s>>>(in, g, part, states, G + 1, res));
     hipError_t e = hipGetLastError();
+    if (e == hipSuccess && marks && marks[0]) e = hipEventRecord(marks[0], s);
     if (e != hipSuccess) return e;
-    FZ_LAUNCH(k_gather<<<g.T, kWG, 0, s>>>(counts, tile_excl, staging, payload, res, g.W));
+    const dim3 grid(g.T + 1);
+    switch (g.ndim) {
+        case 1: FZ_LAUNCH(k_tile_compress<1><<<grid, kWG, 0, s>>>(in, g, eb_rel, part, nparts, res, payload, staging, counts, tail)); break;
+        case 2: FZ_LAUNCH(k_tile_compress<2><<<grid, kWG, 0, s>>>(in, g, eb_rel, part, nparts, res, payload, staging, counts, tail)); break;
+        default: FZ_LAUNCH(k_tile_compress<3><<<grid, kWG, 0, s>>>(in, g, eb_rel, part, nparts, res, payload, staging, counts, tail)); break;
+    }
+    e = hipGetLastError();
+    if (e == hipSuccess && marks && marks[1]) e = hipEventRecord(marks[1], s);
+    if (e != hipSuccess || g.T == 0) return e;
+    FZ_LAUNCH(k_scan_gather<<<G, kGatherThreads, 0, s>>>(counts, g.T, gather_tpw(g.T), staging, payload, states, res, g.W));
     return hipGetLastError();
 }
 
