@@ -112,6 +112,30 @@ def cpu_baseline(x, dims, eb, seconds):
                 sample=f"{reps} full-field OpenMP oracle compress passes over the same field ({el:.1f} s)")
 
 
+def pmc_target(D):
+    """Counter-collection workload: calibration copy (256 MiB read + 256 MiB write through
+    hipMemcpyAsync's copy kernel), then 3 fused compress calls on each workload."""
+    import torch
+
+    import fzgpu
+    a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    for name in D.CONFIGS:
+        x, dims, eb = D.config_field(name)
+        d = torch.from_numpy(x.ravel()).cuda()
+        plan = fzgpu.Plan(*dims)
+        payload, tail, res = plan.alloc_outputs()
+        for _ in range(3):
+            plan.compress_async(d, eb, payload, tail, res)
+        r = fzgpu.read_result(res)
+        print(json.dumps({"workload": name, "payload_bytes": r["payload_bytes"], "n": int(x.size)}), flush=True)
+        del d
+
+
 def load_traffic(kernel, workload):
     p = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{workload}.json")
     if not os.path.exists(p):
@@ -131,6 +155,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--pmc-target", action="store_true",
+                    help="profiling target: a 256 MiB device copy (counter calibration), then 3 fused compress "
+                         "calls per workload, no timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (fields are independent; "
@@ -141,6 +168,9 @@ def main():
 
     from fzgpu import datagen as D
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.pmc_target:
+        pmc_target(D)
+        return
     log(f"generating {args.workload}")
     x, dims, eb = D.config_field(args.workload)
     x = x.ravel()

@@ -97,6 +97,7 @@ struct fz_plan_s {
     int        mode;
     int        prof = 0;                    // per-kernel events enabled
     int        nev = 0;                     // events recorded by the last call
+    int        paired = 0;                  // last call recorded (start, stop) pairs per kernel
     hipEvent_t ev[kMaxProfEvents] = {};
 };
 
@@ -105,7 +106,10 @@ namespace {
 inline hipError_t prof_mark(fz_plan p, hipStream_t s, bool first)
 {
     if (!p->prof) return hipSuccess;
-    if (first) p->nev = 0;
+    if (first) {
+        p->nev = 0;
+        p->paired = 0;
+    }
     if (p->nev >= kMaxProfEvents) return hipSuccess;
     return hipEventRecord(p->ev[p->nev++], s);
 }
@@ -173,8 +177,11 @@ fz_status fz_plan_kernel_times(const fz_plan plan, float* ms, int capacity, int*
     *count = 0;
     if (!plan->prof || plan->nev < 2) return FZ_ERR_NOT_READY;
     FZ_TRY(hipEventSynchronize(plan->ev[plan->nev - 1]));
-    const int n = plan->nev - 1;
-    for (int i = 0; i < n && i < capacity; i++) FZ_TRY(hipEventElapsedTime(&ms[i], plan->ev[i], plan->ev[i + 1]));
+    const int n = plan->paired ? plan->nev / 2 : plan->nev - 1;
+    for (int i = 0; i < n && i < capacity; i++) {
+        if (plan->paired) FZ_TRY(hipEventElapsedTime(&ms[i], plan->ev[2 * i], plan->ev[2 * i + 1]));
+        else FZ_TRY(hipEventElapsedTime(&ms[i], plan->ev[i], plan->ev[i + 1]));
+    }
     *count = n < capacity ? n : capacity;
     return FZ_SUCCESS;
 }
@@ -223,12 +230,14 @@ fz_status fz_compress_f32(const fz_plan plan, const float* d_in, double eb_rel, 
     const Layout& L = plan->L;
     uint8_t*      ws = static_cast<uint8_t*>(d_ws);
     uint64_t*     tstate = reinterpret_cast<uint64_t*>(ws + L.tstate);
-    FZ_TRY(prof_mark(plan, stream, true));
-    hipEvent_t marks[2] = {nullptr, nullptr};
-    for (int i = 0; i < 2 && plan->prof && plan->nev < kMaxProfEvents; i++) marks[i] = plan->ev[plan->nev++];
+    hipEvent_t* ev = nullptr;
+    if (plan->prof) {  // dispatch-timestamped start / stop of each kernel
+        ev = plan->ev;
+        plan->nev = g.T == 0 ? 4 : 6;
+        plan->paired = 1;
+    }
     FZ_TRY(launch_compress(d_in, g, eb_rel, reinterpret_cast<float4*>(ws + L.part), d_result, d_payload, d_tail,
-                           ws + L.stage, reinterpret_cast<uint32_t*>(ws + L.counts), tstate, marks, stream));
-    FZ_TRY(prof_mark(plan, stream, false));
+                           ws + L.stage, reinterpret_cast<uint32_t*>(ws + L.counts), tstate, ev, stream));
     return FZ_SUCCESS;
 }
 

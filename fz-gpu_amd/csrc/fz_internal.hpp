@@ -39,11 +39,10 @@ inline uint32_t range_grid(uint32_t len)
 // -------- launchers (return hipError_t of the launch)
 hipError_t launch_range(const float* in, const Geom& g, double eb_rel, float4* part, fz_result* res,
                         uint64_t* zero_states, uint32_t nzero, uint32_t* zero_ticket, hipStream_t s);
-// Fused compress (3 kernels).  marks[0] / marks[1] (may be null) are recorded after the first
-// and second kernel for per-kernel timing.
+// Fused compress (3 kernels).  ev (may be null) holds start / stop events of each kernel.
 hipError_t launch_compress(const float* in, const Geom& g, double eb_rel, float4* part, fz_result* res,
                            uint8_t* payload, uint16_t* tail, uint8_t* staging, uint32_t* counts, uint64_t* states,
-                           hipEvent_t* marks, hipStream_t s);
+                           hipEvent_t* ev, hipStream_t s);
 uint32_t   gather_grid(uint32_t T);
 // r_host / r_amax: ebx2_r and max|x| when res is null (the per-stage entry point)
 hipError_t launch_dualquant(const float* in, const Geom& g, float r_host, float r_amax, const fz_result* res,

@@ -9,6 +9,7 @@
 //   k_scan_init / k_scan  exclusive u32 sum, single pass (the cub::DeviceScan call, fz.cu:188-192)
 //   k_encode              compaction of flagged 16-byte blocks (fz.cu:49-59)
 //   k_tile_scan, k_decode_unshuffle, k_reconstruct_*   the inverse
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "fz_internal.hpp"
@@ -22,6 +23,17 @@ namespace fz {
         (void)hipGetLastError();           \
         __VA_ARGS__;                       \
     } while (0)
+
+// Launch with dispatch-timestamped start / stop events when both are given (per-kernel
+// timing without extra packets between kernels), plain launch otherwise.
+template <typename K, typename... A>
+static hipError_t launch_timed(K kernel, dim3 grid, dim3 block, hipStream_t s, hipEvent_t t0, hipEvent_t t1, A... args)
+{
+    (void)hipGetLastError();
+    if (t0 && t1) hipExtLaunchKernelGGL(kernel, grid, block, 0, s, t0, t1, 0, args...);
+    else kernel<<<grid, block, 0, s>>>(args...);
+    return hipGetLastError();
+}
 
 // 16-byte global accesses at 4-byte alignment (the encoded blocks follow 4 W bytes of flags)
 typedef uint32_t u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
@@ -488,28 +500,28 @@ inline uint32_t gather_tpw(uint32_t T) { return (T + kGatherWG - 1) / kGatherWG;
 
 uint32_t gather_grid(uint32_t T) { return T == 0 ? 0u : (T + gather_tpw(T) - 1) / gather_tpw(T); }
 
-// Fused compress: range partials, tiles, scan + placement.
+// Fused compress: range partials, tiles, scan + placement.  ev (may be null) holds start /
+// stop events for the three kernels.
 hipError_t launch_compress(const float* in, const Geom& g, double eb_rel, float4* part, fz_result* res,
                            uint8_t* payload, uint16_t* tail, uint8_t* staging, uint32_t* counts, uint64_t* states,
-                           hipEvent_t* marks, hipStream_t s)
+                           hipEvent_t* ev, hipStream_t s)
 {
+    hipEvent_t none[6] = {};
+    if (!ev) ev = none;
     const uint32_t nparts = range_grid(g.len);
     const uint32_t G = gather_grid(g.T);
-    FZ_LAUNCH(k_range_part<<<nparts, kRangeWG, 0, s>>>(in, g, part, states, G + 1, res));
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess && marks && marks[0]) e = hipEventRecord(marks[0], s);
+    hipError_t e = launch_timed(k_range_part, dim3(nparts), dim3(kRangeWG), s, ev[0], ev[1], in, g, part, states,
+                                G + 1, res);
     if (e != hipSuccess) return e;
-    const dim3 grid(g.T + 1);
+    const dim3 grid(g.T + 1), blk(kWG);
     switch (g.ndim) {
-        case 1: FZ_LAUNCH(k_tile_compress<1><<<grid, kWG, 0, s>>>(in, g, eb_rel, part, nparts, res, payload, staging, counts, tail)); break;
-        case 2: FZ_LAUNCH(k_tile_compress<2><<<grid, kWG, 0, s>>>(in, g, eb_rel, part, nparts, res, payload, staging, counts, tail)); break;
-        default: FZ_LAUNCH(k_tile_compress<3><<<grid, kWG, 0, s>>>(in, g, eb_rel, part, nparts, res, payload, staging, counts, tail)); break;
+        case 1: e = launch_timed(k_tile_compress<1>, grid, blk, s, ev[2], ev[3], in, g, eb_rel, (const float4*)part, nparts, res, payload, staging, counts, tail); break;
+        case 2: e = launch_timed(k_tile_compress<2>, grid, blk, s, ev[2], ev[3], in, g, eb_rel, (const float4*)part, nparts, res, payload, staging, counts, tail); break;
+        default: e = launch_timed(k_tile_compress<3>, grid, blk, s, ev[2], ev[3], in, g, eb_rel, (const float4*)part, nparts, res, payload, staging, counts, tail); break;
     }
-    e = hipGetLastError();
-    if (e == hipSuccess && marks && marks[1]) e = hipEventRecord(marks[1], s);
     if (e != hipSuccess || g.T == 0) return e;
-    FZ_LAUNCH(k_scan_gather<<<G, kGatherThreads, 0, s>>>(counts, g.T, gather_tpw(g.T), staging, payload, states, res, g.W));
-    return hipGetLastError();
+    return launch_timed(k_scan_gather, dim3(G), dim3(kGatherThreads), s, ev[4], ev[5], (const uint32_t*)counts, g.T,
+                        gather_tpw(g.T), (const uint8_t*)staging, payload, states, res, g.W);
 }
 
 // ============================================================================ plain stages
