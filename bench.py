@@ -96,6 +96,110 @@ def bench_field(x, dims, eb, steps, warmup, fused=True, rotate_bytes=1.2e9, labe
     return out
 
 
+def _graph_of(fn):
+    """torch CUDA graph of fn() (a loop of per-field library calls on the current stream)."""
+    import torch
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        fn()
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        fn()
+    torch.cuda.synchronize()
+    return g
+
+
+def bench_batch(steps, warmup, nfields=256):
+    """The uneven batch: nfields fields (1D/2D/3D, 64K-64M floats, eb in {1e-2,1e-3,1e-4}),
+    generated on the device.  Batched = one captured graph of the batched kernels; baseline =
+    the plain per-field pipeline, launched field by field from the host and also captured as
+    one graph (no host launch cost).  Every field's bytes are compared."""
+    import torch
+
+    import fzgpu
+    from fzgpu import datagen as D
+    spec = D.batch_fields(nfields)
+    shapes = [f["dims"] for f in spec]
+    ebs = [f["eb"] for f in spec]
+    t0 = time.perf_counter()
+    d = [D.generate_gpu(f["kind"], f["dims"], f["seed"]) for f in spec]
+    torch.cuda.synchronize()
+    log(f"  generated {nfields} fields in {time.perf_counter() - t0:.1f} s")
+    in_bytes = sum(4 * t.numel() for t in d)
+    plan = fzgpu.BatchPlan(shapes)
+    out = plan.alloc_outputs()
+    tune = plan.autotune(d, ebs, out, reps=2)
+    # group-count sweep: eager launches and graph replays
+    sweep = {}
+    for ng in (1, 2, 4, 8, 16):
+        plan.set_tuning(groups=ng)
+        plan.compress_async(d, ebs, out)
+        ms_e, _ = time_steps(lambda k: plan.compress_async(d, ebs, out), max(3, steps // 2))
+        gg = plan.compress_graph(d, ebs, out)
+        gg.launch()
+        ms_g, _ = time_steps(lambda k: gg.launch(), max(3, steps // 2))
+        gg.close()
+        sweep[str(ng)] = {"eager_ms": round(ms_e, 4), "graph_ms": round(ms_g, 4)}
+    plan.set_tuning(groups=tune["groups"])
+    g = plan.compress_graph(d, ebs, out)
+    for _ in range(warmup):
+        g.launch()
+    ms_batch, _ = time_steps(lambda k: g.launch(), steps)
+    plan.profile(True)
+    kt = [0.0] * 7
+    for _ in range(3):
+        plan.compress_async(d, ebs, out)
+        kt = [a + b / 3 for a, b in zip(kt, plan.kernel_times())]
+    plan.profile(False)
+    res = fzgpu.read_results(out.res, nfields)
+    assert all(r["status"] == 0 for r in res)
+    pay_bytes = sum(r["payload_bytes"] for r in res)
+    # per-field plain pipeline (and the fused single-field path) on the same inputs
+    plans = [fzgpu.Plan(*s) for s in shapes]
+    outs = [p.alloc_outputs() for p in plans]
+
+    def per_field(fused):
+        for i in range(nfields):
+            plans[i].compress_async(d[i], ebs[i], *outs[i], fused=fused)
+
+    pf_steps = max(2, steps // 4)
+    per_field(False)
+    ms_plain_host, _ = time_steps(lambda k: per_field(False), pf_steps)
+    mismatch = 0
+    for i in range(nfields):
+        r = fzgpu.read_result(outs[i][2])
+        n = r["payload_bytes"]
+        same = (r == res[i] and torch.equal(outs[i][0][:n], out.payloads[i][:n]) and
+                torch.equal(outs[i][1][: r["tail_len"]], out.tails[i][: r["tail_len"]]))
+        mismatch += 0 if same else 1
+    g_plain = _graph_of(lambda: per_field(False))
+    ms_plain_graph, _ = time_steps(lambda k: g_plain.replay(), pf_steps)
+    per_field(True)
+    ms_fused_host, _ = time_steps(lambda k: per_field(True), pf_steps)
+    g_fused = _graph_of(lambda: per_field(True))
+    ms_fused_graph, _ = time_steps(lambda k: g_fused.replay(), pf_steps)
+    gbps = lambda ms: round(in_bytes / (ms * 1e-3) / 1e9, 2)  # noqa: E731
+    names = ["range", "range_final", "tile_1d", "tile_2d", "tile_3d", "scan", "gather"]
+    r = {
+        "fields": nfields, "input_GB": round(in_bytes / 1e9, 3), "CR": round(in_bytes / pay_bytes, 4),
+        "batched_graph_ms": round(ms_batch, 4), "batched_GBps": gbps(ms_batch),
+        "batched_alg_frac": round((2 * in_bytes + pay_bytes) / (ms_batch * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+        "per_field_plain_host_GBps": gbps(ms_plain_host), "per_field_plain_graph_GBps": gbps(ms_plain_graph),
+        "per_field_fused_host_GBps": gbps(ms_fused_host), "per_field_fused_graph_GBps": gbps(ms_fused_graph),
+        "speedup_vs_plain_host": round(ms_plain_host / ms_batch, 3),
+        "speedup_vs_plain_graph": round(ms_plain_graph / ms_batch, 3),
+        "speedup_vs_fused_graph": round(ms_fused_graph / ms_batch, 3),
+        "kernel_ms": {k: round(v, 5) for k, v in zip(names, kt)},
+        "tuning": tune, "groups_sweep": sweep, "bytes_mismatched_fields": mismatch,
+    }
+    del g, g_plain, g_fused, plans, outs, out, plan, d
+    torch.cuda.empty_cache()
+    return r
+
+
 def cpu_baseline(x, dims, eb, seconds):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -155,6 +259,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 256-field uneven batch")
+    ap.add_argument("--batch-only", action="store_true", help="only the 256-field uneven batch (prints its JSON)")
+    ap.add_argument("--batch-fields", type=int, default=256)
     ap.add_argument("--pmc-target", action="store_true",
                     help="profiling target: a 256 MiB device copy (counter calibration), then 3 fused compress "
                          "calls per workload, no timing")
@@ -170,6 +277,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.pmc_target:
         pmc_target(D)
+        return
+    if args.batch_only:
+        print(json.dumps(bench_batch(args.steps, args.warmup, args.batch_fields)), flush=True)
         return
     log(f"generating {args.workload}")
     x, dims, eb = D.config_field(args.workload)
@@ -210,6 +320,9 @@ def main():
                                                                      (r["kernel_ms"][1] * 1e-3) / 1e9 /
                                                                      HBM_PEAK_GBPS, 4),
                                           "decompress_GBps": round(r["decompress_gbps"], 2)}
+    if not args.no_batch:
+        log(f"timing the {args.batch_fields}-field uneven batch")
+        extra["uneven_batch"] = bench_batch(min(args.steps, 20), min(args.warmup, 3), args.batch_fields)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")

@@ -427,3 +427,612 @@ fz_status fz_encode_blocks(const uint8_t* d_in, uint8_t* d_out, const uint32_t* 
 }
 
 }  // extern "C"
+
+// ============================================================================ batched form
+
+#include <algorithm>
+#include <vector>
+
+constexpr int kBatchSlots = 7;  // event pairs: compress kernels (decompress uses the first 5)
+
+struct fz_batch_plan_s {
+    uint32_t              nf = 0;
+    int                   mode = 0;
+    std::vector<BField>   fields;   // sorted by dimensionality (stable)
+    std::vector<uint32_t> ustart, rstart, pstart[kNumChunks];
+    uint32_t              P[kNumChunks] = {};
+    uint64_t              total_len = 0;
+    BatchShape            shape{};
+    fz_batch_tuning       tune{};
+    struct {
+        size_t fields, ustart, pstart[kNumChunks], rstart, tables_end, ptrs, prm, part, counts, dsts, texcl, staging,
+            total;
+    } L{};
+    std::vector<BPtrs>      host_ptrs;  // staging of the call's pointer records (sorted order)
+    BatchGroup              all{};      // every field as one group
+    std::vector<BatchGroup> groups;     // the tuned grouping
+    BatchStreams            st{};       // created at the first grouped call
+    bool                    st_ready = false;
+    int                prof = 0;
+    int                nslots = 0;  // event pairs recorded by the last call
+    hipEvent_t         ev[2 * kBatchSlots] = {};
+};
+
+struct fz_graph_s {
+    hipGraph_t     graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+bool pow2_upto16(uint32_t v) { return v == 1 || v == 2 || v == 4 || v == 8 || v == 16; }
+
+bool tuning_ok(const fz_batch_tuning& t)
+{
+    bool chunk = false;
+    for (int c = 0; c < kNumChunks; c++) chunk |= t.range_chunk == kRangeChunks[c];
+    return chunk && pow2_upto16(t.gather_waves) && pow2_upto16(t.groups) && t.xcd_order[0] <= 1 &&
+           t.xcd_order[1] <= 1 && t.xcd_order[2] <= 1;
+}
+
+// fields [f_lo, f_hi) of the sorted order as one group
+BatchGroup make_group(const fz_batch_plan p, uint32_t f_lo, uint32_t f_hi)
+{
+    BatchGroup G{};
+    G.f_lo = f_lo;
+    G.f_hi = f_hi;
+    G.u_lo = p->ustart[f_lo];
+    G.u_hi = p->ustart[f_hi];
+    for (int c = 0; c < kNumChunks; c++) {
+        G.p_lo[c] = p->pstart[c][f_lo];
+        G.p_hi[c] = p->pstart[c][f_hi];
+    }
+    for (int nd = 1; nd <= 3; nd++) {
+        const uint32_t lo = std::max(f_lo, p->shape.f_lo[nd]), hi = std::min(f_hi, p->shape.f_hi[nd]);
+        if (lo < hi) {
+            G.df_lo[nd] = lo;
+            G.df_hi[nd] = hi;
+            G.du_lo[nd] = p->ustart[lo];
+            G.du_hi[nd] = p->ustart[hi];
+        }
+    }
+    return G;
+}
+
+// up to n groups of consecutive fields with about equal units
+void build_groups(fz_batch_plan p, uint32_t n)
+{
+    p->groups.clear();
+    const uint64_t U = p->shape.U;
+    uint32_t       f = 0;
+    for (uint32_t g = 0; g < n && f < p->nf; g++) {
+        const uint64_t target = U * (g + 1) / n;
+        uint32_t       e = f + 1;
+        while (e < p->nf && (g == n - 1 || p->ustart[e] < target)) e++;
+        p->groups.push_back(make_group(p, f, e));
+        f = e;
+    }
+}
+
+hipError_t ensure_streams(fz_batch_plan p)
+{
+    if (p->st_ready) return hipSuccess;
+    BatchStreams& st = p->st;
+    hipError_t    e = hipStreamCreateWithFlags(&st.range, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st.gather, hipStreamNonBlocking);
+    hipEvent_t* evs[3] = {&st.fork, &st.join_r, &st.join_g};
+    for (int i = 0; i < 3 && e == hipSuccess; i++) e = hipEventCreateWithFlags(evs[i], hipEventDisableTiming);
+    for (int i = 0; i < kMaxGroups && e == hipSuccess; i++) {
+        e = hipEventCreateWithFlags(&st.r[i], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&st.t[i], hipEventDisableTiming);
+    }
+    p->st_ready = e == hipSuccess;
+    return e;
+}
+
+void destroy_streams(fz_batch_plan p)
+{
+    BatchStreams& st = p->st;
+    if (st.range) (void)hipStreamDestroy(st.range);
+    if (st.gather) (void)hipStreamDestroy(st.gather);
+    hipEvent_t evs[3] = {st.fork, st.join_r, st.join_g};
+    for (hipEvent_t e : evs)
+        if (e) (void)hipEventDestroy(e);
+    for (int i = 0; i < kMaxGroups; i++) {
+        if (st.r[i]) (void)hipEventDestroy(st.r[i]);
+        if (st.t[i]) (void)hipEventDestroy(st.t[i]);
+    }
+    st = BatchStreams{};
+    p->st_ready = false;
+}
+
+void apply_tuning(fz_batch_plan p)
+{
+    int c = 0;
+    while (kRangeChunks[c] != p->tune.range_chunk) c++;
+    p->shape.chunk = kRangeChunks[c];
+    p->shape.P = p->P[c];
+    for (int nd = 1; nd <= 3; nd++) p->shape.xcd[nd] = p->tune.xcd_order[nd - 1];
+    p->shape.gather_waves = p->tune.gather_waves;
+    build_groups(p, p->tune.groups);
+}
+
+int chunk_index(const fz_batch_plan p)
+{
+    int c = 0;
+    while (kRangeChunks[c] != p->tune.range_chunk) c++;
+    return c;
+}
+
+BatchDev batch_dev(const fz_batch_plan p, void* d_ws, fz_result* res)
+{
+    uint8_t* ws = static_cast<uint8_t*>(d_ws);
+    BatchDev d{};
+    d.fields = reinterpret_cast<const BField*>(ws + p->L.fields);
+    d.ustart = reinterpret_cast<const uint32_t*>(ws + p->L.ustart);
+    d.pstart = reinterpret_cast<const uint32_t*>(ws + p->L.pstart[chunk_index(p)]);
+    d.rstart = reinterpret_cast<const uint32_t*>(ws + p->L.rstart);
+    d.ptrs = reinterpret_cast<BPtrs*>(ws + p->L.ptrs);
+    d.prm = reinterpret_cast<BParams*>(ws + p->L.prm);
+    d.part = reinterpret_cast<float4*>(ws + p->L.part);
+    d.counts = reinterpret_cast<uint32_t*>(ws + p->L.counts);
+    d.dsts = reinterpret_cast<uint64_t*>(ws + p->L.dsts);
+    d.texcl = reinterpret_cast<uint32_t*>(ws + p->L.texcl);
+    d.staging = ws + p->L.staging;
+    d.res = res;
+    return d;
+}
+
+inline uint64_t tail_codes(const Geom& g) { return (uint64_t)g.len - (uint64_t)kTileCodes * g.T; }
+
+fz_status batch_compress_impl(const fz_batch_plan plan, const float* const* d_in, const double* eb_rel, void* d_ws,
+                              size_t ws_bytes, uint8_t* const* d_payload, uint16_t* const* d_tail,
+                              fz_result* d_results, hipStream_t stream, bool timed)
+{
+    if (!plan || !d_in || !eb_rel || !d_ws || !d_payload || !d_results) return FZ_ERR_INVALID_ARG;
+    if (!aligned(d_ws, 256) || !aligned(d_results, 8)) return FZ_ERR_INVALID_ARG;
+    if (ws_bytes < plan->L.total) return FZ_ERR_WORKSPACE_TOO_SMALL;
+    for (uint32_t k = 0; k < plan->nf; k++) {
+        const BField&  F = plan->fields[k];
+        const uint32_t c = F.caller;
+        const bool     want_tail = plan->mode == FZ_MODE_FIXED && tail_codes(F.g) > 0;
+        if (!d_in[c] || !aligned(d_in[c], 16) || !d_payload[c] || !aligned(d_payload[c], 16)) return FZ_ERR_INVALID_ARG;
+        if (!(eb_rel[c] > 0.0) || !std::isfinite(eb_rel[c])) return FZ_ERR_INVALID_ARG;
+        if (want_tail && (!d_tail || !d_tail[c] || !aligned(d_tail[c], 2))) return FZ_ERR_INVALID_ARG;
+        plan->host_ptrs[k] = BPtrs{d_in[c], d_payload[c], want_tail ? d_tail[c] : nullptr, eb_rel[c]};
+    }
+    const BatchDev d = batch_dev(plan, d_ws, d_results);
+    FZ_TRY(launch_batch_set(plan->host_ptrs.data(), plan->nf, d.ptrs, stream));
+    const int c = chunk_index(plan);
+    if (timed && plan->prof) {  // one group on the caller's stream, an event pair per kernel
+        plan->nslots = kBatchSlots;
+        FZ_TRY(launch_batch_compress(d, plan->shape, &plan->all, 1, c, nullptr, plan->ev, stream));
+        return FZ_SUCCESS;
+    }
+    if (plan->groups.size() <= 1) {
+        FZ_TRY(launch_batch_compress(d, plan->shape, &plan->all, 1, c, nullptr, nullptr, stream));
+        return FZ_SUCCESS;
+    }
+    FZ_TRY(ensure_streams(plan));
+    FZ_TRY(launch_batch_compress(d, plan->shape, plan->groups.data(), (int)plan->groups.size(), c, &plan->st, nullptr,
+                                 stream));
+    return FZ_SUCCESS;
+}
+
+fz_status batch_decompress_impl(const fz_batch_plan plan, const uint8_t* const* d_payload,
+                                const uint16_t* const* d_tail, const float* ebx2, void* d_ws, size_t ws_bytes,
+                                float* const* d_out, hipStream_t stream, bool timed)
+{
+    if (!plan || !d_payload || !ebx2 || !d_ws || !d_out) return FZ_ERR_INVALID_ARG;
+    if (!aligned(d_ws, 256)) return FZ_ERR_INVALID_ARG;
+    if (ws_bytes < plan->L.total) return FZ_ERR_WORKSPACE_TOO_SMALL;
+    for (uint32_t k = 0; k < plan->nf; k++) {
+        const BField&  F = plan->fields[k];
+        const uint32_t c = F.caller;
+        if (!d_payload[c] || !aligned(d_payload[c], 16) || !d_out[c] || !aligned(d_out[c], 16))
+            return FZ_ERR_INVALID_ARG;
+        if (!(ebx2[c] > 0.0f) || !std::isfinite(ebx2[c])) return FZ_ERR_INVALID_ARG;
+        const uint16_t* t = (plan->mode == FZ_MODE_FIXED && d_tail) ? d_tail[c] : nullptr;
+        if (t && !aligned(t, 2)) return FZ_ERR_INVALID_ARG;
+        plan->host_ptrs[k] = BPtrs{d_payload[c], d_out[c], const_cast<uint16_t*>(t), (double)ebx2[c]};
+    }
+    const BatchDev d = batch_dev(plan, d_ws, nullptr);
+    FZ_TRY(launch_batch_set(plan->host_ptrs.data(), plan->nf, d.ptrs, stream));
+    hipEvent_t* ev = nullptr;
+    if (timed && plan->prof) {
+        ev = plan->ev;
+        plan->nslots = 5;
+    }
+    FZ_TRY(launch_batch_decompress(d, plan->shape, ev, stream));
+    return FZ_SUCCESS;
+}
+
+// Captures body(stream) on a private stream into an instantiated graph.
+template <typename F>
+fz_status capture_graph(F body, fz_graph* out)
+{
+    if (!out) return FZ_ERR_INVALID_ARG;
+    *out = nullptr;
+    hipStream_t s = nullptr;
+    FZ_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(s);
+        return hip_st(e);
+    }
+    const fz_status st = body(s);
+    hipGraph_t      g = nullptr;
+    e = hipStreamEndCapture(s, &g);
+    (void)hipStreamDestroy(s);
+    if (st != FZ_SUCCESS || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return st != FZ_SUCCESS ? st : hip_st(e);
+    }
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return hip_st(e);
+    }
+    fz_graph gr = new (std::nothrow) fz_graph_s;
+    if (!gr) {
+        (void)hipGraphExecDestroy(x);
+        (void)hipGraphDestroy(g);
+        return FZ_ERR_CAPACITY;
+    }
+    gr->graph = g;
+    gr->exec = x;
+    *out = gr;
+    return FZ_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+fz_status fz_batch_plan_create(fz_batch_plan* plan, const fz_field_shape* shapes, uint32_t nfields, int mode)
+{
+    if (!plan) return FZ_ERR_INVALID_ARG;
+    *plan = nullptr;
+    if (!shapes || nfields == 0 || nfields > kBatchMaxFields) return FZ_ERR_INVALID_ARG;
+    if (mode != FZ_MODE_FIXED && mode != FZ_MODE_LITERAL) return FZ_ERR_INVALID_ARG;
+    std::vector<Geom> geo(nfields);
+    for (uint32_t i = 0; i < nfields; i++) {
+        const fz_field_shape& s = shapes[i];
+        const uint64_t        len = s.len ? s.len : (uint64_t)s.x * s.y * s.z;
+        if (!geom_ok(s.x, s.y, s.z, len)) return FZ_ERR_INVALID_ARG;
+        geo[i] = make_geom(s.x, s.y, s.z, len, mode);
+    }
+    fz_batch_plan p = new (std::nothrow) fz_batch_plan_s;
+    if (!p) return FZ_ERR_CAPACITY;
+    p->nf = nfields;
+    p->mode = mode;
+    std::vector<uint32_t> order(nfields);
+    for (uint32_t i = 0; i < nfields; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return geo[a].ndim < geo[b].ndim; });
+    p->fields.resize(nfields);
+    p->ustart.resize(nfields + 1);
+    p->rstart.resize(nfields + 1);
+    for (int c = 0; c < kNumChunks; c++) p->pstart[c].resize(nfields + 1);
+    uint64_t u = 0, r = 0, parts[kNumChunks] = {};
+    BatchShape& b = p->shape;
+    b.nf = nfields;
+    for (uint32_t k = 0; k < nfields; k++) {
+        const Geom& g = geo[order[k]];
+        BField      F{};
+        F.g = g;
+        F.caller = order[k];
+        F.ustart = (uint32_t)u;
+        if (g.ndim == 1) {
+            F.rdims[0] = (g.xyz + 4095) / 4096;
+            F.rdims[1] = F.rdims[2] = 1;
+        } else if (g.ndim == 2) {
+            F.rdims[0] = (g.X + kWG - 1) / kWG;
+            F.rdims[1] = (g.Y + 15) / 16;
+            F.rdims[2] = 1;
+        } else {
+            F.rdims[0] = (g.X + 31) / 32;
+            F.rdims[1] = (g.Y + 7) / 8;
+            F.rdims[2] = (g.Z + 7) / 8;
+        }
+        const int nd = g.ndim;
+        if (k == 0 || geo[order[k - 1]].ndim != nd) {
+            b.f_lo[nd] = k;
+            b.u_lo[nd] = (uint32_t)u;
+            b.r_lo[nd] = (uint32_t)r;
+        }
+        p->ustart[k] = (uint32_t)u;
+        p->rstart[k] = (uint32_t)r;
+        for (int c = 0; c < kNumChunks; c++) {
+            p->pstart[c][k] = (uint32_t)parts[c];
+            parts[c] += (g.len + kRangeChunks[c] - 1) / kRangeChunks[c];
+        }
+        u += (uint64_t)g.T + 1;
+        r += (uint64_t)F.rdims[0] * F.rdims[1] * F.rdims[2];
+        p->total_len += g.len;
+        b.f_hi[nd] = k + 1;
+        b.u_hi[nd] = (uint32_t)u;
+        b.r_hi[nd] = (uint32_t)r;
+        p->fields[k] = F;
+        if (u > 0xFFFFFFF0ull || r > 0xFFFFFFF0ull || parts[0] > 0x7FFFFFFFull) {
+            delete p;
+            return FZ_ERR_CAPACITY;
+        }
+    }
+    p->ustart[nfields] = (uint32_t)u;
+    p->rstart[nfields] = (uint32_t)r;
+    for (int c = 0; c < kNumChunks; c++) {
+        p->pstart[c][nfields] = (uint32_t)parts[c];
+        p->P[c] = (uint32_t)parts[c];
+    }
+    b.U = (uint32_t)u;
+    // workspace: tables (uploaded by fz_batch_workspace_init), then per-call regions
+    auto& L = p->L;
+    size_t o = 0;
+    L.fields = o; o = al256(o + sizeof(BField) * nfields);
+    L.ustart = o; o = al256(o + 4ull * (nfields + 1));
+    for (int c = 0; c < kNumChunks; c++) {
+        L.pstart[c] = o;
+        o = al256(o + 4ull * (nfields + 1));
+    }
+    L.rstart = o; o = al256(o + 4ull * (nfields + 1));
+    L.tables_end = o;
+    L.ptrs = o;    o = al256(o + sizeof(BPtrs) * nfields);
+    L.prm = o;     o = al256(o + sizeof(BParams) * nfields);
+    L.part = o;    o = al256(o + 16ull * p->P[0]);
+    L.counts = o;  o = al256(o + 4ull * u);
+    L.dsts = o;    o = al256(o + 8ull * u);
+    L.texcl = o;   o = al256(o + 4ull * u);
+    L.staging = o; o = al256(o + 4096ull * u);
+    L.total = o;
+    p->host_ptrs.resize(nfields);
+    p->tune.range_chunk = 65536;
+    p->tune.xcd_order[0] = p->tune.xcd_order[1] = p->tune.xcd_order[2] = 1;
+    p->tune.gather_waves = 16;
+    p->tune.groups = 1;
+    p->all = make_group(p, 0, nfields);
+    apply_tuning(p);
+    *plan = p;
+    return FZ_SUCCESS;
+}
+
+fz_status fz_batch_plan_destroy(fz_batch_plan plan)
+{
+    if (!plan) return FZ_SUCCESS;
+    for (int i = 0; i < 2 * kBatchSlots; i++)
+        if (plan->ev[i]) (void)hipEventDestroy(plan->ev[i]);
+    destroy_streams(plan);
+    delete plan;
+    return FZ_SUCCESS;
+}
+
+fz_status fz_batch_plan_sizes(const fz_batch_plan plan, fz_batch_sizes* s)
+{
+    if (!plan || !s) return FZ_ERR_INVALID_ARG;
+    s->workspace_bytes = plan->L.total;
+    s->total_len = plan->total_len;
+    s->units = plan->shape.U;
+    s->nfields = plan->nf;
+    s->range_parts = plan->shape.P;
+    return FZ_SUCCESS;
+}
+
+fz_status fz_batch_field_sizes(const fz_batch_plan plan, uint32_t field, fz_sizes* s)
+{
+    if (!plan || !s || field >= plan->nf) return FZ_ERR_INVALID_ARG;
+    for (const BField& F : plan->fields) {
+        if (F.caller != field) continue;
+        const Geom& g = F.g;
+        s->len = g.len;
+        s->tiles = g.T;
+        s->flag_words = g.W;
+        s->byte_flags = 32ull * g.W;
+        s->payload_max_bytes = 4ull * g.W + 4096ull * g.T;
+        s->tail_max_codes = tail_codes(g);
+        s->workspace_bytes = plan->L.total;
+        return FZ_SUCCESS;
+    }
+    return FZ_ERR_INVALID_ARG;
+}
+
+fz_status fz_batch_get_tuning(const fz_batch_plan plan, fz_batch_tuning* t)
+{
+    if (!plan || !t) return FZ_ERR_INVALID_ARG;
+    *t = plan->tune;
+    return FZ_SUCCESS;
+}
+
+fz_status fz_batch_set_tuning(fz_batch_plan plan, const fz_batch_tuning* t)
+{
+    if (!plan || !t || !tuning_ok(*t)) return FZ_ERR_INVALID_ARG;
+    plan->tune = *t;
+    apply_tuning(plan);
+    return FZ_SUCCESS;
+}
+
+fz_status fz_batch_workspace_init(const fz_batch_plan plan, void* d_ws, size_t ws_bytes, hipStream_t stream)
+{
+    if (!plan || !d_ws || !aligned(d_ws, 256)) return FZ_ERR_INVALID_ARG;
+    if (ws_bytes < plan->L.total) return FZ_ERR_WORKSPACE_TOO_SMALL;
+    const auto&          L = plan->L;
+    std::vector<uint8_t> t(L.tables_end, 0);
+    std::memcpy(t.data() + L.fields, plan->fields.data(), sizeof(BField) * plan->nf);
+    std::memcpy(t.data() + L.ustart, plan->ustart.data(), 4ull * (plan->nf + 1));
+    for (int c = 0; c < kNumChunks; c++)
+        std::memcpy(t.data() + L.pstart[c], plan->pstart[c].data(), 4ull * (plan->nf + 1));
+    std::memcpy(t.data() + L.rstart, plan->rstart.data(), 4ull * (plan->nf + 1));
+    FZ_TRY(hipMemcpyAsync(d_ws, t.data(), t.size(), hipMemcpyHostToDevice, stream));
+    FZ_TRY(hipStreamSynchronize(stream));
+    return FZ_SUCCESS;
+}
+
+fz_status fz_batch_compress_f32(const fz_batch_plan plan, const float* const* d_in, const double* eb_rel,
+                                void* d_ws, size_t ws_bytes, uint8_t* const* d_payload, uint16_t* const* d_tail,
+                                fz_result* d_results, hipStream_t stream)
+{
+    return batch_compress_impl(plan, d_in, eb_rel, d_ws, ws_bytes, d_payload, d_tail, d_results, stream, true);
+}
+
+fz_status fz_batch_decompress_f32(const fz_batch_plan plan, const uint8_t* const* d_payload,
+                                  const uint16_t* const* d_tail, const float* ebx2, void* d_ws, size_t ws_bytes,
+                                  float* const* d_out, hipStream_t stream)
+{
+    return batch_decompress_impl(plan, d_payload, d_tail, ebx2, d_ws, ws_bytes, d_out, stream, true);
+}
+
+fz_status fz_batch_profile(fz_batch_plan plan, int enable)
+{
+    if (!plan) return FZ_ERR_INVALID_ARG;
+    if (enable && !plan->ev[0])
+        for (int i = 0; i < 2 * kBatchSlots; i++) FZ_TRY(hipEventCreate(&plan->ev[i]));
+    plan->prof = enable ? 1 : 0;
+    plan->nslots = 0;
+    return FZ_SUCCESS;
+}
+
+fz_status fz_batch_kernel_times(const fz_batch_plan plan, float* ms, int capacity, int* count)
+{
+    if (!plan || !count) return FZ_ERR_INVALID_ARG;
+    *count = 0;
+    if (!plan->prof || plan->nslots == 0) return FZ_ERR_NOT_READY;
+    FZ_TRY(hipEventSynchronize(plan->ev[2 * plan->nslots - 1]));
+    int n = 0;
+    for (; n < plan->nslots && n < capacity; n++) FZ_TRY(hipEventElapsedTime(&ms[n], plan->ev[2 * n], plan->ev[2 * n + 1]));
+    *count = n;
+    return FZ_SUCCESS;
+}
+
+fz_status fz_batch_autotune(fz_batch_plan plan, const float* const* d_in, const double* eb_rel, void* d_ws,
+                            size_t ws_bytes, uint8_t* const* d_payload, uint16_t* const* d_tail, fz_result* d_results,
+                            int reps, fz_batch_tuning* chosen, hipStream_t stream)
+{
+    if (!plan || reps < 1) return FZ_ERR_INVALID_ARG;
+    const int             prof0 = plan->prof;
+    const fz_batch_tuning start = plan->tune;
+    fz_status             st = fz_batch_profile(plan, 1);
+    // mean per-slot ms over reps runs at the current tuning
+    auto run = [&](float out[kBatchSlots]) -> fz_status {
+        for (int i = 0; i < kBatchSlots; i++) out[i] = 0;
+        for (int r = 0; r < reps; r++) {
+            fz_status s = batch_compress_impl(plan, d_in, eb_rel, d_ws, ws_bytes, d_payload, d_tail, d_results,
+                                              stream, true);
+            if (s != FZ_SUCCESS) return s;
+            float ms[kBatchSlots];
+            int   n = 0;
+            s = fz_batch_kernel_times(plan, ms, kBatchSlots, &n);
+            if (s != FZ_SUCCESS) return s;
+            for (int i = 0; i < n; i++) out[i] += ms[i] / reps;
+        }
+        return FZ_SUCCESS;
+    };
+    float t[kBatchSlots];
+    if (st == FZ_SUCCESS) st = run(t);  // warm-up
+    fz_batch_tuning best = start;
+    // range chunk: range + range-final time
+    float best_ms = 1e30f;
+    for (int c = 0; c < kNumChunks && st == FZ_SUCCESS; c++) {
+        plan->tune.range_chunk = kRangeChunks[c];
+        apply_tuning(plan);
+        st = run(t);
+        if (st == FZ_SUCCESS && t[0] + t[1] < best_ms) {
+            best_ms = t[0] + t[1];
+            best.range_chunk = kRangeChunks[c];
+        }
+    }
+    plan->tune.range_chunk = best.range_chunk;
+    // tile order per dimensionality: tile-kernel time of each
+    float xt[2][3] = {};
+    for (uint32_t v = 0; v < 2 && st == FZ_SUCCESS; v++) {
+        for (int nd = 0; nd < 3; nd++) plan->tune.xcd_order[nd] = v;
+        apply_tuning(plan);
+        st = run(t);
+        for (int nd = 0; nd < 3; nd++) xt[v][nd] = t[2 + nd];
+    }
+    for (int nd = 0; nd < 3; nd++) best.xcd_order[nd] = xt[1][nd] <= xt[0][nd] ? 1u : 0u;
+    for (int nd = 0; nd < 3; nd++) plan->tune.xcd_order[nd] = best.xcd_order[nd];
+    // gather width
+    best_ms = 1e30f;
+    const uint32_t gws[3] = {4, 8, 16};
+    for (int i = 0; i < 3 && st == FZ_SUCCESS; i++) {
+        plan->tune.gather_waves = gws[i];
+        apply_tuning(plan);
+        st = run(t);
+        if (st == FZ_SUCCESS && t[6] < best_ms) {
+            best_ms = t[6];
+            best.gather_waves = gws[i];
+        }
+    }
+    // group count: whole-call time (profiling off, so the grouped schedule runs)
+    plan->tune.gather_waves = best.gather_waves;
+    (void)fz_batch_profile(plan, 0);
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (st == FZ_SUCCESS) st = hip_st(hipEventCreate(&t0));
+    if (st == FZ_SUCCESS) st = hip_st(hipEventCreate(&t1));
+    best_ms = 1e30f;
+    const uint32_t gs[5] = {1, 2, 4, 8, 16};
+    for (int i = 0; i < 5 && st == FZ_SUCCESS; i++) {
+        plan->tune.groups = gs[i];
+        apply_tuning(plan);
+        for (int r = 0; r <= reps && st == FZ_SUCCESS; r++) {  // the first run warms the streams up
+            if (r == 1) st = hip_st(hipEventRecord(t0, stream));
+            if (st == FZ_SUCCESS)
+                st = batch_compress_impl(plan, d_in, eb_rel, d_ws, ws_bytes, d_payload, d_tail, d_results, stream,
+                                         false);
+        }
+        if (st == FZ_SUCCESS) st = hip_st(hipEventRecord(t1, stream));
+        if (st == FZ_SUCCESS) st = hip_st(hipEventSynchronize(t1));
+        float ms = 0;
+        if (st == FZ_SUCCESS) st = hip_st(hipEventElapsedTime(&ms, t0, t1));
+        if (st == FZ_SUCCESS && ms < best_ms * 0.98f) {  // a group count must win by 2 %
+            best_ms = ms;
+            best.groups = gs[i];
+        }
+    }
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    plan->tune = st == FZ_SUCCESS ? best : start;
+    apply_tuning(plan);
+    (void)fz_batch_profile(plan, prof0);
+    if (st == FZ_SUCCESS && chosen) *chosen = plan->tune;
+    return st;
+}
+
+fz_status fz_batch_compress_graph(const fz_batch_plan plan, const float* const* d_in, const double* eb_rel,
+                                  void* d_ws, size_t ws_bytes, uint8_t* const* d_payload, uint16_t* const* d_tail,
+                                  fz_result* d_results, fz_graph* graph)
+{
+    return capture_graph(
+        [&](hipStream_t s) {
+            return batch_compress_impl(plan, d_in, eb_rel, d_ws, ws_bytes, d_payload, d_tail, d_results, s, false);
+        },
+        graph);
+}
+
+fz_status fz_batch_decompress_graph(const fz_batch_plan plan, const uint8_t* const* d_payload,
+                                    const uint16_t* const* d_tail, const float* ebx2, void* d_ws, size_t ws_bytes,
+                                    float* const* d_out, fz_graph* graph)
+{
+    return capture_graph(
+        [&](hipStream_t s) {
+            return batch_decompress_impl(plan, d_payload, d_tail, ebx2, d_ws, ws_bytes, d_out, s, false);
+        },
+        graph);
+}
+
+fz_status fz_graph_launch(fz_graph graph, hipStream_t stream)
+{
+    if (!graph || !graph->exec) return FZ_ERR_INVALID_ARG;
+    FZ_TRY(hipGraphLaunch(graph->exec, stream));
+    return FZ_SUCCESS;
+}
+
+fz_status fz_graph_destroy(fz_graph graph)
+{
+    if (!graph) return FZ_SUCCESS;
+    if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+    if (graph->graph) (void)hipGraphDestroy(graph->graph);
+    delete graph;
+    return FZ_SUCCESS;
+}
+
+}  // extern "C"

@@ -408,6 +408,304 @@ __device__ __forceinline__ void raw_compute(const RawTile<NDIM>& R, uint32_t p0,
     pack8(c, w);
 }
 
+// ---- any X >= 8 (2D / 3D): neighbour rows at 4-byte alignment, per-position block edges
+//
+// With X >= 8 the 8 positions p0..p0+7 change row at most once (at position k = X - x0).
+// Bit i of the masks below describes position p0+i: kx = has an in-block west neighbour
+// (x % B != 0), ky = has an in-block row above (y % B != 0), kz = has an in-block plane
+// below (z % B != 0); B = 16 (2D) or 8 (3D).
+
+typedef float f32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
+
+__device__ __forceinline__ void load8u(const float* __restrict__ p, float v[8])
+{
+    const f32x4a4 a = *reinterpret_cast<const f32x4a4*>(p);
+    const f32x4a4 b = *reinterpret_cast<const f32x4a4*>(p + 4);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+
+// Requires X >= 8 and p0 >= X + 1 (2D) / XY + X + 1 (3D).
+template <int NDIM>
+__device__ __forceinline__ void raw_load_u(const float* __restrict__ in, uint32_t p0, const Geom& g, RawTile<NDIM>& R)
+{
+    load8(in + p0, R.v);
+    load8u(in + (p0 - g.X), R.vn);
+    R.vw = in[p0 - 1];
+    R.vnw = in[p0 - g.X - 1];
+    if constexpr (NDIM == 3) {
+        load8u(in + (p0 - g.XY), R.vu);
+        load8u(in + (p0 - g.XY - g.X), R.vun);
+        R.vuw = in[p0 - g.XY - 1];
+        R.vunw = in[p0 - g.XY - g.X - 1];
+    }
+}
+
+template <int NDIM>
+__device__ __forceinline__ void edge_masks(uint32_t p0, const Geom& g, uint32_t& kx, uint32_t& ky, uint32_t& kz)
+{
+    constexpr uint32_t BM = NDIM == 2 ? 15u : 7u;
+    const uint32_t q = fdiv(p0, g.divX);
+    const uint32_t x0 = p0 - q * g.X;
+    uint32_t       y0 = q, z0 = 0;
+    if constexpr (NDIM == 3) {
+        z0 = fdiv(q, g.divY);
+        y0 = q - z0 * g.Y;
+    }
+    const uint32_t k = g.X - x0;                          // first position on the next row (>= 1)
+    const uint32_t lo = k >= 8 ? 0xFFu : (1u << k) - 1u;  // positions on the row of p0
+    const uint32_t hi = 0xFFu & ~lo;
+    const uint32_t i0 = (BM + 1 - (x0 & BM)) & BM;        // first x-edge on the row of p0
+    const uint32_t ex = (i0 < 8 && i0 < k ? 1u << i0 : 0u) | (k < 8 ? 1u << k : 0u);
+    uint32_t       y1 = y0 + 1, z1 = z0;
+    if constexpr (NDIM == 3) {
+        if (y1 == g.Y) {
+            y1 = 0;
+            z1 = z0 + 1;
+        }
+    }
+    const uint32_t ey = ((y0 & BM) == 0 ? lo : 0u) | ((y1 & BM) == 0 ? hi : 0u);
+    const uint32_t ez = ((z0 & BM) == 0 ? lo : 0u) | ((z1 & BM) == 0 ? hi : 0u);
+    kx = ~ex;
+    ky = ~ey;
+    kz = ~ez;
+}
+
+// all-ones when bit i of m is set, else 0
+__device__ __forceinline__ uint32_t bitmask(uint32_t m, int i) { return (uint32_t)(-(int32_t)((m >> i) & 1u)); }
+
+// v if the mask is all-ones, +0.0f if it is 0
+__device__ __forceinline__ float keep(float v, uint32_t m) { return __uint_as_float(__float_as_uint(v) & m); }
+
+// 2D, any X >= 8: same float operations as raw_compute (and dq8); a neighbour outside the
+// block enters as the value +0.0f (masked before the prequant: prequant(+0) = +0).
+__device__ __forceinline__ void raw_compute_u(const RawTile<2>& R, uint32_t p0, const Geom& g, float r, uint32_t w[4],
+                                              uint32_t& ovf)
+{
+    const uint32_t nz = bits_or(R.v) | bits_or(R.vn) | ((__float_as_uint(R.vw) | __float_as_uint(R.vnw)) & 0x7fffffffu);
+    if (ballot64(nz != 0) == 0) {
+        w[0] = w[1] = w[2] = w[3] = 0;
+        return;
+    }
+    uint32_t kx, ky, kz;
+    edge_masks<2>(p0, g, kx, ky, kz);
+    uint32_t       c[8];
+    float          S[9], N[9];  // positions p0-1+j; the west column has the masks of position 0
+    const uint32_t mx0 = bitmask(kx, 0), my0 = bitmask(ky, 0);
+    S[0] = prequant(keep(R.vw, mx0), r);
+    N[0] = prequant(keep(R.vnw, mx0 & my0), r);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        S[i + 1] = prequant(R.v[i], r);
+        N[i + 1] = prequant(keep(R.vn[i], bitmask(ky, i)), r);
+    }
+    float c1[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) c1[i] = S[i + 1] - N[i + 1];
+    c[0] = code_fast(c1[0] - (S[0] - N[0]), true, ovf);
+#pragma unroll
+    for (int i = 1; i < 8; i++) c[i] = code_fast(c1[i] - keep(c1[i - 1], bitmask(kx, i)), true, ovf);
+    pack8(c, w);
+}
+
+// ---- packed 16-bit path for small prequantised values
+//
+// When every |s| <= 16383 (1D), 8191 (2D) or 4095 (3D), every delta satisfies |d| <= 32764,
+// the reference's float arithmetic on these integers is exact, and the exact d equals the
+// result of the same sums taken modulo 2^16.  The Lorenzo deltas are then computed two
+// positions per instruction on int16 pairs, in any order of the terms:
+//   Q = S - N (2D),  Q = S - N - U + UN (3D),  Q = S (1D)   per column, neighbours outside the
+//   block being 0;  d = Q - (has west ? Q of the west column : 0).
+// (For 3D this is the reference's d = s - (NW3 - A - B - C + D + E + F) regrouped by column.)
+// The code pair is |d| | sign(d) << 15 per half, i.e. the packed code word itself; no
+// overflow is possible.
+
+typedef unsigned short v2u16 __attribute__((ext_vector_type(2)));
+typedef short          v2i16 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ int32_t pq_i(float x, float r)
+{
+    const float t = __fmul_rn(x, r);
+    return (int32_t)__fadd_rn(t, __builtin_copysignf(0.49999997f, t));  // truncating conversion
+}
+
+// (lo, hi) low 16 bits each -> one word
+__device__ __forceinline__ uint32_t pk2(int32_t lo, int32_t hi)
+{
+    return __builtin_amdgcn_perm((uint32_t)hi, (uint32_t)lo, 0x05040100u);
+}
+
+__device__ __forceinline__ uint32_t pk_sub(uint32_t a, uint32_t b)
+{
+    v2u16 x, y;
+    __builtin_memcpy(&x, &a, 4);
+    __builtin_memcpy(&y, &b, 4);
+    const v2u16 z = x - y;
+    uint32_t    o;
+    __builtin_memcpy(&o, &z, 4);
+    return o;
+}
+
+// sign-magnitude codes of two int16 deltas (|d| < 32768); sign kept where `sign` has bit 15 / 31
+__device__ __forceinline__ uint32_t pk_code(uint32_t d, uint32_t sign)
+{
+    v2i16 x;
+    __builtin_memcpy(&x, &d, 4);
+    const v2i16 zero = {0, 0};
+    const v2i16 neg = __builtin_bit_cast(v2i16, __builtin_bit_cast(v2u16, zero) - __builtin_bit_cast(v2u16, x));
+    const v2i16 a = __builtin_elementwise_max(x, neg);
+    uint32_t    m;
+    __builtin_memcpy(&m, &a, 4);
+    return m | (d & sign);
+}
+
+// w[k] = codes of positions 2k, 2k+1 from the column values Q[j] of positions p0-1+j and the
+// "has west" flags of positions 2k, 2k+1 (as 16-bit half masks)
+__device__ __forceinline__ void pk_deltas(const int32_t Q[9], const uint32_t wmask[4], const uint32_t sign[4],
+                                          uint32_t w[4])
+{
+    uint32_t q2[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) q2[k] = pk2(Q[2 * k + 1], Q[2 * k + 2]);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t west = k == 0 ? __builtin_amdgcn_perm(q2[0], (uint32_t)Q[0], 0x05040100u)
+                                     : __builtin_amdgcn_alignbyte(q2[k], q2[k - 1], 2);
+        w[k] = pk_code(pk_sub(q2[k], west & wmask[k]), sign[k]);
+    }
+}
+
+// half masks of 8-bit position flags: pair k = positions 2k (low) and 2k+1 (high)
+__device__ __forceinline__ uint32_t pair_mask(uint32_t m, int k)
+{
+    return __builtin_amdgcn_perm(bitmask(m, 2 * k + 1), bitmask(m, 2 * k), 0x05040100u);
+}
+
+// AL: X % 4 == 0 layout (raw_load); otherwise the any-X layout (raw_load_u, X >= 8).
+template <int NDIM, bool AL>
+__device__ __forceinline__ void raw_compute_pk(const RawTile<NDIM>& R, uint32_t p0, const Geom& g, float r,
+                                               uint32_t w[4])
+{
+    uint32_t nz = bits_or(R.v) | (__float_as_uint(R.vw) & 0x7fffffffu);
+    if constexpr (NDIM >= 2) nz |= bits_or(R.vn) | (__float_as_uint(R.vnw) & 0x7fffffffu);
+    if constexpr (NDIM == 3)
+        nz |= bits_or(R.vu) | bits_or(R.vun) | ((__float_as_uint(R.vuw) | __float_as_uint(R.vunw)) & 0x7fffffffu);
+    if (ballot64(nz != 0) == 0) {
+        w[0] = w[1] = w[2] = w[3] = 0;
+        return;
+    }
+    int32_t  Q[9];
+    uint32_t wmask[4] = {~0u, ~0u, ~0u, ~0u};
+    uint32_t sign[4] = {0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
+    if constexpr (NDIM == 1) {
+        Q[0] = (p0 & 255u) != 0 ? pq_i(R.vw, r) : 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) Q[i + 1] = pq_i(R.v[i], r);
+        if (g.mode != 0) {  // LITERAL: sign only at positions % 4 == 0 (lorenzo_var.cuh:51-63)
+            sign[0] = sign[2] = 0x00008000u;
+            sign[1] = sign[3] = 0u;
+        }
+    } else {
+        // per-position keep masks (bit i = position p0+i): in-block west / row above / plane below
+        uint32_t kx, ky, kz = 0xFFu;
+        if constexpr (AL) {
+            constexpr uint32_t BM = NDIM == 2 ? 15u : 7u;
+            kx = ky = 0xFFu;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const uint32_t p = p0 + 4 * h;
+                const uint32_t q = fdiv(p, g.divX);
+                const uint32_t x = p - q * g.X;
+                uint32_t       y = q, z = 0;
+                if constexpr (NDIM == 3) {
+                    z = fdiv(q, g.divY);
+                    y = q - z * g.Y;
+                }
+                const uint32_t half = 0xFu << (4 * h);
+                if ((x & BM) == 0) kx &= ~(1u << (4 * h));
+                if ((y & BM) == 0) ky &= ~half;
+                if ((z & BM) == 0) kz &= ~half;
+            }
+        } else {
+            edge_masks<NDIM>(p0, g, kx, ky, kz);
+        }
+        const uint32_t mx0 = bitmask(kx, 0), my0 = bitmask(ky, 0), mz0 = bitmask(kz, 0);
+        Q[0] = pq_i(keep(R.vw, mx0), r) - pq_i(keep(R.vnw, mx0 & my0), r);
+        if constexpr (NDIM == 3)
+            Q[0] = Q[0] - pq_i(keep(R.vuw, mx0 & mz0), r) + pq_i(keep(R.vunw, mx0 & my0 & mz0), r);
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const uint32_t my = bitmask(ky, i);
+            Q[i + 1] = pq_i(R.v[i], r) - pq_i(keep(R.vn[i], my), r);
+            if constexpr (NDIM == 3) {
+                const uint32_t mz = bitmask(kz, i);
+                Q[i + 1] = Q[i + 1] - pq_i(keep(R.vu[i], mz), r) + pq_i(keep(R.vun[i], my & mz), r);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) wmask[k] = pair_mask(kx, k);
+    }
+    pk_deltas(Q, wmask, sign, w);
+}
+
+// ---- 3D column form in float for |s| < 2^21
+//
+// Every partial sum of the reference's predictor (7 terms of |s| < 2^21) is an integer below
+// 2^24, so the float arithmetic is exact in any order and equals
+//   Q = ((S - N) - U) + UN per column,  d = Q - (has west ? Q of the west column : 0),
+// each step exact as well (|Q| < 2^23, |d| < 2^24).  Same masks as the packed path.
+template <bool AL>
+__device__ __forceinline__ void raw_compute_q3(const RawTile<3>& R, uint32_t p0, const Geom& g, float r, uint32_t w[4],
+                                               uint32_t& ovf)
+{
+    uint32_t nz = bits_or(R.v) | bits_or(R.vn) | bits_or(R.vu) | bits_or(R.vun) |
+                  ((__float_as_uint(R.vw) | __float_as_uint(R.vnw) | __float_as_uint(R.vuw) | __float_as_uint(R.vunw)) &
+                   0x7fffffffu);
+    if (ballot64(nz != 0) == 0) {
+        w[0] = w[1] = w[2] = w[3] = 0;
+        return;
+    }
+    uint32_t kx, ky, kz;
+    if constexpr (AL) {
+        kx = ky = kz = 0xFFu;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const uint32_t p = p0 + 4 * h;
+            const uint32_t q = fdiv(p, g.divX);
+            const uint32_t x = p - q * g.X;
+            const uint32_t z = fdiv(q, g.divY);
+            const uint32_t y = q - z * g.Y;
+            if ((x & 7u) == 0) kx &= ~(1u << (4 * h));
+            if ((y & 7u) == 0) ky &= ~(0xFu << (4 * h));
+            if ((z & 7u) == 0) kz &= ~(0xFu << (4 * h));
+        }
+    } else {
+        edge_masks<3>(p0, g, kx, ky, kz);
+    }
+    const uint32_t mx0 = bitmask(kx, 0), my0 = bitmask(ky, 0), mz0 = bitmask(kz, 0);
+    float          prev = ((prequant(keep(R.vw, mx0), r) - prequant(keep(R.vnw, mx0 & my0), r)) -
+                  prequant(keep(R.vuw, mx0 & mz0), r)) +
+                 prequant(keep(R.vunw, mx0 & my0 & mz0), r);
+    uint32_t c[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const uint32_t my = bitmask(ky, i), mz = bitmask(kz, i);
+        const float    q = ((prequant(R.v[i], r) - prequant(keep(R.vn[i], my), r)) - prequant(keep(R.vu[i], mz), r)) +
+                        prequant(keep(R.vun[i], my & mz), r);
+        c[i] = code_fast(q - keep(prev, bitmask(kx, i)), true, ovf);
+        prev = q;
+    }
+    pack8(c, w);
+}
+
+__device__ __forceinline__ void dq8_u(const float* __restrict__ in, uint32_t p0, const Geom& g, float r, uint32_t w[4],
+                                      uint32_t& ovf)
+{
+    RawTile<2> R;
+    raw_load_u<2>(in, p0, g, R);
+    raw_compute_u(R, p0, g, r, w, ovf);
+}
+
 __device__ __forceinline__ void dq8_fast1(const float* __restrict__ in, uint32_t p0, const Geom& g, float r,
                                           uint32_t w[4], uint32_t& ovf)
 {

@@ -61,4 +61,96 @@ hipError_t launch_decode_unshuffle(const uint8_t* payload, const uint32_t* tile_
 hipError_t launch_reconstruct(const uint16_t* codes, float* out, const Geom& g, float ebx2, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 
+// ============================================================================ batched form
+//
+// Many fields of uneven shapes compressed by one launch sequence.  Fields are ordered by
+// dimensionality (1D, 2D, 3D).  Every field owns T + 1 consecutive "units" of one global
+// unit space (its T full tiles, then the unit that handles the partial tile); a unit's
+// 4096-byte staging slot, block count and destination address sit at its unit index.
+
+constexpr uint32_t kBatchMaxFields = 4096;  // two wave_upper rounds
+constexpr int      kPtrChunk = 56;          // pointer records per k_batch_set launch (1792 B of arguments)
+constexpr int      kNumChunks = 3;          // range-partial sizes the workspace holds tables for
+constexpr uint32_t kRangeChunks[kNumChunks] = {16384, 65536, 262144};
+
+struct BField {        // per field, uploaded once per workspace (128 B)
+    Geom     g;        // 72 B
+    uint32_t caller;   // index of the field in the caller's arrays (its result record)
+    uint32_t ustart;   // first global unit
+    uint32_t rdims[3]; // reconstruct workgroups along x, y, z
+    uint32_t pad[9];
+};
+static_assert(sizeof(BField) == 128, "BField layout");
+
+struct BPtrs {         // per field and call (32 B)
+    const void* src;   // compress: input floats; decompress: payload
+    void*       dst;   // compress: payload; decompress: output floats
+    uint16_t*   tail;  // FIXED-mode tail codes (null when the field has none)
+    double      param; // compress: eb_rel; decompress: ebx2
+};
+static_assert(sizeof(BPtrs) == 32, "BPtrs layout");
+
+struct BParams {       // per field, written by the range kernel of the call
+    float    r;        // ebx2_r
+    uint32_t status;
+    uint32_t level;    // dual-quant paths allowed by the bound on |s| (EbParams::level)
+    uint32_t pad;
+};
+
+struct PtrChunk {
+    BPtrs p[kPtrChunk];
+};
+
+// device addresses of one batch workspace
+struct BatchDev {
+    const BField*   fields;
+    const uint32_t* ustart;  // nf + 1, global unit of each field (sorted order)
+    const uint32_t* pstart;  // nf + 1, first range partial of each field (selected chunk)
+    const uint32_t* rstart;  // nf + 1, first reconstruct workgroup of each field
+    BPtrs*          ptrs;
+    BParams*        prm;
+    float4*         part;
+    uint32_t*       counts;  // per unit
+    uint64_t*       dsts;    // per unit: payload address of its first block
+    uint32_t*       texcl;   // per unit: blocks before the tile in its field (decompress)
+    uint8_t*        staging; // 4096 B per unit (compress staging, decompress codes)
+    fz_result*      res;     // caller's result array (compress)
+};
+
+struct BatchShape {
+    uint32_t nf;
+    uint32_t f_lo[4], f_hi[4];  // fields of dimensionality d (index 1..3)
+    uint32_t u_lo[4], u_hi[4];  // their units
+    uint32_t r_lo[4], r_hi[4];  // their reconstruct workgroups
+    uint32_t U;                 // all units
+    uint32_t P;                 // range partials
+    uint32_t chunk;             // floats per range partial
+    uint32_t xcd[4];            // tile order per dimensionality: 1 XCD-sliced, 0 linear
+    uint32_t gather_waves;      // units per gather workgroup
+};
+
+// A group of consecutive fields (sorted order) and the units / range partials they own.
+constexpr int kMaxGroups = 16;
+struct BatchGroup {
+    uint32_t f_lo, f_hi, u_lo, u_hi;
+    uint32_t p_lo[kNumChunks], p_hi[kNumChunks];
+    uint32_t df_lo[4], df_hi[4];  // its fields of dimensionality d (index 1..3)
+    uint32_t du_lo[4], du_hi[4];  // their units
+};
+
+// Streams and events of the grouped schedule: `range` and `gather` fork from the caller's
+// stream and join back into it.
+struct BatchStreams {
+    hipStream_t range, gather;
+    hipEvent_t  fork, join_r, join_g, r[kMaxGroups], t[kMaxGroups];
+};
+
+hipError_t launch_batch_set(const BPtrs* sorted, uint32_t nf, BPtrs* dev, hipStream_t s);
+// Groups run in order; with st == null everything runs on s (and ev, when given, receives
+// start / stop pairs of range_part, range_final, tile 1D/2D/3D, scan, gather of a single group).
+hipError_t launch_batch_compress(const BatchDev& d, const BatchShape& b, const BatchGroup* groups, int ngroups,
+                                 int chunk, const BatchStreams* st, hipEvent_t* ev, hipStream_t s);
+// ev (may be null): start / stop pairs of dscan, decode, reconstruct 1D/2D/3D
+hipError_t launch_batch_decompress(const BatchDev& d, const BatchShape& b, hipEvent_t* ev, hipStream_t s);
+
 }  // namespace fz

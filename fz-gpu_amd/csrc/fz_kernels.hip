@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include "fz_internal.hpp"
+#include "fz_tile.hpp"
 
 namespace fz {
 
@@ -35,21 +36,6 @@ static hipError_t launch_timed(K kernel, dim3 grid, dim3 block, hipStream_t s, h
     return hipGetLastError();
 }
 
-// 16-byte global accesses at 4-byte alignment (the encoded blocks follow 4 W bytes of flags)
-typedef uint32_t u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
-
-__device__ __forceinline__ void st16(uint8_t* p, uint4 v)
-{
-    *reinterpret_cast<u32x4a4*>(p) = u32x4a4{v.x, v.y, v.z, v.w};
-}
-__device__ __forceinline__ uint4 ld16(const uint8_t* p)
-{
-    const u32x4a4 t = *reinterpret_cast<const u32x4a4*>(p);
-    return make_uint4(t.x, t.y, t.z, t.w);
-}
-
-__device__ __forceinline__ uint32_t nonfinite(float v) { return (__float_as_uint(v) & 0x7f800000u) == 0x7f800000u; }
-
 // ============================================================================ range
 
 // Value range in two kernels: k_range_part (per-workgroup min / max / non-finite flag, plain
@@ -66,178 +52,37 @@ __device__ __forceinline__ uint32_t nonfinite(float v) { return (__float_as_uint
 // placed round-robin over the 8 XCDs, so the slices an XCD reduces are the tiles it then
 // compresses (a placement hint only; any placement gives the same result).
 
-__device__ __forceinline__ uint32_t region_start(uint32_t x, uint32_t n)
-{
-    return x * (n >> 3) + min(x, n & 7u);
-}
-
-// tile of workgroup b < n in the XCD-aware order (bijective on [0, n))
-__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n)
-{
-    return region_start(b & 7u, n) + (b >> 3);
-}
-
 constexpr int kRangeWG = 1024;  // threads per partial-range workgroup
 
 __global__ __launch_bounds__(kRangeWG) void k_range_part(const float* __restrict__ in, Geom g,
                                                          float4* __restrict__ part, uint64_t* zero_states,
                                                          uint32_t nzero, fz_result* zero_ovf)
 {
-    __shared__ float    s_lo[kRangeWG / 64], s_hi[kRangeWG / 64];
-    __shared__ uint32_t s_bad[kRangeWG / 64];
     const uint32_t tid = threadIdx.x;
     if (blockIdx.x == 0) {  // clear the look-back state and overflow count of the kernels that follow
         for (uint32_t i = tid; i < nzero; i += kRangeWG) zero_states[i] = 0;
         if (tid == 0 && zero_ovf) zero_ovf->overflow = 0;
     }
-    // this workgroup's span of float4s: slice (b >> 3) of region (b & 7), regions in tile units
+    // this workgroup's span: slice (b >> 3) of region (b & 7), regions in tile units
     const uint32_t nunits = g.T + 1;  // the last unit is the partial tile
     const uint32_t x = blockIdx.x & 7u, i = blockIdx.x >> 3, per = gridDim.x >> 3;
     const uint32_t r_lo = region_start(x, nunits), r_hi = region_start(x + 1, nunits);
     const uint32_t u_lo = r_lo + (uint32_t)(((uint64_t)(r_hi - r_lo) * i) / per);
     const uint32_t u_hi = r_lo + (uint32_t)(((uint64_t)(r_hi - r_lo) * (i + 1)) / per);
     const uint32_t f_lo = min(u_lo * (uint32_t)kTileCodes, g.len), f_hi = min(u_hi * (uint32_t)kTileCodes, g.len);
-    float    lo = __builtin_huge_valf(), hi = -__builtin_huge_valf();
-    uint32_t bad = 0;
-    const uint32_t v_lo = (f_lo + 3) / 4, v_hi = f_hi / 4;  // whole float4s inside [f_lo, f_hi)
-    const float4*  in4 = reinterpret_cast<const float4*>(in);
-    constexpr uint32_t U = 4;
-    if (v_lo < v_hi) {
-        for (uint32_t base = v_lo + tid; base < v_hi; base += kRangeWG * U) {
-            float4 v[U];
-#pragma unroll
-            for (uint32_t j = 0; j < U; j++) v[j] = in4[min(base + j * kRangeWG, v_hi - 1)];  // re-reads are harmless
-#pragma unroll
-            for (uint32_t j = 0; j < U; j++) {
-                lo = fminf(lo, fminf(fminf(v[j].x, v[j].y), fminf(v[j].z, v[j].w)));
-                hi = fmaxf(hi, fmaxf(fmaxf(v[j].x, v[j].y), fmaxf(v[j].z, v[j].w)));
-                bad |= nonfinite(v[j].x) | nonfinite(v[j].y) | nonfinite(v[j].z) | nonfinite(v[j].w);
-            }
-        }
-    }
-    // scalars at the span's edges (fewer than 4 each side)
-    const uint32_t e0 = min(4 * v_lo, f_hi), e1 = max(4 * v_hi, e0);
-    for (uint32_t k = f_lo + tid; k < e0; k += kRangeWG) {
-        lo = fminf(lo, in[k]); hi = fmaxf(hi, in[k]); bad |= nonfinite(in[k]);
-    }
-    for (uint32_t k = e1 + tid; k < f_hi; k += kRangeWG) {
-        lo = fminf(lo, in[k]); hi = fmaxf(hi, in[k]); bad |= nonfinite(in[k]);
-    }
-    lo = wave_min(lo);
-    hi = wave_max(hi);
-    bad = ballot64(bad != 0) != 0;
-    const uint32_t wv = tid >> 6;
-    if ((tid & 63) == 0) {
-        s_lo[wv] = lo;
-        s_hi[wv] = hi;
-        s_bad[wv] = bad;
-    }
-    __syncthreads();
-    if (tid < 64) {
-        const bool live = tid < kRangeWG / 64;
-        lo = wave_min(live ? s_lo[tid] : __builtin_huge_valf());
-        hi = wave_max(live ? s_hi[tid] : -__builtin_huge_valf());
-        bad = ballot64(live && s_bad[tid] != 0) != 0;
-        if (tid == 0) part[blockIdx.x] = make_float4(lo, hi, __uint_as_float(bad), 0.0f);
-    }
-}
-
-// Reduces the partials of k_range_part inside any 256-thread workgroup (nparts <= 256) and
-// derives the eb parameters in thread 0.  Returns them to every thread.
-struct EbParams {
-    float    lo, hi, r, ebx2;
-    double   eb;
-    uint32_t status;
-    bool     safe;
-};
-
-__device__ __forceinline__ EbParams reduce_partials(const float4* __restrict__ part, uint32_t nparts, double eb_rel)
-{
-    __shared__ float    s_lo[4], s_hi[4];
-    __shared__ uint32_t s_bad[4];
-    __shared__ EbParams s_eb;
-    const uint32_t tid = threadIdx.x, wv = tid >> 6;
-    float    lo = __builtin_huge_valf(), hi = -__builtin_huge_valf();
-    uint32_t bad = 0;
-    if (tid < nparts) {
-        const float4 p = part[tid];
-        lo = p.x;
-        hi = p.y;
-        bad = __float_as_uint(p.z);
-    }
-    lo = wave_min(lo);
-    hi = wave_max(hi);
-    bad = ballot64(bad != 0) != 0;
-    if ((tid & 63) == 0) {
-        s_lo[wv] = lo;
-        s_hi[wv] = hi;
-        s_bad[wv] = bad;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        EbParams e;
-        e.lo = fminf(fminf(s_lo[0], s_lo[1]), fminf(s_lo[2], s_lo[3]));
-        e.hi = fmaxf(fmaxf(s_hi[0], s_hi[1]), fmaxf(s_hi[2], s_hi[3]));
-        const uint32_t b = s_bad[0] | s_bad[1] | s_bad[2] | s_bad[3];
-        const float    range = e.hi - e.lo;
-        e.eb = eb_rel * (double)range;
-        const double e2 = e.eb * 2.0;
-        const double e2r = 1.0 / e2;
-        e.r = (float)e2r;
-        e.ebx2 = (float)e2;
-        e.status = b ? ST_NONFINITE : (range == 0.0f ? ST_CONSTANT : ST_SUCCESS);
-        e.safe = fmaxf(__builtin_fabsf(e.lo), __builtin_fabsf(e.hi)) * e.r < 8388608.0f;
-        s_eb = e;
-    }
-    __syncthreads();
-    return s_eb;
+    range_span<kRangeWG>(in, f_lo, f_hi, part + blockIdx.x);
 }
 
 __global__ __launch_bounds__(kWG) void k_range_final(const float4* __restrict__ part, uint32_t nparts, Geom g,
                                                      double eb_rel, fz_result* res, uint64_t* zero_states,
                                                      uint32_t nzero, uint32_t* zero_ticket)
 {
-    __shared__ float    s_lo[4], s_hi[4];
-    __shared__ uint32_t s_bad[4];
     const uint32_t tid = threadIdx.x;
     for (uint32_t i = tid; i < nzero; i += kWG) zero_states[i] = 0;
     if (tid == 0 && zero_ticket) *zero_ticket = 0;
-    float    lo = __builtin_huge_valf(), hi = -__builtin_huge_valf();
-    uint32_t bad = 0;
-    for (uint32_t i = tid; i < nparts; i += kWG) {
-        const float4 p = part[i];
-        lo = fminf(lo, p.x);
-        hi = fmaxf(hi, p.y);
-        bad |= __float_as_uint(p.z);
-    }
-    lo = wave_min(lo);
-    hi = wave_max(hi);
-    bad = ballot64(bad != 0) != 0;
-    const uint32_t wv = tid >> 6;
-    if ((tid & 63) == 0) {
-        s_lo[wv] = lo;
-        s_hi[wv] = hi;
-        s_bad[wv] = bad;
-    }
-    __syncthreads();
+    const EbParams e = reduce_partials(part, nparts, eb_rel, g.ndim);
     if (tid == 0) {
-        lo = fminf(fminf(s_lo[0], s_lo[1]), fminf(s_lo[2], s_lo[3]));
-        hi = fmaxf(fmaxf(s_hi[0], s_hi[1]), fmaxf(s_hi[2], s_hi[3]));
-        bad = s_bad[0] | s_bad[1] | s_bad[2] | s_bad[3];
-        const float  range = hi - lo;
-        const double eb = eb_rel * (double)range;
-        const double e2 = eb * 2.0;
-        const double e2r = 1.0 / e2;
-        res->status = bad ? ST_NONFINITE : (range == 0.0f ? ST_CONSTANT : ST_SUCCESS);
-        res->ndim = (uint32_t)g.ndim;
-        res->vmin = lo;
-        res->vmax = hi;
-        res->ebx2_r = (float)e2r;
-        res->ebx2 = (float)e2;
-        res->eb_abs = eb;
-        res->nblocks = 0;
-        res->payload_bytes = 0;
-        res->tail_len = g.mode == 0 ? (uint64_t)(g.len - (uint32_t)kTileCodes * g.T) : 0;
+        write_header(res, e, g);
         res->overflow = 0;
     }
 }
@@ -263,65 +108,6 @@ hipError_t launch_range(const float* in, const Geom& g, double eb_rel, float4* p
 //                    look-back across workgroups) and the copy of every staged block to
 //                    payload + 4 W + 16 * (global rank)  (encodeDebug's placement, fz.cu:56-57).
 
-// Codes of the len - 2048 T positions no tile covers: returned in `tail` in FIXED mode
-// (the reference never encodes them, fz.cu:173).  Also clears the bitflag words of the
-// partial tile (fz.cu:167 memsets them) and finishes the record when there is no tile.
-template <int NDIM>
-__device__ void tail_work(const float* __restrict__ in, const Geom& g, float r, fz_result* res, uint32_t* bwords,
-                          uint16_t* tail, uint32_t& ovf)
-{
-    const uint32_t tid = threadIdx.x;
-    const uint32_t t0 = (uint32_t)kTileCodes * g.T;
-    const uint32_t tail_len = g.len - t0;
-    if (8 * tid < tail_len) {
-        uint32_t w[4];
-        dq8<NDIM>(in, t0 + 8 * tid, g, r, w, ovf);
-        if (g.mode == 0 && tail) {
-#pragma unroll
-            for (int i = 0; i < 8; i++)
-                if (8 * tid + i < tail_len) tail[8 * tid + i] = (uint16_t)(w[i >> 1] >> (16 * (i & 1)));
-        }
-    }
-    for (uint32_t i = 8 * g.T + tid; i < g.W; i += kWG) bwords[i] = 0;
-    if (res && g.T == 0 && tid == 0) {
-        res->nblocks = 0;
-        res->payload_bytes = 4ull * g.W;
-    }
-}
-
-// Which dual-quant path a tile takes (uniform over the workgroup): 2 = fast (aligned float4
-// neighbour windows), 1 = unguarded generic, 0 = guarded generic.  geom_path ignores the
-// value range; the fast and unguarded paths also need every |s| < 2^23 (`safe`).
-template <int NDIM>
-__device__ __forceinline__ int geom_path(const Geom& g, uint32_t t0)
-{
-    if (t0 + (uint32_t)kTileCodes > g.xyz) return 0;
-    if constexpr (NDIM == 1) return 2;
-    const uint32_t halo = (NDIM == 2) ? g.X + 1 : g.XY + g.X + 1;
-    if (t0 < halo) return 0;
-    return (g.X & 3u) == 0 ? 2 : 1;
-}
-
-template <int NDIM>
-__device__ __forceinline__ int tile_path(const Geom& g, uint32_t t0, bool safe)
-{
-    return safe ? geom_path<NDIM>(g, t0) : 0;
-}
-
-template <int NDIM>
-__device__ __forceinline__ void dq8_tile(const float* __restrict__ in, uint32_t p0, const Geom& g, float r, int path,
-                                         uint32_t w[4], uint32_t& ovf)
-{
-    if (path == 2) {
-        if constexpr (NDIM == 1) dq8_fast1(in, p0, g, r, w, ovf);
-        else dq8_a4<NDIM>(in, p0, g, r, w, ovf);
-    } else if (path == 1) {
-        dq8<NDIM, false>(in, p0, g, r, w, ovf);
-    } else {
-        dq8<NDIM, true>(in, p0, g, r, w, ovf);
-    }
-}
-
 template <int NDIM>
 __global__ __launch_bounds__(kWG) void k_tile_compress(const float* __restrict__ in, Geom g, double eb_rel,
                                                        const float4* __restrict__ part, uint32_t nparts,
@@ -329,81 +115,29 @@ __global__ __launch_bounds__(kWG) void k_tile_compress(const float* __restrict__
                                                        uint8_t* __restrict__ staging, uint32_t* __restrict__ counts,
                                                        uint16_t* __restrict__ tail)
 {
-    __shared__ uint32_t sh[32 * kLdsPitch];
-    __shared__ uint32_t sh_wcnt[4];
-    __shared__ uint32_t sh_ovf;
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const bool     is_tail = blockIdx.x >= g.T;
-    // XCD-aware order: an XCD walks its eighth of the field, so the neighbour rows a tile
-    // re-reads were fetched into the same L2 moments before
+    const bool is_tail = blockIdx.x >= g.T;
+    // XCD-aware order: an XCD walks its eighth of the field
     const uint32_t tile = is_tail ? g.T : xcd_tile(blockIdx.x, g.T);
-    const uint32_t t0 = tile * (uint32_t)kTileCodes, p0 = t0 + 8 * tid;
-    const int      gp = is_tail ? 0 : geom_path<NDIM>(g, t0);
+    const int      gp = is_tail ? 0 : geom_path<NDIM>(g, tile * (uint32_t)kTileCodes);
     // the tile's loads need no eb: issue them before reducing the range partials
     RawTile<NDIM> R;
-    if (gp == 2) raw_load<NDIM>(in, p0, g, R);
-    const EbParams e = reduce_partials(part, nparts, eb_rel);
-    if (blockIdx.x == 0 && tid == 0) {
-        res->status = e.status;
-        res->ndim = (uint32_t)g.ndim;
-        res->vmin = e.lo;
-        res->vmax = e.hi;
-        res->ebx2_r = e.r;
-        res->ebx2 = e.ebx2;
-        res->eb_abs = e.eb;
-        res->nblocks = 0;
-        res->payload_bytes = 0;
-        res->tail_len = g.mode == 0 ? (uint64_t)(g.len - (uint32_t)kTileCodes * g.T) : 0;
-    }
+    raw_prefetch<NDIM>(in, tile * (uint32_t)kTileCodes + 8 * threadIdx.x, g, gp, R);
+    const EbParams e = reduce_partials_256(part, nparts, eb_rel, NDIM);
+    if (blockIdx.x == 0 && threadIdx.x == 0) write_header(res, e, g);
     if (e.status != ST_SUCCESS) return;  // uniform
-    const float r = e.r;
-    uint32_t*   bwords = reinterpret_cast<uint32_t*>(payload);
-    uint32_t    ovf = 0;
-    if (tid == 0) sh_ovf = 0;
-
+    unsigned long long* ovf = reinterpret_cast<unsigned long long*>(&res->overflow);
+    uint32_t*           bwords = reinterpret_cast<uint32_t*>(payload);
     if (is_tail) {
-        tail_work<NDIM>(in, g, r, res, bwords, tail, ovf);
-        ovf = wave_sum(ovf);
-        if (lane == 0 && ovf) atomicAdd(reinterpret_cast<unsigned long long*>(&res->overflow), (unsigned long long)ovf);
+        tail_unit<NDIM>(in, g, e.r, bwords, tail, ovf);
+        if (g.T == 0 && threadIdx.x == 0) {
+            res->nblocks = 0;
+            res->payload_bytes = 4ull * g.W;
+        }
         return;
     }
-
-    // 1. dual-quant: words 4t..4t+3 of the tile
-    uint32_t w[4];
-    if (gp == 2 && e.safe) raw_compute<NDIM>(R, p0, g, r, w, ovf);
-    else dq8_tile<NDIM>(in, p0, g, r, e.safe ? gp : 0, w, ovf);
-
-    // 2. bit transpose of group k = t/8 (skipped for an all-zero wave); register q then holds
-    //    o[32(4L+q) + k]
-    const uint32_t L = tid & 7, k = tid >> 3;
-    if (ballot64((w[0] | w[1] | w[2] | w[3]) != 0) != 0) transpose4(w, L);
-#pragma unroll
-    for (int q = 0; q < 4; q++) sh[kLdsPitch * (4 * L + q) + k] = w[q];
-    ovf = wave_sum(ovf);
-    if (lane == 0 && ovf) atomicAdd(&sh_ovf, ovf);
-    __syncthreads();
-
-    // 3. thread i owns 16-byte block i = o[4i .. 4i+3]; flag = block != 0 (fz.cu:84-91)
-    const uint32_t c = tid >> 3, k0 = 4 * (tid & 7);
-    const uint4    blk = make_uint4(sh[kLdsPitch * c + k0], sh[kLdsPitch * c + k0 + 1], sh[kLdsPitch * c + k0 + 2],
-                                    sh[kLdsPitch * c + k0 + 3]);
-    const bool     f = (blk.x | blk.y | blk.z | blk.w) != 0;
-    const uint64_t mask = ballot64(f);
-    // bitflag word 8*tile + y holds flags 32y..32y+31 (LSB first, fz.cu:95-99): wave wv covers
-    // words 2wv (lanes 0-31) and 2wv+1 (lanes 32-63)
-    if (lane < 2) bwords[8 * tile + 2 * wv + lane] = lane ? (uint32_t)(mask >> 32) : (uint32_t)mask;
-    const uint32_t rank = rank_below(mask);
-    if (lane == 0) sh_wcnt[wv] = (uint32_t)__popcll(mask);
-    __syncthreads();
-    const uint32_t c0 = sh_wcnt[0], c1 = sh_wcnt[1], c2 = sh_wcnt[2], c3 = sh_wcnt[3];
-    const uint32_t woff = (wv > 0 ? c0 : 0u) + (wv > 1 ? c1 : 0u) + (wv > 2 ? c2 : 0u);
-    if (tid == 0) {
-        counts[tile] = c0 + c1 + c2 + c3;
-        if (sh_ovf) atomicAdd(reinterpret_cast<unsigned long long*>(&res->overflow), (unsigned long long)sh_ovf);
-    }
-    // 4. in-tile compaction into the tile's staging slot
-    if (f) *reinterpret_cast<uint4*>(staging + (size_t)tile * 4096 + 16 * (woff + rank)) = blk;
+    tile_encode<NDIM>(in, g, tile, gp, R, e.r, e.level, bwords, staging + (size_t)tile * 4096, counts + tile, ovf);
 }
+
 
 // Scan of the tile counts + placement of the staged blocks.  The grid is small (<= kGatherWG
 // workgroups, each a contiguous run of tiles) and chains its look-back in blockIdx order: at
@@ -804,44 +538,16 @@ hipError_t launch_tile_scan(const uint32_t* bitflags, uint32_t T, uint32_t* tile
     return hipGetLastError();
 }
 
-// Scatter the tile's nonzero blocks back (block i: payload + 4W + 16 * (tile_excl + rank)),
-// unshuffle, store the 1024 code words.  The extra workgroup writes the tail codes (or
-// zeros when there is no tail buffer).
+// Inverse of the tile compress: the extra workgroup writes the tail codes (or zeros when
+// there is no tail buffer).
 __global__ __launch_bounds__(kWG) void k_decode_unshuffle(const uint8_t* __restrict__ payload,
                                                           const uint32_t* __restrict__ tile_excl,
                                                           const uint16_t* __restrict__ tail, Geom g,
                                                           uint32_t* __restrict__ codes)
 {
-    __shared__ uint32_t sh[32 * kLdsPitch];
-    __shared__ uint32_t sh_w[4];
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, tile = blockIdx.x;
-    if (tile >= g.T) {
-        uint16_t*      c16 = reinterpret_cast<uint16_t*>(codes);
-        const uint32_t t0 = kTileCodes * g.T;
-        for (uint32_t i = tid; t0 + i < g.len; i += kWG) c16[t0 + i] = tail ? tail[i] : (uint16_t)0;
-        return;
-    }
-    const uint32_t* bwords = reinterpret_cast<const uint32_t*>(payload);
-    const uint32_t  word = bwords[8 * tile + (tid >> 5)];
-    const bool      f = (word >> (tid & 31)) & 1u;
-    const uint64_t  mask = ballot64(f);
-    if (lane == 0) sh_w[wv] = (uint32_t)__popcll(mask);
-    __syncthreads();
-    const uint32_t woff = (wv > 0 ? sh_w[0] : 0u) + (wv > 1 ? sh_w[1] : 0u) + (wv > 2 ? sh_w[2] : 0u);
-    uint4          blk = make_uint4(0, 0, 0, 0);
-    if (f) blk = ld16(payload + 4ull * g.W + 16ull * (tile_excl[tile] + woff + rank_below(mask)));
-    const uint32_t c = tid >> 3, k0 = 4 * (tid & 7);
-    sh[kLdsPitch * c + k0] = blk.x;
-    sh[kLdsPitch * c + k0 + 1] = blk.y;
-    sh[kLdsPitch * c + k0 + 2] = blk.z;
-    sh[kLdsPitch * c + k0 + 3] = blk.w;
-    __syncthreads();
-    const uint32_t L = tid & 7, k = tid >> 3;
-    uint32_t       w[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) w[q] = sh[kLdsPitch * (4 * L + q) + k];
-    transpose4(w, L);
-    *reinterpret_cast<uint4*>(codes + (size_t)tile * kTileWords + 4 * tid) = make_uint4(w[0], w[1], w[2], w[3]);
+    const uint32_t tile = blockIdx.x;
+    if (tile >= g.T) decode_tail(tail, g, reinterpret_cast<uint16_t*>(codes));
+    else decode_tile(payload, g.W, tile, tile_excl[tile], codes + (size_t)tile * kTileWords);
 }
 
 hipError_t launch_decode_unshuffle(const uint8_t* payload, const uint32_t* tile_excl, const uint16_t* tail,
@@ -851,123 +557,22 @@ hipError_t launch_decode_unshuffle(const uint8_t* payload, const uint32_t* tile_
     return hipGetLastError();
 }
 
-__device__ __forceinline__ int32_t decode(uint32_t c)
-{
-    const int32_t m = (int32_t)(c & 0x7fffu);
-    return (c & 0x8000u) ? -m : m;
-}
-
-// 1D: 16 consecutive codes per thread, 16 threads per 256-block (block-local inclusive
-// prefix sum), x = (float)s * ebx2.
 __global__ __launch_bounds__(kWG) void k_reconstruct_1d(const uint16_t* __restrict__ codes, float* __restrict__ out,
                                                         Geom g, float ebx2)
 {
-    const uint32_t p0 = (blockIdx.x * kWG + threadIdx.x) * 16;
-    int32_t        s[16];
-    if (p0 + 16 <= g.xyz) {
-        const uint4 a = *reinterpret_cast<const uint4*>(codes + p0);
-        const uint4 b = *reinterpret_cast<const uint4*>(codes + p0 + 8);
-        const uint32_t wd[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            s[2 * j] = decode(wd[j] & 0xffffu);
-            s[2 * j + 1] = decode(wd[j] >> 16);
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < 16; j++) s[j] = (p0 + j < g.xyz) ? decode(codes[p0 + j]) : 0;
-    }
-#pragma unroll
-    for (int j = 1; j < 16; j++) s[j] += s[j - 1];
-    // exclusive prefix of the 16 thread totals of this 256-block (lanes 16-aligned)
-    int32_t t = s[15];
-#pragma unroll
-    for (int d = 1; d < 16; d <<= 1) {
-        const int32_t o = __shfl_up(t, d, 16);
-        if ((threadIdx.x & 15) >= (uint32_t)d) t += o;
-    }
-    const int32_t ex = t - s[15];
-    if (p0 + 16 <= g.xyz) {
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-            *reinterpret_cast<float4*>(out + p0 + 4 * j) =
-                make_float4((float)(ex + s[4 * j]) * ebx2, (float)(ex + s[4 * j + 1]) * ebx2,
-                            (float)(ex + s[4 * j + 2]) * ebx2, (float)(ex + s[4 * j + 3]) * ebx2);
-    } else {
-#pragma unroll
-        for (int j = 0; j < 16; j++)
-            if (p0 + j < g.xyz) out[p0 + j] = (float)(ex + s[j]) * ebx2;
-    }
+    recon_1d(codes, out, g, ebx2, blockIdx.x);
 }
 
-// 2D: workgroup = 16 rows x 256 columns (16 Lorenzo blocks); thread = one column of a block.
-// Prefix over y in registers, then over x across the 16 lanes of the block.
 __global__ __launch_bounds__(kWG) void k_reconstruct_2d(const uint16_t* __restrict__ codes, float* __restrict__ out,
                                                         Geom g, float ebx2)
 {
-    const uint32_t x = blockIdx.x * kWG + threadIdx.x, y0 = blockIdx.y * 16;
-    const bool     xin = x < g.X;
-    int32_t        s[16];
-#pragma unroll
-    for (int r = 0; r < 16; r++)
-        s[r] = (xin && y0 + r < g.Y) ? decode(codes[(size_t)(y0 + r) * g.X + x]) : 0;
-#pragma unroll
-    for (int r = 1; r < 16; r++) s[r] += s[r - 1];
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-#pragma unroll
-        for (int d = 1; d < 16; d <<= 1) {
-            const int32_t o = __shfl_up(s[r], d, 16);
-            if ((threadIdx.x & 15) >= (uint32_t)d) s[r] += o;
-        }
-    }
-    if (xin) {
-#pragma unroll
-        for (int r = 0; r < 16; r++)
-            if (y0 + r < g.Y) out[(size_t)(y0 + r) * g.X + x] = (float)s[r] * ebx2;
-    }
+    recon_2d(codes, out, g, ebx2, blockIdx.x, blockIdx.y);
 }
 
-// 3D: workgroup = 32 (x) x 8 (y) x 8 (z) like the reference's (32,1,8) block.  Thread (tx, ty)
-// holds the z column of (x, y): prefix over z in registers, over x across 8-lane groups; then
-// an LDS transpose puts the y column in registers for the last prefix.
 __global__ __launch_bounds__(kWG) void k_reconstruct_3d(const uint16_t* __restrict__ codes, float* __restrict__ out,
                                                         Geom g, float ebx2)
 {
-    __shared__ int32_t sh[8][8][33];
-    const uint32_t tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-    const uint32_t x = blockIdx.x * 32 + tx, y0 = blockIdx.y * 8, z0 = blockIdx.z * 8;
-    const bool     xin = x < g.X;
-    int32_t        s[8];
-    {
-        const uint32_t y = y0 + ty;
-#pragma unroll
-        for (int z = 0; z < 8; z++)
-            s[z] = (xin && y < g.Y && z0 + z < g.Z) ? decode(codes[(size_t)(z0 + z) * g.XY + (size_t)y * g.X + x]) : 0;
-    }
-#pragma unroll
-    for (int z = 1; z < 8; z++) s[z] += s[z - 1];
-#pragma unroll
-    for (int z = 0; z < 8; z++) {
-#pragma unroll
-        for (int d = 1; d < 8; d <<= 1) {
-            const int32_t o = __shfl_up(s[z], d, 8);
-            if ((tx & 7) >= (uint32_t)d) s[z] += o;
-        }
-        sh[z][ty][tx] = s[z];
-    }
-    __syncthreads();
-    const uint32_t tz = ty;
-    int32_t        v[8];
-#pragma unroll
-    for (int y = 0; y < 8; y++) v[y] = sh[tz][y][tx];
-#pragma unroll
-    for (int y = 1; y < 8; y++) v[y] += v[y - 1];
-    if (xin && z0 + tz < g.Z) {
-#pragma unroll
-        for (int y = 0; y < 8; y++)
-            if (y0 + y < g.Y) out[(size_t)(z0 + tz) * g.XY + (size_t)(y0 + y) * g.X + x] = (float)v[y] * ebx2;
-    }
+    recon_3d(codes, out, g, ebx2, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 hipError_t launch_reconstruct(const uint16_t* codes, float* out, const Geom& g, float ebx2, hipStream_t s)

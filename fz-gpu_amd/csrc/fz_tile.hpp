@@ -1,0 +1,538 @@
+// fz_tile.hpp -- per-workgroup device routines shared by the single-field kernels
+// (fz_kernels.hip) and the batched kernels (fz_batch.hip).  Both kernel families call the same
+// bodies, so a field compressed alone and the same field compressed inside a batch produce
+// the same bytes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/fz_capi.h"
+#include "fz_device.hpp"
+
+namespace fz {
+
+// 16-byte global accesses at 4-byte alignment (the encoded blocks follow 4 W bytes of flags)
+typedef uint32_t u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
+
+__device__ __forceinline__ void st16(uint8_t* p, uint4 v)
+{
+    *reinterpret_cast<u32x4a4*>(p) = u32x4a4{v.x, v.y, v.z, v.w};
+}
+__device__ __forceinline__ uint4 ld16(const uint8_t* p)
+{
+    const u32x4a4 t = *reinterpret_cast<const u32x4a4*>(p);
+    return make_uint4(t.x, t.y, t.z, t.w);
+}
+
+__device__ __forceinline__ uint32_t nonfinite(float v) { return (__float_as_uint(v) & 0x7f800000u) == 0x7f800000u; }
+
+// ---------------------------------------------------------------------------- XCD-aware order
+
+// Workgroups are placed round-robin over the 8 XCDs (each with its own L2).  xcd_tile maps
+// workgroup b of n to a tile such that XCD b % 8 walks the (b % 8)-th contiguous eighth of
+// the tile range: the neighbour rows a Lorenzo tile re-reads were fetched into the same L2
+// moments before.  It is a bijection on [0, n); any placement gives the same results.
+__device__ __forceinline__ uint32_t region_start(uint32_t x, uint32_t n)
+{
+    return x * (n >> 3) + min(x, n & 7u);
+}
+
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n)
+{
+    return region_start(b & 7u, n) + (b >> 3);
+}
+
+// Largest i in [lo, hi) with a[i] <= v, for a nondecreasing a with a[lo] <= v.  Called by
+// all 64 lanes of a wave with uniform arguments; returns a uniform value.  Each round narrows
+// the range 64-fold with one load per lane (two rounds cover 4096 entries).
+__device__ __forceinline__ uint32_t wave_upper(const uint32_t* __restrict__ a, uint32_t lo, uint32_t hi, uint32_t v)
+{
+    const uint32_t lane = lane_id();
+    while (hi - lo > 64u) {
+        const uint32_t step = (hi - lo + 63u) >> 6;
+        const uint32_t i = lo + lane * step;
+        const uint32_t c = (uint32_t)__popcll(ballot64(i < hi && a[i] <= v));
+        lo += (c - 1u) * step;
+        hi = min(hi, lo + step);
+    }
+    const uint32_t i = lo + lane;
+    return __builtin_amdgcn_readfirstlane(lo + (uint32_t)__popcll(ballot64(i < hi && a[i] <= v)) - 1u);
+}
+
+// ---------------------------------------------------------------------------- value range
+
+// min / max / non-finite flag of in[f_lo, f_hi), reduced over the NT threads of the
+// workgroup; thread 0 stores (lo, hi, bad-as-bits, 0) to *out.  float4 loads over the
+// 16-byte-aligned middle, scalar loads at the edges.
+template <int NT>
+__device__ __forceinline__ void range_span(const float* __restrict__ in, uint32_t f_lo, uint32_t f_hi,
+                                           float4* __restrict__ out)
+{
+    __shared__ float    s_lo[NT / 64], s_hi[NT / 64];
+    __shared__ uint32_t s_bad[NT / 64];
+    const uint32_t tid = threadIdx.x;
+    float    lo = __builtin_huge_valf(), hi = -__builtin_huge_valf();
+    uint32_t bad = 0;
+    const uint32_t v_lo = (f_lo + 3) / 4, v_hi = f_hi / 4;  // whole float4s inside [f_lo, f_hi)
+    const float4*  in4 = reinterpret_cast<const float4*>(in);
+    constexpr uint32_t U = 4;
+    if (v_lo < v_hi) {
+        for (uint32_t base = v_lo + tid; base < v_hi; base += NT * U) {
+            float4 v[U];
+#pragma unroll
+            for (uint32_t j = 0; j < U; j++) v[j] = in4[min(base + j * NT, v_hi - 1)];  // re-reads are harmless
+#pragma unroll
+            for (uint32_t j = 0; j < U; j++) {
+                lo = fminf(lo, fminf(fminf(v[j].x, v[j].y), fminf(v[j].z, v[j].w)));
+                hi = fmaxf(hi, fmaxf(fmaxf(v[j].x, v[j].y), fmaxf(v[j].z, v[j].w)));
+                bad |= nonfinite(v[j].x) | nonfinite(v[j].y) | nonfinite(v[j].z) | nonfinite(v[j].w);
+            }
+        }
+    }
+    const uint32_t e0 = min(4 * v_lo, f_hi), e1 = max(4 * v_hi, e0);
+    for (uint32_t k = f_lo + tid; k < e0; k += NT) {
+        lo = fminf(lo, in[k]); hi = fmaxf(hi, in[k]); bad |= nonfinite(in[k]);
+    }
+    for (uint32_t k = e1 + tid; k < f_hi; k += NT) {
+        lo = fminf(lo, in[k]); hi = fmaxf(hi, in[k]); bad |= nonfinite(in[k]);
+    }
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    bad = ballot64(bad != 0) != 0;
+    const uint32_t wv = tid >> 6;
+    if ((tid & 63) == 0) {
+        s_lo[wv] = lo;
+        s_hi[wv] = hi;
+        s_bad[wv] = bad;
+    }
+    __syncthreads();
+    if (tid < 64) {
+        const bool live = tid < NT / 64;
+        lo = wave_min(live ? s_lo[tid] : __builtin_huge_valf());
+        hi = wave_max(live ? s_hi[tid] : -__builtin_huge_valf());
+        bad = ballot64(live && s_bad[tid] != 0) != 0;
+        if (tid == 0) *out = make_float4(lo, hi, __uint_as_float(bad), 0.0f);
+    }
+}
+
+// eb parameters of a field from its value range (fz.cu:137, 149; lorenzo_var.cuh:562-563):
+//   eb_abs = eb_rel * (double)(max - min);  ebx2 = eb_abs * 2;  ebx2_r = 1 / ebx2  (double),
+//   both narrowed to float.  `level` bounds every |s| = |round(x * ebx2_r)| (|fl(x r)| <=
+//   fl(max|x| r) by monotone rounding) and picks the dual-quant paths a tile may take:
+//     3  |s| <= 16383 / 8191 / 4095 (1D / 2D / 3D): packed 16-bit path
+//     2  |s| < 2^21: 3D column form in float
+//     1  |s| < 2^23: fast float paths in the reference's operation order
+//     0  otherwise: guarded generic path
+struct EbParams {
+    float    lo, hi, r, ebx2;
+    double   eb;
+    uint32_t status;
+    int      level;
+};
+
+__device__ __forceinline__ EbParams eb_params(float lo, float hi, uint32_t bad, double eb_rel, int ndim)
+{
+    EbParams e;
+    e.lo = lo;
+    e.hi = hi;
+    const float range = hi - lo;
+    e.eb = eb_rel * (double)range;
+    const double e2 = e.eb * 2.0;
+    const double e2r = 1.0 / e2;
+    e.r = (float)e2r;
+    e.ebx2 = (float)e2;
+    e.status = bad ? ST_NONFINITE : (range == 0.0f ? ST_CONSTANT : ST_SUCCESS);
+    const float m = fmaxf(__builtin_fabsf(lo), __builtin_fabsf(hi)) * e.r;
+    e.level = m < (ndim == 1 ? 16383.0f : (ndim == 2 ? 8191.0f : 4095.0f)) ? 3
+              : m < 2097151.0f                                            ? 2
+              : m < 8388608.0f                                            ? 1
+                                                                          : 0;
+    return e;
+}
+
+// Reduces the partials part[0, nparts) inside a 256-thread workgroup and returns the eb
+// parameters to every thread.  Any nparts (a loop of 256 per round).
+__device__ __forceinline__ EbParams reduce_partials(const float4* __restrict__ part, uint32_t nparts, double eb_rel,
+                                                    int ndim)
+{
+    __shared__ float    s_lo[4], s_hi[4];
+    __shared__ uint32_t s_bad[4];
+    __shared__ EbParams s_eb;
+    const uint32_t tid = threadIdx.x, wv = tid >> 6;
+    float    lo = __builtin_huge_valf(), hi = -__builtin_huge_valf();
+    uint32_t bad = 0;
+    for (uint32_t i = tid; i < nparts; i += kWG) {
+        const float4 p = part[i];
+        lo = fminf(lo, p.x);
+        hi = fmaxf(hi, p.y);
+        bad |= __float_as_uint(p.z);
+    }
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    bad = ballot64(bad != 0) != 0;
+    if ((tid & 63) == 0) {
+        s_lo[wv] = lo;
+        s_hi[wv] = hi;
+        s_bad[wv] = bad;
+    }
+    __syncthreads();
+    if (tid == 0)
+        s_eb = eb_params(fminf(fminf(s_lo[0], s_lo[1]), fminf(s_lo[2], s_lo[3])),
+                         fmaxf(fmaxf(s_hi[0], s_hi[1]), fmaxf(s_hi[2], s_hi[3])),
+                         s_bad[0] | s_bad[1] | s_bad[2] | s_bad[3], eb_rel, ndim);
+    __syncthreads();
+    return s_eb;
+}
+
+// Same result for nparts <= 256, reduced by wave 0 alone (4 loads per lane issued together)
+// while the other waves go straight to the barrier: the form used by every tile workgroup.
+__device__ __forceinline__ EbParams reduce_partials_256(const float4* __restrict__ part, uint32_t nparts,
+                                                        double eb_rel, int ndim)
+{
+    __shared__ EbParams s_eb;
+    const uint32_t tid = threadIdx.x;
+    if (tid < 64) {
+        float4 p[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t i = tid + 64 * j;
+            p[j] = i < nparts ? part[i] : make_float4(__builtin_huge_valf(), -__builtin_huge_valf(), 0.0f, 0.0f);
+        }
+        float lo = fminf(fminf(p[0].x, p[1].x), fminf(p[2].x, p[3].x));
+        float hi = fmaxf(fmaxf(p[0].y, p[1].y), fmaxf(p[2].y, p[3].y));
+        const uint32_t bad =
+            ballot64((__float_as_uint(p[0].z) | __float_as_uint(p[1].z) | __float_as_uint(p[2].z) |
+                      __float_as_uint(p[3].z)) != 0) != 0;
+        lo = wave_min(lo);
+        hi = wave_max(hi);
+        if (tid == 0) s_eb = eb_params(lo, hi, bad, eb_rel, ndim);
+    }
+    __syncthreads();
+    return s_eb;
+}
+
+// The result record as the range fixes it; nblocks / payload_bytes are filled in by the scan.
+__device__ __forceinline__ void write_header(fz_result* res, const EbParams& e, const Geom& g)
+{
+    res->status = e.status;
+    res->ndim = (uint32_t)g.ndim;
+    res->vmin = e.lo;
+    res->vmax = e.hi;
+    res->ebx2_r = e.r;
+    res->ebx2 = e.ebx2;
+    res->eb_abs = e.eb;
+    res->nblocks = 0;
+    res->payload_bytes = 0;
+    res->tail_len = g.mode == 0 ? (uint64_t)(g.len - (uint32_t)kTileCodes * g.T) : 0;
+}
+
+// ---------------------------------------------------------------------------- compress tiles
+
+// Which dual-quant path a tile takes (uniform over the workgroup): 2 = fast with aligned
+// float4 neighbour windows (1D, or X % 4 == 0), 3 = fast with 4-byte-aligned neighbour
+// windows and per-position block edges (X >= 8), 1 = unguarded generic, 0 = guarded generic.
+// geom_path ignores the value range; paths 1-3 also need every |s| < 2^23 (`safe`).
+template <int NDIM>
+__device__ __forceinline__ int geom_path(const Geom& g, uint32_t t0)
+{
+    if (t0 + (uint32_t)kTileCodes > g.xyz) return 0;
+    if constexpr (NDIM == 1) return 2;
+    const uint32_t halo = (NDIM == 2) ? g.X + 1 : g.XY + g.X + 1;
+    if (t0 < halo) return 0;
+    return (g.X & 3u) == 0 ? 2 : (g.X >= 8 ? 3 : 1);
+}
+
+// issue the raw loads of a fast-path tile (gp 2 or 3)
+template <int NDIM>
+__device__ __forceinline__ void raw_prefetch(const float* __restrict__ in, uint32_t p0, const Geom& g, int gp,
+                                             RawTile<NDIM>& R)
+{
+    if (gp == 2) raw_load<NDIM>(in, p0, g, R);
+    if constexpr (NDIM > 1)
+        if (gp == 3) raw_load_u<NDIM>(in, p0, g, R);
+}
+
+template <int NDIM>
+__device__ __forceinline__ int tile_path(const Geom& g, uint32_t t0, bool safe)
+{
+    return safe ? geom_path<NDIM>(g, t0) : 0;
+}
+
+template <int NDIM>
+__device__ __forceinline__ void dq8_tile(const float* __restrict__ in, uint32_t p0, const Geom& g, float r, int path,
+                                         uint32_t w[4], uint32_t& ovf)
+{
+    if (path == 2) {
+        if constexpr (NDIM == 1) dq8_fast1(in, p0, g, r, w, ovf);
+        else dq8_a4<NDIM>(in, p0, g, r, w, ovf);
+    } else if (path == 3) {
+        if constexpr (NDIM == 2) dq8_u(in, p0, g, r, w, ovf);
+        else if constexpr (NDIM == 3) dq8<3, false>(in, p0, g, r, w, ovf);
+        else dq8_fast1(in, p0, g, r, w, ovf);  // not reached: 1D never takes path 3
+    } else if (path == 1) {
+        dq8<NDIM, false>(in, p0, g, r, w, ovf);
+    } else {
+        dq8<NDIM, true>(in, p0, g, r, w, ovf);
+    }
+}
+
+// The unit after the last full tile: codes of the len - 2048 T positions no tile covers,
+// returned in `tail` in FIXED mode (the reference never encodes them, fz.cu:173), and the
+// bitflag words of the partial tile cleared (fz.cu:167 memsets them).
+template <int NDIM>
+__device__ __forceinline__ void tail_unit(const float* __restrict__ in, const Geom& g, float r,
+                                          uint32_t* __restrict__ bwords, uint16_t* __restrict__ tail,
+                                          unsigned long long* ovf_out)
+{
+    const uint32_t tid = threadIdx.x;
+    const uint32_t t0 = (uint32_t)kTileCodes * g.T;
+    const uint32_t tail_len = g.len - t0;
+    uint32_t       ovf = 0;
+    if (8 * tid < tail_len) {
+        uint32_t w[4];
+        dq8<NDIM>(in, t0 + 8 * tid, g, r, w, ovf);
+        if (g.mode == 0 && tail) {
+#pragma unroll
+            for (int i = 0; i < 8; i++)
+                if (8 * tid + i < tail_len) tail[8 * tid + i] = (uint16_t)(w[i >> 1] >> (16 * (i & 1)));
+        }
+    }
+    for (uint32_t i = 8 * g.T + tid; i < g.W; i += kWG) bwords[i] = 0;
+    ovf = wave_sum(ovf);
+    if ((tid & 63) == 0 && ovf) atomicAdd(ovf_out, (unsigned long long)ovf);
+}
+
+// One full tile: dual-quant -> 32x32 bit transpose -> block flags -> in-tile compaction.
+//   bwords[8 tile .. 8 tile + 8)  the tile's bitflag words (fz.cu:95-99, LSB first)
+//   slot[0, 16 count)              its nonzero 16-byte blocks in order (fz.cu:56-57 order)
+//   *count_out                     count
+// `R` holds the tile's raw loads when gp is 2 or 3 (raw_prefetch, issued by the caller before
+// its other latency-bound work).
+template <int NDIM>
+__device__ __forceinline__ void tile_encode(const float* __restrict__ in, const Geom& g, uint32_t tile, int gp,
+                                            const RawTile<NDIM>& R, float r, int level,
+                                            uint32_t* __restrict__ bwords, uint8_t* __restrict__ slot,
+                                            uint32_t* __restrict__ count_out, unsigned long long* ovf_out)
+{
+    __shared__ uint32_t sh[32 * kLdsPitch];
+    __shared__ uint32_t sh_wcnt[4], sh_wovf[4];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint32_t p0 = tile * (uint32_t)kTileCodes + 8 * tid;
+    uint32_t       ovf = 0;
+
+    // 1. dual-quant: words 4t..4t+3 of the tile
+    uint32_t w[4];
+    if constexpr (NDIM == 1) {
+        if (gp == 2 && level >= 3) raw_compute_pk<1, true>(R, p0, g, r, w);
+        else if (gp == 2 && level >= 1) raw_compute<1>(R, p0, g, r, w, ovf);
+        else dq8_tile<1>(in, p0, g, r, level >= 1 ? gp : 0, w, ovf);
+    } else if constexpr (NDIM == 2) {
+        if (gp == 2 && level >= 3) raw_compute_pk<2, true>(R, p0, g, r, w);
+        else if (gp == 3 && level >= 3) raw_compute_pk<2, false>(R, p0, g, r, w);
+        else if (gp == 2 && level >= 1) raw_compute<2>(R, p0, g, r, w, ovf);
+        else if (gp == 3 && level >= 1) raw_compute_u(R, p0, g, r, w, ovf);
+        else dq8_tile<2>(in, p0, g, r, level >= 1 ? gp : 0, w, ovf);
+    } else {
+        if (gp == 2 && level >= 3) raw_compute_pk<3, true>(R, p0, g, r, w);
+        else if (gp == 3 && level >= 3) raw_compute_pk<3, false>(R, p0, g, r, w);
+        else if (gp == 2 && level >= 2) raw_compute_q3<true>(R, p0, g, r, w, ovf);
+        else if (gp == 3 && level >= 2) raw_compute_q3<false>(R, p0, g, r, w, ovf);
+        else if (gp == 2 && level >= 1) raw_compute<3>(R, p0, g, r, w, ovf);
+        else dq8_tile<3>(in, p0, g, r, level >= 1 ? (gp == 3 ? 1 : gp) : 0, w, ovf);
+    }
+
+    // 2. bit transpose of group k = t/8 (skipped for an all-zero wave); register q then holds
+    //    o[32(4L+q) + k]
+    const uint32_t L = tid & 7, k = tid >> 3;
+    if (ballot64((w[0] | w[1] | w[2] | w[3]) != 0) != 0) transpose4(w, L);
+#pragma unroll
+    for (int q = 0; q < 4; q++) sh[kLdsPitch * (4 * L + q) + k] = w[q];
+    if (ballot64(ovf != 0) != 0) ovf = wave_sum(ovf);  // overflows are rare: skip the sum otherwise
+    if (lane == 0) sh_wovf[wv] = ovf;
+    __syncthreads();
+
+    // 3. thread i owns 16-byte block i = o[4i .. 4i+3]; flag = block != 0 (fz.cu:84-91)
+    const uint32_t c = tid >> 3, k0 = 4 * (tid & 7);
+    const uint4    blk = make_uint4(sh[kLdsPitch * c + k0], sh[kLdsPitch * c + k0 + 1], sh[kLdsPitch * c + k0 + 2],
+                                    sh[kLdsPitch * c + k0 + 3]);
+    const bool     f = (blk.x | blk.y | blk.z | blk.w) != 0;
+    const uint64_t mask = ballot64(f);
+    // bitflag word 8*tile + y holds flags 32y..32y+31: wave wv covers words 2wv (lanes 0-31)
+    // and 2wv+1 (lanes 32-63)
+    if (lane < 2) bwords[8 * tile + 2 * wv + lane] = lane ? (uint32_t)(mask >> 32) : (uint32_t)mask;
+    const uint32_t rank = rank_below(mask);
+    if (lane == 0) sh_wcnt[wv] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    const uint32_t c0 = sh_wcnt[0], c1 = sh_wcnt[1], c2 = sh_wcnt[2], c3 = sh_wcnt[3];
+    const uint32_t woff = (wv > 0 ? c0 : 0u) + (wv > 1 ? c1 : 0u) + (wv > 2 ? c2 : 0u);
+    if (tid == 0) {
+        *count_out = c0 + c1 + c2 + c3;
+        const uint32_t o = sh_wovf[0] + sh_wovf[1] + sh_wovf[2] + sh_wovf[3];
+        if (o) atomicAdd(ovf_out, (unsigned long long)o);
+    }
+    // 4. in-tile compaction into the tile's staging slot
+    if (f) *reinterpret_cast<uint4*>(slot + 16 * (woff + rank)) = blk;
+}
+
+// ---------------------------------------------------------------------------- decompress tiles
+
+// Scatter a tile's nonzero blocks back (block i at payload + 4W + 16 (excl + rank)),
+// unshuffle, store its 1024 code words to out_words.
+__device__ __forceinline__ void decode_tile(const uint8_t* __restrict__ payload, uint32_t W, uint32_t tile,
+                                            uint32_t excl, uint32_t* __restrict__ out_words)
+{
+    __shared__ uint32_t sh[32 * kLdsPitch];
+    __shared__ uint32_t sh_w[4];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint32_t* bwords = reinterpret_cast<const uint32_t*>(payload);
+    const uint32_t  word = bwords[8 * tile + (tid >> 5)];
+    const bool      f = (word >> (tid & 31)) & 1u;
+    const uint64_t  mask = ballot64(f);
+    if (lane == 0) sh_w[wv] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    const uint32_t woff = (wv > 0 ? sh_w[0] : 0u) + (wv > 1 ? sh_w[1] : 0u) + (wv > 2 ? sh_w[2] : 0u);
+    uint4          blk = make_uint4(0, 0, 0, 0);
+    if (f) blk = ld16(payload + 4ull * W + 16ull * (excl + woff + rank_below(mask)));
+    const uint32_t c = tid >> 3, k0 = 4 * (tid & 7);
+    sh[kLdsPitch * c + k0] = blk.x;
+    sh[kLdsPitch * c + k0 + 1] = blk.y;
+    sh[kLdsPitch * c + k0 + 2] = blk.z;
+    sh[kLdsPitch * c + k0 + 3] = blk.w;
+    __syncthreads();
+    const uint32_t L = tid & 7, k = tid >> 3;
+    uint32_t       w[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) w[q] = sh[kLdsPitch * (4 * L + q) + k];
+    transpose4(w, L);
+    *reinterpret_cast<uint4*>(out_words + 4 * tid) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// Codes of the positions after the last full tile: the trailer, or 0 without one.
+__device__ __forceinline__ void decode_tail(const uint16_t* __restrict__ tail, const Geom& g,
+                                            uint16_t* __restrict__ codes)
+{
+    const uint32_t t0 = kTileCodes * g.T;
+    for (uint32_t i = threadIdx.x; t0 + i < g.len; i += kWG) codes[t0 + i] = tail ? tail[i] : (uint16_t)0;
+}
+
+__device__ __forceinline__ int32_t decode(uint32_t c)
+{
+    const int32_t m = (int32_t)(c & 0x7fffu);
+    return (c & 0x8000u) ? -m : m;
+}
+
+// 1D: workgroup wg covers codes [4096 wg, 4096 wg + 4096): 16 consecutive codes per thread,
+// 16 threads per 256-block (block-local inclusive prefix sum), x = (float)s * ebx2.
+__device__ __forceinline__ void recon_1d(const uint16_t* __restrict__ codes, float* __restrict__ out, const Geom& g,
+                                         float ebx2, uint32_t wg)
+{
+    const uint32_t p0 = (wg * kWG + threadIdx.x) * 16;
+    int32_t        s[16];
+    if (p0 + 16 <= g.xyz) {
+        const uint4 a = *reinterpret_cast<const uint4*>(codes + p0);
+        const uint4 b = *reinterpret_cast<const uint4*>(codes + p0 + 8);
+        const uint32_t wd[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            s[2 * j] = decode(wd[j] & 0xffffu);
+            s[2 * j + 1] = decode(wd[j] >> 16);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 16; j++) s[j] = (p0 + j < g.xyz) ? decode(codes[p0 + j]) : 0;
+    }
+#pragma unroll
+    for (int j = 1; j < 16; j++) s[j] += s[j - 1];
+    // exclusive prefix of the 16 thread totals of this 256-block (lanes 16-aligned)
+    int32_t t = s[15];
+#pragma unroll
+    for (int d = 1; d < 16; d <<= 1) {
+        const int32_t o = __shfl_up(t, d, 16);
+        if ((threadIdx.x & 15) >= (uint32_t)d) t += o;
+    }
+    const int32_t ex = t - s[15];
+    if (p0 + 16 <= g.xyz) {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            *reinterpret_cast<float4*>(out + p0 + 4 * j) =
+                make_float4((float)(ex + s[4 * j]) * ebx2, (float)(ex + s[4 * j + 1]) * ebx2,
+                            (float)(ex + s[4 * j + 2]) * ebx2, (float)(ex + s[4 * j + 3]) * ebx2);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+            if (p0 + j < g.xyz) out[p0 + j] = (float)(ex + s[j]) * ebx2;
+    }
+}
+
+// 2D: workgroup (bx, by) = 16 rows x 256 columns (16 Lorenzo blocks); thread = one column of a
+// block.  Prefix over y in registers, then over x across the 16 lanes of the block.
+__device__ __forceinline__ void recon_2d(const uint16_t* __restrict__ codes, float* __restrict__ out, const Geom& g,
+                                         float ebx2, uint32_t bx, uint32_t by)
+{
+    const uint32_t x = bx * kWG + threadIdx.x, y0 = by * 16;
+    const bool     xin = x < g.X;
+    int32_t        s[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++)
+        s[r] = (xin && y0 + r < g.Y) ? decode(codes[(size_t)(y0 + r) * g.X + x]) : 0;
+#pragma unroll
+    for (int r = 1; r < 16; r++) s[r] += s[r - 1];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+#pragma unroll
+        for (int d = 1; d < 16; d <<= 1) {
+            const int32_t o = __shfl_up(s[r], d, 16);
+            if ((threadIdx.x & 15) >= (uint32_t)d) s[r] += o;
+        }
+    }
+    if (xin) {
+#pragma unroll
+        for (int r = 0; r < 16; r++)
+            if (y0 + r < g.Y) out[(size_t)(y0 + r) * g.X + x] = (float)s[r] * ebx2;
+    }
+}
+
+// 3D: workgroup (bx, by, bz) = 32 (x) x 8 (y) x 8 (z) like the reference's (32,1,8) block.
+// Thread (tx, ty) holds the z column of (x, y): prefix over z in registers, over x across
+// 8-lane groups; then an LDS transpose puts the y column in registers for the last prefix.
+__device__ __forceinline__ void recon_3d(const uint16_t* __restrict__ codes, float* __restrict__ out, const Geom& g,
+                                         float ebx2, uint32_t bx, uint32_t by, uint32_t bz)
+{
+    __shared__ int32_t sh[8][8][33];
+    const uint32_t tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const uint32_t x = bx * 32 + tx, y0 = by * 8, z0 = bz * 8;
+    const bool     xin = x < g.X;
+    int32_t        s[8];
+    {
+        const uint32_t y = y0 + ty;
+#pragma unroll
+        for (int z = 0; z < 8; z++)
+            s[z] = (xin && y < g.Y && z0 + z < g.Z) ? decode(codes[(size_t)(z0 + z) * g.XY + (size_t)y * g.X + x]) : 0;
+    }
+#pragma unroll
+    for (int z = 1; z < 8; z++) s[z] += s[z - 1];
+#pragma unroll
+    for (int z = 0; z < 8; z++) {
+#pragma unroll
+        for (int d = 1; d < 8; d <<= 1) {
+            const int32_t o = __shfl_up(s[z], d, 8);
+            if ((tx & 7) >= (uint32_t)d) s[z] += o;
+        }
+        sh[z][ty][tx] = s[z];
+    }
+    __syncthreads();
+    const uint32_t tz = ty;
+    int32_t        v[8];
+#pragma unroll
+    for (int y = 0; y < 8; y++) v[y] = sh[tz][y][tx];
+#pragma unroll
+    for (int y = 1; y < 8; y++) v[y] += v[y - 1];
+    if (xin && z0 + tz < g.Z) {
+#pragma unroll
+        for (int y = 0; y < 8; y++)
+            if (y0 + y < g.Y) out[(size_t)(z0 + tz) * g.XY + (size_t)(y0 + y) * g.X + x] = (float)v[y] * ebx2;
+    }
+}
+
+}  // namespace fz

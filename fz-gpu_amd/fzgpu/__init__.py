@@ -10,6 +10,7 @@ Mirrors of the reference's surface:
     bitshuffle_bitflag(...)             -> fz.cu:61 bitshuffleAndBitflag
     exclusive_sum(...)                  -> cub::DeviceScan::ExclusiveSum (fz.cu:188-192)
     encode_blocks(...)                  -> fz.cu:49 encodeDebug
+    BatchPlan                           -> runSzs over many fields in one launch sequence
 """
 import ctypes
 import os
@@ -58,7 +59,30 @@ EXPORTS = ["fz_version", "fz_status_string", "fz_last_error", "fz_plan_create", 
            "fz_workspace_init", "fz_compress_f32", "fz_compress_plain_f32", "fz_decompress_f32",
            "fz_compress_f32_sync", "fz_lorenzo_construct_f32", "fz_lorenzo_reconstruct_f32",
            "fz_bitshuffle_bitflag", "fz_bitunshuffle", "fz_exclusive_sum_u32", "fz_encode_blocks",
-           "fz_plan_profile", "fz_plan_kernel_times"]
+           "fz_plan_profile", "fz_plan_kernel_times",
+           "fz_batch_plan_create", "fz_batch_plan_destroy", "fz_batch_plan_sizes", "fz_batch_field_sizes",
+           "fz_batch_get_tuning", "fz_batch_set_tuning", "fz_batch_workspace_init", "fz_batch_compress_f32",
+           "fz_batch_decompress_f32", "fz_batch_autotune", "fz_batch_profile", "fz_batch_kernel_times",
+           "fz_batch_compress_graph", "fz_batch_decompress_graph", "fz_graph_launch", "fz_graph_destroy"]
+
+
+class FieldShape(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_uint32), ("y", ctypes.c_uint32), ("z", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("len", ctypes.c_uint64)]
+
+
+class BatchSizes(ctypes.Structure):
+    _fields_ = [("workspace_bytes", ctypes.c_uint64), ("total_len", ctypes.c_uint64), ("units", ctypes.c_uint64),
+                ("nfields", ctypes.c_uint32), ("range_parts", ctypes.c_uint32)]
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("range_chunk", ctypes.c_uint32), ("xcd_order", ctypes.c_uint32 * 3),
+                ("gather_waves", ctypes.c_uint32), ("groups", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+
+    def as_dict(self):
+        return {"range_chunk": self.range_chunk, "xcd_order": list(self.xcd_order),
+                "gather_waves": self.gather_waves, "groups": self.groups}
 
 _lib = None
 
@@ -99,6 +123,22 @@ def lib():
     L.fz_encode_blocks.argtypes = [vp, vp, vp, u64, vp]
     L.fz_plan_profile.argtypes = [vp, i32]
     L.fz_plan_kernel_times.argtypes = [vp, ctypes.POINTER(f32), i32, ctypes.POINTER(i32)]
+    L.fz_batch_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(FieldShape), u32, i32]
+    L.fz_batch_plan_destroy.argtypes = [vp]
+    L.fz_batch_plan_sizes.argtypes = [vp, ctypes.POINTER(BatchSizes)]
+    L.fz_batch_field_sizes.argtypes = [vp, u32, ctypes.POINTER(Sizes)]
+    L.fz_batch_get_tuning.argtypes = [vp, ctypes.POINTER(BatchTuning)]
+    L.fz_batch_set_tuning.argtypes = [vp, ctypes.POINTER(BatchTuning)]
+    L.fz_batch_workspace_init.argtypes = [vp, vp, ctypes.c_size_t, vp]
+    L.fz_batch_compress_f32.argtypes = [vp, vp, vp, vp, ctypes.c_size_t, vp, vp, vp, vp]
+    L.fz_batch_decompress_f32.argtypes = [vp, vp, vp, vp, vp, ctypes.c_size_t, vp, vp]
+    L.fz_batch_autotune.argtypes = [vp, vp, vp, vp, ctypes.c_size_t, vp, vp, vp, i32, ctypes.POINTER(BatchTuning), vp]
+    L.fz_batch_profile.argtypes = [vp, i32]
+    L.fz_batch_kernel_times.argtypes = [vp, ctypes.POINTER(f32), i32, ctypes.POINTER(i32)]
+    L.fz_batch_compress_graph.argtypes = [vp, vp, vp, vp, ctypes.c_size_t, vp, vp, vp, ctypes.POINTER(vp)]
+    L.fz_batch_decompress_graph.argtypes = [vp, vp, vp, vp, vp, ctypes.c_size_t, vp, ctypes.POINTER(vp)]
+    L.fz_graph_launch.argtypes = [vp, vp]
+    L.fz_graph_destroy.argtypes = [vp]
     for name in EXPORTS:
         if name not in ("fz_version", "fz_status_string", "fz_last_error"):
             getattr(L, name).restype = i32
@@ -217,6 +257,203 @@ def read_result(res_tensor):
     """Copy the 64-byte device result record to the host (synchronises)."""
     raw = bytes(res_tensor[:64].cpu().numpy().tobytes())
     return Result.from_buffer_copy(raw).as_dict()
+
+
+def read_results(res_tensor, n):
+    """n consecutive 64-byte device result records -> list of dicts (synchronises)."""
+    raw = bytes(res_tensor[: 64 * n].cpu().numpy().tobytes())
+    return [Result.from_buffer_copy(raw[64 * i: 64 * i + 64]).as_dict() for i in range(n)]
+
+
+def _ptr_array(tensors):
+    arr = (ctypes.c_void_p * len(tensors))()
+    for i, t in enumerate(tensors):
+        arr[i] = None if t is None else t.data_ptr()
+    return arr
+
+
+def _al(n, a=256):
+    return (int(n) + a - 1) // a * a
+
+
+class Graph:
+    """An instantiated hipGraph of one batched call on fixed buffers."""
+
+    def __init__(self, handle, keep):
+        self.handle = handle
+        self._keep = keep  # the buffers the graph's kernels address
+
+    def launch(self, stream=None):
+        _check(lib().fz_graph_launch(self.handle, _stream(stream)), "fz_graph_launch")
+
+    def close(self):
+        if self.handle is not None and _lib is not None:
+            _lib.fz_graph_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BatchOutputs:
+    """Device outputs of a batched compress: one payload arena and one tail arena with a
+    256-byte-aligned slice per field, and the result records."""
+
+    def __init__(self, plan):
+        import torch
+        self.payload_off, self.tail_off = [], []
+        po = to = 0
+        for i in range(plan.nf):
+            fs = plan.field_sizes(i)
+            self.payload_off.append((po, fs.payload_max_bytes))
+            self.tail_off.append((to, fs.tail_max_codes))
+            po += _al(fs.payload_max_bytes + 16)
+            to += _al(2 * fs.tail_max_codes + 2)
+        self.payload_arena = torch.empty(max(po, 256), dtype=torch.uint8, device=plan.device)
+        self.tail_arena = torch.empty(max(to, 256), dtype=torch.uint8, device=plan.device)
+        self.payloads = [self.payload_arena[o: o + n + 16] for o, n in self.payload_off]
+        self.tails = [self.tail_arena[o: o + 2 * n + 2].view(torch.int16) for o, n in self.tail_off]
+        self.res = torch.zeros(64 * plan.nf, dtype=torch.uint8, device=plan.device)
+
+
+class BatchPlan:
+    """Many fields of uneven shapes (list of (x, y, z) or (x, y, z, len)) compressed by one
+    launch sequence; each field's bytes equal those of Plan on that field alone."""
+
+    def __init__(self, shapes, mode=MODE_FIXED, device="cuda", workspace=True):
+        self.shapes = [tuple(int(v) for v in s) for s in shapes]
+        self.nf = len(self.shapes)
+        self.mode = mode
+        self.device = device
+        arr = (FieldShape * max(self.nf, 1))()
+        for i, s in enumerate(self.shapes):
+            arr[i].x, arr[i].y, arr[i].z = s[0], s[1], s[2]
+            arr[i].len = s[3] if len(s) > 3 else 0
+        h = ctypes.c_void_p()
+        _check(lib().fz_batch_plan_create(ctypes.byref(h), arr, self.nf, mode), "fz_batch_plan_create")
+        self.handle = h
+        bs = BatchSizes()
+        _check(lib().fz_batch_plan_sizes(h, ctypes.byref(bs)), "fz_batch_plan_sizes")
+        self.sizes = bs
+        self.ws = None
+        if workspace:
+            self.ws = _empty(bs.workspace_bytes, device)
+            _check(lib().fz_batch_workspace_init(h, _ptr(self.ws), self.ws.numel(), _stream()),
+                   "fz_batch_workspace_init")
+
+    def close(self):
+        if self.handle is not None and _lib is not None:
+            _lib.fz_batch_plan_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def field_sizes(self, i):
+        s = Sizes()
+        _check(lib().fz_batch_field_sizes(self.handle, i, ctypes.byref(s)), "fz_batch_field_sizes")
+        return s
+
+    def lens(self):
+        return [self.field_sizes(i).len for i in range(self.nf)]
+
+    @property
+    def tuning(self):
+        t = BatchTuning()
+        _check(lib().fz_batch_get_tuning(self.handle, ctypes.byref(t)), "fz_batch_get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, range_chunk=None, xcd_order=None, gather_waves=None, groups=None):
+        t = BatchTuning()
+        _check(lib().fz_batch_get_tuning(self.handle, ctypes.byref(t)), "fz_batch_get_tuning")
+        if range_chunk is not None:
+            t.range_chunk = range_chunk
+        if xcd_order is not None:
+            for i in range(3):
+                t.xcd_order[i] = xcd_order[i]
+        if gather_waves is not None:
+            t.gather_waves = gather_waves
+        if groups is not None:
+            t.groups = groups
+        _check(lib().fz_batch_set_tuning(self.handle, ctypes.byref(t)), "fz_batch_set_tuning")
+
+    def alloc_outputs(self):
+        return BatchOutputs(self)
+
+    def _compress_args(self, inputs, ebs, out):
+        assert len(inputs) == self.nf and len(ebs) == self.nf
+        ins = _ptr_array(inputs)
+        eb = (ctypes.c_double * self.nf)(*[float(e) for e in ebs])
+        pays = _ptr_array(out.payloads)
+        tails = _ptr_array(out.tails)
+        return ins, eb, pays, tails
+
+    def compress_async(self, inputs, ebs, out, stream=None):
+        ins, eb, pays, tails = self._compress_args(inputs, ebs, out)
+        _check(lib().fz_batch_compress_f32(self.handle, ins, eb, _ptr(self.ws), self.ws.numel(), pays, tails,
+                                           _ptr(out.res), _stream(stream)), "fz_batch_compress_f32")
+
+    def compress(self, inputs, ebs, stream=None):
+        """Returns (outputs, list of result dicts); payload i = outputs.payloads[i][:payload_bytes]."""
+        out = self.alloc_outputs()
+        self.compress_async(inputs, ebs, out, stream)
+        return out, read_results(out.res, self.nf)
+
+    def compress_graph(self, inputs, ebs, out):
+        ins, eb, pays, tails = self._compress_args(inputs, ebs, out)
+        h = ctypes.c_void_p()
+        _check(lib().fz_batch_compress_graph(self.handle, ins, eb, _ptr(self.ws), self.ws.numel(), pays, tails,
+                                             _ptr(out.res), ctypes.byref(h)), "fz_batch_compress_graph")
+        return Graph(h, (list(inputs), out, self.ws))
+
+    def _decompress_args(self, payloads, tails, ebx2s, outs):
+        assert len(payloads) == self.nf and len(ebx2s) == self.nf and len(outs) == self.nf
+        pays = _ptr_array(payloads)
+        tl = _ptr_array([t if (t is not None and t.numel() > 0) else None for t in tails]) if tails else None
+        e = (ctypes.c_float * self.nf)(*[float(v) for v in ebx2s])
+        return pays, tl, e, _ptr_array(outs)
+
+    def decompress_async(self, payloads, tails, ebx2s, outs, stream=None):
+        pays, tl, e, o = self._decompress_args(payloads, tails, ebx2s, outs)
+        _check(lib().fz_batch_decompress_f32(self.handle, pays, tl, e, _ptr(self.ws), self.ws.numel(), o,
+                                             _stream(stream)), "fz_batch_decompress_f32")
+
+    def decompress(self, payloads, tails, ebx2s, stream=None):
+        import torch
+        outs = [torch.empty(max(s[0] * s[1] * s[2], 4), dtype=torch.float32, device=self.device)
+                for s in self.shapes]
+        self.decompress_async(payloads, tails, ebx2s, outs, stream)
+        return [o[: s[0] * s[1] * s[2]] for o, s in zip(outs, self.shapes)]
+
+    def decompress_graph(self, payloads, tails, ebx2s, outs):
+        pays, tl, e, o = self._decompress_args(payloads, tails, ebx2s, outs)
+        h = ctypes.c_void_p()
+        _check(lib().fz_batch_decompress_graph(self.handle, pays, tl, e, _ptr(self.ws), self.ws.numel(), o,
+                                               ctypes.byref(h)), "fz_batch_decompress_graph")
+        return Graph(h, (list(payloads), list(tails or []), list(outs), self.ws))
+
+    def autotune(self, inputs, ebs, out, reps=3, stream=None):
+        ins, eb, pays, tails = self._compress_args(inputs, ebs, out)
+        t = BatchTuning()
+        _check(lib().fz_batch_autotune(self.handle, ins, eb, _ptr(self.ws), self.ws.numel(), pays, tails,
+                                       _ptr(out.res), int(reps), ctypes.byref(t), _stream(stream)),
+               "fz_batch_autotune")
+        return t.as_dict()
+
+    def profile(self, enable=True):
+        _check(lib().fz_batch_profile(self.handle, 1 if enable else 0), "fz_batch_profile")
+
+    def kernel_times(self):
+        buf = (ctypes.c_float * 8)()
+        cnt = ctypes.c_int(0)
+        _check(lib().fz_batch_kernel_times(self.handle, buf, 8, ctypes.byref(cnt)), "fz_batch_kernel_times")
+        return [buf[i] for i in range(cnt.value)]
 
 
 # ---------------------------------------------------------------- per-stage mirrors

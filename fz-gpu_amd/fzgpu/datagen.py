@@ -150,3 +150,141 @@ def generate(kind, dims, seed, **kw):
 def config_field(name):
     dims, kind, seed, eb = CONFIGS[name]
     return generate(kind, dims, seed), dims, eb
+
+
+# ---------------------------------------------------------------- the uneven batch
+
+BATCH_EBS = (1e-2, 1e-3, 1e-4)
+BATCH_KINDS = ("waves", "peaks", "blobs")
+
+
+def batch_fields(nfields=256, lo=2 ** 16, hi=2 ** 26, split=(77, 102, 77), seed=2000):
+    """Shapes, content generator, seed and eb of every field of the uneven batch.
+
+    Sizes are log-uniform in [lo, hi] elements; the 1D / 2D / 3D split (scaled to nfields)
+    is a seeded permutation; aspect ratios are log-uniform in [1/8, 8] per axis pair; eb is
+    uniform over BATCH_EBS; the generator cycles waves / peaks / blobs; field i uses seed
+    seed + i.  Returns a list of dicts (dims, kind, seed, eb)."""
+    tot = sum(split)
+    counts = [int(round(nfields * s / tot)) for s in split]
+    counts[1] = nfields - counts[0] - counts[2]
+    nds = np.array([1] * counts[0] + [2] * counts[1] + [3] * counts[2])
+    np.random.default_rng(seed).shuffle(nds)
+    lr = (math.log(1 / 8), math.log(8))
+    out = []
+    for i in range(nfields):
+        r = np.random.default_rng(seed + i)
+        n = int(round(math.exp(r.uniform(math.log(lo), math.log(hi)))))
+        nd = int(nds[i])
+        if nd == 1:
+            dims = (n, 1, 1)
+        elif nd == 2:
+            a = math.exp(r.uniform(*lr))
+            y = max(2, int(round(math.sqrt(n / a))))
+            dims = (max(2, int(round(n / y))), y, 1)
+        else:
+            a1, a2 = math.exp(r.uniform(*lr)), math.exp(r.uniform(*lr))
+            z = max(2, int(round((n / (a1 * a2 * a2)) ** (1 / 3))))
+            y = max(2, int(round(a2 * z)))
+            dims = (max(2, int(round(n / (y * z)))), y, z)
+        out.append(dict(dims=dims, kind=BATCH_KINDS[i % 3], seed=seed + i, eb=BATCH_EBS[int(r.integers(0, 3))]))
+    return out
+
+
+# ---------------------------------------------------------------- device-side generators
+#
+# Same families as above, built on the GPU with torch so that multi-GB batches are made in
+# seconds: a smooth field is a sum of plane waves evaluated from the flat index in chunks.
+
+def _smooth_gpu(shp, rng, nmodes, lam_lo, lam_hi, device, chunk=1 << 24):
+    import torch
+    nd = len(shp)
+    n = int(np.prod(shp))
+    strides = [int(np.prod(shp[a + 1:])) for a in range(nd)]
+    modes = []
+    for _ in range(nmodes):
+        lam = math.exp(rng.uniform(math.log(lam_lo), math.log(lam_hi)))
+        v = rng.normal(size=nd)
+        v /= np.linalg.norm(v) + 1e-300
+        modes.append((lam, [float(2 * math.pi / lam * c) for c in v], float(rng.uniform(0, 2 * math.pi))))
+    out = torch.empty(n, dtype=torch.float32, device=device)
+    for c0 in range(0, n, chunk):
+        idx = torch.arange(c0, min(n, c0 + chunk), device=device, dtype=torch.int64)
+        coords = [((idx // strides[a]) % shp[a]).to(torch.float32) for a in range(nd)]
+        acc = torch.zeros(idx.numel(), dtype=torch.float32, device=device)
+        for lam, k, phi in modes:
+            ph = torch.full_like(acc, phi)
+            for a in range(nd):
+                ph.add_(coords[a], alpha=k[a])
+            acc.add_(torch.cos(ph), alpha=lam)
+        out[c0:c0 + idx.numel()] = acc
+    return out
+
+
+def _quantile_gpu(t, q, gen, sample=1 << 20):
+    import torch
+    if t.numel() > sample:
+        t = t[torch.randint(0, t.numel(), (sample,), device=t.device, generator=gen)]
+    k = min(t.numel(), max(1, int(q * t.numel())))
+    return torch.kthvalue(t.float().cpu(), k).values.item()
+
+
+def _gen(device, seed):
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    return g
+
+
+def waves_gpu(dims, seed, device="cuda", nmodes=32, zero_frac=0.15, noise=1e-3):
+    import torch
+    rng, gen = np.random.default_rng(seed), _gen(device, seed)
+    shp = _squeeze_shape(dims)
+    f = _smooth_gpu(shp, rng, nmodes, 20.0, 900.0, device)
+    lo, hi = f.min(), f.max()
+    f.sub_(lo).div_(torch.clamp(hi - lo, min=1e-30))
+    f.add_(torch.randn(f.numel(), device=device, generator=gen), alpha=noise).clamp_(0.0, 1.0)
+    m = _smooth_gpu(shp, rng, 8, 300.0, max(300.0, 2.0 * max(shp)), device)
+    f[m < _quantile_gpu(m, zero_frac, gen)] = 0.0
+    return f
+
+
+def peaks_gpu(dims, seed, device="cuda", zero_frac=0.97, bg_rows=0.05, bg_seg=0.40, bg_sigma=0.5):
+    import torch
+    rng, gen = np.random.default_rng(seed), _gen(device, seed)
+    shp = _squeeze_shape(dims)
+    s = _smooth_gpu(shp, rng, 32, 4.0, 40.0, device)
+    thr = _quantile_gpu(s, zero_frac, gen)
+    u = _smooth_gpu(shp, rng, 4, 200.0, max(200.0, 2.0 * max(shp)), device)
+    u.sub_(u.min()).div_(torch.clamp(u.max() - u.min(), min=1e-30))
+    h = torch.pow(10.0, 1.0 + 3.0 * u)
+    f = torch.where(s > thr, (s - thr) / max(float(s.max()) - thr, 1e-30) * h, torch.zeros_like(s))
+    rowlen = shp[-1]
+    nrows = f.numel() // rowlen
+    nbg = int(round(bg_rows * nrows)) if nrows > 1 else 0
+    if nbg:
+        seg = max(1, int(bg_seg * rowlen))
+        rows = torch.randperm(nrows, device=device, generator=gen)[:nbg]
+        start = torch.randint(0, rowlen - seg + 1, (nbg,), device=device, generator=gen)
+        idx = (rows * rowlen + start).unsqueeze(1) + torch.arange(seg, device=device).unsqueeze(0)
+        f.view(-1).index_add_(0, idx.reshape(-1), bg_sigma * torch.randn(idx.numel(), device=device, generator=gen))
+    return f
+
+
+def blobs_gpu(dims, seed, device="cuda", vmax=1e-3, zero_frac=0.70):
+    import torch
+    rng, gen = np.random.default_rng(seed), _gen(device, seed)
+    shp = _squeeze_shape(dims)
+    f = _smooth_gpu(shp, rng, 16, 20.0, 160.0, device)
+    f.sub_(f.min())
+    f.mul_(vmax / max(float(f.max()), 1e-30))
+    f[f < _quantile_gpu(f, zero_frac, gen)] = 0.0
+    return f
+
+
+GPU_GENERATORS = {"waves": waves_gpu, "peaks": peaks_gpu, "blobs": blobs_gpu}
+
+
+def generate_gpu(kind, dims, seed, device="cuda"):
+    """float32 cuda tensor of x*y*z values (flat, x fastest)."""
+    return GPU_GENERATORS[kind](dims, seed, device).contiguous()

@@ -164,6 +164,92 @@ fz_status fz_exclusive_sum_u32(void* d_temp, size_t* temp_bytes, const uint32_t*
 fz_status fz_encode_blocks(const uint8_t* d_in, uint8_t* d_out, const uint32_t* d_presum, uint64_t num_blocks,
                            hipStream_t stream);
 
+/* ---- batched form ----
+ * No counterpart in the reference, which compresses one field per process (fz.cu:226-237).
+ * It runs runSzs (fz.cu:113-209) over many fields of uneven shapes and dimensionality in
+ * one launch sequence: each field's payload, tail codes and result equal those of
+ * fz_compress_f32 / fz_compress_plain_f32 on that field alone, byte for byte.
+ * Host arrays (d_in, eb_rel, d_payload, d_tail, ebx2, d_out) hold one entry per field, in
+ * the order of the shapes given to fz_batch_plan_create; d_results is a device array of
+ * nfields records in that order.  Per-field buffer sizes are those of fz_batch_field_sizes. */
+typedef struct fz_batch_plan_s* fz_batch_plan;
+
+typedef struct fz_field_shape {
+    uint32_t x, y, z; /* x fastest, as fz_plan_create */
+    uint32_t reserved;
+    uint64_t len;     /* floats in the field, >= x*y*z; 0 means x*y*z */
+} fz_field_shape;
+
+typedef struct fz_batch_sizes {
+    uint64_t workspace_bytes;
+    uint64_t total_len;  /* floats over all fields */
+    uint64_t units;      /* sum over fields of (full tiles + 1) */
+    uint32_t nfields;
+    uint32_t range_parts;
+} fz_batch_sizes;
+
+/* Launch knobs; any setting gives the same bytes. */
+typedef struct fz_batch_tuning {
+    uint32_t range_chunk;  /* floats per range partial: 16384, 65536 or 262144 */
+    uint32_t xcd_order[3]; /* tile order of 1D, 2D, 3D fields: 1 XCD-sliced, 0 linear */
+    uint32_t gather_waves; /* units per gather workgroup: 1, 2, 4, 8 or 16 */
+    uint32_t groups;       /* field groups pipelined over three streams: 1, 2, 4, 8 or 16 */
+    uint32_t reserved[3];
+} fz_batch_tuning;
+
+/* At most 4096 fields; every field as fz_plan_create accepts it. */
+fz_status fz_batch_plan_create(fz_batch_plan* plan, const fz_field_shape* shapes, uint32_t nfields, int mode);
+fz_status fz_batch_plan_destroy(fz_batch_plan plan);
+fz_status fz_batch_plan_sizes(const fz_batch_plan plan, fz_batch_sizes* sizes);
+fz_status fz_batch_field_sizes(const fz_batch_plan plan, uint32_t field, fz_sizes* sizes);
+fz_status fz_batch_get_tuning(const fz_batch_plan plan, fz_batch_tuning* tuning);
+fz_status fz_batch_set_tuning(fz_batch_plan plan, const fz_batch_tuning* tuning);
+
+/* Writes the plan's field tables into a freshly allocated workspace (one host-to-device
+ * copy; waits for the stream).  Call once per workspace, outside any graph capture. */
+fz_status fz_batch_workspace_init(const fz_batch_plan plan, void* d_ws, size_t ws_bytes, hipStream_t stream);
+
+/* Compress every field.  Launches kernels only (the per-field pointers travel as kernel
+ * arguments), so the call can be captured into a hipGraph.  One workspace serves one
+ * stream at a time.  With tuning.groups > 1 the fields are split into that many groups and
+ * the call forks onto two streams owned by the plan (joined back into `stream` before it
+ * returns): the value range of one group runs beside the tiles of the previous one and the
+ * placement of a group's blocks beside the tiles of the next.  Calls on one plan are then
+ * issued from one host thread at a time. */
+fz_status fz_batch_compress_f32(const fz_batch_plan plan, const float* const* d_in, const double* eb_rel,
+                                void* d_ws, size_t ws_bytes, uint8_t* const* d_payload, uint16_t* const* d_tail,
+                                fz_result* d_results, hipStream_t stream);
+
+/* Decompress every field (ebx2[i] = fz_result.ebx2 of field i; d_tail entries may be NULL). */
+fz_status fz_batch_decompress_f32(const fz_batch_plan plan, const uint8_t* const* d_payload,
+                                  const uint16_t* const* d_tail, const float* ebx2, void* d_ws, size_t ws_bytes,
+                                  float* const* d_out, hipStream_t stream);
+
+/* Times the tuning candidates on these buffers (reps runs each, per-kernel events), keeps
+ * the fastest in the plan and returns it in *chosen (may be NULL).  Waits for the stream. */
+fz_status fz_batch_autotune(fz_batch_plan plan, const float* const* d_in, const double* eb_rel, void* d_ws,
+                            size_t ws_bytes, uint8_t* const* d_payload, uint16_t* const* d_tail, fz_result* d_results,
+                            int reps, fz_batch_tuning* chosen, hipStream_t stream);
+
+/* Per-kernel timing of the most recent batched call, as fz_plan_profile / _kernel_times.
+ * Compress slots: range, range final, tiles 1D, 2D, 3D, scan, gather.  Decompress slots:
+ * flag scan, decode, reconstruct 1D, 2D, 3D.  A kernel with no work reports ~0.  While
+ * profiling is on, compress runs as one group on the caller's stream. */
+fz_status fz_batch_profile(fz_batch_plan plan, int enable);
+fz_status fz_batch_kernel_times(const fz_batch_plan plan, float* ms, int capacity, int* count);
+
+/* hipGraph of one batched call on fixed buffers, captured once; launching it replays the
+ * call's kernels on those buffers. */
+typedef struct fz_graph_s* fz_graph;
+fz_status fz_batch_compress_graph(const fz_batch_plan plan, const float* const* d_in, const double* eb_rel,
+                                  void* d_ws, size_t ws_bytes, uint8_t* const* d_payload, uint16_t* const* d_tail,
+                                  fz_result* d_results, fz_graph* graph);
+fz_status fz_batch_decompress_graph(const fz_batch_plan plan, const uint8_t* const* d_payload,
+                                    const uint16_t* const* d_tail, const float* ebx2, void* d_ws, size_t ws_bytes,
+                                    float* const* d_out, fz_graph* graph);
+fz_status fz_graph_launch(fz_graph graph, hipStream_t stream);
+fz_status fz_graph_destroy(fz_graph graph);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,3 +53,55 @@ def test_status_strings():
     L = fzgpu.lib()
     assert L.fz_status_string(4) == b"constant field (value range is 0)"
     assert L.fz_status_string(0) == b"success"
+
+
+# ---------------------------------------------------------------- batched form (host side)
+
+def test_batch_plan_sizes_follow_the_per_field_formulas():
+    shapes = [(3600, 1800, 1), (5000, 1, 1), (33, 20, 17), (2047, 1, 1), (1000, 3, 1, 6000)]
+    bp = fzgpu.BatchPlan(shapes, workspace=False)
+    lens = [6480000, 5000, 33 * 20 * 17, 2047, 6000]
+    assert bp.sizes.nfields == 5 and bp.sizes.total_len == sum(lens)
+    assert bp.sizes.units == sum(n // 2048 + 1 for n in lens)
+    assert bp.sizes.workspace_bytes >= 4096 * bp.sizes.units
+    for i, s in enumerate(shapes):
+        single = fzgpu.Plan(*s[:3], length=lens[i], workspace=False).sizes
+        fs = bp.field_sizes(i)
+        for k in ("len", "tiles", "flag_words", "payload_max_bytes", "tail_max_codes"):
+            assert getattr(fs, k) == getattr(single, k), (i, k)
+
+
+def test_batch_tuning_validation():
+    bp = fzgpu.BatchPlan([(5000, 1, 1), (64, 64, 64)], workspace=False)
+    t = bp.tuning
+    assert t == {"range_chunk": 65536, "xcd_order": [1, 1, 1], "gather_waves": 16, "groups": 1}
+    bp.set_tuning(range_chunk=16384, xcd_order=(0, 1, 0), gather_waves=4, groups=8)
+    assert bp.tuning == {"range_chunk": 16384, "xcd_order": [0, 1, 0], "gather_waves": 4, "groups": 8}
+    parts16 = bp.sizes.range_parts
+    for bad in [dict(range_chunk=1000), dict(gather_waves=3), dict(xcd_order=(2, 0, 0)), dict(groups=3),
+                dict(groups=0)]:
+        with pytest.raises(fzgpu.FzError) as e:
+            bp.set_tuning(**bad)
+        assert e.value.status == 1
+    bp.set_tuning(range_chunk=262144)
+    assert fzgpu.BatchPlan([(5000, 1, 1), (64, 64, 64)], workspace=False).sizes.range_parts >= bp.sizes.range_parts
+    assert parts16 >= 2
+
+
+@pytest.mark.parametrize("shapes", [[], [(0, 1, 1)], [(4, 4, 4, 10)], [(8, 1, 1)] * 4097])
+def test_batch_plan_rejects_bad_input(shapes):
+    with pytest.raises(fzgpu.FzError) as e:
+        fzgpu.BatchPlan(shapes, workspace=False)
+    assert e.value.status == 1
+
+
+def test_batch_shapes_of_the_uneven_workload():
+    from fzgpu import datagen as D
+    spec = D.batch_fields()
+    nd = [1 if f["dims"][1] == f["dims"][2] == 1 else (2 if f["dims"][2] == 1 else 3) for f in spec]
+    assert (nd.count(1), nd.count(2), nd.count(3)) == (77, 102, 77)
+    n = [f["dims"][0] * f["dims"][1] * f["dims"][2] for f in spec]
+    assert min(n) >= 2 ** 16 * 0.9 and max(n) <= 2 ** 26 * 1.1
+    assert {f["eb"] for f in spec} == {1e-2, 1e-3, 1e-4}
+    bp = fzgpu.BatchPlan([f["dims"] for f in spec], workspace=False)
+    assert bp.sizes.total_len == sum(n)

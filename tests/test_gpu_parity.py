@@ -43,7 +43,7 @@ def _np(t, dtype=None):
 
 SHAPES = [(5000, 1, 1), (4096, 1, 1), (2047, 1, 1), (2048, 1, 1), (9, 1, 1), (100, 77, 1), (16, 16, 1),
           (17, 33, 1), (3600, 20, 1), (1, 3000, 1), (33, 20, 17), (64, 64, 64), (500, 50, 10), (1, 1, 5000),
-          (7, 1, 900)]
+          (7, 1, 900), (3601, 37, 1), (45, 46, 47), (9, 50, 40), (13, 1, 700), (8, 300, 3), (1030, 9, 11)]
 
 
 def _check_against_oracle(x, dims, eb, mode, fused, length=None):
@@ -193,3 +193,26 @@ def test_full_size_configs_bit_exact(name):
     err = float(np.abs(xr - x).max())
     say(f"  CR {4 * x.size / r['payload_bytes']:.3f} maxerr/eb {err / r['eb_abs']:.6f}")
     assert err <= tol
+
+
+# ---------------------------------------------------------------- path selection by value range
+
+PACKED_LIMIT = {1: 16383.0, 2: 8191.0, 3: 4095.0}
+
+
+@pytest.mark.parametrize("dims", [(5000, 1, 1), (300, 77, 1), (301, 77, 1), (40, 30, 20), (41, 30, 20),
+                                  (9, 50, 40)])
+@pytest.mark.parametrize("scale", [0.99, 1.01, 300.0, 1000.0, 3000.0])
+def test_fused_paths_around_the_packed_bound(dims, scale):
+    """Uniform +-1 data, eb chosen so that max|x * ebx2_r| sits just inside / outside the
+    packed 16-bit bound (deltas up to ~2^15), in the 3D column-form band (|s| < 2^21), in
+    the reference-order float band (|s| < 2^23) and beyond 2^23 (guarded generic path)."""
+    rng = np.random.default_rng(sum(dims) + int(scale * 100))
+    n = dims[0] * dims[1] * dims[2]
+    x = rng.uniform(-1.0, 1.0, n).astype(np.float32)
+    nd = 1 if dims[1] == dims[2] == 1 else (2 if dims[2] == 1 else 3)
+    amax, rng_v = float(np.abs(x).max()), float(x.max() - x.min())
+    eb = amax / (2.0 * rng_v * PACKED_LIMIT[nd] * scale)
+    _check_against_oracle(x, dims, eb, O.MODE_FIXED, fused=True)
+    if nd == 1:
+        _check_against_oracle(x, dims, eb, O.MODE_LITERAL, fused=True)
