@@ -200,6 +200,38 @@ def bench_batch(steps, warmup, nfields=256):
     return r
 
 
+def bench_serving(rates=(50, 100, 200, 500, 1000, 2000), duration_s=1.0, pool_fields=48):
+    """Poisson request streams (80% compress / 20% decompress of archives already served)
+    over a device-resident pool of fields log-uniform in 256 KB - 1 GB, under a 200 GB HBM
+    budget; latency is arrival to completion."""
+    import torch
+
+    from fzgpu import serve as S
+    t0 = time.perf_counter()
+    pool = S.field_pool(pool_fields)
+    torch.cuda.synchronize()
+    log(f"  pool of {pool_fields} fields ({sum(4 * x.numel() for _, x in pool) / 1e9:.2f} GB) in "
+        f"{time.perf_counter() - t0:.1f} s")
+    out = {"pool_GB": round(sum(4 * x.numel() for _, x in pool) / 1e9, 3), "duration_s": duration_s}
+    # warm-up: every pool field compressed and decompressed alone, so that the caching
+    # allocator holds blocks of every request size before the timed streams
+    warm = S.Server(S.DeviceBackend(), hbm_budget_bytes=200e9, slots=3, max_fields=1, max_archives=len(pool))
+    warm.run([S.Request(i, "compress", d, 1e-3, x, 0.0) for i, (d, x) in enumerate(pool)], realtime=False)
+    arch = [r.result for r, _ in warm.archives]
+    warm2 = S.Server(S.DeviceBackend(), hbm_budget_bytes=200e9, slots=3, max_fields=1)
+    warm2.run([S.Request(i, "decompress", a.dims, 1e-3, a, 0.0) for i, a in enumerate(arch)], realtime=False)
+    del warm, warm2, arch
+    for k, rate in enumerate(rates):
+        reqs = S.make_requests(rate, duration_s, pool, seed=3000 + k)
+        srv = S.Server(S.DeviceBackend(), hbm_budget_bytes=200e9, slots=3, max_fields=256)
+        srv.run(reqs, realtime=True, on_arrival=S.pick_archive(4000 + k))
+        out[f"rate_{rate}"] = S.summarize(srv, reqs)
+        log(f"  rate {rate}/s: {out[f'rate_{rate}']['latency']}")
+        del srv, reqs
+        torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(x, dims, eb, seconds):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -262,6 +294,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-field uneven batch")
     ap.add_argument("--batch-only", action="store_true", help="only the 256-field uneven batch (prints its JSON)")
     ap.add_argument("--batch-fields", type=int, default=256)
+    ap.add_argument("--no-serve", action="store_true", help="skip the request-serving leg")
+    ap.add_argument("--serve-only", action="store_true", help="only the request-serving leg (prints its JSON)")
     ap.add_argument("--pmc-target", action="store_true",
                     help="profiling target: a 256 MiB device copy (counter calibration), then 3 fused compress "
                          "calls per workload, no timing")
@@ -280,6 +314,9 @@ def main():
         return
     if args.batch_only:
         print(json.dumps(bench_batch(args.steps, args.warmup, args.batch_fields)), flush=True)
+        return
+    if args.serve_only:
+        print(json.dumps(bench_serving()), flush=True)
         return
     log(f"generating {args.workload}")
     x, dims, eb = D.config_field(args.workload)
@@ -323,6 +360,9 @@ def main():
     if not args.no_batch:
         log(f"timing the {args.batch_fields}-field uneven batch")
         extra["uneven_batch"] = bench_batch(min(args.steps, 20), min(args.warmup, 3), args.batch_fields)
+    if not args.no_serve:
+        log("serving loop")
+        extra["serving"] = bench_serving()
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")

@@ -1,0 +1,379 @@
+"""Request serving on top of the batched C ABI: continuous batching under an HBM budget.
+
+Compress and decompress requests arrive over time.  A request is admitted when its device
+memory (outputs and its share of the batch workspace) fits in what the budget has left;
+admitted requests of one kind that are waiting together form the next batch, which runs on
+its own stream through fz_batch_compress_f32 / fz_batch_decompress_f32 while earlier batches
+are still on the device.  When a batch's event completes, its requests finish, its memory
+returns to the budget and freed slots admit the next batch.  Inputs and archives stay
+device-resident: a compress request reads a field that is already in HBM and leaves its
+archive (payload + tail + result record) in HBM for later decompress requests.
+
+The scheduler only talks to a backend (launch a batch, poll it, account memory), so its
+admission and ordering rules run on the CPU with a simulated backend in the tests.
+"""
+import collections
+import dataclasses
+import math
+import time
+
+import numpy as np
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    kind: str                 # "compress" | "decompress"
+    dims: tuple               # (x, y, z)
+    eb: float = 1e-3          # compress: value-range-relative error bound
+    source: object = None     # compress: input tensor (device); decompress: the Archive
+    arrival: float = 0.0      # seconds, on the server's clock
+    admitted: float = math.nan
+    done: float = math.nan
+    batch: int = -1
+    result: object = None     # compress: Archive; decompress: output tensor
+
+    @property
+    def n(self):
+        return self.dims[0] * self.dims[1] * self.dims[2]
+
+    @property
+    def input_bytes(self):
+        return 4 * self.n
+
+    @property
+    def latency(self):
+        return self.done - self.arrival
+
+
+@dataclasses.dataclass
+class Archive:
+    """A compressed field kept on the device: payload[:payload_bytes], tail[:tail_len]."""
+    dims: tuple
+    payload: object
+    tail: object
+    record: dict
+    keep: object = None       # the batch buffers the views above live in
+
+    @property
+    def nbytes(self):
+        return int(self.record["payload_bytes"]) + 2 * int(self.record["tail_len"])
+
+
+class HbmBudget:
+    """Byte accounting of the device memory the server may hold."""
+
+    def __init__(self, total_bytes):
+        self.total = int(total_bytes)
+        self.used = 0
+        self.peak = 0
+
+    def fits(self, nbytes):
+        return self.used + nbytes <= self.total
+
+    def take(self, nbytes):
+        assert self.fits(nbytes)
+        self.used += int(nbytes)
+        self.peak = max(self.peak, self.used)
+
+    def give(self, nbytes):
+        self.used -= int(nbytes)
+        assert self.used >= 0
+
+
+def request_bytes(req):
+    """(workspace, output) device bytes of a request, upper bounds: its share of the batch
+    workspace (staging / codes, 4096 B per tile, plus counters) and its output (the archive
+    slot of a compress request, the floats of a decompress request)."""
+    n = req.n
+    tiles = n // 2048 + 1
+    w = (n + 255) // 256
+    ws = 4096 * tiles + 32 * tiles + 1024
+    out = (4 * w + 4096 * tiles + 4096 + 64) if req.kind == "compress" else 4 * n + 64
+    return ws, out
+
+
+class Server:
+    """Continuous-batching scheduler.
+
+    backend: object with
+        launch(kind, requests) -> handle     (allocates outputs, enqueues the batch)
+        poll(handle) -> bool                 (True when the batch finished on the device)
+        finish(handle, requests) -> None     (fills request.result)
+    slots: batches allowed on the device at once; max_fields: fields per batch.
+    """
+
+    def __init__(self, backend, hbm_budget_bytes=200e9, slots=3, max_fields=256, max_archives=64,
+                 clock=time.perf_counter, sleep=time.sleep):
+        self.backend = backend
+        self.budget = HbmBudget(hbm_budget_bytes)
+        self.max_archives = int(max_archives)
+        self.archives = collections.deque()  # (request, bytes) of compress results kept on the device
+        self.slots = int(slots)
+        self.max_fields = int(max_fields)
+        self.clock = clock
+        self.sleep = sleep
+        self.t0 = None
+        self.waiting = {"compress": collections.deque(), "decompress": collections.deque()}
+        self.inflight = []        # (handle, kind, requests, bytes)
+        self.completed = []
+        self.batches = []         # (batch id, kind, n requests, input bytes, launch time, finish time)
+        self.max_inflight = 0
+
+    def now(self):
+        return self.clock() - self.t0
+
+    def submit(self, req):
+        if sum(request_bytes(req)) > self.budget.total:
+            raise ValueError(f"request {req.rid} needs more device memory than the whole budget")
+        self.waiting[req.kind].append(req)
+
+    def _form(self, kind):
+        """FIFO batch of waiting requests of `kind` that fits the budget (a request that does
+        not fit blocks the ones behind it, so admission order is arrival order)."""
+        q = self.waiting[kind]
+        take, nbytes = [], 0
+        while q and len(take) < self.max_fields:
+            need = sum(request_bytes(q[0]))
+            while not self.budget.fits(nbytes + need) and self.archives:
+                self._evict()
+            if not self.budget.fits(nbytes + need):
+                break
+            take.append(q.popleft())
+            nbytes += need
+        return take, nbytes
+
+    def _evict(self):
+        # a decompress request that already picked this archive keeps its buffers alive until
+        # it finishes (they are counted once, here, until eviction)
+        req, nbytes = self.archives.popleft()
+        req.result = None
+        self.budget.give(nbytes)
+
+    def archive_pool(self):
+        """Compress results currently kept on the device."""
+        return [r for r, _ in self.archives]
+
+    def _launch_ready(self):
+        launched = False
+        # the kind whose oldest request waited longest goes first
+        kinds = sorted((k for k in self.waiting if self.waiting[k]), key=lambda k: self.waiting[k][0].arrival)
+        for kind in kinds:
+            if len(self.inflight) >= self.slots:
+                break
+            reqs, nbytes = self._form(kind)
+            if not reqs:
+                continue
+            self.budget.take(nbytes)
+            t = self.now()
+            bid = len(self.batches)
+            for r in reqs:
+                r.admitted = t
+                r.batch = bid
+            handle = self.backend.launch(kind, reqs)
+            self.inflight.append((handle, kind, reqs, nbytes, bid))
+            self.batches.append([bid, kind, len(reqs), sum(r.input_bytes for r in reqs), t, math.nan])
+            self.max_inflight = max(self.max_inflight, len(self.inflight))
+            launched = True
+        return launched
+
+    def _poll(self):
+        done_any = False
+        still = []
+        for item in self.inflight:
+            handle, kind, reqs, nbytes, bid = item
+            if self.backend.poll(handle):
+                t = self.now()
+                self.backend.finish(handle, reqs)
+                for r in reqs:
+                    r.done = t
+                self.completed.extend(reqs)
+                self.budget.give(nbytes)
+                if kind == "compress" and self.max_archives > 0:  # archives stay on the device, in the budget
+                    for r in reqs:
+                        out = request_bytes(r)[1]
+                        while self.archives and (len(self.archives) >= self.max_archives or not self.budget.fits(out)):
+                            self._evict()
+                        if self.budget.fits(out):
+                            self.budget.take(out)
+                            self.archives.append((r, out))
+                self.batches[bid][5] = t
+                done_any = True
+            else:
+                still.append(item)
+        self.inflight = still
+        return done_any
+
+    def run(self, requests, realtime=True, on_arrival=None):
+        """Serve `requests` (arrival times in seconds from the start).  With realtime, a
+        request becomes visible when the clock passes its arrival; otherwise all are
+        visible at once.  on_arrival(server, req) may complete the request before it queues
+        (a decompress request picks its archive there)."""
+        pending = sorted(requests, key=lambda r: r.arrival)
+        self.t0 = self.clock()
+        i = 0
+        while i < len(pending) or any(self.waiting.values()) or self.inflight:
+            t = self.now()
+            while i < len(pending) and (not realtime or pending[i].arrival <= t):
+                if on_arrival:
+                    on_arrival(self, pending[i])
+                self.submit(pending[i])
+                i += 1
+            self._poll()
+            self._launch_ready()
+            if not self.inflight and not any(self.waiting.values()) and i < len(pending) and realtime:
+                # idle until the next arrival
+                dt = pending[i].arrival - self.now()
+                if dt > 2e-4:
+                    self.sleep(dt - 1e-4)
+        return self.completed
+
+
+def latency_stats(reqs):
+    lat = np.array([r.latency for r in reqs], dtype=np.float64) * 1e3
+    if lat.size == 0:
+        return {"n": 0}
+    return {"n": int(lat.size), "p50_ms": round(float(np.percentile(lat, 50)), 4),
+            "p99_ms": round(float(np.percentile(lat, 99)), 4), "mean_ms": round(float(lat.mean()), 4),
+            "max_ms": round(float(lat.max()), 4)}
+
+
+def poisson_arrivals(rate, duration_s, rng):
+    """Arrival times of a Poisson process of `rate` per second over [0, duration_s)."""
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= duration_s:
+            return out
+        out.append(t)
+
+
+# ---------------------------------------------------------------- the device backend
+
+class DeviceBackend:
+    """Batches through the batched C ABI, one stream per batch from a small stream pool."""
+
+    def __init__(self, mode=0, nstreams=3, device="cuda"):
+        import torch
+        self.torch = torch
+        self.mode = mode
+        self.device = device
+        self.streams = [torch.cuda.Stream(device=device) for _ in range(nstreams)]
+        self.busy = [False] * nstreams
+
+    def launch(self, kind, reqs):
+        import fzgpu
+        torch = self.torch
+        si = self.busy.index(False)
+        self.busy[si] = True
+        s = self.streams[si]
+        with torch.cuda.stream(s):
+            plan = fzgpu.BatchPlan([r.dims for r in reqs], mode=self.mode)
+            if kind == "compress":
+                out = plan.alloc_outputs()
+                plan.compress_async([r.source for r in reqs], [r.eb for r in reqs], out, stream=s)
+                keep = (plan, out)
+            else:
+                pays = [r.source.payload for r in reqs]
+                tails = [r.source.tail for r in reqs]
+                outs = [torch.empty(max(r.n, 4), dtype=torch.float32, device=self.device) for r in reqs]
+                plan.decompress_async(pays, tails, [r.source.record["ebx2"] for r in reqs], outs, stream=s)
+                keep = (plan, outs)
+            ev = torch.cuda.Event()
+            ev.record(s)
+        return {"si": si, "ev": ev, "keep": keep, "kind": kind}
+
+    def poll(self, h):
+        return h["ev"].query()
+
+    def finish(self, h, reqs):
+        import fzgpu
+        self.busy[h["si"]] = False
+        plan, out = h["keep"]
+        if h["kind"] == "compress":
+            recs = fzgpu.read_results(out.res, len(reqs))
+            for i, r in enumerate(reqs):
+                rec = recs[i]
+                r.result = Archive(r.dims, out.payloads[i][: rec["payload_bytes"]], out.tails[i][: rec["tail_len"]],
+                                   rec, keep=out)
+        else:
+            for i, r in enumerate(reqs):
+                r.result = out[i][: r.n]
+
+
+# ---------------------------------------------------------------- the serving workload
+
+SERVE_EBS = (1e-2, 1e-3, 1e-4)
+
+
+def field_pool(nfields=48, lo=2 ** 16, hi=2 ** 28, seed=3000, device="cuda"):
+    """Device-resident input fields: sizes log-uniform in [lo, hi] floats (256 KB - 1 GB),
+    1D / 2D / 3D shapes in turn, content from the device generators."""
+    from . import datagen as D
+    rng = np.random.default_rng(seed)
+    lr = (math.log(1 / 8), math.log(8))
+    pool = []
+    for i in range(nfields):
+        n = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        nd = 1 + i % 3
+        if nd == 1:
+            dims = (n, 1, 1)
+        elif nd == 2:
+            y = max(2, int(round(math.sqrt(n / math.exp(rng.uniform(*lr))))))
+            dims = (max(2, n // y), y, 1)
+        else:
+            a1, a2 = math.exp(rng.uniform(*lr)), math.exp(rng.uniform(*lr))
+            z = max(2, int(round((n / (a1 * a2 * a2)) ** (1 / 3))))
+            y = max(2, int(round(a2 * z)))
+            dims = (max(2, n // (y * z)), y, z)
+        pool.append((dims, D.generate_gpu(D.BATCH_KINDS[i % 3], dims, seed + i, device)))
+    return pool
+
+
+def make_requests(rate, duration_s, pool, seed, decompress_share=0.2):
+    """Poisson arrivals; each is a decompress request with probability decompress_share
+    (its archive is picked when it arrives), else a compress request of a pool field."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i, t in enumerate(poisson_arrivals(rate, duration_s, rng)):
+        dims, x = pool[int(rng.integers(0, len(pool)))]
+        eb = SERVE_EBS[int(rng.integers(0, len(SERVE_EBS)))]
+        kind = "decompress" if rng.random() < decompress_share else "compress"
+        reqs.append(Request(i, kind, dims, eb, x, t))
+    return reqs
+
+
+def pick_archive(seed):
+    """on_arrival hook: a decompress request reads a random archive the server holds (a
+    compress request of its field when there is none yet)."""
+    rng = np.random.default_rng(seed)
+
+    def hook(server, req):
+        if req.kind != "decompress":
+            return
+        pool = [r for r in server.archive_pool() if r.result is not None]
+        if not pool:
+            req.kind = "compress"
+            return
+        src = pool[int(rng.integers(0, len(pool)))]
+        req.dims = src.dims
+        req.source = src.result
+    return hook
+
+
+def summarize(server, reqs):
+    done = [r for r in reqs if not math.isnan(r.done)]
+    t_end = max(r.done for r in done) if done else 0.0
+    t_start = min(r.arrival for r in done) if done else 0.0
+    nbytes = sum(r.input_bytes for r in done)
+    sizes = [b[2] for b in server.batches]
+    return {
+        "requests": len(done), "compress": sum(r.kind == "compress" for r in done),
+        "decompress": sum(r.kind == "decompress" for r in done),
+        "latency": latency_stats(done),
+        "latency_compress": latency_stats([r for r in done if r.kind == "compress"]),
+        "latency_decompress": latency_stats([r for r in done if r.kind == "decompress"]),
+        "GBps": round(nbytes / max(t_end - t_start, 1e-9) / 1e9, 2),
+        "batches": len(server.batches), "mean_batch": round(float(np.mean(sizes)), 2) if sizes else 0,
+        "max_inflight_batches": server.max_inflight, "hbm_peak_GB": round(server.budget.peak / 1e9, 3),
+    }

@@ -1,0 +1,120 @@
+"""Scheduler rules of the serving loop, on the CPU with a simulated backend: FIFO admission
+per kind, the HBM budget and the slot limit are never exceeded, archives are evicted to make
+room, latencies are consistent, and an oversized request is refused."""
+import math
+
+import numpy as np
+import pytest
+
+from fzgpu import serve as S
+
+
+class FakeClock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        self.t += 1e-6  # a spinning scheduler loop takes time too
+        return self.t
+
+    def sleep(self, dt):
+        self.t += dt
+
+
+class FakeBackend:
+    """A batch takes 1 ms + 1 ns per input byte of simulated time; the clock advances on poll."""
+
+    def __init__(self, clock, server_ref):
+        self.clock = clock
+        self.server_ref = server_ref
+        self.launched = []
+
+    def launch(self, kind, reqs):
+        srv = self.server_ref[0]
+        assert srv.budget.used <= srv.budget.total
+        self.launched.append((kind, [r.rid for r in reqs]))
+        return {"end": self.clock.t + 1e-3 + 1e-9 * sum(r.input_bytes for r in reqs), "kind": kind}
+
+    def poll(self, h):
+        self.clock.t += 1e-5
+        return self.clock.t >= h["end"]
+
+    def finish(self, h, reqs):
+        for r in reqs:
+            r.result = S.Archive(r.dims, None, None, {"payload_bytes": 10, "tail_len": 0}) if h["kind"] == "compress" \
+                else "floats"
+
+
+def _server(budget=200e9, slots=3, max_fields=256, max_archives=64):
+    clock = FakeClock()
+    ref = [None]
+    be = FakeBackend(clock, ref)
+    srv = S.Server(be, hbm_budget_bytes=budget, slots=slots, max_fields=max_fields, max_archives=max_archives,
+                   clock=clock, sleep=clock.sleep)
+    ref[0] = srv
+    return srv, be, clock
+
+
+def _reqs(n, seed=0, dims=(1 << 20, 1, 1), decompress=False):
+    rng = np.random.default_rng(seed)
+    out = []
+    t = 0.0
+    for i in range(n):
+        t += rng.exponential(1e-4)
+        out.append(S.Request(i, "decompress" if decompress and i % 5 == 4 else "compress", dims, 1e-3, None, t))
+    return out
+
+
+def test_fifo_order_and_all_complete():
+    srv, be, _ = _server(slots=2, max_fields=4)
+    reqs = _reqs(40)
+    done = srv.run(reqs, realtime=True)
+    assert len(done) == 40
+    order = [rid for kind, ids in be.launched for rid in ids]
+    assert order == sorted(order)
+    assert srv.max_inflight <= 2
+    assert all(len(ids) <= 4 for _, ids in be.launched)
+    for r in done:
+        assert r.arrival <= r.admitted <= r.done
+
+
+def test_budget_limits_admission():
+    one = sum(S.request_bytes(S.Request(0, "compress", (1 << 20, 1, 1))))
+    srv, be, _ = _server(budget=3.5 * one, slots=8, max_archives=0)
+    srv.run(_reqs(20), realtime=False)
+    assert all(len(ids) <= 3 for _, ids in be.launched)
+    assert srv.budget.peak <= srv.budget.total
+    assert srv.budget.used == 0
+
+
+def test_archives_are_evicted_to_make_room():
+    one = sum(S.request_bytes(S.Request(0, "compress", (1 << 20, 1, 1))))
+    srv, _, _ = _server(budget=4 * one, slots=1, max_fields=2, max_archives=100)
+    srv.run(_reqs(30), realtime=False)
+    assert len(srv.completed) == 30
+    assert srv.budget.peak <= srv.budget.total
+    assert len(srv.archives) >= 1 and sum(b for _, b in srv.archives) <= srv.budget.total
+
+
+def test_decompress_requests_pick_archives():
+    srv, be, _ = _server(slots=3)
+    reqs = _reqs(50, seed=3, decompress=True)
+    srv.run(reqs, realtime=False, on_arrival=S.pick_archive(5))
+    kinds = {k for k, _ in be.launched}
+    assert "compress" in kinds
+    assert all(r.kind == "compress" or isinstance(r.source, S.Archive) for r in reqs)
+
+
+def test_oversized_request_is_refused():
+    srv, _, _ = _server(budget=1e6)
+    with pytest.raises(ValueError):
+        srv.run([S.Request(0, "compress", (1 << 22, 1, 1))], realtime=False)
+
+
+def test_latency_stats_and_poisson():
+    rng = np.random.default_rng(1)
+    t = S.poisson_arrivals(1000.0, 2.0, rng)
+    assert 1800 < len(t) < 2200 and all(a < b for a, b in zip(t, t[1:]))
+    reqs = [S.Request(i, "compress", (10, 1, 1), arrival=0.0, done=(i + 1) * 1e-3) for i in range(100)]
+    st = S.latency_stats(reqs)
+    assert st["n"] == 100 and math.isclose(st["p50_ms"], 50.5, rel_tol=1e-6) and st["p99_ms"] <= 100.0
