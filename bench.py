@@ -272,6 +272,24 @@ def pmc_target(D):
         del d
 
 
+def batch_target(D):
+    """Counter / trace target for the batched kernels: 3 compress calls of the 256-field
+    uneven batch at the default tuning (one group on one stream)."""
+    import torch
+
+    import fzgpu
+    spec = D.batch_fields()
+    d = [D.generate_gpu(f["kind"], f["dims"], f["seed"]) for f in spec]
+    plan = fzgpu.BatchPlan([f["dims"] for f in spec])
+    out = plan.alloc_outputs()
+    for _ in range(3):
+        plan.compress_async(d, [f["eb"] for f in spec], out)
+    torch.cuda.synchronize()
+    res = fzgpu.read_results(out.res, len(spec))
+    print(json.dumps({"workload": "uneven-batch-256", "input_bytes": sum(4 * t.numel() for t in d),
+                      "payload_bytes": sum(r["payload_bytes"] for r in res)}), flush=True)
+
+
 def load_traffic(kernel, workload):
     p = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{workload}.json")
     if not os.path.exists(p):
@@ -294,6 +312,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-field uneven batch")
     ap.add_argument("--batch-only", action="store_true", help="only the 256-field uneven batch (prints its JSON)")
     ap.add_argument("--batch-fields", type=int, default=256)
+    ap.add_argument("--batch-target", action="store_true",
+                    help="profiling target: the 256-field uneven batch, default tuning (one group), 3 compress calls")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-serving leg")
     ap.add_argument("--serve-only", action="store_true", help="only the request-serving leg (prints its JSON)")
     ap.add_argument("--pmc-target", action="store_true",
@@ -314,6 +334,9 @@ def main():
         return
     if args.batch_only:
         print(json.dumps(bench_batch(args.steps, args.warmup, args.batch_fields)), flush=True)
+        return
+    if args.batch_target:
+        batch_target(D)
         return
     if args.serve_only:
         print(json.dumps(bench_serving()), flush=True)
