@@ -56,7 +56,7 @@ def kernel_profile(plan, fn, reps):
     return [a / reps for a in acc]
 
 
-def bench_field(x, dims, eb, steps, warmup, fused=True, rotate_bytes=1.2e9, label=""):
+def bench_field(x, dims, eb, steps, warmup, fused=True, rotate_bytes=1.2e9, label="", wave_tile=0):
     import numpy as np
     import torch
 
@@ -67,6 +67,7 @@ def bench_field(x, dims, eb, steps, warmup, fused=True, rotate_bytes=1.2e9, labe
     base = torch.from_numpy(np.ascontiguousarray(x.ravel())).to("cuda")
     copies = [base] + [base.clone() for _ in range(ncopies - 1)]
     plan = fzgpu.Plan(*dims)
+    plan.set_tile_layout(wave_tile)
     payload, tail, res = plan.alloc_outputs()
 
     def step(k):

@@ -131,6 +131,42 @@ __global__ __launch_bounds__(kWG) void k_batch_tile(BatchDev d, uint32_t u_lo, u
                       ovf);
 }
 
+// One wave per unit, four units per workgroup, no workgroup barrier: every wave finds its field,
+// issues its first quarter's loads, then encodes its tile (tile_encode_w) or the tail unit.
+template <int NDIM>
+__global__ __launch_bounds__(kWG) void k_batch_tile_w(BatchDev d, uint32_t u_lo, uint32_t nunits, uint32_t f_lo,
+                                                      uint32_t f_hi, uint32_t xcd)
+{
+    __shared__ uint32_t sh[4][32 * kLdsPitch];
+    const uint32_t wv = threadIdx.x >> 6;
+    const uint32_t ngroups = (nunits + 3) / 4;
+    const uint32_t local = 4 * (xcd ? xcd_tile(blockIdx.x, ngroups) : blockIdx.x) + wv;
+    if (local >= nunits) return;
+    const uint32_t gu = u_lo + local;
+    const uint32_t f = wave_upper(d.ustart, f_lo, f_hi, gu);
+    const Geom     g = d.fields[f].g;
+    const uint32_t tile = gu - d.ustart[f];
+    const bool     is_tail = tile >= g.T;
+    const BPtrs    q = d.ptrs[f];
+    const float*   in = static_cast<const float*>(q.src);
+    const int      gp = is_tail ? 0 : geom_path<NDIM>(g, tile * (uint32_t)kTileCodes);
+    RawTile<NDIM>  R[2];
+    if (gp >= 2) raw_prefetch<NDIM>(in, tile * (uint32_t)kTileCodes + 8 * lane_id(), g, gp, R[0]);
+    const BParams p = d.prm[f];
+    if (p.status != ST_SUCCESS || is_tail) {
+        if (lane_id() == 0) d.counts[gu] = 0;
+        if (p.status != ST_SUCCESS) return;
+    }
+    unsigned long long* ovf = reinterpret_cast<unsigned long long*>(&d.res[d.fields[f].caller].overflow);
+    uint32_t*           bwords = static_cast<uint32_t*>(q.dst);
+    if (is_tail) {
+        tail_unit_w<NDIM>(in, g, p.r, bwords, q.tail, ovf);
+        return;
+    }
+    tile_encode_w<NDIM>(in, g, tile, gp, R, p.r, (int)p.level, bwords, d.staging + (size_t)gu * 4096, d.counts + gu,
+                        ovf, sh[wv]);
+}
+
 // One workgroup per field: exclusive scan of its tile counts (16 per thread per round) into
 // the payload address of every tile's first block: payload + 4 W + 16 * (blocks before it)
 // (encodeDebug's placement, fz.cu:56-57); then the field's block count and payload size.
@@ -223,11 +259,19 @@ static hipError_t group_tiles(const BatchDev& d, const BatchShape& b, const Batc
             e = skip_pair(s, ev, 1 + nd);
             continue;
         }
-        const dim3 grid(n), blk(kWG);
         const uint32_t ul = G.du_lo[nd], fl = G.df_lo[nd], fh = G.df_hi[nd], x = b.xcd[nd];
-        if (nd == 1) e = launch_pair(k_batch_tile<1>, grid, blk, s, ev, 2, d, ul, n, fl, fh, x);
-        else if (nd == 2) e = launch_pair(k_batch_tile<2>, grid, blk, s, ev, 3, d, ul, n, fl, fh, x);
-        else e = launch_pair(k_batch_tile<3>, grid, blk, s, ev, 4, d, ul, n, fl, fh, x);
+        const dim3     blk(kWG);
+        if (b.wave_tile[nd]) {
+            const dim3 grid((n + 3) / 4);
+            if (nd == 1) e = launch_pair(k_batch_tile_w<1>, grid, blk, s, ev, 2, d, ul, n, fl, fh, x);
+            else if (nd == 2) e = launch_pair(k_batch_tile_w<2>, grid, blk, s, ev, 3, d, ul, n, fl, fh, x);
+            else e = launch_pair(k_batch_tile_w<3>, grid, blk, s, ev, 4, d, ul, n, fl, fh, x);
+        } else {
+            const dim3 grid(n);
+            if (nd == 1) e = launch_pair(k_batch_tile<1>, grid, blk, s, ev, 2, d, ul, n, fl, fh, x);
+            else if (nd == 2) e = launch_pair(k_batch_tile<2>, grid, blk, s, ev, 3, d, ul, n, fl, fh, x);
+            else e = launch_pair(k_batch_tile<3>, grid, blk, s, ev, 4, d, ul, n, fl, fh, x);
+        }
     }
     return e;
 }

@@ -95,6 +95,7 @@ struct fz_plan_s {
     Geom       g;
     Layout     L;
     int        mode;
+    int        wave_tile = 0;               // tile-kernel layout of the fused compress
     int        prof = 0;                    // per-kernel events enabled
     int        nev = 0;                     // events recorded by the last call
     int        paired = 0;                  // last call recorded (start, stop) pairs per kernel
@@ -158,6 +159,13 @@ fz_status fz_plan_destroy(fz_plan plan)
     for (int i = 0; i < kMaxProfEvents; i++)
         if (plan->ev[i]) (void)hipEventDestroy(plan->ev[i]);
     delete plan;
+    return FZ_SUCCESS;
+}
+
+fz_status fz_plan_set_tile_layout(fz_plan plan, int wave_tile)
+{
+    if (!plan || (wave_tile != 0 && wave_tile != 1)) return FZ_ERR_INVALID_ARG;
+    plan->wave_tile = wave_tile;
     return FZ_SUCCESS;
 }
 
@@ -237,7 +245,8 @@ fz_status fz_compress_f32(const fz_plan plan, const float* d_in, double eb_rel, 
         plan->paired = 1;
     }
     FZ_TRY(launch_compress(d_in, g, eb_rel, reinterpret_cast<float4*>(ws + L.part), d_result, d_payload, d_tail,
-                           ws + L.stage, reinterpret_cast<uint32_t*>(ws + L.counts), tstate, ev, stream));
+                           ws + L.stage, reinterpret_cast<uint32_t*>(ws + L.counts), tstate, plan->wave_tile, ev,
+                           stream));
     return FZ_SUCCESS;
 }
 
@@ -472,7 +481,8 @@ bool tuning_ok(const fz_batch_tuning& t)
     bool chunk = false;
     for (int c = 0; c < kNumChunks; c++) chunk |= t.range_chunk == kRangeChunks[c];
     return chunk && pow2_upto16(t.gather_waves) && pow2_upto16(t.groups) && t.xcd_order[0] <= 1 &&
-           t.xcd_order[1] <= 1 && t.xcd_order[2] <= 1;
+           t.xcd_order[1] <= 1 && t.xcd_order[2] <= 1 && t.wave_tile[0] <= 1 && t.wave_tile[1] <= 1 &&
+           t.wave_tile[2] <= 1;
 }
 
 // fields [f_lo, f_hi) of the sorted order as one group
@@ -552,7 +562,10 @@ void apply_tuning(fz_batch_plan p)
     while (kRangeChunks[c] != p->tune.range_chunk) c++;
     p->shape.chunk = kRangeChunks[c];
     p->shape.P = p->P[c];
-    for (int nd = 1; nd <= 3; nd++) p->shape.xcd[nd] = p->tune.xcd_order[nd - 1];
+    for (int nd = 1; nd <= 3; nd++) {
+        p->shape.xcd[nd] = p->tune.xcd_order[nd - 1];
+        p->shape.wave_tile[nd] = p->tune.wave_tile[nd - 1];
+    }
     p->shape.gather_waves = p->tune.gather_waves;
     build_groups(p, p->tune.groups);
 }
@@ -940,16 +953,24 @@ fz_status fz_batch_autotune(fz_batch_plan plan, const float* const* d_in, const 
         }
     }
     plan->tune.range_chunk = best.range_chunk;
-    // tile order per dimensionality: tile-kernel time of each
-    float xt[2][3] = {};
-    for (uint32_t v = 0; v < 2 && st == FZ_SUCCESS; v++) {
-        for (int nd = 0; nd < 3; nd++) plan->tune.xcd_order[nd] = v;
+    // tile order x tile layout per dimensionality: tile-kernel time of each (v = 2 layout + order)
+    float xt[4][3] = {};
+    for (uint32_t v = 0; v < 4 && st == FZ_SUCCESS; v++) {
+        for (int nd = 0; nd < 3; nd++) {
+            plan->tune.xcd_order[nd] = v & 1u;
+            plan->tune.wave_tile[nd] = v >> 1;
+        }
         apply_tuning(plan);
         st = run(t);
         for (int nd = 0; nd < 3; nd++) xt[v][nd] = t[2 + nd];
     }
-    for (int nd = 0; nd < 3; nd++) best.xcd_order[nd] = xt[1][nd] <= xt[0][nd] ? 1u : 0u;
-    for (int nd = 0; nd < 3; nd++) plan->tune.xcd_order[nd] = best.xcd_order[nd];
+    for (int nd = 0; nd < 3; nd++) {
+        uint32_t bv = 1;  // XCD-sliced, one workgroup per tile on ties
+        for (uint32_t v = 0; v < 4; v++)
+            if (xt[v][nd] < xt[bv][nd]) bv = v;
+        best.xcd_order[nd] = plan->tune.xcd_order[nd] = bv & 1u;
+        best.wave_tile[nd] = plan->tune.wave_tile[nd] = bv >> 1;
+    }
     // gather width
     best_ms = 1e30f;
     const uint32_t gws[3] = {4, 8, 16};

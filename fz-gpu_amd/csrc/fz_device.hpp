@@ -89,6 +89,35 @@ __device__ __forceinline__ float wave_max(float v)
     return v;
 }
 
+// min and max over the 64 lanes, both returned to every lane, through DPP (VALU-only: row
+// shifts within 16-lane rows, then the row broadcasts of lanes 15 and 31, then lane 63).
+// A lane whose DPP source does not exist takes the identity (+inf for min, -inf for max).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_or(float identity, float v)
+{
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(identity), __float_as_int(v), CTRL, ROWS, 0xF, false));
+}
+
+__device__ __forceinline__ void wave_minmax(float& lo, float& hi)
+{
+    constexpr float PI = __builtin_huge_valf(), NI = -__builtin_huge_valf();
+    lo = fminf(lo, dpp_or<0x111, 0xF>(PI, lo));  // row_shr:1
+    hi = fmaxf(hi, dpp_or<0x111, 0xF>(NI, hi));
+    lo = fminf(lo, dpp_or<0x112, 0xF>(PI, lo));  // row_shr:2
+    hi = fmaxf(hi, dpp_or<0x112, 0xF>(NI, hi));
+    lo = fminf(lo, dpp_or<0x114, 0xF>(PI, lo));  // row_shr:4
+    hi = fmaxf(hi, dpp_or<0x114, 0xF>(NI, hi));
+    lo = fminf(lo, dpp_or<0x118, 0xF>(PI, lo));  // row_shr:8: lane 15 of a row holds the row's result
+    hi = fmaxf(hi, dpp_or<0x118, 0xF>(NI, hi));
+    lo = fminf(lo, dpp_or<0x142, 0xA>(PI, lo));  // row_bcast:15 into rows 1 and 3
+    hi = fmaxf(hi, dpp_or<0x142, 0xA>(NI, hi));
+    lo = fminf(lo, dpp_or<0x143, 0xC>(PI, lo));  // row_bcast:31 into rows 2 and 3: lane 63 holds all
+    hi = fmaxf(hi, dpp_or<0x143, 0xC>(NI, hi));
+    lo = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lo), 63));
+    hi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hi), 63));
+}
+
 // inclusive scan over the 64 lanes
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
 {

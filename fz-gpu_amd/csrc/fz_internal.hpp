@@ -39,10 +39,11 @@ inline uint32_t range_grid(uint32_t len)
 // -------- launchers (return hipError_t of the launch)
 hipError_t launch_range(const float* in, const Geom& g, double eb_rel, float4* part, fz_result* res,
                         uint64_t* zero_states, uint32_t nzero, uint32_t* zero_ticket, hipStream_t s);
-// Fused compress (3 kernels).  ev (may be null) holds start / stop events of each kernel.
+// Fused compress (3 kernels).  wave_tile: 1 = one wave per tile (k_tile_compress_w), 0 = one
+// workgroup per tile.  ev (may be null) holds start / stop events of each kernel.
 hipError_t launch_compress(const float* in, const Geom& g, double eb_rel, float4* part, fz_result* res,
                            uint8_t* payload, uint16_t* tail, uint8_t* staging, uint32_t* counts, uint64_t* states,
-                           hipEvent_t* ev, hipStream_t s);
+                           int wave_tile, hipEvent_t* ev, hipStream_t s);
 uint32_t   gather_grid(uint32_t T);
 // r_host / r_amax: ebx2_r and max|x| when res is null (the per-stage entry point)
 hipError_t launch_dualquant(const float* in, const Geom& g, float r_host, float r_amax, const fz_result* res,
@@ -126,6 +127,7 @@ struct BatchShape {
     uint32_t P;                 // range partials
     uint32_t chunk;             // floats per range partial
     uint32_t xcd[4];            // tile order per dimensionality: 1 XCD-sliced, 0 linear
+    uint32_t wave_tile[4];      // tile layout per dimensionality: 1 one wave per tile, 0 four
     uint32_t gather_waves;      // units per gather workgroup
 };
 

@@ -139,6 +139,39 @@ __global__ __launch_bounds__(kWG) void k_tile_compress(const float* __restrict__
 }
 
 
+// One wave per tile: a workgroup carries four consecutive units (the tail unit included) and
+// reduces the range partials once for them; the waves never wait for each other afterwards.
+template <int NDIM>
+__global__ __launch_bounds__(kWG) void k_tile_compress_w(const float* __restrict__ in, Geom g, double eb_rel,
+                                                         const float4* __restrict__ part, uint32_t nparts,
+                                                         fz_result* res, uint8_t* __restrict__ payload,
+                                                         uint8_t* __restrict__ staging, uint32_t* __restrict__ counts,
+                                                         uint16_t* __restrict__ tail)
+{
+    __shared__ uint32_t sh[4][32 * kLdsPitch];
+    const uint32_t wv = threadIdx.x >> 6;
+    const uint32_t unit = 4 * xcd_tile(blockIdx.x, (g.T + 4) / 4) + wv;  // units 0 .. T (T = the tail unit)
+    const bool     is_tile = unit < g.T;
+    const int      gp = is_tile ? geom_path<NDIM>(g, unit * (uint32_t)kTileCodes) : 0;
+    RawTile<NDIM>  R[2];
+    if (gp >= 2) raw_prefetch<NDIM>(in, unit * (uint32_t)kTileCodes + 8 * lane_id(), g, gp, R[0]);
+    const EbParams e = reduce_partials_256(part, nparts, eb_rel, NDIM);
+    if (blockIdx.x == 0 && threadIdx.x == 0) write_header(res, e, g);
+    if (e.status != ST_SUCCESS || unit > g.T) return;  // uniform per wave
+    unsigned long long* ovf = reinterpret_cast<unsigned long long*>(&res->overflow);
+    uint32_t*           bwords = reinterpret_cast<uint32_t*>(payload);
+    if (!is_tile) {
+        tail_unit_w<NDIM>(in, g, e.r, bwords, tail, ovf);
+        if (g.T == 0 && lane_id() == 0) {
+            res->nblocks = 0;
+            res->payload_bytes = 4ull * g.W;
+        }
+        return;
+    }
+    tile_encode_w<NDIM>(in, g, unit, gp, R, e.r, e.level, bwords, staging + (size_t)unit * 4096, counts + unit, ovf,
+                        sh[wv]);
+}
+
 // Scan of the tile counts + placement of the staged blocks.  The grid is small (<= kGatherWG
 // workgroups, each a contiguous run of tiles) and chains its look-back in blockIdx order: at
 // most kGatherWG - 1 workgroups can be waiting, far fewer than the device holds, so a
@@ -238,7 +271,7 @@ uint32_t gather_grid(uint32_t T) { return T == 0 ? 0u : (T + gather_tpw(T) - 1) 
 // stop events for the three kernels.
 hipError_t launch_compress(const float* in, const Geom& g, double eb_rel, float4* part, fz_result* res,
                            uint8_t* payload, uint16_t* tail, uint8_t* staging, uint32_t* counts, uint64_t* states,
-                           hipEvent_t* ev, hipStream_t s)
+                           int wave_tile, hipEvent_t* ev, hipStream_t s)
 {
     hipEvent_t none[6] = {};
     if (!ev) ev = none;
@@ -247,6 +280,17 @@ hipError_t launch_compress(const float* in, const Geom& g, double eb_rel, float4
     hipError_t e = launch_timed(k_range_part, dim3(nparts), dim3(kRangeWG), s, ev[0], ev[1], in, g, part, states,
                                 G + 1, res);
     if (e != hipSuccess) return e;
+    if (wave_tile) {
+        const dim3 grid((g.T + 4) / 4), blk(kWG);
+        switch (g.ndim) {
+            case 1: e = launch_timed(k_tile_compress_w<1>, grid, blk, s, ev[2], ev[3], in, g, eb_rel, (const float4*)part, nparts, res, payload, staging, counts, tail); break;
+            case 2: e = launch_timed(k_tile_compress_w<2>, grid, blk, s, ev[2], ev[3], in, g, eb_rel, (const float4*)part, nparts, res, payload, staging, counts, tail); break;
+            default: e = launch_timed(k_tile_compress_w<3>, grid, blk, s, ev[2], ev[3], in, g, eb_rel, (const float4*)part, nparts, res, payload, staging, counts, tail); break;
+        }
+        if (e != hipSuccess || g.T == 0) return e;
+        return launch_timed(k_scan_gather, dim3(G), dim3(kGatherThreads), s, ev[4], ev[5], (const uint32_t*)counts,
+                            g.T, gather_tpw(g.T), (const uint8_t*)staging, payload, states, res, g.W);
+    }
     const dim3 grid(g.T + 1), blk(kWG);
     switch (g.ndim) {
         case 1: e = launch_timed(k_tile_compress<1>, grid, blk, s, ev[2], ev[3], in, g, eb_rel, (const float4*)part, nparts, res, payload, staging, counts, tail); break;

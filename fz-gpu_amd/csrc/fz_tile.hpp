@@ -97,8 +97,7 @@ __device__ __forceinline__ void range_span(const float* __restrict__ in, uint32_
     for (uint32_t k = e1 + tid; k < f_hi; k += NT) {
         lo = fminf(lo, in[k]); hi = fmaxf(hi, in[k]); bad |= nonfinite(in[k]);
     }
-    lo = wave_min(lo);
-    hi = wave_max(hi);
+    wave_minmax(lo, hi);
     bad = ballot64(bad != 0) != 0;
     const uint32_t wv = tid >> 6;
     if ((tid & 63) == 0) {
@@ -109,8 +108,9 @@ __device__ __forceinline__ void range_span(const float* __restrict__ in, uint32_
     __syncthreads();
     if (tid < 64) {
         const bool live = tid < NT / 64;
-        lo = wave_min(live ? s_lo[tid] : __builtin_huge_valf());
-        hi = wave_max(live ? s_hi[tid] : -__builtin_huge_valf());
+        lo = live ? s_lo[tid] : __builtin_huge_valf();
+        hi = live ? s_hi[tid] : -__builtin_huge_valf();
+        wave_minmax(lo, hi);
         bad = ballot64(live && s_bad[tid] != 0) != 0;
         if (tid == 0) *out = make_float4(lo, hi, __uint_as_float(bad), 0.0f);
     }
@@ -168,8 +168,7 @@ __device__ __forceinline__ EbParams reduce_partials(const float4* __restrict__ p
         hi = fmaxf(hi, p.y);
         bad |= __float_as_uint(p.z);
     }
-    lo = wave_min(lo);
-    hi = wave_max(hi);
+    wave_minmax(lo, hi);
     bad = ballot64(bad != 0) != 0;
     if ((tid & 63) == 0) {
         s_lo[wv] = lo;
@@ -204,8 +203,7 @@ __device__ __forceinline__ EbParams reduce_partials_256(const float4* __restrict
         const uint32_t bad =
             ballot64((__float_as_uint(p[0].z) | __float_as_uint(p[1].z) | __float_as_uint(p[2].z) |
                       __float_as_uint(p[3].z)) != 0) != 0;
-        lo = wave_min(lo);
-        hi = wave_max(hi);
+        wave_minmax(lo, hi);
         if (tid == 0) s_eb = eb_params(lo, hi, bad, eb_rel, ndim);
     }
     __syncthreads();
@@ -373,6 +371,119 @@ __device__ __forceinline__ void tile_encode(const float* __restrict__ in, const 
     }
     // 4. in-tile compaction into the tile's staging slot
     if (f) *reinterpret_cast<uint4*>(slot + 16 * (woff + rank)) = blk;
+}
+
+// ---------------------------------------------------------------------------- one wave per tile
+//
+// The same tile encoded by one wave instead of four, with no workgroup barrier.  Lane L
+// computes the 8 codes at tile_start + 512 m + 8 L for the quarters m = 0..3 -- exactly the
+// positions thread 64 m + L computes in tile_encode, through the same per-thread paths -- and
+// issues the raw loads of quarter m + 1 before computing quarter m.  Each quarter is transposed
+// in registers and written to the wave's own LDS region; lane L then owns blocks L + 64 j
+// (j = 0..3): ballot j gives bitflag words 2j and 2j+1, and the ranks of the nonzero blocks come
+// from the popcounts of ballots < j plus mbcnt.  `sh` is the wave's 32 x kLdsPitch words.
+
+// dual-quant of one 8-code group with the raw loads in R (fast paths) or loading itself
+template <int NDIM>
+__device__ __forceinline__ void dq8_level(const float* __restrict__ in, uint32_t p0, const Geom& g, int gp,
+                                          const RawTile<NDIM>& R, float r, int level, uint32_t w[4], uint32_t& ovf)
+{
+    if constexpr (NDIM == 1) {
+        if (gp == 2 && level >= 3) raw_compute_pk<1, true>(R, p0, g, r, w);
+        else if (gp == 2 && level >= 1) raw_compute<1>(R, p0, g, r, w, ovf);
+        else dq8_tile<1>(in, p0, g, r, level >= 1 ? gp : 0, w, ovf);
+    } else if constexpr (NDIM == 2) {
+        if (gp == 2 && level >= 3) raw_compute_pk<2, true>(R, p0, g, r, w);
+        else if (gp == 3 && level >= 3) raw_compute_pk<2, false>(R, p0, g, r, w);
+        else if (gp == 2 && level >= 1) raw_compute<2>(R, p0, g, r, w, ovf);
+        else if (gp == 3 && level >= 1) raw_compute_u(R, p0, g, r, w, ovf);
+        else dq8_tile<2>(in, p0, g, r, level >= 1 ? gp : 0, w, ovf);
+    } else {
+        if (gp == 2 && level >= 3) raw_compute_pk<3, true>(R, p0, g, r, w);
+        else if (gp == 3 && level >= 3) raw_compute_pk<3, false>(R, p0, g, r, w);
+        else if (gp == 2 && level >= 2) raw_compute_q3<true>(R, p0, g, r, w, ovf);
+        else if (gp == 3 && level >= 2) raw_compute_q3<false>(R, p0, g, r, w, ovf);
+        else if (gp == 2 && level >= 1) raw_compute<3>(R, p0, g, r, w, ovf);
+        else dq8_tile<3>(in, p0, g, r, level >= 1 ? (gp == 3 ? 1 : gp) : 0, w, ovf);
+    }
+}
+
+// LDS writes of the lanes of this wave visible to its other lanes
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// gp = geom_path of the tile; R[0] holds quarter 0's raw loads when gp >= 2 (issued by the
+// caller with raw_prefetch at tile_start + 8 lane, before its other latency-bound work).
+template <int NDIM>
+__device__ __forceinline__ void tile_encode_w(const float* __restrict__ in, const Geom& g, uint32_t tile, int gp,
+                                              RawTile<NDIM> (&R)[2], float r, int level,
+                                              uint32_t* __restrict__ bwords, uint8_t* __restrict__ slot,
+                                              uint32_t* __restrict__ count_out, unsigned long long* ovf_out,
+                                              uint32_t* __restrict__ sh)
+{
+    const uint32_t lane = lane_id(), L8 = lane & 7, k8 = lane >> 3;
+    const uint32_t t0 = tile * (uint32_t)kTileCodes;
+    const bool     fast = level >= 1 && gp >= 2;
+    uint32_t       ovf = 0;
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const uint32_t p0 = t0 + 512 * m + 8 * lane;
+        if (fast && m < 3) raw_prefetch<NDIM>(in, p0 + 512, g, gp, R[(m + 1) & 1]);
+        uint32_t w[4];
+        dq8_level<NDIM>(in, p0, g, gp, R[m & 1], r, level, w, ovf);
+        if (ballot64((w[0] | w[1] | w[2] | w[3]) != 0) != 0) transpose4(w, L8);
+#pragma unroll
+        for (int q = 0; q < 4; q++) sh[kLdsPitch * (4 * L8 + q) + 8 * m + k8] = w[q];
+    }
+    wave_lds_sync();
+    uint32_t base = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint32_t i = lane + 64 * j, c = i >> 3, k0 = 4 * (i & 7);
+        const uint4    blk = make_uint4(sh[kLdsPitch * c + k0], sh[kLdsPitch * c + k0 + 1], sh[kLdsPitch * c + k0 + 2],
+                                        sh[kLdsPitch * c + k0 + 3]);
+        const bool     f = (blk.x | blk.y | blk.z | blk.w) != 0;
+        const uint64_t mask = ballot64(f);
+        if (lane < 2) bwords[8 * tile + 2 * j + lane] = lane ? (uint32_t)(mask >> 32) : (uint32_t)mask;
+        if (f) *reinterpret_cast<uint4*>(slot + 16 * (base + rank_below(mask))) = blk;
+        base += (uint32_t)__popcll(mask);
+    }
+    if (lane == 0) *count_out = base;
+    if (ballot64(ovf != 0) != 0) {
+        ovf = wave_sum(ovf);
+        if (lane == 0) atomicAdd(ovf_out, (unsigned long long)ovf);
+    }
+    wave_lds_sync();  // the region is rewritten by this wave's next tile
+}
+
+// The unit after the last full tile, by one wave (see tail_unit).
+template <int NDIM>
+__device__ __forceinline__ void tail_unit_w(const float* __restrict__ in, const Geom& g, float r,
+                                            uint32_t* __restrict__ bwords, uint16_t* __restrict__ tail,
+                                            unsigned long long* ovf_out)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t t0 = (uint32_t)kTileCodes * g.T;
+    const uint32_t tail_len = g.len - t0;
+    uint32_t       ovf = 0;
+    for (uint32_t j = lane; 8 * j < tail_len; j += 64) {
+        uint32_t w[4];
+        dq8<NDIM>(in, t0 + 8 * j, g, r, w, ovf);
+        if (g.mode == 0 && tail) {
+#pragma unroll
+            for (int i = 0; i < 8; i++)
+                if (8 * j + i < tail_len) tail[8 * j + i] = (uint16_t)(w[i >> 1] >> (16 * (i & 1)));
+        }
+    }
+    for (uint32_t i = 8 * g.T + lane; i < g.W; i += 64) bwords[i] = 0;
+    if (ballot64(ovf != 0) != 0) {
+        ovf = wave_sum(ovf);
+        if (lane == 0) atomicAdd(ovf_out, (unsigned long long)ovf);
+    }
 }
 
 // ---------------------------------------------------------------------------- decompress tiles

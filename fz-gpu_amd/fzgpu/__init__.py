@@ -59,7 +59,7 @@ EXPORTS = ["fz_version", "fz_status_string", "fz_last_error", "fz_plan_create", 
            "fz_workspace_init", "fz_compress_f32", "fz_compress_plain_f32", "fz_decompress_f32",
            "fz_compress_f32_sync", "fz_lorenzo_construct_f32", "fz_lorenzo_reconstruct_f32",
            "fz_bitshuffle_bitflag", "fz_bitunshuffle", "fz_exclusive_sum_u32", "fz_encode_blocks",
-           "fz_plan_profile", "fz_plan_kernel_times",
+           "fz_plan_set_tile_layout", "fz_plan_profile", "fz_plan_kernel_times",
            "fz_batch_plan_create", "fz_batch_plan_destroy", "fz_batch_plan_sizes", "fz_batch_field_sizes",
            "fz_batch_get_tuning", "fz_batch_set_tuning", "fz_batch_workspace_init", "fz_batch_compress_f32",
            "fz_batch_decompress_f32", "fz_batch_autotune", "fz_batch_profile", "fz_batch_kernel_times",
@@ -78,11 +78,12 @@ class BatchSizes(ctypes.Structure):
 
 class BatchTuning(ctypes.Structure):
     _fields_ = [("range_chunk", ctypes.c_uint32), ("xcd_order", ctypes.c_uint32 * 3),
-                ("gather_waves", ctypes.c_uint32), ("groups", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+                ("gather_waves", ctypes.c_uint32), ("groups", ctypes.c_uint32), ("wave_tile", ctypes.c_uint32 * 3)]
 
     def as_dict(self):
         return {"range_chunk": self.range_chunk, "xcd_order": list(self.xcd_order),
-                "gather_waves": self.gather_waves, "groups": self.groups}
+                "gather_waves": self.gather_waves, "groups": self.groups,
+                "wave_tile": list(self.wave_tile)}
 
 _lib = None
 
@@ -122,6 +123,7 @@ def lib():
     L.fz_exclusive_sum_u32.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t), vp, vp, u64, vp]
     L.fz_encode_blocks.argtypes = [vp, vp, vp, u64, vp]
     L.fz_plan_profile.argtypes = [vp, i32]
+    L.fz_plan_set_tile_layout.argtypes = [vp, i32]
     L.fz_plan_kernel_times.argtypes = [vp, ctypes.POINTER(f32), i32, ctypes.POINTER(i32)]
     L.fz_batch_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(FieldShape), u32, i32]
     L.fz_batch_plan_destroy.argtypes = [vp]
@@ -205,6 +207,10 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+    def set_tile_layout(self, wave_tile):
+        """Tile-kernel layout of the fused compress: 1 = one wave per tile, 0 = one workgroup."""
+        _check(lib().fz_plan_set_tile_layout(self.handle, int(wave_tile)), "fz_plan_set_tile_layout")
 
     def profile(self, enable=True):
         _check(lib().fz_plan_profile(self.handle, 1 if enable else 0), "fz_plan_profile")
@@ -369,7 +375,7 @@ class BatchPlan:
         _check(lib().fz_batch_get_tuning(self.handle, ctypes.byref(t)), "fz_batch_get_tuning")
         return t.as_dict()
 
-    def set_tuning(self, range_chunk=None, xcd_order=None, gather_waves=None, groups=None):
+    def set_tuning(self, range_chunk=None, xcd_order=None, gather_waves=None, groups=None, wave_tile=None):
         t = BatchTuning()
         _check(lib().fz_batch_get_tuning(self.handle, ctypes.byref(t)), "fz_batch_get_tuning")
         if range_chunk is not None:
@@ -381,6 +387,9 @@ class BatchPlan:
             t.gather_waves = gather_waves
         if groups is not None:
             t.groups = groups
+        if wave_tile is not None:
+            for i in range(3):
+                t.wave_tile[i] = wave_tile[i]
         _check(lib().fz_batch_set_tuning(self.handle, ctypes.byref(t)), "fz_batch_set_tuning")
 
     def alloc_outputs(self):

@@ -123,6 +123,11 @@ fz_status fz_compress_f32_sync(const fz_plan plan, const float* d_in, double eb_
                                uint8_t* d_payload, uint16_t* d_tail, fz_result* d_result, fz_result* h_result,
                                hipStream_t stream);
 
+/* Launch knob of fz_compress_f32 (no counterpart in the reference); both settings give the
+ * same bytes.  wave_tile = 0: one 256-thread workgroup per 2048-code tile; 1: one 64-lane wave
+ * per tile, four tiles per workgroup, no workgroup barrier inside a tile. */
+fz_status fz_plan_set_tile_layout(fz_plan plan, int wave_tile);
+
 /* Per-kernel timing (the reference's cuda_timer_t around its Lorenzo launch, timer.hh:45-85,
  * extended to every kernel): when enabled, the compress/decompress calls on this plan record
  * an event before their first kernel and after each kernel; fz_plan_kernel_times waits for the
@@ -194,7 +199,7 @@ typedef struct fz_batch_tuning {
     uint32_t xcd_order[3]; /* tile order of 1D, 2D, 3D fields: 1 XCD-sliced, 0 linear */
     uint32_t gather_waves; /* units per gather workgroup: 1, 2, 4, 8 or 16 */
     uint32_t groups;       /* field groups pipelined over three streams: 1, 2, 4, 8 or 16 */
-    uint32_t reserved[3];
+    uint32_t wave_tile[3]; /* tile layout of 1D, 2D, 3D fields: 1 one wave per tile, 0 four waves per tile */
 } fz_batch_tuning;
 
 /* At most 4096 fields; every field as fz_plan_create accepts it. */

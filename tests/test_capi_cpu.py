@@ -43,6 +43,16 @@ def test_plan_rejects_bad_geometry(args):
     assert e.value.status == 1
 
 
+def test_tile_layout_knob_is_validated():
+    p = fzgpu.Plan(100, 77, 1, workspace=False)
+    p.set_tile_layout(1)
+    p.set_tile_layout(0)
+    for bad in (2, -1):
+        with pytest.raises(fzgpu.FzError) as e:
+            p.set_tile_layout(bad)
+        assert e.value.status == 1
+
+
 def test_scan_size_query_needs_no_device():
     n = ctypes.c_size_t(0)
     assert fzgpu.lib().fz_exclusive_sum_u32(None, ctypes.byref(n), None, None, 1 << 20, None) == 0
@@ -74,12 +84,14 @@ def test_batch_plan_sizes_follow_the_per_field_formulas():
 def test_batch_tuning_validation():
     bp = fzgpu.BatchPlan([(5000, 1, 1), (64, 64, 64)], workspace=False)
     t = bp.tuning
-    assert t == {"range_chunk": 65536, "xcd_order": [1, 1, 1], "gather_waves": 16, "groups": 1}
-    bp.set_tuning(range_chunk=16384, xcd_order=(0, 1, 0), gather_waves=4, groups=8)
-    assert bp.tuning == {"range_chunk": 16384, "xcd_order": [0, 1, 0], "gather_waves": 4, "groups": 8}
+    assert t == {"range_chunk": 65536, "xcd_order": [1, 1, 1], "gather_waves": 16, "groups": 1,
+                 "wave_tile": [0, 0, 0]}
+    bp.set_tuning(range_chunk=16384, xcd_order=(0, 1, 0), gather_waves=4, groups=8, wave_tile=(1, 0, 1))
+    assert bp.tuning == {"range_chunk": 16384, "xcd_order": [0, 1, 0], "gather_waves": 4, "groups": 8,
+                         "wave_tile": [1, 0, 1]}
     parts16 = bp.sizes.range_parts
     for bad in [dict(range_chunk=1000), dict(gather_waves=3), dict(xcd_order=(2, 0, 0)), dict(groups=3),
-                dict(groups=0)]:
+                dict(groups=0), dict(wave_tile=(0, 2, 0))]:
         with pytest.raises(fzgpu.FzError) as e:
             bp.set_tuning(**bad)
         assert e.value.status == 1

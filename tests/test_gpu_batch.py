@@ -108,15 +108,16 @@ def test_same_bytes_under_every_tuning():
     d = [_to_dev(x) for x in xs]
     base_out, base_res = plan.compress(d, ebs)
     base = [_field_bytes(base_out, base_res, i) for i in range(len(MIX))]
-    for chunk, xcd, gw, ng in [(16384, (0, 0, 0), 1, 1), (262144, (1, 0, 1), 4, 2), (65536, (0, 1, 0), 2, 4),
-                               (16384, (1, 1, 1), 8, 16), (65536, (1, 1, 1), 16, 8)]:
-        say(f"tuning chunk={chunk} xcd={xcd} gather_waves={gw} groups={ng}")
-        plan.set_tuning(range_chunk=chunk, xcd_order=xcd, gather_waves=gw, groups=ng)
+    for chunk, xcd, gw, ng, wt in [(16384, (0, 0, 0), 1, 1, (1, 1, 1)), (262144, (1, 0, 1), 4, 2, (0, 1, 0)),
+                                   (65536, (0, 1, 0), 2, 4, (1, 0, 1)), (16384, (1, 1, 1), 8, 16, (1, 1, 1)),
+                                   (65536, (1, 1, 1), 16, 8, (0, 0, 0)), (65536, (1, 1, 1), 16, 1, (1, 1, 1))]:
+        say(f"tuning chunk={chunk} xcd={xcd} gather_waves={gw} groups={ng} wave_tile={wt}")
+        plan.set_tuning(range_chunk=chunk, xcd_order=xcd, gather_waves=gw, groups=ng, wave_tile=wt)
         out, res = plan.compress(d, ebs)
         assert res == base_res
         for i in range(len(MIX)):
             p, t = _field_bytes(out, res, i)
-            assert np.array_equal(p, base[i][0]) and np.array_equal(t, base[i][1]), (i, chunk, xcd, gw, ng)
+            assert np.array_equal(p, base[i][0]) and np.array_equal(t, base[i][1]), (i, chunk, xcd, gw, ng, wt)
 
 
 def test_failed_fields_do_not_disturb_the_others():

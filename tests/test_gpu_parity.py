@@ -46,12 +46,13 @@ SHAPES = [(5000, 1, 1), (4096, 1, 1), (2047, 1, 1), (2048, 1, 1), (9, 1, 1), (10
           (7, 1, 900), (3601, 37, 1), (45, 46, 47), (9, 50, 40), (13, 1, 700), (8, 300, 3), (1030, 9, 11)]
 
 
-def _check_against_oracle(x, dims, eb, mode, fused, length=None):
+def _check_against_oracle(x, dims, eb, mode, fused, length=None, wave_tile=0):
     n = x.size
-    say(f"case dims={dims} len={n} eb={eb} mode={mode} fused={fused}")
+    say(f"case dims={dims} len={n} eb={eb} mode={mode} fused={fused} wave_tile={wave_tile}")
     ref = O.compress(x, dims, eb, mode)
     assert ref["status"] == 0
     plan = fzgpu.Plan(*dims, length=n, mode=mode)
+    plan.set_tile_layout(wave_tile)
     payload, tail, r = plan.compress(_to_dev(x), eb, fused=fused)
     info = ref["info"]
     for k_gpu, k_ref in [("nblocks", "K"), ("payload_bytes", "payload_bytes"), ("overflow", "overflow"),
@@ -67,9 +68,10 @@ def _check_against_oracle(x, dims, eb, mode, fused, length=None):
 
 @pytest.mark.parametrize("dims", SHAPES)
 @pytest.mark.parametrize("mode", [O.MODE_FIXED, O.MODE_LITERAL])
-def test_fused_compress_bit_exact(dims, mode):
+@pytest.mark.parametrize("wave_tile", [0, 1])
+def test_fused_compress_bit_exact(dims, mode, wave_tile):
     x = _field(dims, sum(dims) + mode)
-    _check_against_oracle(x, dims, 1e-3, mode, fused=True)
+    _check_against_oracle(x, dims, 1e-3, mode, fused=True, wave_tile=wave_tile)
 
 
 @pytest.mark.parametrize("dims", SHAPES)
@@ -183,10 +185,11 @@ def test_full_size_configs_bit_exact(name):
     assert ref["status"] == 0
     plan = fzgpu.Plan(*dims)
     d = _to_dev(x)
-    for fused in (True, False):
+    for fused, wave_tile in ((True, 0), (True, 1), (False, 0)):
+        plan.set_tile_layout(wave_tile)
         payload, tail, r = plan.compress(d, eb, fused=fused)
         assert r["payload_bytes"] == ref["info"]["payload_bytes"]
-        assert np.array_equal(_np(payload), ref["payload"]), fused
+        assert np.array_equal(_np(payload), ref["payload"]), (fused, wave_tile)
         assert np.array_equal(_np(tail, np.uint16), ref["tail"])
     xr = _np(plan.decompress(payload, tail, r["ebx2"]))
     tol = r["eb_abs"] + 4 * 2.0**-24 * float(np.abs(x).max())
@@ -203,7 +206,8 @@ PACKED_LIMIT = {1: 16383.0, 2: 8191.0, 3: 4095.0}
 @pytest.mark.parametrize("dims", [(5000, 1, 1), (300, 77, 1), (301, 77, 1), (40, 30, 20), (41, 30, 20),
                                   (9, 50, 40)])
 @pytest.mark.parametrize("scale", [0.99, 1.01, 300.0, 1000.0, 3000.0])
-def test_fused_paths_around_the_packed_bound(dims, scale):
+@pytest.mark.parametrize("wave_tile", [0, 1])
+def test_fused_paths_around_the_packed_bound(dims, scale, wave_tile):
     """Uniform +-1 data, eb chosen so that max|x * ebx2_r| sits just inside / outside the
     packed 16-bit bound (deltas up to ~2^15), in the 3D column-form band (|s| < 2^21), in
     the reference-order float band (|s| < 2^23) and beyond 2^23 (guarded generic path)."""
@@ -213,6 +217,6 @@ def test_fused_paths_around_the_packed_bound(dims, scale):
     nd = 1 if dims[1] == dims[2] == 1 else (2 if dims[2] == 1 else 3)
     amax, rng_v = float(np.abs(x).max()), float(x.max() - x.min())
     eb = amax / (2.0 * rng_v * PACKED_LIMIT[nd] * scale)
-    _check_against_oracle(x, dims, eb, O.MODE_FIXED, fused=True)
+    _check_against_oracle(x, dims, eb, O.MODE_FIXED, fused=True, wave_tile=wave_tile)
     if nd == 1:
-        _check_against_oracle(x, dims, eb, O.MODE_LITERAL, fused=True)
+        _check_against_oracle(x, dims, eb, O.MODE_LITERAL, fused=True, wave_tile=wave_tile)
