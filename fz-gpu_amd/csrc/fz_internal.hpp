@@ -24,12 +24,12 @@ struct Ctr {
 constexpr uint32_t kScanChunk = 4096;  // items per workgroup of the u32 scan
 constexpr uint32_t kGatherWG = 256;  // max workgroups of the compress scan + gather
 
-// partial-range workgroups: a multiple of 8 (one slice per XCD region), ~16K floats each
 // partial-range workgroups (1024 threads each): a multiple of 8 (one slice per XCD region),
-// ~64K floats each, at most 256 (the tile kernel reduces one partial per thread)
+// at least ~16K floats each, at most 256 (the tile kernel reduces one partial per thread), so
+// every CU takes a slice once the field has 4M floats
 inline uint32_t range_grid(uint32_t len)
 {
-    uint64_t g = ((uint64_t)len + 65535) / 65536;
+    uint64_t g = ((uint64_t)len + 16383) / 16384;
     g = (g + 7) & ~7ull;
     if (g < 8) g = 8;
     if (g > kRangeMaxWG) g = kRangeMaxWG;

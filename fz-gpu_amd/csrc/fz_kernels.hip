@@ -70,7 +70,7 @@ __global__ __launch_bounds__(kRangeWG) void k_range_part(const float* __restrict
     const uint32_t u_lo = r_lo + (uint32_t)(((uint64_t)(r_hi - r_lo) * i) / per);
     const uint32_t u_hi = r_lo + (uint32_t)(((uint64_t)(r_hi - r_lo) * (i + 1)) / per);
     const uint32_t f_lo = min(u_lo * (uint32_t)kTileCodes, g.len), f_hi = min(u_hi * (uint32_t)kTileCodes, g.len);
-    range_span<kRangeWG>(in, f_lo, f_hi, part + blockIdx.x);
+    range_span<kRangeWG, 8>(in, f_lo, f_hi, part + blockIdx.x);
 }
 
 __global__ __launch_bounds__(kWG) void k_range_final(const float4* __restrict__ part, uint32_t nparts, Geom g,
@@ -188,7 +188,6 @@ __global__ __launch_bounds__(kGatherThreads) void k_scan_gather(const uint32_t* 
     __shared__ uint32_t sh_w[kGW];
     __shared__ uint32_t sh_excl;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    if (res->status != ST_SUCCESS) return;
     const uint32_t wg = blockIdx.x;
     const uint32_t t_lo = wg * tpw, t_hi = min(T, t_lo + tpw);
     // exclusive prefix over the waves' partial sums (in every thread), and their total
@@ -203,6 +202,69 @@ __global__ __launch_bounds__(kGatherThreads) void k_scan_gather(const uint32_t* 
         total = tot;
         return pre;
     };
+    auto publish_total = [&](uint32_t excl, uint32_t agg) {
+        if (wg == gridDim.x - 1) {
+            res->nblocks = (uint64_t)excl + agg;
+            res->payload_bytes = 4ull * W + 16ull * ((uint64_t)excl + agg);
+        }
+    };
+    if (t_hi - t_lo <= (uint32_t)kGatherThreads) {
+        // One run: the block sources depend only on the run's own counts, so their loads are
+        // issued before the look-back and its latency overlaps theirs.
+        const uint32_t t = t_lo + tid;
+        const uint32_t cnt = t < t_hi ? counts[t] : 0u;  // issued before the status is known
+        if (res->status != ST_SUCCESS) return;
+        const uint32_t incl = wave_incl_scan(cnt);
+        if (lane == 63) sh_w[wv] = incl;
+        __syncthreads();
+        uint32_t run;
+        sh_off[tid] = wave_prefix(run) + incl - cnt;
+        __syncthreads();
+        auto source = [&](uint32_t j) {  // staging offset of block j of the run
+            uint32_t lo = 0, hi = kGatherThreads - 1;  // last tile of the run whose first block is <= j
+#pragma unroll
+            for (int it = 0; it < 10; it++) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (sh_off[mid] <= j) lo = mid;
+                else hi = mid - 1;
+            }
+            return (t_lo + lo) * 4096u + 16u * (j - sh_off[lo]);
+        };
+        uint4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint32_t j = tid + u * kGatherThreads;
+            if (j < run) v[u] = *reinterpret_cast<const uint4*>(staging + source(j));
+        }
+        if (wv == 0) {
+            const uint32_t excl = lookback_wide<4>(states, wg, run);
+            if (lane == 0) {
+                sh_excl = excl;
+                publish_total(excl, run);
+            }
+        }
+        __syncthreads();
+        uint8_t* out = payload + 4ull * W + 16ull * sh_excl;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint32_t j = tid + u * kGatherThreads;
+            if (j < run) st16(out + 16ull * j, v[u]);
+        }
+        for (uint32_t j0 = 4 * kGatherThreads; j0 < run; j0 += 4 * kGatherThreads) {
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t j = j0 + tid + u * kGatherThreads;
+                if (j < run) v[u] = *reinterpret_cast<const uint4*>(staging + source(j));
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t j = j0 + tid + u * kGatherThreads;
+                if (j < run) st16(out + 16ull * j, v[u]);
+            }
+        }
+        return;
+    }
+    if (res->status != ST_SUCCESS) return;
     uint32_t part = 0;
     for (uint32_t t = t_lo + tid; t < t_hi; t += kGatherThreads) part += counts[t];
     part = wave_sum(part);
@@ -214,10 +276,7 @@ __global__ __launch_bounds__(kGatherThreads) void k_scan_gather(const uint32_t* 
         const uint32_t excl = lookback_wide<4>(states, wg, agg);
         if (lane == 0) {
             sh_excl = excl;
-            if (wg == gridDim.x - 1) {
-                res->nblocks = (uint64_t)excl + agg;
-                res->payload_bytes = 4ull * W + 16ull * ((uint64_t)excl + agg);
-            }
+            publish_total(excl, agg);
         }
     }
     __syncthreads();

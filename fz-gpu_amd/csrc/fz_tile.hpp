@@ -65,7 +65,7 @@ __device__ __forceinline__ uint32_t wave_upper(const uint32_t* __restrict__ a, u
 // min / max / non-finite flag of in[f_lo, f_hi), reduced over the NT threads of the
 // workgroup; thread 0 stores (lo, hi, bad-as-bits, 0) to *out.  float4 loads over the
 // 16-byte-aligned middle, scalar loads at the edges.
-template <int NT>
+template <int NT, uint32_t U = 4>  // U float4 loads in flight per thread
 __device__ __forceinline__ void range_span(const float* __restrict__ in, uint32_t f_lo, uint32_t f_hi,
                                            float4* __restrict__ out)
 {
@@ -76,7 +76,6 @@ __device__ __forceinline__ void range_span(const float* __restrict__ in, uint32_
     uint32_t bad = 0;
     const uint32_t v_lo = (f_lo + 3) / 4, v_hi = f_hi / 4;  // whole float4s inside [f_lo, f_hi)
     const float4*  in4 = reinterpret_cast<const float4*>(in);
-    constexpr uint32_t U = 4;
     if (v_lo < v_hi) {
         for (uint32_t base = v_lo + tid; base < v_hi; base += NT * U) {
             float4 v[U];

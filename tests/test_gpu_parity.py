@@ -220,3 +220,26 @@ def test_fused_paths_around_the_packed_bound(dims, scale, wave_tile):
     _check_against_oracle(x, dims, eb, O.MODE_FIXED, fused=True, wave_tile=wave_tile)
     if nd == 1:
         _check_against_oracle(x, dims, eb, O.MODE_LITERAL, fused=True, wave_tile=wave_tile)
+
+
+# ---------------------------------------------------------------- more tiles than one gather run
+
+def test_field_beyond_one_gather_run():
+    """A 2.2 GB field (T = 269,531 tiles > 256 x 1024): every placement workgroup of the fused
+    compress walks several runs of 1024 tiles.  Its archive must equal the plain pipeline's,
+    byte for byte, and decompress within the error bound."""
+    import torch
+    dims = (24000, 23000, 1)
+    n = dims[0] * dims[1]
+    assert n // 2048 > 256 * 1024
+    say(f"large field dims={dims} n={n}")
+    x = D.generate_gpu("peaks", dims, 77)
+    plan = fzgpu.Plan(*dims)
+    pf, tf, rf = plan.compress(x, 1e-3, fused=True)
+    pp, tp, rp = plan.compress(x, 1e-3, fused=False)
+    assert rf == rp
+    assert torch.equal(pf, pp) and torch.equal(tf, tp)
+    xr = plan.decompress(pf, tf, rf["ebx2"])
+    err = float((xr - x).abs().max())
+    assert err <= rf["eb_abs"] + 4 * 2.0**-24 * float(x.abs().max())
+    say(f"  CR {4 * n / rf['payload_bytes']:.3f} maxerr/eb {err / rf['eb_abs']:.6f}")
