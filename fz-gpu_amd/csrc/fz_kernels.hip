@@ -113,7 +113,7 @@ __global__ __launch_bounds__(kWG) void k_tile_compress(const float* __restrict__
                                                        const float4* __restrict__ part, uint32_t nparts,
                                                        fz_result* res, uint8_t* __restrict__ payload,
                                                        uint8_t* __restrict__ staging, uint32_t* __restrict__ counts,
-                                                       uint16_t* __restrict__ tail)
+                                                       uint16_t* __restrict__ tail, uint32_t* run_sums, uint32_t tpw)
 {
     const bool is_tail = blockIdx.x >= g.T;
     // XCD-aware order: an XCD walks its eighth of the field
@@ -135,7 +135,8 @@ __global__ __launch_bounds__(kWG) void k_tile_compress(const float* __restrict__
         }
         return;
     }
-    tile_encode<NDIM>(in, g, tile, gp, R, e.r, e.level, bwords, staging + (size_t)tile * 4096, counts + tile, ovf);
+    tile_encode<NDIM>(in, g, tile, gp, R, e.r, e.level, bwords, staging + (size_t)tile * 4096, counts + tile, ovf,
+                      run_sums + tile / tpw);
 }
 
 
@@ -146,7 +147,7 @@ __global__ __launch_bounds__(kWG) void k_tile_compress_w(const float* __restrict
                                                          const float4* __restrict__ part, uint32_t nparts,
                                                          fz_result* res, uint8_t* __restrict__ payload,
                                                          uint8_t* __restrict__ staging, uint32_t* __restrict__ counts,
-                                                         uint16_t* __restrict__ tail)
+                                                         uint16_t* __restrict__ tail, uint32_t* run_sums, uint32_t tpw)
 {
     __shared__ uint32_t sh[4][32 * kLdsPitch];
     const uint32_t wv = threadIdx.x >> 6;
@@ -169,137 +170,74 @@ __global__ __launch_bounds__(kWG) void k_tile_compress_w(const float* __restrict
         return;
     }
     tile_encode_w<NDIM>(in, g, unit, gp, R, e.r, e.level, bwords, staging + (size_t)unit * 4096, counts + unit, ovf,
-                        sh[wv]);
+                        sh[wv], run_sums + unit / tpw);
 }
 
-// Scan of the tile counts + placement of the staged blocks.  The grid is small (<= kGatherWG
-// workgroups, each a contiguous run of tiles) and chains its look-back in blockIdx order: at
-// most kGatherWG - 1 workgroups can be waiting, far fewer than the device holds, so a
-// workgroup that has not started yet always finds a free slot and the chain completes.
-constexpr int kGatherThreads = 1024;  // threads per scan + gather workgroup
+// Placement of the staged blocks.  Workgroup b owns the run of tiles [b tpw, (b+1) tpw) (at
+// most kGatherWG runs); the tile kernel has added each tile's block count to run_sums[run], so
+// the blocks before run b are the sum of run_sums[0, b): one load per thread, no chain between
+// workgroups.  Within the run, a scan of the tile counts gives each staged block its rank.
+constexpr int kGatherThreads = 1024;  // threads per placement workgroup
 constexpr int kGW = kGatherThreads / 64;
 
 __global__ __launch_bounds__(kGatherThreads) void k_scan_gather(const uint32_t* __restrict__ counts, uint32_t T,
                                                                 uint32_t tpw, const uint8_t* __restrict__ staging,
-                                                                uint8_t* __restrict__ payload, uint64_t* states,
-                                                                fz_result* res, uint32_t W)
+                                                                uint8_t* __restrict__ payload,
+                                                                const uint32_t* __restrict__ run_sums, fz_result* res,
+                                                                uint32_t W)
 {
     __shared__ uint32_t sh_off[kGatherThreads];
-    __shared__ uint32_t sh_w[kGW];
-    __shared__ uint32_t sh_excl;
+    __shared__ uint32_t sh_w[kGW], sh_b[kGW];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const uint32_t wg = blockIdx.x;
     const uint32_t t_lo = wg * tpw, t_hi = min(T, t_lo + tpw);
-    // exclusive prefix over the waves' partial sums (in every thread), and their total
-    auto wave_prefix = [&](uint32_t& total) {
+    // exclusive prefix over the waves' values in sh (in every thread), and their total
+    auto wave_prefix = [&](const uint32_t* sh, uint32_t& total) {
         uint32_t pre = 0, tot = 0;
 #pragma unroll
         for (int i = 0; i < kGW; i++) {
-            const uint32_t v = sh_w[i];
+            const uint32_t v = sh[i];
             pre += (uint32_t)i < wv ? v : 0u;
             tot += v;
         }
         total = tot;
         return pre;
     };
-    auto publish_total = [&](uint32_t excl, uint32_t agg) {
-        if (wg == gridDim.x - 1) {
-            res->nblocks = (uint64_t)excl + agg;
-            res->payload_bytes = 4ull * W + 16ull * ((uint64_t)excl + agg);
-        }
-    };
-    if (t_hi - t_lo <= (uint32_t)kGatherThreads) {
-        // One run: the block sources depend only on the run's own counts, so their loads are
-        // issued before the look-back and its latency overlaps theirs.
-        const uint32_t t = t_lo + tid;
-        const uint32_t cnt = t < t_hi ? counts[t] : 0u;  // issued before the status is known
-        if (res->status != ST_SUCCESS) return;
-        const uint32_t incl = wave_incl_scan(cnt);
-        if (lane == 63) sh_w[wv] = incl;
-        __syncthreads();
-        uint32_t run;
-        sh_off[tid] = wave_prefix(run) + incl - cnt;
-        __syncthreads();
-        auto source = [&](uint32_t j) {  // staging offset of block j of the run
-            uint32_t lo = 0, hi = kGatherThreads - 1;  // last tile of the run whose first block is <= j
-#pragma unroll
-            for (int it = 0; it < 10; it++) {
-                const uint32_t mid = (lo + hi + 1) >> 1;
-                if (sh_off[mid] <= j) lo = mid;
-                else hi = mid - 1;
-            }
-            return (t_lo + lo) * 4096u + 16u * (j - sh_off[lo]);
-        };
-        uint4 v[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const uint32_t j = tid + u * kGatherThreads;
-            if (j < run) v[u] = *reinterpret_cast<const uint4*>(staging + source(j));
-        }
-        if (wv == 0) {
-            const uint32_t excl = lookback_wide<4>(states, wg, run);
-            if (lane == 0) {
-                sh_excl = excl;
-                publish_total(excl, run);
-            }
-        }
-        __syncthreads();
-        uint8_t* out = payload + 4ull * W + 16ull * sh_excl;
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const uint32_t j = tid + u * kGatherThreads;
-            if (j < run) st16(out + 16ull * j, v[u]);
-        }
-        for (uint32_t j0 = 4 * kGatherThreads; j0 < run; j0 += 4 * kGatherThreads) {
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const uint32_t j = j0 + tid + u * kGatherThreads;
-                if (j < run) v[u] = *reinterpret_cast<const uint4*>(staging + source(j));
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const uint32_t j = j0 + tid + u * kGatherThreads;
-                if (j < run) st16(out + 16ull * j, v[u]);
-            }
-        }
-        return;
-    }
+    // first run of up to 1024 tiles and the blocks of the runs before this one, loaded before
+    // the status is known (both buffers are always valid memory)
+    uint32_t cnt = t_lo + tid < t_hi ? counts[t_lo + tid] : 0u;
+    const uint32_t before = tid < wg ? run_sums[tid] : 0u;  // wg < kGatherWG <= kGatherThreads
     if (res->status != ST_SUCCESS) return;
-    uint32_t part = 0;
-    for (uint32_t t = t_lo + tid; t < t_hi; t += kGatherThreads) part += counts[t];
-    part = wave_sum(part);
-    if (lane == 0) sh_w[wv] = part;
-    __syncthreads();
-    uint32_t agg;
-    (void)wave_prefix(agg);
-    if (wv == 0) {
-        const uint32_t excl = lookback_wide<4>(states, wg, agg);
-        if (lane == 0) {
-            sh_excl = excl;
-            publish_total(excl, agg);
-        }
-    }
-    __syncthreads();
-    uint8_t* out = payload + 4ull * W + 16ull * sh_excl;
-    uint32_t base = 0;
-    // runs of 1024 tiles: offsets in LDS, then 4 independent blocks per thread per step
+    const uint32_t b = wave_sum(before);
+    if (lane == 0) sh_b[wv] = b;
+    uint32_t excl = 0, base = 0;
     for (uint32_t r0 = t_lo; r0 < t_hi; r0 += kGatherThreads) {
-        const uint32_t t = r0 + tid;
-        const uint32_t cnt = t < t_hi ? counts[t] : 0u;
+        if (r0 != t_lo) {
+            __syncthreads();  // the previous run's sh_w / sh_off reads are done
+            cnt = r0 + tid < t_hi ? counts[r0 + tid] : 0u;
+        }
         const uint32_t incl = wave_incl_scan(cnt);
-        __syncthreads();
         if (lane == 63) sh_w[wv] = incl;
         __syncthreads();
+        if (r0 == t_lo) {
+            uint32_t e;
+            (void)wave_prefix(sh_b, e);
+            excl = e;
+            if (wg == gridDim.x - 1 && tid == 0) {
+                const uint64_t K = (uint64_t)excl + run_sums[wg];
+                res->nblocks = K;
+                res->payload_bytes = 4ull * W + 16ull * K;
+            }
+        }
         uint32_t run;
-        sh_off[tid] = wave_prefix(run) + incl - cnt;
+        sh_off[tid] = wave_prefix(sh_w, run) + incl - cnt;
         __syncthreads();
+        uint8_t* out = payload + 4ull * W + 16ull * ((uint64_t)excl + base);
         for (uint32_t j0 = 0; j0 < run; j0 += 4 * kGatherThreads) {
             uint32_t src[4];
-            bool     ok[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const uint32_t j = j0 + tid + u * kGatherThreads;
-                ok[u] = j < run;
                 uint32_t lo = 0, hi = kGatherThreads - 1;  // last tile of the run whose first block is <= j
 #pragma unroll
                 for (int it = 0; it < 10; it++) {
@@ -312,13 +250,14 @@ __global__ __launch_bounds__(kGatherThreads) void k_scan_gather(const uint32_t* 
             uint4 v[4];
 #pragma unroll
             for (int u = 0; u < 4; u++)
-                if (ok[u]) v[u] = *reinterpret_cast<const uint4*>(staging + src[u]);
+                if (j0 + tid + u * kGatherThreads < run) v[u] = *reinterpret_cast<const uint4*>(staging + src[u]);
 #pragma unroll
-            for (int u = 0; u < 4; u++)
-                if (ok[u]) st16(out + 16ull * (base + j0 + tid + u * kGatherThreads), v[u]);
+            for (int u = 0; u < 4; u++) {
+                const uint32_t j = j0 + tid + u * kGatherThreads;
+                if (j < run) st16(out + 16ull * j, v[u]);
+            }
         }
         base += run;
-        __syncthreads();
     }
 }
 
@@ -339,26 +278,30 @@ hipError_t launch_compress(const float* in, const Geom& g, double eb_rel, float4
     hipError_t e = launch_timed(k_range_part, dim3(nparts), dim3(kRangeWG), s, ev[0], ev[1], in, g, part, states,
                                 G + 1, res);
     if (e != hipSuccess) return e;
+    // per-run block sums of the placement kernel, in the look-back state words k_range_part
+    // clears (G + 1 u64 >= G u32)
+    uint32_t* const sums = reinterpret_cast<uint32_t*>(states);
+    const uint32_t  tpw = gather_tpw(g.T);
     if (wave_tile) {
         const dim3 grid((g.T + 4) / 4), blk(kWG);
         switch (g.ndim) {
-            case 1: e = launch_timed(k_tile_compress_w<1>, grid, blk, s, ev[2], ev[3], in, g, eb_rel, (const float4*)part, nparts, res, payload, staging, counts, tail); break;
-            case 2: e = launch_timed(k_tile_compress_w<2>, grid, blk, s, ev[2], ev[3], in, g, eb_rel, (const float4*)part, nparts, res, payload, staging, counts, tail); break;
-            default: e = launch_timed(k_tile_compress_w<3>, grid, blk, s, ev[2], ev[3], in, g, eb_rel, (const float4*)part, nparts, res, payload, staging, counts, tail); break;
+            case 1: e = launch_timed(k_tile_compress_w<1>, grid, blk, s, ev[2], ev[3], in, g, eb_rel, (const float4*)part, nparts, res, payload, staging, counts, tail, sums, tpw); break;
+            case 2: e = launch_timed(k_tile_compress_w<2>, grid, blk, s, ev[2], ev[3], in, g, eb_rel, (const float4*)part, nparts, res, payload, staging, counts, tail, sums, tpw); break;
+            default: e = launch_timed(k_tile_compress_w<3>, grid, blk, s, ev[2], ev[3], in, g, eb_rel, (const float4*)part, nparts, res, payload, staging, counts, tail, sums, tpw); break;
         }
         if (e != hipSuccess || g.T == 0) return e;
         return launch_timed(k_scan_gather, dim3(G), dim3(kGatherThreads), s, ev[4], ev[5], (const uint32_t*)counts,
-                            g.T, gather_tpw(g.T), (const uint8_t*)staging, payload, states, res, g.W);
+                            g.T, tpw, (const uint8_t*)staging, payload, (const uint32_t*)sums, res, g.W);
     }
     const dim3 grid(g.T + 1), blk(kWG);
     switch (g.ndim) {
-        case 1: e = launch_timed(k_tile_compress<1>, grid, blk, s, ev[2], ev[3], in, g, eb_rel, (const float4*)part, nparts, res, payload, staging, counts, tail); break;
-        case 2: e = launch_timed(k_tile_compress<2>, grid, blk, s, ev[2], ev[3], in, g, eb_rel, (const float4*)part, nparts, res, payload, staging, counts, tail); break;
-        default: e = launch_timed(k_tile_compress<3>, grid, blk, s, ev[2], ev[3], in, g, eb_rel, (const float4*)part, nparts, res, payload, staging, counts, tail); break;
+        case 1: e = launch_timed(k_tile_compress<1>, grid, blk, s, ev[2], ev[3], in, g, eb_rel, (const float4*)part, nparts, res, payload, staging, counts, tail, sums, tpw); break;
+        case 2: e = launch_timed(k_tile_compress<2>, grid, blk, s, ev[2], ev[3], in, g, eb_rel, (const float4*)part, nparts, res, payload, staging, counts, tail, sums, tpw); break;
+        default: e = launch_timed(k_tile_compress<3>, grid, blk, s, ev[2], ev[3], in, g, eb_rel, (const float4*)part, nparts, res, payload, staging, counts, tail, sums, tpw); break;
     }
     if (e != hipSuccess || g.T == 0) return e;
     return launch_timed(k_scan_gather, dim3(G), dim3(kGatherThreads), s, ev[4], ev[5], (const uint32_t*)counts, g.T,
-                        gather_tpw(g.T), (const uint8_t*)staging, payload, states, res, g.W);
+                        tpw, (const uint8_t*)staging, payload, (const uint32_t*)sums, res, g.W);
 }
 
 // ============================================================================ plain stages

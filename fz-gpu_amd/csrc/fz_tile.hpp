@@ -304,13 +304,16 @@ __device__ __forceinline__ void tail_unit(const float* __restrict__ in, const Ge
 //   bwords[8 tile .. 8 tile + 8)  the tile's bitflag words (fz.cu:95-99, LSB first)
 //   slot[0, 16 count)              its nonzero 16-byte blocks in order (fz.cu:56-57 order)
 //   *count_out                     count
+//   *run_sum                       += count (when run_sum is not null; relaxed atomic, read
+//                                  only after the kernel ends)
 // `R` holds the tile's raw loads when gp is 2 or 3 (raw_prefetch, issued by the caller before
 // its other latency-bound work).
 template <int NDIM>
 __device__ __forceinline__ void tile_encode(const float* __restrict__ in, const Geom& g, uint32_t tile, int gp,
                                             const RawTile<NDIM>& R, float r, int level,
                                             uint32_t* __restrict__ bwords, uint8_t* __restrict__ slot,
-                                            uint32_t* __restrict__ count_out, unsigned long long* ovf_out)
+                                            uint32_t* __restrict__ count_out, unsigned long long* ovf_out,
+                                            uint32_t* run_sum = nullptr)
 {
     __shared__ uint32_t sh[32 * kLdsPitch];
     __shared__ uint32_t sh_wcnt[4], sh_wovf[4];
@@ -364,7 +367,9 @@ __device__ __forceinline__ void tile_encode(const float* __restrict__ in, const 
     const uint32_t c0 = sh_wcnt[0], c1 = sh_wcnt[1], c2 = sh_wcnt[2], c3 = sh_wcnt[3];
     const uint32_t woff = (wv > 0 ? c0 : 0u) + (wv > 1 ? c1 : 0u) + (wv > 2 ? c2 : 0u);
     if (tid == 0) {
-        *count_out = c0 + c1 + c2 + c3;
+        const uint32_t total = c0 + c1 + c2 + c3;
+        *count_out = total;
+        if (run_sum && total) atomicAdd(run_sum, total);
         const uint32_t o = sh_wovf[0] + sh_wovf[1] + sh_wovf[2] + sh_wovf[3];
         if (o) atomicAdd(ovf_out, (unsigned long long)o);
     }
@@ -422,7 +427,7 @@ __device__ __forceinline__ void tile_encode_w(const float* __restrict__ in, cons
                                               RawTile<NDIM> (&R)[2], float r, int level,
                                               uint32_t* __restrict__ bwords, uint8_t* __restrict__ slot,
                                               uint32_t* __restrict__ count_out, unsigned long long* ovf_out,
-                                              uint32_t* __restrict__ sh)
+                                              uint32_t* __restrict__ sh, uint32_t* run_sum = nullptr)
 {
     const uint32_t lane = lane_id(), L8 = lane & 7, k8 = lane >> 3;
     const uint32_t t0 = tile * (uint32_t)kTileCodes;
@@ -451,7 +456,10 @@ __device__ __forceinline__ void tile_encode_w(const float* __restrict__ in, cons
         if (f) *reinterpret_cast<uint4*>(slot + 16 * (base + rank_below(mask))) = blk;
         base += (uint32_t)__popcll(mask);
     }
-    if (lane == 0) *count_out = base;
+    if (lane == 0) {
+        *count_out = base;
+        if (run_sum && base) atomicAdd(run_sum, base);
+    }
     if (ballot64(ovf != 0) != 0) {
         ovf = wave_sum(ovf);
         if (lane == 0) atomicAdd(ovf_out, (unsigned long long)ovf);
