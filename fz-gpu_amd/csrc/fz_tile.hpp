@@ -119,6 +119,7 @@ __device__ __forceinline__ void range_span(const float* __restrict__ in, uint32_
 //   eb_abs = eb_rel * (double)(max - min);  ebx2 = eb_abs * 2;  ebx2_r = 1 / ebx2  (double),
 //   both narrowed to float.  `level` bounds every |s| = |round(x * ebx2_r)| (|fl(x r)| <=
 //   fl(max|x| r) by monotone rounding) and picks the dual-quant paths a tile may take:
+//     4  as 3, and min >= 0: packed path with the two-instruction rounding (pq_pos)
 //     3  |s| <= 16383 / 8191 / 4095 (1D / 2D / 3D): packed 16-bit path
 //     2  |s| < 2^21: 3D column form in float
 //     1  |s| < 2^23: fast float paths in the reference's operation order
@@ -143,7 +144,7 @@ __device__ __forceinline__ EbParams eb_params(float lo, float hi, uint32_t bad, 
     e.ebx2 = (float)e2;
     e.status = bad ? ST_NONFINITE : (range == 0.0f ? ST_CONSTANT : ST_SUCCESS);
     const float m = fmaxf(__builtin_fabsf(lo), __builtin_fabsf(hi)) * e.r;
-    e.level = m < (ndim == 1 ? 16383.0f : (ndim == 2 ? 8191.0f : 4095.0f)) ? 3
+    e.level = m < (ndim == 1 ? 16383.0f : (ndim == 2 ? 8191.0f : 4095.0f)) ? (lo >= 0.0f ? 4 : 3)
               : m < 2097151.0f                                            ? 2
               : m < 8388608.0f                                            ? 1
                                                                           : 0;
@@ -300,6 +301,14 @@ __device__ __forceinline__ void tail_unit(const float* __restrict__ in, const Ge
     if ((tid & 63) == 0 && ovf) atomicAdd(ovf_out, (unsigned long long)ovf);
 }
 
+template <int NDIM, bool AL>
+__device__ __forceinline__ void pk_dispatch(const RawTile<NDIM>& R, uint32_t p0, const Geom& g, float r, int level,
+                                            uint32_t w[4])
+{
+    if (level == 4) raw_compute_pk<NDIM, AL, true>(R, p0, g, r, w);
+    else raw_compute_pk<NDIM, AL, false>(R, p0, g, r, w);
+}
+
 // One full tile: dual-quant -> 32x32 bit transpose -> block flags -> in-tile compaction.
 //   bwords[8 tile .. 8 tile + 8)  the tile's bitflag words (fz.cu:95-99, LSB first)
 //   slot[0, 16 count)              its nonzero 16-byte blocks in order (fz.cu:56-57 order)
@@ -324,18 +333,18 @@ __device__ __forceinline__ void tile_encode(const float* __restrict__ in, const 
     // 1. dual-quant: words 4t..4t+3 of the tile
     uint32_t w[4];
     if constexpr (NDIM == 1) {
-        if (gp == 2 && level >= 3) raw_compute_pk<1, true>(R, p0, g, r, w);
+        if (gp == 2 && level >= 3) pk_dispatch<1, true>(R, p0, g, r, level, w);
         else if (gp == 2 && level >= 1) raw_compute<1>(R, p0, g, r, w, ovf);
         else dq8_tile<1>(in, p0, g, r, level >= 1 ? gp : 0, w, ovf);
     } else if constexpr (NDIM == 2) {
-        if (gp == 2 && level >= 3) raw_compute_pk<2, true>(R, p0, g, r, w);
-        else if (gp == 3 && level >= 3) raw_compute_pk<2, false>(R, p0, g, r, w);
+        if (gp == 2 && level >= 3) pk_dispatch<2, true>(R, p0, g, r, level, w);
+        else if (gp == 3 && level >= 3) pk_dispatch<2, false>(R, p0, g, r, level, w);
         else if (gp == 2 && level >= 1) raw_compute<2>(R, p0, g, r, w, ovf);
         else if (gp == 3 && level >= 1) raw_compute_u(R, p0, g, r, w, ovf);
         else dq8_tile<2>(in, p0, g, r, level >= 1 ? gp : 0, w, ovf);
     } else {
-        if (gp == 2 && level >= 3) raw_compute_pk<3, true>(R, p0, g, r, w);
-        else if (gp == 3 && level >= 3) raw_compute_pk<3, false>(R, p0, g, r, w);
+        if (gp == 2 && level >= 3) pk_dispatch<3, true>(R, p0, g, r, level, w);
+        else if (gp == 3 && level >= 3) pk_dispatch<3, false>(R, p0, g, r, level, w);
         else if (gp == 2 && level >= 2) raw_compute_q3<true>(R, p0, g, r, w, ovf);
         else if (gp == 3 && level >= 2) raw_compute_q3<false>(R, p0, g, r, w, ovf);
         else if (gp == 2 && level >= 1) raw_compute<3>(R, p0, g, r, w, ovf);
@@ -393,18 +402,18 @@ __device__ __forceinline__ void dq8_level(const float* __restrict__ in, uint32_t
                                           const RawTile<NDIM>& R, float r, int level, uint32_t w[4], uint32_t& ovf)
 {
     if constexpr (NDIM == 1) {
-        if (gp == 2 && level >= 3) raw_compute_pk<1, true>(R, p0, g, r, w);
+        if (gp == 2 && level >= 3) pk_dispatch<1, true>(R, p0, g, r, level, w);
         else if (gp == 2 && level >= 1) raw_compute<1>(R, p0, g, r, w, ovf);
         else dq8_tile<1>(in, p0, g, r, level >= 1 ? gp : 0, w, ovf);
     } else if constexpr (NDIM == 2) {
-        if (gp == 2 && level >= 3) raw_compute_pk<2, true>(R, p0, g, r, w);
-        else if (gp == 3 && level >= 3) raw_compute_pk<2, false>(R, p0, g, r, w);
+        if (gp == 2 && level >= 3) pk_dispatch<2, true>(R, p0, g, r, level, w);
+        else if (gp == 3 && level >= 3) pk_dispatch<2, false>(R, p0, g, r, level, w);
         else if (gp == 2 && level >= 1) raw_compute<2>(R, p0, g, r, w, ovf);
         else if (gp == 3 && level >= 1) raw_compute_u(R, p0, g, r, w, ovf);
         else dq8_tile<2>(in, p0, g, r, level >= 1 ? gp : 0, w, ovf);
     } else {
-        if (gp == 2 && level >= 3) raw_compute_pk<3, true>(R, p0, g, r, w);
-        else if (gp == 3 && level >= 3) raw_compute_pk<3, false>(R, p0, g, r, w);
+        if (gp == 2 && level >= 3) pk_dispatch<3, true>(R, p0, g, r, level, w);
+        else if (gp == 3 && level >= 3) pk_dispatch<3, false>(R, p0, g, r, level, w);
         else if (gp == 2 && level >= 2) raw_compute_q3<true>(R, p0, g, r, w, ovf);
         else if (gp == 3 && level >= 2) raw_compute_q3<false>(R, p0, g, r, w, ovf);
         else if (gp == 2 && level >= 1) raw_compute<3>(R, p0, g, r, w, ovf);

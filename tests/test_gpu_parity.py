@@ -243,3 +243,26 @@ def test_field_beyond_one_gather_run():
     err = float((xr - x).abs().max())
     assert err <= rf["eb_abs"] + 4 * 2.0**-24 * float(x.abs().max())
     say(f"  CR {4 * n / rf['payload_bytes']:.3f} maxerr/eb {err / rf['eb_abs']:.6f}")
+
+
+# ---------------------------------------------------------------- rounding ties
+
+@pytest.mark.parametrize("dims", [(40000, 1, 1), (64, 300, 1), (101, 300, 1), (32, 32, 24), (37, 30, 20)])
+@pytest.mark.parametrize("sign", ["nonneg", "mixed"])
+@pytest.mark.parametrize("wave_tile", [0, 1])
+def test_rounding_ties_on_the_packed_paths(dims, sign, wave_tile):
+    """Value range 1024 and eb 2^-11 make ebx2 = 1 and ebx2_r = 1 exactly, so x * ebx2_r = x:
+    values at k + 0.5 and one ulp to either side put every prequantization on or next to a
+    rounding tie.  Non-negative fields take the two-instruction rounding, mixed-sign fields
+    the copysign form; both must match the oracle's roundf byte for byte."""
+    rng = np.random.default_rng(sum(dims) + (0 if sign == "nonneg" else 1))
+    n = dims[0] * dims[1] * dims[2]
+    base = (rng.integers(0, 1024, n) + 0.5).astype(np.float32)
+    pick = rng.integers(0, 4, n)
+    x = np.where(pick == 1, np.nextafter(base, np.float32(np.inf)),
+                 np.where(pick == 2, np.nextafter(base, np.float32(-np.inf)),
+                          np.where(pick == 3, np.floor(base), base))).astype(np.float32)
+    x[0], x[n // 2] = 0.0, 1024.0
+    if sign == "mixed":
+        x = (x - np.float32(512.0)).astype(np.float32)
+    _check_against_oracle(x, dims, 2.0**-11, O.MODE_FIXED, fused=True, wave_tile=wave_tile)
