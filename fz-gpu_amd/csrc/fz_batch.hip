@@ -115,7 +115,7 @@ __global__ __launch_bounds__(kWG) void k_batch_tile(BatchDev d, uint32_t u_lo, u
     const float*   in = static_cast<const float*>(q.src);
     const int      gp = is_tail ? 0 : geom_path<NDIM>(g, tile * (uint32_t)kTileCodes);
     RawTile<NDIM>  R;
-    raw_prefetch<NDIM>(in, tile * (uint32_t)kTileCodes + 8 * threadIdx.x, g, gp, R);
+    raw_prefetch<NDIM>(in, tile * (uint32_t)kTileCodes, 8 * threadIdx.x, g, gp, R);
     const BParams p = d.prm[f];
     if (p.status != ST_SUCCESS || is_tail) {
         if (threadIdx.x == 0) d.counts[gu] = 0;
@@ -127,8 +127,13 @@ __global__ __launch_bounds__(kWG) void k_batch_tile(BatchDev d, uint32_t u_lo, u
         tail_unit<NDIM>(in, g, p.r, bwords, q.tail, ovf);
         return;
     }
-    tile_encode<NDIM>(in, g, tile, gp, R, p.r, (int)p.level, bwords, d.staging + (size_t)gu * 4096, d.counts + gu,
-                      ovf);
+    uint8_t* const slot = d.staging + (size_t)gu * 4096;
+    if constexpr (NDIM == 3) {  // one branch per path family: see tile_encode
+        if (gp < 2) tile_encode<3, 2>(in, g, tile, gp, R, p.r, (int)p.level, bwords, slot, d.counts + gu, ovf);
+        else tile_encode<3, 1>(in, g, tile, gp, R, p.r, (int)p.level, bwords, slot, d.counts + gu, ovf);
+    } else {
+        tile_encode<NDIM>(in, g, tile, gp, R, p.r, (int)p.level, bwords, slot, d.counts + gu, ovf);
+    }
 }
 
 // One wave per unit, four units per workgroup, no workgroup barrier: every wave finds its field,
@@ -138,7 +143,7 @@ __global__ __launch_bounds__(kWG) void k_batch_tile_w(BatchDev d, uint32_t u_lo,
                                                       uint32_t f_hi, uint32_t xcd)
 {
     __shared__ uint32_t sh[4][32 * kLdsPitch];
-    const uint32_t wv = threadIdx.x >> 6;
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
     const uint32_t ngroups = (nunits + 3) / 4;
     const uint32_t local = 4 * (xcd ? xcd_tile(blockIdx.x, ngroups) : blockIdx.x) + wv;
     if (local >= nunits) return;
@@ -151,7 +156,7 @@ __global__ __launch_bounds__(kWG) void k_batch_tile_w(BatchDev d, uint32_t u_lo,
     const float*   in = static_cast<const float*>(q.src);
     const int      gp = is_tail ? 0 : geom_path<NDIM>(g, tile * (uint32_t)kTileCodes);
     RawTile<NDIM>  R[2];
-    if (gp >= 2) raw_prefetch<NDIM>(in, tile * (uint32_t)kTileCodes + 8 * lane_id(), g, gp, R[0]);
+    if (gp >= 2) raw_prefetch<NDIM>(in, tile * (uint32_t)kTileCodes, 8 * lane_id(), g, gp, R[0]);
     const BParams p = d.prm[f];
     if (p.status != ST_SUCCESS || is_tail) {
         if (lane_id() == 0) d.counts[gu] = 0;

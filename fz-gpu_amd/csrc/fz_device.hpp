@@ -324,7 +324,36 @@ struct RawTile<3> {
     float vw, vnw, vuw, vunw;
 };
 
-// Requires the fast-path geometry (tile_path == 2 ignoring the value range).
+// Raw loads of the thread at offset lo (a multiple of 8, < 2048) of the tile that starts at
+// t0 (uniform across the caller's wave).  The row bases are uniform, so each load is a scalar
+// base, one 32-bit lane offset and a constant.  Requires the fast-path geometry (tile_path == 2
+// ignoring the value range).
+template <int NDIM>
+__device__ __forceinline__ void raw_load_t(const float* __restrict__ in, uint32_t t0, uint32_t lo, const Geom& g,
+                                           RawTile<NDIM>& R)
+{
+    const float* own = in + t0;
+    load8(own + lo, R.v);
+    if constexpr (NDIM == 1) {
+        // lo == 0 starts a 256-block (t0 % 2048 == 0): its west value is never used
+        R.vw = lo ? (own + lo)[-1] : 0.0f;
+    } else {
+        const float* n = own - g.X;
+        load8(n + lo, R.vn);
+        R.vw = (own + lo)[-1];
+        R.vnw = (n + lo)[-1];
+        if constexpr (NDIM == 3) {
+            const float* u = own - g.XY;
+            const float* un = u - g.X;
+            load8(u + lo, R.vu);
+            load8(un + lo, R.vun);
+            R.vuw = (u + lo)[-1];
+            R.vunw = (un + lo)[-1];
+        }
+    }
+}
+
+// As raw_load_t at any position p0 (a multiple of 8).
 template <int NDIM>
 __device__ __forceinline__ void raw_load(const float* __restrict__ in, uint32_t p0, const Geom& g, RawTile<NDIM>& R)
 {
@@ -455,6 +484,88 @@ __device__ __forceinline__ void load8u(const float* __restrict__ p, float v[8])
 }
 
 // Requires X >= 8 and p0 >= X + 1 (2D) / XY + X + 1 (3D).
+template <int NDIM>
+__device__ __forceinline__ void raw_load_ut(const float* __restrict__ in, uint32_t t0, uint32_t lo, const Geom& g,
+                                            RawTile<NDIM>& R)
+{
+    const float* own = in + t0;
+    const float* n = own - g.X;
+    load8(own + lo, R.v);
+    load8u(n + lo, R.vn);
+    R.vw = (own + lo)[-1];
+    R.vnw = (n + lo)[-1];
+    if constexpr (NDIM == 3) {
+        const float* u = own - g.XY;
+        const float* un = u - g.X;
+        load8u(u + lo, R.vu);
+        load8u(un + lo, R.vun);
+        R.vuw = (u + lo)[-1];
+        R.vunw = (un + lo)[-1];
+    }
+}
+
+// ---- generic 3D path on a RawTile
+//
+// The generic 3D loop of dq8<3> (lorenzo_var.cuh:271-283 order, any value range) with its
+// 36 raw values taken from a RawTile<3> that the fast paths' loads filled, so that a kernel
+// holding the tile's raw loads spends no extra registers on it.
+
+// prequantized windows: s[8] own, n / u / un[9] at p - X, p - XY, p - XY - X for
+// p = p0-1 .. p0+7, sw the west of p0
+struct G3 {
+    float s[8], n[9], u[9], un[9], sw;
+};
+
+__device__ __forceinline__ void g3_from_raw(const RawTile<3>& R, float r, G3& q)
+{
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        q.s[i] = prequant(R.v[i], r);
+        q.n[i + 1] = prequant(R.vn[i], r);
+        q.u[i + 1] = prequant(R.vu[i], r);
+        q.un[i + 1] = prequant(R.vun[i], r);
+    }
+    q.sw = prequant(R.vw, r);
+    q.n[0] = prequant(R.vnw, r);
+    q.u[0] = prequant(R.vuw, r);
+    q.un[0] = prequant(R.vunw, r);
+}
+
+// codes of p0 .. p0+7 (p0 < xyz) from the windows, exactly as dq8<3>
+__device__ __forceinline__ void g3_codes(const G3& Q, uint32_t p0, const Geom& g, uint32_t w[4], uint32_t& ovf)
+{
+    const uint32_t q = fdiv(p0, g.divX);
+    uint32_t       x = p0 - q * g.X;
+    uint32_t       zz = fdiv(q, g.divY);
+    uint32_t       y = q - zz * g.Y;
+    uint32_t       c[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const bool  bx = (x & 7u) != 0, by = (y & 7u) != 0, bz = (zz & 7u) != 0;
+        const float west = (i == 0) ? Q.sw : Q.s[i - 1];
+        float       p = (bz && by && bx) ? Q.un[i] : 0.0f;  // NW3 = s[z-1][y-1][x-1]
+        p = p - ((by && bx) ? Q.n[i] : 0.0f);                // A   = s[z][y-1][x-1]
+        p = p - ((bz && bx) ? Q.u[i] : 0.0f);                // B   = s[z-1][y][x-1]
+        p = p - ((bz && by) ? Q.un[i + 1] : 0.0f);           // C   = s[z-1][y-1][x]
+        p = p + (bx ? west : 0.0f);                          // D   = s[z][y][x-1]
+        p = p + (by ? Q.n[i + 1] : 0.0f);                    // E   = s[z][y-1][x]
+        p = p + (bz ? Q.u[i + 1] : 0.0f);                    // F   = s[z-1][y][x]
+        const float d = Q.s[i] - p;
+        const bool  live = p0 + i < g.xyz;
+        c[i] = live ? code_of(d, true) : 0u;
+        ovf += live ? is_ovf(d) : 0u;
+        if (++x == g.X) {
+            x = 0;
+            if (++y == g.Y) {
+                y = 0;
+                ++zz;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) w[k] = c[2 * k] | (c[2 * k + 1] << 16);
+}
+
 template <int NDIM>
 __device__ __forceinline__ void raw_load_u(const float* __restrict__ in, uint32_t p0, const Geom& g, RawTile<NDIM>& R)
 {

@@ -121,7 +121,7 @@ __global__ __launch_bounds__(kWG) void k_tile_compress(const float* __restrict__
     const int      gp = is_tail ? 0 : geom_path<NDIM>(g, tile * (uint32_t)kTileCodes);
     // the tile's loads need no eb: issue them before reducing the range partials
     RawTile<NDIM> R;
-    raw_prefetch<NDIM>(in, tile * (uint32_t)kTileCodes + 8 * threadIdx.x, g, gp, R);
+    raw_prefetch<NDIM>(in, tile * (uint32_t)kTileCodes, 8 * threadIdx.x, g, gp, R);
     const EbParams e = reduce_partials_256(part, nparts, eb_rel, NDIM);
     if (blockIdx.x == 0 && threadIdx.x == 0) write_header(res, e, g);
     if (e.status != ST_SUCCESS) return;  // uniform
@@ -135,8 +135,14 @@ __global__ __launch_bounds__(kWG) void k_tile_compress(const float* __restrict__
         }
         return;
     }
-    tile_encode<NDIM>(in, g, tile, gp, R, e.r, e.level, bwords, staging + (size_t)tile * 4096, counts + tile, ovf,
-                      run_sums + tile / tpw);
+    uint8_t* const  slot = staging + (size_t)tile * 4096;
+    uint32_t* const sum = run_sums + tile / tpw;
+    if constexpr (NDIM == 3) {  // one branch per path family: see tile_encode
+        if (gp < 2) tile_encode<3, 2>(in, g, tile, gp, R, e.r, e.level, bwords, slot, counts + tile, ovf, sum);
+        else tile_encode<3, 1>(in, g, tile, gp, R, e.r, e.level, bwords, slot, counts + tile, ovf, sum);
+    } else {
+        tile_encode<NDIM>(in, g, tile, gp, R, e.r, e.level, bwords, slot, counts + tile, ovf, sum);
+    }
 }
 
 
@@ -150,12 +156,12 @@ __global__ __launch_bounds__(kWG) void k_tile_compress_w(const float* __restrict
                                                          uint16_t* __restrict__ tail, uint32_t* run_sums, uint32_t tpw)
 {
     __shared__ uint32_t sh[4][32 * kLdsPitch];
-    const uint32_t wv = threadIdx.x >> 6;
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
     const uint32_t unit = 4 * xcd_tile(blockIdx.x, (g.T + 4) / 4) + wv;  // units 0 .. T (T = the tail unit)
     const bool     is_tile = unit < g.T;
     const int      gp = is_tile ? geom_path<NDIM>(g, unit * (uint32_t)kTileCodes) : 0;
     RawTile<NDIM>  R[2];
-    if (gp >= 2) raw_prefetch<NDIM>(in, unit * (uint32_t)kTileCodes + 8 * lane_id(), g, gp, R[0]);
+    if (gp >= 2) raw_prefetch<NDIM>(in, unit * (uint32_t)kTileCodes, 8 * lane_id(), g, gp, R[0]);
     const EbParams e = reduce_partials_256(part, nparts, eb_rel, NDIM);
     if (blockIdx.x == 0 && threadIdx.x == 0) write_header(res, e, g);
     if (e.status != ST_SUCCESS || unit > g.T) return;  // uniform per wave

@@ -241,14 +241,15 @@ __device__ __forceinline__ int geom_path(const Geom& g, uint32_t t0)
     return (g.X & 3u) == 0 ? 2 : (g.X >= 8 ? 3 : 1);
 }
 
-// issue the raw loads of a fast-path tile (gp 2 or 3)
+// issue the raw loads of a fast-path tile (gp 2 or 3): the thread at offset lo of the tile
+// starting at t0 (uniform across the wave)
 template <int NDIM>
-__device__ __forceinline__ void raw_prefetch(const float* __restrict__ in, uint32_t p0, const Geom& g, int gp,
-                                             RawTile<NDIM>& R)
+__device__ __forceinline__ void raw_prefetch(const float* __restrict__ in, uint32_t t0, uint32_t lo, const Geom& g,
+                                             int gp, RawTile<NDIM>& R)
 {
-    if (gp == 2) raw_load<NDIM>(in, p0, g, R);
+    if (gp == 2) raw_load_t<NDIM>(in, t0, lo, g, R);
     if constexpr (NDIM > 1)
-        if (gp == 3) raw_load_u<NDIM>(in, p0, g, R);
+        if (gp == 3) raw_load_ut<NDIM>(in, t0, lo, g, R);
 }
 
 template <int NDIM>
@@ -301,6 +302,18 @@ __device__ __forceinline__ void tail_unit(const float* __restrict__ in, const Ge
     if ((tid & 63) == 0 && ovf) atomicAdd(ovf_out, (unsigned long long)ovf);
 }
 
+// The 3D generic path of a fast-geometry tile (gp 2 or 3, whose raw loads R holds): level 0,
+// and level 1 on the any-X layout.
+__device__ __forceinline__ void generic3_raw(uint32_t p0, const Geom& g, float r, const RawTile<3>& R, uint32_t w[4],
+                                             uint32_t& ovf)
+{
+    w[0] = w[1] = w[2] = w[3] = 0;
+    if (p0 >= g.xyz) return;
+    G3 q;
+    g3_from_raw(R, r, q);
+    g3_codes(q, p0, g, w, ovf);
+}
+
 template <int NDIM, bool AL>
 __device__ __forceinline__ void pk_dispatch(const RawTile<NDIM>& R, uint32_t p0, const Geom& g, float r, int level,
                                             uint32_t w[4])
@@ -316,8 +329,11 @@ __device__ __forceinline__ void pk_dispatch(const RawTile<NDIM>& R, uint32_t p0,
 //   *run_sum                       += count (when run_sum is not null; relaxed atomic, read
 //                                  only after the kernel ends)
 // `R` holds the tile's raw loads when gp is 2 or 3 (raw_prefetch, issued by the caller before
-// its other latency-bound work).
-template <int NDIM>
+// its other latency-bound work).  PATHS (3D): 1 compiles only the paths of gp 2 / 3 tiles, 2
+// only the generic path with its own loads (gp 0 / 1); a kernel that branches between the two
+// at its top keeps the register count of the larger one instead of their union (69 against
+// 87 VGPRs: 7 waves per SIMD instead of 5).
+template <int NDIM, int PATHS = 0>
 __device__ __forceinline__ void tile_encode(const float* __restrict__ in, const Geom& g, uint32_t tile, int gp,
                                             const RawTile<NDIM>& R, float r, int level,
                                             uint32_t* __restrict__ bwords, uint8_t* __restrict__ slot,
@@ -342,13 +358,16 @@ __device__ __forceinline__ void tile_encode(const float* __restrict__ in, const 
         else if (gp == 2 && level >= 1) raw_compute<2>(R, p0, g, r, w, ovf);
         else if (gp == 3 && level >= 1) raw_compute_u(R, p0, g, r, w, ovf);
         else dq8_tile<2>(in, p0, g, r, level >= 1 ? gp : 0, w, ovf);
+    } else if constexpr (PATHS == 2) {
+        dq8_tile<3>(in, p0, g, r, level >= 1 ? gp : 0, w, ovf);
     } else {
         if (gp == 2 && level >= 3) pk_dispatch<3, true>(R, p0, g, r, level, w);
         else if (gp == 3 && level >= 3) pk_dispatch<3, false>(R, p0, g, r, level, w);
         else if (gp == 2 && level >= 2) raw_compute_q3<true>(R, p0, g, r, w, ovf);
         else if (gp == 3 && level >= 2) raw_compute_q3<false>(R, p0, g, r, w, ovf);
         else if (gp == 2 && level >= 1) raw_compute<3>(R, p0, g, r, w, ovf);
-        else dq8_tile<3>(in, p0, g, r, level >= 1 ? (gp == 3 ? 1 : gp) : 0, w, ovf);
+        else if (gp >= 2) generic3_raw(p0, g, r, R, w, ovf);
+        else dq8_tile<3>(in, p0, g, r, level >= 1 ? gp : 0, w, ovf);  // PATHS == 0 only
     }
 
     // 2. bit transpose of group k = t/8 (skipped for an all-zero wave); register q then holds
@@ -417,7 +436,8 @@ __device__ __forceinline__ void dq8_level(const float* __restrict__ in, uint32_t
         else if (gp == 2 && level >= 2) raw_compute_q3<true>(R, p0, g, r, w, ovf);
         else if (gp == 3 && level >= 2) raw_compute_q3<false>(R, p0, g, r, w, ovf);
         else if (gp == 2 && level >= 1) raw_compute<3>(R, p0, g, r, w, ovf);
-        else dq8_tile<3>(in, p0, g, r, level >= 1 ? (gp == 3 ? 1 : gp) : 0, w, ovf);
+        else if (gp >= 2) generic3_raw(p0, g, r, R, w, ovf);
+        else dq8_tile<3>(in, p0, g, r, level >= 1 ? gp : 0, w, ovf);
     }
 }
 
@@ -440,12 +460,12 @@ __device__ __forceinline__ void tile_encode_w(const float* __restrict__ in, cons
 {
     const uint32_t lane = lane_id(), L8 = lane & 7, k8 = lane >> 3;
     const uint32_t t0 = tile * (uint32_t)kTileCodes;
-    const bool     fast = level >= 1 && gp >= 2;
     uint32_t       ovf = 0;
 #pragma unroll
     for (int m = 0; m < 4; m++) {
         const uint32_t p0 = t0 + 512 * m + 8 * lane;
-        if (fast && m < 3) raw_prefetch<NDIM>(in, p0 + 512, g, gp, R[(m + 1) & 1]);
+        // every gp >= 2 path reads R (the 3D generic one too), whatever the level
+        if (gp >= 2 && m < 3) raw_prefetch<NDIM>(in, t0, 512 * (m + 1) + 8 * lane, g, gp, R[(m + 1) & 1]);
         uint32_t w[4];
         dq8_level<NDIM>(in, p0, g, gp, R[m & 1], r, level, w, ovf);
         if (ballot64((w[0] | w[1] | w[2] | w[3]) != 0) != 0) transpose4(w, L8);
