@@ -119,10 +119,21 @@ __global__ __launch_bounds__(kWG) void k_tile_compress(const float* __restrict__
     // XCD-aware order: an XCD walks its eighth of the field
     const uint32_t tile = is_tail ? g.T : xcd_tile(blockIdx.x, g.T);
     const int      gp = is_tail ? 0 : geom_path<NDIM>(g, tile * (uint32_t)kTileCodes);
-    // the tile's loads need no eb: issue them before reducing the range partials
+    // 1D / 2D: wave 0's partial loads first, then the tile's raw loads (they need no eb), so
+    // the partials return first and the eb parameters are computed while the tile is in
+    // flight.  3D issues the raw loads first: holding the partials across them would cost it
+    // a wave per SIMD (86 against 69 VGPRs).
+    Partials      P;
     RawTile<NDIM> R;
-    raw_prefetch<NDIM>(in, tile * (uint32_t)kTileCodes, 8 * threadIdx.x, g, gp, R);
-    const EbParams e = reduce_partials_256(part, nparts, eb_rel, NDIM);
+    if constexpr (NDIM < 3) {
+        partials_load(part, nparts, P);
+        __builtin_amdgcn_sched_barrier(0);
+        raw_prefetch<NDIM>(in, tile * (uint32_t)kTileCodes, 8 * threadIdx.x, g, gp, R);
+    } else {
+        raw_prefetch<NDIM>(in, tile * (uint32_t)kTileCodes, 8 * threadIdx.x, g, gp, R);
+        partials_load(part, nparts, P);
+    }
+    const EbParams e = partials_reduce(P, eb_rel, NDIM);
     if (blockIdx.x == 0 && threadIdx.x == 0) write_header(res, e, g);
     if (e.status != ST_SUCCESS) return;  // uniform
     unsigned long long* ovf = reinterpret_cast<unsigned long long*>(&res->overflow);
@@ -160,9 +171,11 @@ __global__ __launch_bounds__(kWG) void k_tile_compress_w(const float* __restrict
     const uint32_t unit = 4 * xcd_tile(blockIdx.x, (g.T + 4) / 4) + wv;  // units 0 .. T (T = the tail unit)
     const bool     is_tile = unit < g.T;
     const int      gp = is_tile ? geom_path<NDIM>(g, unit * (uint32_t)kTileCodes) : 0;
-    RawTile<NDIM>  R[2];
+    RawTile<NDIM> R[2];
     if (gp >= 2) raw_prefetch<NDIM>(in, unit * (uint32_t)kTileCodes, 8 * lane_id(), g, gp, R[0]);
-    const EbParams e = reduce_partials_256(part, nparts, eb_rel, NDIM);
+    Partials P;
+    partials_load(part, nparts, P);
+    const EbParams e = partials_reduce(P, eb_rel, NDIM);
     if (blockIdx.x == 0 && threadIdx.x == 0) write_header(res, e, g);
     if (e.status != ST_SUCCESS || unit > g.T) return;  // uniform per wave
     unsigned long long* ovf = reinterpret_cast<unsigned long long*>(&res->overflow);

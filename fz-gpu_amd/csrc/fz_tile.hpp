@@ -184,20 +184,32 @@ __device__ __forceinline__ EbParams reduce_partials(const float4* __restrict__ p
     return s_eb;
 }
 
-// Same result for nparts <= 256, reduced by wave 0 alone (4 loads per lane issued together)
-// while the other waves go straight to the barrier: the form used by every tile workgroup.
-__device__ __forceinline__ EbParams reduce_partials_256(const float4* __restrict__ part, uint32_t nparts,
-                                                        double eb_rel, int ndim)
+// Same result for nparts <= 256, reduced by wave 0 alone while the other waves go straight to
+// the barrier: the form used by every tile workgroup.  Wave 0 loads the partials first
+// (partials_load, 4 loads per lane), before the tile's raw loads, so that they return first
+// (a wave's loads return in order) and the eb parameters are ready while the tile is still
+// in flight; partials_reduce then consumes them.
+struct Partials {
+    float4 p[4];
+};
+
+__device__ __forceinline__ void partials_load(const float4* __restrict__ part, uint32_t nparts, Partials& P)
+{
+    if (threadIdx.x < 64) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t i = threadIdx.x + 64 * j;
+            P.p[j] = i < nparts ? part[i] : make_float4(__builtin_huge_valf(), -__builtin_huge_valf(), 0.0f, 0.0f);
+        }
+    }
+}
+
+__device__ __forceinline__ EbParams partials_reduce(const Partials& P, double eb_rel, int ndim)
 {
     __shared__ EbParams s_eb;
     const uint32_t tid = threadIdx.x;
     if (tid < 64) {
-        float4 p[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const uint32_t i = tid + 64 * j;
-            p[j] = i < nparts ? part[i] : make_float4(__builtin_huge_valf(), -__builtin_huge_valf(), 0.0f, 0.0f);
-        }
+        const float4* p = P.p;
         float lo = fminf(fminf(p[0].x, p[1].x), fminf(p[2].x, p[3].x));
         float hi = fmaxf(fmaxf(p[0].y, p[1].y), fmaxf(p[2].y, p[3].y));
         const uint32_t bad =
@@ -242,14 +254,16 @@ __device__ __forceinline__ int geom_path(const Geom& g, uint32_t t0)
 }
 
 // issue the raw loads of a fast-path tile (gp 2 or 3): the thread at offset lo of the tile
-// starting at t0 (uniform across the wave)
+// starting at t0 (uniform across the wave).  One load sequence for both layouts: the
+// neighbour rows use 4-byte-aligned dwordx4 loads, the same instruction an aligned row gets,
+// so gp 2 and gp 3 need no separate branches (whose merge would wait on the loads at once).
 template <int NDIM>
 __device__ __forceinline__ void raw_prefetch(const float* __restrict__ in, uint32_t t0, uint32_t lo, const Geom& g,
                                              int gp, RawTile<NDIM>& R)
 {
-    if (gp == 2) raw_load_t<NDIM>(in, t0, lo, g, R);
-    if constexpr (NDIM > 1)
-        if (gp == 3) raw_load_ut<NDIM>(in, t0, lo, g, R);
+    if (gp < 2) return;
+    if constexpr (NDIM == 1) raw_load_t<1>(in, t0, lo, g, R);
+    else raw_load_ut<NDIM>(in, t0, lo, g, R);
 }
 
 template <int NDIM>
