@@ -324,6 +324,12 @@ struct RawTile<3> {
     float vw, vnw, vuw, vunw;
 };
 
+// v of the previous lane of the wave (DPP wave_shr:1); lane 0 gets `first`
+__device__ __forceinline__ float from_prev_lane(float v, float first)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(first), __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+
 // Raw loads of the thread at offset lo (a multiple of 8, < 2048) of the tile that starts at
 // t0 (uniform across the caller's wave).  The row bases are uniform, so each load is a scalar
 // base, one 32-bit lane offset and a constant.  Requires the fast-path geometry (tile_path == 2
@@ -335,8 +341,9 @@ __device__ __forceinline__ void raw_load_t(const float* __restrict__ in, uint32_
     const float* own = in + t0;
     load8(own + lo, R.v);
     if constexpr (NDIM == 1) {
-        // lo == 0 starts a 256-block (t0 % 2048 == 0): its west value is never used
-        R.vw = lo ? (own + lo)[-1] : 0.0f;
+        // the west value is the previous lane's last one; lane 0 of a wave starts a 256-block
+        // (its offset is a multiple of 512), so its west value is never used
+        R.vw = from_prev_lane(R.v[7], 0.0f);
     } else {
         const float* n = own - g.X;
         load8(n + lo, R.vn);
@@ -492,15 +499,19 @@ __device__ __forceinline__ void raw_load_ut(const float* __restrict__ in, uint32
     const float* n = own - g.X;
     load8(own + lo, R.v);
     load8u(n + lo, R.vn);
-    R.vw = (own + lo)[-1];
-    R.vnw = (n + lo)[-1];
+    // west values: the previous lane's last value of the same row; lane 0 of the wave reads
+    // the one before its row segment (a uniform address, inside the field since gp >= 2)
+    // (pointer steps, not lo0 - 1: lo0 is 0 for the tile's first wave)
+    const uint32_t lo0 = __builtin_amdgcn_readfirstlane(lo);
+    R.vw = from_prev_lane(R.v[7], (own + lo0)[-1]);
+    R.vnw = from_prev_lane(R.vn[7], (n + lo0)[-1]);
     if constexpr (NDIM == 3) {
         const float* u = own - g.XY;
         const float* un = u - g.X;
         load8u(u + lo, R.vu);
         load8u(un + lo, R.vun);
-        R.vuw = (u + lo)[-1];
-        R.vunw = (un + lo)[-1];
+        R.vuw = from_prev_lane(R.vu[7], (u + lo0)[-1]);
+        R.vunw = from_prev_lane(R.vun[7], (un + lo0)[-1]);
     }
 }
 
