@@ -2,7 +2,8 @@
 //
 // The work queue is the global unit space: every field owns T + 1 consecutive units (its
 // full tiles, then the unit of the partial tile), and the hardware dispatcher hands units to
-// workgroups.  A workgroup finds its field with wave_upper over the fields' first units, then
+// workgroups.  A workgroup reads its field from the per-unit field table (one uniform scalar
+// load, so its raw loads issue after a single short round trip), then
 // runs the same per-tile bodies as the single-field kernels (fz_tile.hpp), so each field's
 // bytes equal those of the per-field path.  Compress:
 //
@@ -107,7 +108,7 @@ __global__ __launch_bounds__(kWG) void k_batch_tile(BatchDev d, uint32_t u_lo, u
 {
     const uint32_t local = xcd ? xcd_tile(blockIdx.x, nunits) : blockIdx.x;
     const uint32_t gu = u_lo + local;
-    const uint32_t f = wave_upper(d.ustart, f_lo, f_hi, gu);
+    const uint32_t f = d.ufield[gu];  // uniform: one scalar load
     const Geom     g = d.fields[f].g;
     const uint32_t tile = gu - d.ustart[f];
     const bool     is_tail = tile >= g.T;
@@ -148,7 +149,7 @@ __global__ __launch_bounds__(kWG) void k_batch_tile_w(BatchDev d, uint32_t u_lo,
     const uint32_t local = 4 * (xcd ? xcd_tile(blockIdx.x, ngroups) : blockIdx.x) + wv;
     if (local >= nunits) return;
     const uint32_t gu = u_lo + local;
-    const uint32_t f = wave_upper(d.ustart, f_lo, f_hi, gu);
+    const uint32_t f = d.ufield[gu];  // uniform: one scalar load
     const Geom     g = d.fields[f].g;
     const uint32_t tile = gu - d.ustart[f];
     const bool     is_tail = tile >= g.T;
@@ -392,7 +393,7 @@ __global__ __launch_bounds__(kBScanWG) void k_batch_dscan(BatchDev d)
 __global__ __launch_bounds__(kWG) void k_batch_decode(BatchDev d, uint32_t nf)
 {
     const uint32_t gu = blockIdx.x;
-    const uint32_t f = wave_upper(d.ustart, 0, nf, gu);
+    const uint32_t f = d.ufield[gu];
     const Geom     g = d.fields[f].g;
     const uint32_t u0 = d.ustart[f], tile = gu - u0;
     const BPtrs    q = d.ptrs[f];

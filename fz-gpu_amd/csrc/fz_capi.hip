@@ -454,8 +454,8 @@ struct fz_batch_plan_s {
     BatchShape            shape{};
     fz_batch_tuning       tune{};
     struct {
-        size_t fields, ustart, pstart[kNumChunks], rstart, tables_end, ptrs, prm, part, counts, dsts, texcl, staging,
-            total;
+        size_t fields, ustart, ufield, pstart[kNumChunks], rstart, tables_end, ptrs, prm, part, counts, dsts, texcl,
+            staging, total;
     } L{};
     std::vector<BPtrs>      host_ptrs;  // staging of the call's pointer records (sorted order)
     BatchGroup              all{};      // every field as one group
@@ -583,6 +583,7 @@ BatchDev batch_dev(const fz_batch_plan p, void* d_ws, fz_result* res)
     BatchDev d{};
     d.fields = reinterpret_cast<const BField*>(ws + p->L.fields);
     d.ustart = reinterpret_cast<const uint32_t*>(ws + p->L.ustart);
+    d.ufield = reinterpret_cast<const uint32_t*>(ws + p->L.ufield);
     d.pstart = reinterpret_cast<const uint32_t*>(ws + p->L.pstart[chunk_index(p)]);
     d.rstart = reinterpret_cast<const uint32_t*>(ws + p->L.rstart);
     d.ptrs = reinterpret_cast<BPtrs*>(ws + p->L.ptrs);
@@ -784,6 +785,7 @@ fz_status fz_batch_plan_create(fz_batch_plan* plan, const fz_field_shape* shapes
     size_t o = 0;
     L.fields = o; o = al256(o + sizeof(BField) * nfields);
     L.ustart = o; o = al256(o + 4ull * (nfields + 1));
+    L.ufield = o; o = al256(o + 4ull * u);
     for (int c = 0; c < kNumChunks; c++) {
         L.pstart[c] = o;
         o = al256(o + 4ull * (nfields + 1));
@@ -871,6 +873,11 @@ fz_status fz_batch_workspace_init(const fz_batch_plan plan, void* d_ws, size_t w
     std::vector<uint8_t> t(L.tables_end, 0);
     std::memcpy(t.data() + L.fields, plan->fields.data(), sizeof(BField) * plan->nf);
     std::memcpy(t.data() + L.ustart, plan->ustart.data(), 4ull * (plan->nf + 1));
+    {
+        uint32_t* uf = reinterpret_cast<uint32_t*>(t.data() + L.ufield);
+        for (uint32_t f = 0; f < plan->nf; f++)
+            for (uint32_t u = plan->ustart[f]; u < plan->ustart[f + 1]; u++) uf[u] = f;
+    }
     for (int c = 0; c < kNumChunks; c++)
         std::memcpy(t.data() + L.pstart[c], plan->pstart[c].data(), 4ull * (plan->nf + 1));
     std::memcpy(t.data() + L.rstart, plan->rstart.data(), 4ull * (plan->nf + 1));

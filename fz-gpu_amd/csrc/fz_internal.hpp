@@ -106,6 +106,7 @@ struct PtrChunk {
 struct BatchDev {
     const BField*   fields;
     const uint32_t* ustart;  // nf + 1, global unit of each field (sorted order)
+    const uint32_t* ufield;  // per unit: its field (sorted order)
     const uint32_t* pstart;  // nf + 1, first range partial of each field (selected chunk)
     const uint32_t* rstart;  // nf + 1, first reconstruct workgroup of each field
     BPtrs*          ptrs;

@@ -475,17 +475,22 @@ __device__ __forceinline__ void tile_encode_w(const float* __restrict__ in, cons
     const uint32_t lane = lane_id(), L8 = lane & 7, k8 = lane >> 3;
     const uint32_t t0 = tile * (uint32_t)kTileCodes;
     uint32_t       ovf = 0;
-#pragma unroll
-    for (int m = 0; m < 4; m++) {
+    // quarter m computes from `cur` while the loads of quarter m + 1 go to `nxt` (explicit
+    // buffers: the indices stay constant even where the compiler does not unroll)
+    auto quarter = [&](int m, const RawTile<NDIM>& cur, RawTile<NDIM>& nxt) {
         const uint32_t p0 = t0 + 512 * m + 8 * lane;
         // every gp >= 2 path reads R (the 3D generic one too), whatever the level
-        if (gp >= 2 && m < 3) raw_prefetch<NDIM>(in, t0, 512 * (m + 1) + 8 * lane, g, gp, R[(m + 1) & 1]);
+        if (gp >= 2 && m < 3) raw_prefetch<NDIM>(in, t0, 512 * (m + 1) + 8 * lane, g, gp, nxt);
         uint32_t w[4];
-        dq8_level<NDIM>(in, p0, g, gp, R[m & 1], r, level, w, ovf);
+        dq8_level<NDIM>(in, p0, g, gp, cur, r, level, w, ovf);
         if (ballot64((w[0] | w[1] | w[2] | w[3]) != 0) != 0) transpose4(w, L8);
 #pragma unroll
         for (int q = 0; q < 4; q++) sh[kLdsPitch * (4 * L8 + q) + 8 * m + k8] = w[q];
-    }
+    };
+    quarter(0, R[0], R[1]);
+    quarter(1, R[1], R[0]);
+    quarter(2, R[0], R[1]);
+    quarter(3, R[1], R[0]);
     wave_lds_sync();
     uint32_t base = 0;
 #pragma unroll
