@@ -109,10 +109,10 @@ __global__ __launch_bounds__(kWG) void k_batch_tile(BatchDev d, uint32_t u_lo, u
     const uint32_t local = xcd ? xcd_tile(blockIdx.x, nunits) : blockIdx.x;
     const uint32_t gu = u_lo + local;
     const uint32_t f = d.ufield[gu];  // uniform: one scalar load
+    const BPtrs    q = d.ptrs[f];     // the field's records, both loaded as soon as f is known
     const Geom     g = d.fields[f].g;
-    const uint32_t tile = gu - d.ustart[f];
+    const uint32_t tile = gu - d.fields[f].ustart;
     const bool     is_tail = tile >= g.T;
-    const BPtrs    q = d.ptrs[f];
     const float*   in = static_cast<const float*>(q.src);
     const int      gp = is_tail ? 0 : geom_path<NDIM>(g, tile * (uint32_t)kTileCodes);
     RawTile<NDIM>  R;
@@ -150,10 +150,10 @@ __global__ __launch_bounds__(kWG) void k_batch_tile_w(BatchDev d, uint32_t u_lo,
     if (local >= nunits) return;
     const uint32_t gu = u_lo + local;
     const uint32_t f = d.ufield[gu];  // uniform: one scalar load
+    const BPtrs    q = d.ptrs[f];     // the field's records, both loaded as soon as f is known
     const Geom     g = d.fields[f].g;
-    const uint32_t tile = gu - d.ustart[f];
+    const uint32_t tile = gu - d.fields[f].ustart;
     const bool     is_tail = tile >= g.T;
-    const BPtrs    q = d.ptrs[f];
     const float*   in = static_cast<const float*>(q.src);
     const int      gp = is_tail ? 0 : geom_path<NDIM>(g, tile * (uint32_t)kTileCodes);
     RawTile<NDIM>  R[2];

@@ -741,7 +741,7 @@ fz_status fz_batch_plan_create(fz_batch_plan* plan, const fz_field_shape* shapes
             F.rdims[0] = (g.xyz + 4095) / 4096;
             F.rdims[1] = F.rdims[2] = 1;
         } else if (g.ndim == 2) {
-            F.rdims[0] = (g.X + kWG - 1) / kWG;
+            F.rdims[0] = (g.X + kRecon2DCols - 1) / kRecon2DCols;
             F.rdims[1] = (g.Y + 15) / 16;
             F.rdims[2] = 1;
         } else {

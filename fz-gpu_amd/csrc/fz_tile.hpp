@@ -631,31 +631,61 @@ __device__ __forceinline__ void recon_1d(const uint16_t* __restrict__ codes, flo
     }
 }
 
-// 2D: workgroup (bx, by) = 16 rows x 256 columns (16 Lorenzo blocks); thread = one column of a
-// block.  Prefix over y in registers, then over x across the 16 lanes of the block.
+// 2D: workgroup (bx, by) = 16 rows x 1024 columns (64 Lorenzo blocks); thread = 4 adjacent
+// columns, so 4 lanes cover a block's 16.  The 16 rows stream through: each row adds its
+// decoded values to the running column sums (prefix over y), then a prefix over x in the
+// thread and across its 4 lanes (two DPP steps) gives the row's values.  Loads are 8 bytes
+// and stores 16 bytes per lane when X % 4 == 0 (stores at 4-byte alignment otherwise).
+__device__ __forceinline__ int32_t quad_incl_scan(int32_t v)
+{
+    const uint32_t q = lane_id() & 3u;
+    const int32_t  a = __builtin_amdgcn_mov_dpp(v, 0x90, 0xF, 0xF, false);  // quad_perm [0,0,1,2]
+    v += q >= 1 ? a : 0;
+    const int32_t b = __builtin_amdgcn_mov_dpp(v, 0x40, 0xF, 0xF, false);  // quad_perm [0,0,0,1]
+    return v + (q >= 2 ? b : 0);
+}
+
 __device__ __forceinline__ void recon_2d(const uint16_t* __restrict__ codes, float* __restrict__ out, const Geom& g,
                                          float ebx2, uint32_t bx, uint32_t by)
 {
-    const uint32_t x = bx * kWG + threadIdx.x, y0 = by * 16;
-    const bool     xin = x < g.X;
-    int32_t        s[16];
-#pragma unroll
-    for (int r = 0; r < 16; r++)
-        s[r] = (xin && y0 + r < g.Y) ? decode(codes[(size_t)(y0 + r) * g.X + x]) : 0;
-#pragma unroll
-    for (int r = 1; r < 16; r++) s[r] += s[r - 1];
+    const uint32_t x0 = bx * kRecon2DCols + 4 * threadIdx.x, y0 = by * 16;
+    const bool     full = x0 + 4 <= g.X;
+    const bool     vec = full && (g.X & 3u) == 0;
+    int32_t        col[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int r = 0; r < 16; r++) {
+        const uint32_t y = y0 + r;
+        if (y >= g.Y) break;  // uniform
+        const size_t row = (size_t)y * g.X;
+        uint32_t     c[4];
+        if (vec) {
+            const uint2 v = *reinterpret_cast<const uint2*>(codes + row + x0);
+            c[0] = v.x & 0xFFFFu;
+            c[1] = v.x >> 16;
+            c[2] = v.y & 0xFFFFu;
+            c[3] = v.y >> 16;
+        } else {
 #pragma unroll
-        for (int d = 1; d < 16; d <<= 1) {
-            const int32_t o = __shfl_up(s[r], d, 16);
-            if ((threadIdx.x & 15) >= (uint32_t)d) s[r] += o;
+            for (int j = 0; j < 4; j++) c[j] = x0 + j < g.X ? codes[row + x0 + j] : 0u;
         }
-    }
-    if (xin) {
+        int32_t p[4];
 #pragma unroll
-        for (int r = 0; r < 16; r++)
-            if (y0 + r < g.Y) out[(size_t)(y0 + r) * g.X + x] = (float)s[r] * ebx2;
+        for (int j = 0; j < 4; j++) {
+            col[j] += decode(c[j]);
+            p[j] = col[j] + (j ? p[j - 1] : 0);
+        }
+        const int32_t excl = quad_incl_scan(p[3]) - p[3];
+        float o[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) o[j] = (float)(p[j] + excl) * ebx2;
+        if (full) {
+            typedef float f32x4a4_t __attribute__((ext_vector_type(4), aligned(4)));
+            *reinterpret_cast<f32x4a4_t*>(out + row + x0) = f32x4a4_t{o[0], o[1], o[2], o[3]};
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (x0 + j < g.X) out[row + x0 + j] = o[j];
+        }
     }
 }
 
