@@ -649,7 +649,7 @@ hipError_t launch_reconstruct(const uint16_t* codes, float* out, const Geom& g, 
         const dim3 grid((g.X + kRecon2DCols - 1) / kRecon2DCols, (g.Y + 15) / 16);
         FZ_LAUNCH(k_reconstruct_2d<<<grid, kWG, 0, s>>>(codes, out, g, ebx2));
     } else {
-        const dim3 grid((g.X + 31) / 32, (g.Y + 7) / 8, (g.Z + 7) / 8);
+        const dim3 grid((g.X + kRecon3DCols - 1) / kRecon3DCols, (g.Y + 7) / 8, (g.Z + 7) / 8);
         FZ_LAUNCH(k_reconstruct_3d<<<grid, kWG, 0, s>>>(codes, out, g, ebx2));
     }
     return hipGetLastError();

@@ -689,45 +689,72 @@ __device__ __forceinline__ void recon_2d(const uint16_t* __restrict__ codes, flo
     }
 }
 
-// 3D: workgroup (bx, by, bz) = 32 (x) x 8 (y) x 8 (z) like the reference's (32,1,8) block.
-// Thread (tx, ty) holds the z column of (x, y): prefix over z in registers, over x across
-// 8-lane groups; then an LDS transpose puts the y column in registers for the last prefix.
+// 3D: workgroup (bx, by, bz) = 128 (x) x 8 (y) x 8 (z); thread (xq, ty) = 4 adjacent columns
+// of row y0 + ty over the 8 z of the strip.  Prefix over z in registers, over x in the thread
+// and with the other half of its 8-wide Lorenzo block (the neighbour lane, one DPP step); an
+// LDS transpose (16-byte accesses) then gives thread (xq, tz) the 8 rows of its z slice for the
+// prefix over y.  Loads are 8 bytes and stores 16 bytes per lane when X % 4 == 0.
 __device__ __forceinline__ void recon_3d(const uint16_t* __restrict__ codes, float* __restrict__ out, const Geom& g,
                                          float ebx2, uint32_t bx, uint32_t by, uint32_t bz)
 {
-    __shared__ int32_t sh[8][8][33];
-    const uint32_t tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-    const uint32_t x = bx * 32 + tx, y0 = by * 8, z0 = bz * 8;
-    const bool     xin = x < g.X;
-    int32_t        s[8];
+    __shared__ int4 sh[8][8][32];  // [z][y][x quad]
+    const uint32_t xq = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const uint32_t x0 = bx * kRecon3DCols + 4 * xq, y0 = by * 8, z0 = bz * 8;
+    const bool     full = x0 + 4 <= g.X;
+    const bool     vec = full && (g.X & 3u) == 0;
     {
         const uint32_t y = y0 + ty;
+        int32_t        col[4] = {0, 0, 0, 0};
 #pragma unroll
-        for (int z = 0; z < 8; z++)
-            s[z] = (xin && y < g.Y && z0 + z < g.Z) ? decode(codes[(size_t)(z0 + z) * g.XY + (size_t)y * g.X + x]) : 0;
-    }
+        for (int z = 0; z < 8; z++) {
+            uint32_t c[4] = {0u, 0u, 0u, 0u};
+            if (y < g.Y && z0 + z < g.Z) {
+                const size_t row = (size_t)(z0 + z) * g.XY + (size_t)y * g.X;
+                if (vec) {
+                    const uint2 v = *reinterpret_cast<const uint2*>(codes + row + x0);
+                    c[0] = v.x & 0xFFFFu;
+                    c[1] = v.x >> 16;
+                    c[2] = v.y & 0xFFFFu;
+                    c[3] = v.y >> 16;
+                } else {
 #pragma unroll
-    for (int z = 1; z < 8; z++) s[z] += s[z - 1];
+                    for (int j = 0; j < 4; j++) c[j] = x0 + j < g.X ? codes[row + x0 + j] : 0u;
+                }
+            }
+            int32_t p[4];
 #pragma unroll
-    for (int z = 0; z < 8; z++) {
-#pragma unroll
-        for (int d = 1; d < 8; d <<= 1) {
-            const int32_t o = __shfl_up(s[z], d, 8);
-            if ((tx & 7) >= (uint32_t)d) s[z] += o;
+            for (int j = 0; j < 4; j++) {
+                col[j] += decode(c[j]);              // prefix over z
+                p[j] = col[j] + (j ? p[j - 1] : 0);  // prefix over the thread's 4 columns
+            }
+            // the odd lane of a pair holds the block's upper 4 columns: add the even lane's total
+            const int32_t lower = __builtin_amdgcn_mov_dpp(p[3], 0xA0, 0xF, 0xF, false);  // quad_perm [0,0,2,2]
+            const int32_t add = (xq & 1u) ? lower : 0;
+            sh[z][ty][xq] = make_int4(p[0] + add, p[1] + add, p[2] + add, p[3] + add);
         }
-        sh[z][ty][tx] = s[z];
     }
     __syncthreads();
     const uint32_t tz = ty;
-    int32_t        v[8];
+    if (z0 + tz >= g.Z) return;
+    int4 acc = make_int4(0, 0, 0, 0);
 #pragma unroll
-    for (int y = 0; y < 8; y++) v[y] = sh[tz][y][tx];
+    for (int y = 0; y < 8; y++) {
+        if (y0 + y >= g.Y) break;  // uniform
+        const int4 v = sh[tz][y][xq];
+        acc.x += v.x;
+        acc.y += v.y;
+        acc.z += v.z;
+        acc.w += v.w;
+        const size_t row = (size_t)(z0 + tz) * g.XY + (size_t)(y0 + y) * g.X;
+        const float  o[4] = {(float)acc.x * ebx2, (float)acc.y * ebx2, (float)acc.z * ebx2, (float)acc.w * ebx2};
+        if (full) {
+            typedef float f32x4a4_t __attribute__((ext_vector_type(4), aligned(4)));
+            *reinterpret_cast<f32x4a4_t*>(out + row + x0) = f32x4a4_t{o[0], o[1], o[2], o[3]};
+        } else {
 #pragma unroll
-    for (int y = 1; y < 8; y++) v[y] += v[y - 1];
-    if (xin && z0 + tz < g.Z) {
-#pragma unroll
-        for (int y = 0; y < 8; y++)
-            if (y0 + y < g.Y) out[(size_t)(z0 + tz) * g.XY + (size_t)(y0 + y) * g.X + x] = (float)v[y] * ebx2;
+            for (int j = 0; j < 4; j++)
+                if (x0 + j < g.X) out[row + x0 + j] = o[j];
+        }
     }
 }
 
