@@ -92,6 +92,14 @@ def bench_field(x, dims, eb, steps, warmup, fused=True, rotate_bytes=1.2e9, labe
     out["decompress_gbps"] = nbytes / (dms * 1e-3) / 1e9
     out["decompress_kernel_ms"] = kernel_profile(plan, lambda k: plan.decompress(p, t, r2["ebx2"], out=dst),
                                                  min(steps, 50))
+    # decompress + quality metrics against the original (one call, one stream)
+    pa = p.clone() if p.data_ptr() % 16 else p
+    d_stats = torch.zeros(160, dtype=torch.uint8, device="cuda")
+    vfn = lambda k: plan.decompress_verify_async(pa, t, r2["ebx2"], base, dst, d_stats)
+    vms, _ = time_steps(vfn, steps)
+    out["decompress_verify_ms"] = vms
+    out["decompress_verify_kernel_ms"] = kernel_profile(plan, vfn, min(steps, 50))
+    out["verify_stats"] = fzgpu.read_stats(d_stats)
     del copies, base
     torch.cuda.empty_cache()
     return out
@@ -233,6 +241,29 @@ def bench_serving(rates=(50, 100, 200, 500, 1000, 2000), duration_s=1.0, pool_fi
     return out
 
 
+def bench_verify(reps=20, sizes=(3600 * 1800, 1 << 24, 1 << 26, 1 << 28)):
+    """fz_verify_f32 (verify.hh:25-83 on the device) on reconstruction-like pairs of several
+    sizes: microseconds per call and GB/s of the 8n bytes it reads."""
+    import torch
+
+    import fzgpu
+    g = torch.Generator(device="cuda").manual_seed(7)
+    res = {}
+    for n in sizes:
+        o = torch.randn(n, device="cuda", generator=g).cumsum(0)
+        x = o + torch.rand(n, device="cuda", generator=g) * 1e-3
+        st = torch.zeros(160, dtype=torch.uint8, device="cuda")
+        temp = fzgpu.verify_async(x, o, st)
+        for _ in range(3):
+            fzgpu.verify_async(x, o, st, temp)
+        ms, _ = time_steps(lambda k: fzgpu.verify_async(x, o, st, temp), reps)
+        res[str(n)] = {"us": round(ms * 1e3, 2), "GBps": round(8 * n / (ms * 1e-3) / 1e9, 1)}
+        log(f"verify n={n}: {ms * 1e3:.1f} us, {8 * n / (ms * 1e-3) / 1e9:.0f} GB/s")
+        del o, x
+        torch.cuda.empty_cache()
+    return res
+
+
 def cpu_baseline(x, dims, eb, seconds):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -317,6 +348,8 @@ def main():
                     help="profiling target: the 256-field uneven batch, default tuning (one group), 3 compress calls")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-serving leg")
     ap.add_argument("--serve-only", action="store_true", help="only the request-serving leg (prints its JSON)")
+    ap.add_argument("--verify-only", action="store_true",
+                    help="only the quality-metric reduction at four sizes (prints its JSON)")
     ap.add_argument("--pmc-target", action="store_true",
                     help="profiling target: a 256 MiB device copy (counter calibration), then 3 fused compress "
                          "calls per workload, no timing")
@@ -338,6 +371,9 @@ def main():
         return
     if args.batch_target:
         batch_target(D)
+        return
+    if args.verify_only:
+        print(json.dumps(bench_verify(max(args.steps, 5))), flush=True)
         return
     if args.serve_only:
         print(json.dumps(bench_serving()), flush=True)
@@ -361,6 +397,11 @@ def main():
         "range_kernel_frac": round(4 * n / (range_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
         "decompress_GBps": round(main_r["decompress_gbps"], 2),
         "decompress_kernel_ms": [round(v, 5) for v in main_r["decompress_kernel_ms"]],
+        "decompress_verify_ms": round(main_r["decompress_verify_ms"], 5),
+        # the last interval is the metrics' two kernels, which read 8n bytes
+        "verify_GBps": round(8 * n / (main_r["decompress_verify_kernel_ms"][-1] * 1e-3) / 1e9, 2),
+        "verify_PSNR_dB": round(main_r["verify_stats"]["reduced"]["PSNR"], 4),
+        "verify_max_rel_err": main_r["verify_stats"]["max_err"]["rel"],
         "input_copies": main_r["copies"],
     }
     log("timing plain pipeline")

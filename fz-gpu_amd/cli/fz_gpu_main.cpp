@@ -5,12 +5,14 @@
 //   --literal      byte-identical reference mode (1D sign loss and dropped tail included)
 //   --plain        run the plain per-stage pipeline instead of the fused kernel
 //   --json         print one JSON line of statistics after the three lines
-// and a decompress mode:
-//   fz-gpu -d <archive> <out.f32>
+// and a decompress mode, with the quality metrics of the reconstruction when the original
+// is given (quality_viewer.hh:97-110 layout, computed on the GPU):
+//   fz-gpu -d <archive> <out.f32> [original.f32]
 #include <hip/hip_runtime.h>
 #include <sys/stat.h>
 
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -48,11 +50,34 @@ static int usage()
 {
     std::fprintf(stderr,
                  "usage: fz-gpu <file> <x> <y> <z> <eb> [-o archive] [--literal] [--plain] [--json]\n"
-                 "       fz-gpu -d <archive> <out.f32>\n");
+                 "       fz-gpu -d <archive> <out.f32> [original.f32]\n");
     return 1;
 }
 
-static int decompress_main(const std::string& arc, const std::string& out)
+// print_metrics_cross + print_metrics_auto (quality_viewer.hh:53-95) for float data
+static void print_metrics(const fz_stats& s, double compressed_bytes, double lag1, double lag2)
+{
+    auto head = [](const char* a, const char* b, const char* c, const char* d, const char* e) {
+        std::printf("  \033[1m\033[31m%-10s %16s %16s %16s %16s\033[0m\n", a, b, c, d, e);
+    };
+    auto row = [](const char* a, double b, double c, double d, double e) {
+        std::printf("  %-10s %16.8g %16.8g %16.8g %16.8g\n", a, b, c, d, e);
+    };
+    std::printf("\nquality metrics (using GPU checker):\n");
+    head("", "data-len", "data-byte", "fp-type?", "");
+    std::printf("  %-10s %16llu %16d %16s\n", "", (unsigned long long)s.len, 4, "yes");
+    head("", "min", "max", "rng", "std");
+    row("origin", s.odata.min, s.odata.max, s.odata.rng, s.odata.std);
+    row("eb-lossy", s.xdata.min, s.xdata.max, s.xdata.rng, s.xdata.std);
+    head("", "abs-val", "abs-idx", "pw-rel", "VS-RNG");
+    row("max-error", s.max_err.abs, (double)s.max_err.idx, s.max_err.pwrrel, s.max_err.rel);
+    head("", "CR", "NRMSE", "cross-cor", "PSNR");
+    row("metrics", s.len * 4.0 / compressed_bytes, s.reduced.NRMSE, s.reduced.coeff, s.reduced.PSNR);
+    head("", "lag1-cor", "lag2-cor", "", "");
+    std::printf("  %-10s %16lf %16lf\n\n", "auto", lag1, lag2);
+}
+
+static int decompress_main(const std::string& arc, const std::string& out, const std::string& original)
 {
     fzar::Header          h;
     std::vector<uint8_t>  payload;
@@ -81,7 +106,46 @@ static int decompress_main(const std::string& arc, const std::string& out)
     hipStream_t st;
     HIPCHK(hipStreamCreate(&st));
     FZCHK(fz_workspace_init(plan, d_ws, sz.workspace_bytes, st));
-    FZCHK(fz_decompress_f32(plan, d_payload, d_tail, h.ebx2, d_ws, sz.workspace_bytes, d_out, st));
+    if (original.empty()) {
+        FZCHK(fz_decompress_f32(plan, d_payload, d_tail, h.ebx2, d_ws, sz.workspace_bytes, d_out, st));
+    } else {
+        if (file_size(original) != (long)(4 * n)) {
+            std::fprintf(stderr, "fz-gpu: %s does not hold %llu floats\n", original.c_str(), (unsigned long long)n);
+            return 2;
+        }
+        std::vector<float> orig(n);
+        FILE*              fo = std::fopen(original.c_str(), "rb");
+        if (!fo || std::fread(orig.data(), 4, n, fo) != n) {
+            std::fprintf(stderr, "fz-gpu: cannot read %s\n", original.c_str());
+            return 2;
+        }
+        std::fclose(fo);
+        float*    d_orig;
+        fz_stats* d_stats;
+        HIPCHK(hipMalloc(&d_orig, 4 * n + 16));
+        HIPCHK(hipMalloc(&d_stats, 3 * sizeof(fz_stats)));
+        HIPCHK(hipMemcpy(d_orig, orig.data(), 4 * n, hipMemcpyHostToDevice));
+        FZCHK(fz_decompress_verify_f32(plan, d_payload, d_tail, h.ebx2, d_ws, sz.workspace_bytes, d_out, d_orig,
+                                       d_stats, st));
+        // lag-1 and lag-2 autocorrelation of the original (quality_viewer.hh:105-108)
+        double lag[2] = {NAN, NAN};
+        size_t tb = 0;
+        FZCHK(fz_verify_f32(nullptr, &tb, nullptr, nullptr, nullptr, n, st));
+        void* d_temp;
+        HIPCHK(hipMalloc(&d_temp, tb));
+        for (int l = 1; l <= 2; l++)
+            if (n > (uint64_t)l) FZCHK(fz_verify_f32(d_temp, &tb, d_stats + l, d_orig, d_orig + l, n - l, st));
+        fz_stats hs[3];
+        HIPCHK(hipMemcpyAsync(hs, d_stats, sizeof(hs), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (int l = 1; l <= 2; l++)
+            if (n > (uint64_t)l) lag[l - 1] = hs[l].reduced.coeff;
+        const double archive_bytes = (double)file_size(arc);
+        print_metrics(hs[0], archive_bytes, lag[0], lag[1]);
+        (void)hipFree(d_temp);
+        (void)hipFree(d_orig);
+        (void)hipFree(d_stats);
+    }
     HIPCHK(hipStreamSynchronize(st));
     std::vector<float> host(n);
     HIPCHK(hipMemcpy(host.data(), d_out, 4 * n, hipMemcpyDeviceToHost));
@@ -103,8 +167,8 @@ static int decompress_main(const std::string& arc, const std::string& out)
 int main(int argc, char* argv[])
 {
     if (argc >= 2 && std::strcmp(argv[1], "-d") == 0) {
-        if (argc != 4) return usage();
-        return decompress_main(argv[2], argv[3]);
+        if (argc != 4 && argc != 5) return usage();
+        return decompress_main(argv[2], argv[3], argc == 5 ? argv[4] : "");
     }
     if (argc < 6) return usage();
     const std::string fileName(argv[1]);

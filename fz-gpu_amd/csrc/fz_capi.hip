@@ -17,7 +17,7 @@ constexpr uint64_t kMaxLen = 0xFFFFF000ull;  // positions are 32-bit inside the 
 inline size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 struct Layout {
-    size_t ctr, part, tstate, sstate, dstate, codes, shuf, F, P, texcl, counts, stage, total;
+    size_t ctr, part, tstate, sstate, dstate, codes, shuf, F, P, texcl, counts, stage, vtemp, total;
     uint32_t n_tstate, n_sstate, n_dstate;
 };
 
@@ -41,6 +41,7 @@ Layout make_layout(const Geom& g)
     L.texcl = o;  o = al256(o + 4ull * (g.T + 1));
     L.counts = o; o = al256(o + 4ull * (g.T + 1));
     L.stage = o;  o = al256(o + 4096ull * g.T + 64);
+    L.vtemp = o;  o = al256(o + verify_workspace_bytes((uint64_t)g.xyz));
     L.total = o;
     return L;
 }
@@ -310,6 +311,36 @@ fz_status fz_decompress_f32(const fz_plan plan, const uint8_t* d_payload, const 
     FZ_TRY(launch_decode_unshuffle(d_payload, texcl, tail, g, codes, stream));
     FZ_TRY(prof_mark(plan, stream, false));
     FZ_TRY(launch_reconstruct(reinterpret_cast<const uint16_t*>(codes), d_out, g, ebx2, stream));
+    FZ_TRY(prof_mark(plan, stream, false));
+    return FZ_SUCCESS;
+}
+
+fz_status fz_verify_f32(void* d_temp, size_t* temp_bytes, fz_stats* d_stats, const float* d_xdata,
+                        const float* d_odata, uint64_t len, hipStream_t stream)
+{
+    if (!temp_bytes || len == 0) return FZ_ERR_INVALID_ARG;
+    const size_t need = verify_workspace_bytes(len);
+    if (!d_temp) {
+        *temp_bytes = need;
+        return FZ_SUCCESS;
+    }
+    if (!d_stats || !d_xdata || !d_odata || !aligned(d_temp, 256) || !aligned(d_stats, 8) ||
+        !aligned(d_xdata, 4) || !aligned(d_odata, 4))
+        return FZ_ERR_INVALID_ARG;
+    if (*temp_bytes < need) return FZ_ERR_WORKSPACE_TOO_SMALL;
+    FZ_TRY(launch_verify(d_xdata, d_odata, len, d_temp, d_stats, stream));
+    return FZ_SUCCESS;
+}
+
+fz_status fz_decompress_verify_f32(const fz_plan plan, const uint8_t* d_payload, const uint16_t* d_tail,
+                                   float ebx2, void* d_ws, size_t ws_bytes, float* d_out, const float* d_odata,
+                                   fz_stats* d_stats, hipStream_t stream)
+{
+    if (!plan || !d_odata || !d_stats || !aligned(d_odata, 4) || !aligned(d_stats, 8)) return FZ_ERR_INVALID_ARG;
+    const fz_status st = fz_decompress_f32(plan, d_payload, d_tail, ebx2, d_ws, ws_bytes, d_out, stream);
+    if (st != FZ_SUCCESS) return st;
+    uint8_t* ws = static_cast<uint8_t*>(d_ws);
+    FZ_TRY(launch_verify(d_out, d_odata, plan->g.xyz, ws + plan->L.vtemp, d_stats, stream));
     FZ_TRY(prof_mark(plan, stream, false));
     return FZ_SUCCESS;
 }

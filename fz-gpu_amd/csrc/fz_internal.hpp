@@ -62,6 +62,10 @@ hipError_t launch_decode_unshuffle(const uint8_t* payload, const uint32_t* tile_
 hipError_t launch_reconstruct(const uint16_t* codes, float* out, const Geom& g, float ebx2, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 
+// Quality metrics (fz_verify.hip): scratch bytes for n elements, and the two-kernel launch.
+size_t     verify_workspace_bytes(uint64_t n);
+hipError_t launch_verify(const float* x, const float* o, uint64_t n, void* temp, fz_stats* stats, hipStream_t s);
+
 // ============================================================================ batched form
 //
 // Many fields of uneven shapes compressed by one launch sequence.  Fields are ordered by

@@ -11,6 +11,8 @@ Mirrors of the reference's surface:
     exclusive_sum(...)                  -> cub::DeviceScan::ExclusiveSum (fz.cu:188-192)
     encode_blocks(...)                  -> fz.cu:49 encodeDebug
     BatchPlan                           -> runSzs over many fields in one launch sequence
+    verify(xdata, odata)                -> verify.hh:25 verify_data (quality metrics)
+    quality_report(xdata, odata, ...)   -> quality_viewer.hh:97 echo_metric_gpu
 """
 import ctypes
 import os
@@ -63,7 +65,43 @@ EXPORTS = ["fz_version", "fz_status_string", "fz_last_error", "fz_plan_create", 
            "fz_batch_plan_create", "fz_batch_plan_destroy", "fz_batch_plan_sizes", "fz_batch_field_sizes",
            "fz_batch_get_tuning", "fz_batch_set_tuning", "fz_batch_workspace_init", "fz_batch_compress_f32",
            "fz_batch_decompress_f32", "fz_batch_autotune", "fz_batch_profile", "fz_batch_kernel_times",
-           "fz_batch_compress_graph", "fz_batch_decompress_graph", "fz_graph_launch", "fz_graph_destroy"]
+           "fz_batch_compress_graph", "fz_batch_decompress_graph", "fz_graph_launch", "fz_graph_destroy",
+           "fz_verify_f32", "fz_decompress_verify_f32"]
+
+
+class _StatsRange(ctypes.Structure):
+    _fields_ = [("min", ctypes.c_double), ("max", ctypes.c_double), ("rng", ctypes.c_double),
+                ("std", ctypes.c_double)]
+
+
+class _StatsReduced(ctypes.Structure):
+    _fields_ = [("PSNR", ctypes.c_double), ("MSE", ctypes.c_double), ("NRMSE", ctypes.c_double),
+                ("coeff", ctypes.c_double)]
+
+
+class _StatsMaxErr(ctypes.Structure):
+    _fields_ = [("abs", ctypes.c_double), ("rel", ctypes.c_double), ("pwrrel", ctypes.c_double),
+                ("idx", ctypes.c_uint64)]
+
+
+class _StatsAutocor(ctypes.Structure):
+    _fields_ = [("lag_one", ctypes.c_double), ("lag_two", ctypes.c_double)]
+
+
+class Stats(ctypes.Structure):
+    """fz_stats = cusz_stats (type.h:185-194) with size_t as uint64."""
+    _fields_ = [("odata", _StatsRange), ("xdata", _StatsRange), ("reduced", _StatsReduced),
+                ("max_err", _StatsMaxErr), ("autocor", _StatsAutocor), ("user_eb", ctypes.c_double),
+                ("len", ctypes.c_uint64)]
+
+    def as_dict(self):
+        def sub(st):
+            return {k: getattr(st, k) for k, _ in st._fields_}
+        return {"odata": sub(self.odata), "xdata": sub(self.xdata), "reduced": sub(self.reduced),
+                "max_err": sub(self.max_err), "len": self.len}
+
+
+assert ctypes.sizeof(Stats) == 160
 
 
 class FieldShape(ctypes.Structure):
@@ -141,6 +179,8 @@ def lib():
     L.fz_batch_decompress_graph.argtypes = [vp, vp, vp, vp, vp, ctypes.c_size_t, vp, ctypes.POINTER(vp)]
     L.fz_graph_launch.argtypes = [vp, vp]
     L.fz_graph_destroy.argtypes = [vp]
+    L.fz_verify_f32.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t), vp, vp, vp, u64, vp]
+    L.fz_decompress_verify_f32.argtypes = [vp, vp, vp, f32, vp, ctypes.c_size_t, vp, vp, vp, vp]
     for name in EXPORTS:
         if name not in ("fz_version", "fz_status_string", "fz_last_error"):
             getattr(L, name).restype = i32
@@ -257,6 +297,98 @@ class Plan:
         _check(lib().fz_decompress_f32(self.handle, _ptr(src), _ptr(t), float(ebx2), _ptr(self.ws), self.ws.numel(),
                                        _ptr(out), _stream(stream)), "fz_decompress")
         return out
+
+    def decompress_verify_async(self, payload, tail, ebx2, odata, out, d_stats, stream=None):
+        """Launch only: decompress into out, then the metrics against odata into d_stats
+        (160 device bytes).  payload must be 16-byte aligned."""
+        t = tail if (tail is not None and tail.numel() > 0) else None
+        _check(lib().fz_decompress_verify_f32(self.handle, _ptr(payload), _ptr(t), float(ebx2), _ptr(self.ws),
+                                              self.ws.numel(), _ptr(out), _ptr(odata), _ptr(d_stats),
+                                              _stream(stream)), "fz_decompress_verify")
+
+    def decompress_verify(self, payload, tail, ebx2, odata, out=None, stream=None):
+        """Decompress, then the quality metrics of the result against odata on the same
+        stream.  Returns (out, stats dict)."""
+        import torch
+        n = self.x * self.y * self.z
+        if out is None:
+            out = torch.empty(n, dtype=torch.float32, device=self.device)
+        src = payload
+        if payload.data_ptr() % 16:
+            src = payload.clone()
+        d_stats = torch.zeros(ctypes.sizeof(Stats), dtype=torch.uint8, device=self.device)
+        self.decompress_verify_async(src, tail, ebx2, odata, out, d_stats, stream)
+        return out, read_stats(d_stats)
+
+
+def read_stats(stats_tensor):
+    """fz_stats bytes on the device -> dict (waits for the stream)."""
+    raw = stats_tensor.cpu().numpy().tobytes()
+    return Stats.from_buffer_copy(raw[: ctypes.sizeof(Stats)]).as_dict()
+
+
+def verify_async(d_xdata, d_odata, d_stats, temp=None, length=None, stream=None):
+    """fz_verify_f32: metrics of d_xdata against d_odata into d_stats (160 device bytes);
+    returns the temp tensor so a caller can reuse it."""
+    n = d_odata.numel() if length is None else int(length)
+    nbytes = ctypes.c_size_t(0)
+    _check(lib().fz_verify_f32(None, ctypes.byref(nbytes), None, None, None, n, None), "fz_verify_f32(query)")
+    if temp is None or temp.numel() < nbytes.value:
+        temp = _empty(nbytes.value, d_odata.device)
+    nbytes = ctypes.c_size_t(temp.numel())
+    _check(lib().fz_verify_f32(_ptr(temp), ctypes.byref(nbytes), _ptr(d_stats), _ptr(d_xdata), _ptr(d_odata), n,
+                               _stream(stream)), "fz_verify_f32")
+    return temp
+
+
+def verify(d_xdata, d_odata, stream=None):
+    """verify_data (verify.hh:25-83) on the device: returns the stats dict."""
+    import torch
+    if d_xdata.numel() != d_odata.numel():
+        raise FzError(1, "verify: arrays of different lengths")
+    d_stats = torch.zeros(ctypes.sizeof(Stats), dtype=torch.uint8, device=d_odata.device)
+    verify_async(d_xdata, d_odata, d_stats, stream=stream)
+    return read_stats(d_stats)
+
+
+def quality_report(d_xdata, d_odata, compressed_bytes=0, stream=None, echo=True):
+    """echo_metric_gpu (quality_viewer.hh:97-110): cross metrics of the reconstruction, then
+    the lag-1 and lag-2 autocorrelation of the original (odata vs odata shifted by 1 and 2),
+    printed in the layout of print_metrics_cross / print_metrics_auto (:53-95)."""
+    x = d_xdata.reshape(-1)
+    o = d_odata.reshape(-1)
+    s = verify(x, o, stream)
+    n = o.numel()
+    s["autocor"] = {"lag_one": verify(o[: n - 1], o[1:], stream)["reduced"]["coeff"] if n > 1 else float("nan"),
+                    "lag_two": verify(o[: n - 2], o[2:], stream)["reduced"]["coeff"] if n > 2 else float("nan")}
+    if echo:
+        print(format_report(s, compressed_bytes))
+    return s
+
+
+def format_report(s, compressed_bytes=0):
+    """Text of print_metrics_cross + print_metrics_auto (quality_viewer.hh:53-95) for float data."""
+    def head(*c):
+        return "  %-10s %16s %16s %16s %16s" % c
+
+    def row(name, *v):
+        return "  %-10s %16.8g %16.8g %16.8g %16.8g" % ((name,) + v)
+
+    cr = (s["len"] * 4.0 / compressed_bytes) if compressed_bytes else float("inf")
+    lines = ["", "quality metrics (using GPU checker):",
+             head("", "data-len", "data-byte", "fp-type?", ""),
+             "  %-10s %16d %16d %16s" % ("", s["len"], 4, "yes"),
+             head("", "min", "max", "rng", "std"),
+             row("origin", s["odata"]["min"], s["odata"]["max"], s["odata"]["rng"], s["odata"]["std"]),
+             row("eb-lossy", s["xdata"]["min"], s["xdata"]["max"], s["xdata"]["rng"], s["xdata"]["std"]),
+             head("", "abs-val", "abs-idx", "pw-rel", "VS-RNG"),
+             row("max-error", s["max_err"]["abs"], s["max_err"]["idx"], s["max_err"]["pwrrel"], s["max_err"]["rel"]),
+             head("", "CR", "NRMSE", "cross-cor", "PSNR"),
+             row("metrics", cr, s["reduced"]["NRMSE"], s["reduced"]["coeff"], s["reduced"]["PSNR"])]
+    if "autocor" in s:
+        lines += [head("", "lag1-cor", "lag2-cor", "", ""),
+                  "  %-10s %16f %16f" % ("auto", s["autocor"]["lag_one"], s["autocor"]["lag_two"])]
+    return "\n".join(lines)
 
 
 def read_result(res_tensor):

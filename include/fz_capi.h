@@ -136,6 +136,47 @@ fz_status fz_plan_set_tile_layout(fz_plan plan, int wave_tile);
 fz_status fz_plan_profile(fz_plan plan, int enable);
 fz_status fz_plan_kernel_times(const fz_plan plan, float* ms, int capacity, int* count);
 
+/* ---- quality metrics of a reconstruction (verify.hh:25-83, quality_viewer.hh:97-110) ---- */
+
+/* cusz_stats (type.h:185-194), size_t as uint64_t. */
+typedef struct fz_stats_range {
+    double min, max, rng, std;
+} fz_stats_range;
+typedef struct fz_stats {
+    fz_stats_range odata, xdata;
+    struct {
+        double PSNR, MSE, NRMSE, coeff;
+    } reduced;
+    struct {
+        double   abs, rel, pwrrel;
+        uint64_t idx;
+    } max_err;
+    struct {
+        double lag_one, lag_two;
+    } autocor;
+    double   user_eb;
+    uint64_t len;
+} fz_stats;
+
+/* verify_data (verify.hh:25) on the device: the metrics of reconstruction d_xdata against
+ * original d_odata, written to *d_stats (device memory) with the reference's definitions:
+ * population std, max_err.rel = max |x-o| / odata range, pwrrel = max over odata != 0 of the
+ * float |x-o|/|o|, idx = first index of the largest |x-o|, PSNR = 20 log10(range) -
+ * 10 log10(MSE), coeff = Pearson correlation.  xdata.max repeats the reference's scan of
+ * odata (verify.hh:44): max(xdata[0], max odata).  autocor and user_eb are not written.
+ * Call with d_temp == NULL to get *temp_bytes (at most 90 KiB), then again to run: one
+ * streaming pass and one finishing workgroup.  Arrays need 4-byte alignment (16 for the
+ * vector path), so the lag-1/lag-2 autocorrelation calls of quality_viewer.hh:105-107
+ * (origin vs origin+1, len-1) work as they are. */
+fz_status fz_verify_f32(void* d_temp, size_t* temp_bytes, fz_stats* d_stats, const float* d_xdata,
+                        const float* d_odata, uint64_t len, hipStream_t stream);
+
+/* fz_decompress_f32 followed on the same stream by fz_verify_f32 of d_out against d_odata
+ * (x*y*z floats; the plan's workspace holds the metric partials). */
+fz_status fz_decompress_verify_f32(const fz_plan plan, const uint8_t* d_payload, const uint16_t* d_tail,
+                                   float ebx2, void* d_ws, size_t ws_bytes, float* d_out, const float* d_odata,
+                                   fz_stats* d_stats, hipStream_t stream);
+
 /* ---- per-stage entry points (the reference's kernel boundaries) ---- */
 
 /* launch_construct_LorenzoI_var<float, uint16_t, float> (lorenzo_var.cuh:520-597).

@@ -8,6 +8,7 @@
  *   fz.cu:187-201            (cub::DeviceScan::ExclusiveSum + host total)
  *   fz.cu:49-59              (encodeDebug)
  *   fz.cu:113-209            (runSzs orchestration, sizes)
+ *   verify.hh:25-83          (verify_data: quality metrics of a reconstruction)
  * Each CUDA workgroup is a loop iteration, each thread a nested loop, each
  * __syncthreads a phase boundary, each __ballot_sync / __shfl_*_sync a loop over lanes.
  * The inverse (absent from the reference binary) follows the block structure of
@@ -594,4 +595,66 @@ int fzo_decompress(const uint8_t* payload, uint64_t payload_bytes, const uint16_
     free(shuffled);
     free(codes);
     return FZO_SUCCESS;
+}
+
+/* verify.hh:25-83.  T = float: abserr is a float (fabs of a float difference, :46), so is
+ * its quotient by fabs(odata[i]) (:48) and its square (:58); every other quantity is a
+ * double.  Two passes as in the reference: the means first (:31-34), then one loop over
+ * extrema, errors and centred products (:39-59). */
+void fzo_verify(const float* xdata, const float* odata, uint64_t len, fzo_stats* s)
+{
+    double max_odata = odata[0], min_odata = odata[0];
+    double max_xdata = xdata[0], min_xdata = xdata[0];
+    double max_abserr = fabsf(xdata[0] - odata[0]);
+
+    double sum_0 = 0, sum_x = 0;
+    for (uint64_t i = 0; i < len; i++) {
+        sum_0 += odata[i];
+        sum_x += xdata[i];
+    }
+    const double mean_odata = sum_0 / (double)len, mean_xdata = sum_x / (double)len;
+    double sum_var_odata = 0, sum_var_xdata = 0, sum_err2 = 0, sum_corr = 0;
+    double max_pwrrel_abserr = 0;
+    uint64_t max_abserr_index = 0;
+    for (uint64_t i = 0; i < len; i++) {
+        const float o = odata[i], x = xdata[i];
+        max_odata = max_odata < o ? o : max_odata;
+        min_odata = min_odata > o ? o : min_odata;
+        /* verify.hh:44 scans odata for the xdata maximum; kept for identical output */
+        max_xdata = max_xdata < o ? o : max_xdata;
+        min_xdata = min_xdata > x ? x : min_xdata;
+
+        const float abserr = fabsf(x - o);
+        if (o != 0) {
+            const double rel_abserr = (float)(abserr / fabsf(o));
+            max_pwrrel_abserr = max_pwrrel_abserr < rel_abserr ? rel_abserr : max_pwrrel_abserr;
+        }
+        max_abserr_index = max_abserr < abserr ? i : max_abserr_index;
+        max_abserr = max_abserr < abserr ? abserr : max_abserr;
+        sum_corr += (o - mean_odata) * (x - mean_xdata);
+        sum_var_odata += (o - mean_odata) * (o - mean_odata);
+        sum_var_xdata += (x - mean_xdata) * (x - mean_xdata);
+        sum_err2 += (float)(abserr * abserr);
+    }
+    const double std_odata = sqrt(sum_var_odata / (double)len);
+    const double std_xdata = sqrt(sum_var_xdata / (double)len);
+    const double ee = sum_corr / (double)len;
+
+    s->len = len;
+    s->odata.max = max_odata;
+    s->odata.min = min_odata;
+    s->odata.rng = max_odata - min_odata;
+    s->odata.std = std_odata;
+    s->xdata.max = max_xdata;
+    s->xdata.min = min_xdata;
+    s->xdata.rng = max_xdata - min_xdata;
+    s->xdata.std = std_xdata;
+    s->max_err.idx = max_abserr_index;
+    s->max_err.abs = max_abserr;
+    s->max_err.rel = max_abserr / s->odata.rng;
+    s->max_err.pwrrel = max_pwrrel_abserr;
+    s->reduced.coeff = ee / std_odata / std_xdata;
+    s->reduced.MSE = sum_err2 / (double)len;
+    s->reduced.NRMSE = sqrt(s->reduced.MSE) / s->odata.rng;
+    s->reduced.PSNR = 20 * log10(s->odata.rng) - 10 * log10(s->reduced.MSE);
 }

@@ -89,6 +89,22 @@ int fzo_decompress(const uint8_t* payload, uint64_t payload_bytes, const uint16_
                    uint64_t len, uint32_t x, uint32_t y, uint32_t z, float ebx2, int nthreads, float* xdata,
                    uint16_t* dbg_codes);
 
+/* Quality metrics of a reconstruction (cusz_stats, type.h:185-194; size_t -> uint64_t). */
+typedef struct {
+    struct { double min, max, rng, std; } odata, xdata;
+    struct { double PSNR, MSE, NRMSE, coeff; } reduced;
+    struct { double abs, rel, pwrrel; uint64_t idx; } max_err;
+    struct { double lag_one, lag_two; } autocor;
+    double   user_eb;
+    uint64_t len;
+} fzo_stats;
+
+/* verify_data (verify.hh:25-83), sequential, in the reference's order and types: double
+ * sums and two passes; float |x-o| and float |x-o|/|o|; max_err.idx is the first index of
+ * the largest error.  xdata.max repeats the reference's scan over odata (verify.hh:44).
+ * autocor and user_eb are not written. */
+void fzo_verify(const float* xdata, const float* odata, uint64_t len, fzo_stats* s);
+
 #ifdef __cplusplus
 }
 #endif

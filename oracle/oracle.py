@@ -31,6 +31,37 @@ class Info(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class _Res(ctypes.Structure):
+    _fields_ = [("min", ctypes.c_double), ("max", ctypes.c_double), ("rng", ctypes.c_double),
+                ("std", ctypes.c_double)]
+
+
+class _Reduced(ctypes.Structure):
+    _fields_ = [("PSNR", ctypes.c_double), ("MSE", ctypes.c_double), ("NRMSE", ctypes.c_double),
+                ("coeff", ctypes.c_double)]
+
+
+class _MaxErr(ctypes.Structure):
+    _fields_ = [("abs", ctypes.c_double), ("rel", ctypes.c_double), ("pwrrel", ctypes.c_double),
+                ("idx", ctypes.c_uint64)]
+
+
+class _Autocor(ctypes.Structure):
+    _fields_ = [("lag_one", ctypes.c_double), ("lag_two", ctypes.c_double)]
+
+
+class Stats(ctypes.Structure):
+    """cusz_stats (type.h:185-194) with size_t as uint64; same layout as fz_stats."""
+    _fields_ = [("odata", _Res), ("xdata", _Res), ("reduced", _Reduced), ("max_err", _MaxErr),
+                ("autocor", _Autocor), ("user_eb", ctypes.c_double), ("len", ctypes.c_uint64)]
+
+    def as_dict(self):
+        def sub(st):
+            return {k: getattr(st, k) for k, _ in st._fields_}
+        return {"odata": sub(self.odata), "xdata": sub(self.xdata), "reduced": sub(self.reduced),
+                "max_err": sub(self.max_err), "len": self.len}
+
+
 _lib = None
 
 
@@ -68,6 +99,8 @@ def lib():
         L.fzo_compress.restype = i32
         L.fzo_decompress.argtypes = [vp, u64, vp, u64, u64, u32, u32, u32, f32, i32, vp, vp]
         L.fzo_decompress.restype = i32
+        L.fzo_verify.argtypes = [vp, vp, u64, ctypes.POINTER(Stats)]
+        L.fzo_verify.restype = None
         _lib = L
     return _lib
 
@@ -188,3 +221,14 @@ def decompress(payload, tail, n, dims, ebx2, nthreads=1, debug=False):
     if debug:
         return rc, out, codes[:n]
     return rc, out
+
+
+def verify(xdata, odata):
+    """verify_data (verify.hh:25-83) of reconstruction xdata against original odata."""
+    x = np.ascontiguousarray(xdata, dtype=np.float32).ravel()
+    o = np.ascontiguousarray(odata, dtype=np.float32).ravel()
+    if x.size != o.size or x.size == 0:
+        raise ValueError("verify needs two non-empty arrays of one length")
+    st = Stats()
+    lib().fzo_verify(_p(x), _p(o), x.size, ctypes.byref(st))
+    return st.as_dict()

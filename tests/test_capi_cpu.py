@@ -117,3 +117,19 @@ def test_batch_shapes_of_the_uneven_workload():
     assert {f["eb"] for f in spec} == {1e-2, 1e-3, 1e-4}
     bp = fzgpu.BatchPlan([f["dims"] for f in spec], workspace=False)
     assert bp.sizes.total_len == sum(n)
+
+
+def test_verify_size_query_and_argument_checks_need_no_device():
+    L = fzgpu.lib()
+    n = ctypes.c_size_t(0)
+    assert L.fz_verify_f32(None, ctypes.byref(n), None, None, None, 1 << 20, None) == 0
+    big = n.value
+    assert 0 < big <= 90 * 1024
+    assert L.fz_verify_f32(None, ctypes.byref(n), None, None, None, 5, None) == 0
+    assert 0 < n.value <= big
+    assert L.fz_verify_f32(None, ctypes.byref(n), None, None, None, 0, None) == 1
+    assert L.fz_verify_f32(None, None, None, None, None, 5, None) == 1
+    # a non-null temp with missing arrays is rejected before any launch
+    assert L.fz_verify_f32(ctypes.c_void_p(256), ctypes.byref(n), None, None, None, 5, None) == 1
+    p = fzgpu.Plan(100, 77, 1, workspace=False)
+    assert L.fz_decompress_verify_f32(p.handle, None, None, 1.0, None, 0, None, None, None, None) == 1
