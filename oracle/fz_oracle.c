@@ -385,6 +385,45 @@ void fzo_encode(const uint8_t* in, uint8_t* out, const uint32_t* presum, uint64_
 
 /* ------------------------------------------------------------------ compress */
 
+/* The stages after the dual-quant (fz.cu:154-205), shared by every mode: bitshuffle + flags
+ * of the full tiles, exclusive scan of the byte flags, payload = B || compacted blocks. */
+static void encode_codes(const uint16_t* codes, uint64_t len, uint8_t* payload, fzo_info* info, int nthreads,
+                         uint32_t* dbg_shuffled, uint32_t* dbg_byteflags, uint32_t* dbg_presum)
+{
+    const uint64_t T = info->T, W = info->W, N = info->N;
+    /* fz.cu:154-155: shuffled buffer starts as a copy of the codes (tail stays raw) */
+    uint8_t* shuffled = (uint8_t*)malloc((size_t)(2 * len + 4));
+    memcpy(shuffled, codes, (size_t)(2 * len));
+    uint32_t* bitflags = (uint32_t*)calloc((size_t)W, sizeof(uint32_t));
+    uint32_t* byteflags = (uint32_t*)calloc((size_t)N, sizeof(uint32_t));
+    fzo_bitshuffle_flags((const uint32_t*)codes, (uint32_t*)shuffled, T, bitflags, byteflags, nthreads);
+
+    /* fz.cu:180-201 */
+    uint32_t* presum = (uint32_t*)calloc((size_t)N + 1, sizeof(uint32_t));
+    fzo_exclusive_sum(byteflags, presum, N, nthreads);
+    presum[N] = presum[N - 1] + byteflags[N - 1];
+    info->K = presum[N];
+    info->payload_bytes = 4 * W + 16 * info->K;
+
+    /* payload = B || compacted blocks */
+    for (uint64_t i = 0; i < W; i++) {
+        const uint32_t v = bitflags[i];
+        payload[4 * i + 0] = (uint8_t)(v);
+        payload[4 * i + 1] = (uint8_t)(v >> 8);
+        payload[4 * i + 2] = (uint8_t)(v >> 16);
+        payload[4 * i + 3] = (uint8_t)(v >> 24);
+    }
+    fzo_encode(shuffled, payload + 4 * W, presum, N, nthreads);
+
+    if (dbg_shuffled) memcpy(dbg_shuffled, shuffled, (size_t)(2 * len));
+    if (dbg_byteflags) memcpy(dbg_byteflags, byteflags, (size_t)(4 * N));
+    if (dbg_presum) memcpy(dbg_presum, presum, (size_t)(4 * (N + 1)));
+    free(shuffled);
+    free(bitflags);
+    free(byteflags);
+    free(presum);
+}
+
 /* runSzs (fz.cu:113-209) without file I/O.  The payload is the reference's output bytes:
  * LE u32 bitflags B[0..W) followed by d_out[0..16K) (size formula fz.cu:208). */
 int fzo_compress(const float* data, uint64_t len, uint32_t x, uint32_t y, uint32_t z, double eb_rel, int mode,
@@ -417,43 +456,163 @@ int fzo_compress(const float* data, uint64_t len, uint32_t x, uint32_t y, uint32
     uint16_t* codes = (uint16_t*)calloc((size_t)len + 2, sizeof(uint16_t));
     info->overflow = fzo_dualquant(data, codes, x, y, z, info->ebx2_r, mode, nthreads);
 
-    /* fz.cu:154-155: shuffled buffer starts as a copy of the codes (tail stays raw) */
-    uint8_t* shuffled = (uint8_t*)malloc((size_t)(2 * len + 4));
-    memcpy(shuffled, codes, (size_t)(2 * len));
-    uint32_t* bitflags = (uint32_t*)calloc((size_t)W, sizeof(uint32_t));
-    uint32_t* byteflags = (uint32_t*)calloc((size_t)N, sizeof(uint32_t));
-    fzo_bitshuffle_flags((const uint32_t*)codes, (uint32_t*)shuffled, T, bitflags, byteflags, nthreads);
-
-    /* fz.cu:180-201 */
-    uint32_t* presum = (uint32_t*)calloc((size_t)N + 1, sizeof(uint32_t));
-    fzo_exclusive_sum(byteflags, presum, N, nthreads);
-    presum[N] = presum[N - 1] + byteflags[N - 1];
-    info->K = presum[N];
-    info->payload_bytes = 4 * W + 16 * info->K;
-
-    /* payload = B || compacted blocks */
-    for (uint64_t i = 0; i < W; i++) {
-        const uint32_t v = bitflags[i];
-        payload[4 * i + 0] = (uint8_t)(v);
-        payload[4 * i + 1] = (uint8_t)(v >> 8);
-        payload[4 * i + 2] = (uint8_t)(v >> 16);
-        payload[4 * i + 3] = (uint8_t)(v >> 24);
-    }
-    fzo_encode(shuffled, payload + 4 * W, presum, N, nthreads);
-
+    encode_codes(codes, len, payload, info, nthreads, dbg_shuffled, dbg_byteflags, dbg_presum);
     if (mode == FZO_MODE_FIXED && tail)
         for (uint64_t i = 0; i < info->tail_len; i++) tail[i] = codes[2048 * T + i];
-
     if (dbg_codes) memcpy(dbg_codes, codes, (size_t)(2 * len));
-    if (dbg_shuffled) memcpy(dbg_shuffled, shuffled, (size_t)(2 * len));
-    if (dbg_byteflags) memcpy(dbg_byteflags, byteflags, (size_t)(4 * N));
-    if (dbg_presum) memcpy(dbg_presum, presum, (size_t)(4 * (N + 1)));
     free(codes);
-    free(shuffled);
-    free(bitflags);
-    free(byteflags);
-    free(presum);
     return info->status = FZO_SUCCESS;
+}
+
+/* ------------------------------------------------------------------ outlier mode */
+
+uint64_t fzo_outlier_max_bytes(uint64_t len) { return 16 + 8 * len; }
+
+/* quantised integer of one value; raw = kept verbatim (not finite, or |fl(v r)| >= 2^24) */
+static inline int32_t quant_i(float v, float r, int* raw)
+{
+    const float t = v * r;
+    if (!isfinite(v) || !(fabsf(t) < 16777216.0f)) {
+        *raw = 1;
+        return 0;
+    }
+    *raw = 0;
+    return (int32_t)roundf(t);
+}
+
+static void put_u32(uint8_t* p, uint32_t v)
+{
+    p[0] = (uint8_t)v;
+    p[1] = (uint8_t)(v >> 8);
+    p[2] = (uint8_t)(v >> 16);
+    p[3] = (uint8_t)(v >> 24);
+}
+
+static uint32_t get_u32(const uint8_t* p)
+{
+    return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+}
+
+static void put_u64(uint8_t* p, uint64_t v)
+{
+    put_u32(p, (uint32_t)v);
+    put_u32(p + 4, (uint32_t)(v >> 32));
+}
+
+static uint64_t get_u64(const uint8_t* p) { return (uint64_t)get_u32(p) | ((uint64_t)get_u32(p + 4) << 32); }
+
+/* Range of the finite values (0, 0 when there are none), eb parameters, then the
+ * block-local Lorenzo deltas of the quantised integers in exact integer arithmetic, in the
+ * reference's block shapes (256 / 16x16 / 8x8x8, neighbours outside the block are 0). */
+int fzo_compress_outlier(const float* data, uint64_t len, uint32_t x, uint32_t y, uint32_t z, double eb_rel,
+                         uint8_t* payload, uint16_t* tail, uint8_t* side, uint64_t side_cap, fzo_info* info,
+                         uint16_t* dbg_codes)
+{
+    memset(info, 0, sizeof(*info));
+    info->len = len;
+    if (len == 0 || x == 0 || y == 0 || z == 0 || (uint64_t)x * y * z > len || !(eb_rel > 0) || !isfinite(eb_rel))
+        return info->status = FZO_ERR_INVALID_ARG;
+    const int      nd = fzo_ndim(x, y, z);
+    const uint64_t xyz = (uint64_t)x * y * z;
+    info->ndim = nd;
+    info->T = fzo_tiles(len);
+    info->W = fzo_flag_words(len);
+    info->N = 32 * info->W;
+    info->tail_len = len - 2048 * info->T;
+
+    float lo = INFINITY, hi = -INFINITY;
+    for (uint64_t i = 0; i < len; i++)
+        if (isfinite(data[i])) {
+            lo = data[i] < lo ? data[i] : lo;
+            hi = data[i] > hi ? data[i] : hi;
+        }
+    if (lo > hi) lo = hi = 0.0f;
+    info->vmin = lo;
+    info->vmax = hi;
+    const float range = hi - lo;
+    info->range = range;
+    if (!isfinite(range)) return info->status = FZO_ERR_NONFINITE; /* the range itself overflows */
+    if (range == 0.0f) {
+        info->eb_abs = 0.0;
+        info->ebx2_r = 0.0f;
+        info->ebx2 = 0.0f; /* constant: every quantised integer is 0, the inverse fills vmin */
+    } else {
+        fzo_eb_params(range, eb_rel, &info->eb_abs, &info->ebx2_r, &info->ebx2);
+    }
+
+    int32_t*  q = (int32_t*)calloc((size_t)xyz, sizeof(int32_t));
+    uint8_t*  raw = (uint8_t*)calloc((size_t)xyz, 1);
+    uint16_t* codes = (uint16_t*)calloc((size_t)len + 2, sizeof(uint16_t));
+    for (uint64_t i = 0; i < xyz; i++) {
+        int r;
+        q[i] = quant_i(data[i], info->ebx2_r, &r);
+        raw[i] = (uint8_t)r;
+        info->n_raw += (uint64_t)r;
+    }
+    uint32_t* idx_d = (uint32_t*)malloc((size_t)(4 * xyz + 4));
+    int32_t*  val_d = (int32_t*)malloc((size_t)(4 * xyz + 4));
+    uint64_t  nd_out = 0;
+    const uint64_t sy = x, sz = (uint64_t)x * y;
+    for (uint64_t p = 0; p < xyz; p++) {
+        int64_t d;
+        if (nd == 1) {
+            d = (int64_t)q[p] - ((p % 256) ? q[p - 1] : 0);
+        } else if (nd == 2) {
+            const uint64_t ix = p % x, iy = p / x;
+            const int bx = (ix % 16) != 0, by = (iy % 16) != 0;
+            d = (int64_t)q[p] - (bx ? q[p - 1] : 0) - (by ? q[p - sy] : 0) + ((bx && by) ? q[p - sy - 1] : 0);
+        } else {
+            const uint64_t ix = p % x, iy = (p / x) % y, iz = p / sz;
+            const int bx = (ix % 8) != 0, by = (iy % 8) != 0, bz = (iz % 8) != 0;
+            d = (int64_t)q[p] - (bx ? q[p - 1] : 0) - (by ? q[p - sy] : 0) - (bz ? q[p - sz] : 0) +
+                ((bx && by) ? q[p - sy - 1] : 0) + ((bx && bz) ? q[p - sz - 1] : 0) +
+                ((by && bz) ? q[p - sz - sy] : 0) - ((bx && by && bz) ? q[p - sz - sy - 1] : 0);
+        }
+        if (d >= 32768 || d <= -32768) {
+            idx_d[nd_out] = (uint32_t)p;
+            val_d[nd_out] = (int32_t)d;
+            nd_out++;
+            codes[p] = 0;
+        } else {
+            codes[p] = (uint16_t)((d < 0 ? -d : d) | (d < 0 ? 0x8000 : 0));
+        }
+    }
+    info->overflow = nd_out;
+    info->side_bytes = 16 + 8 * (nd_out + info->n_raw);
+
+    encode_codes(codes, len, payload, info, 1, NULL, NULL, NULL);
+    if (tail)
+        for (uint64_t i = 0; i < info->tail_len; i++) tail[i] = codes[2048 * info->T + i];
+    if (dbg_codes) memcpy(dbg_codes, codes, (size_t)(2 * len));
+
+    int rc = FZO_SUCCESS;
+    if (info->side_bytes > side_cap) {
+        rc = FZO_ERR_CAPACITY;
+    } else if (side) {
+        const uint64_t nr = info->n_raw;
+        uint8_t*       o = side;
+        put_u64(o, nd_out);
+        put_u64(o + 8, nr);
+        for (uint64_t k = 0; k < nd_out; k++) {
+            put_u32(o + 16 + 4 * k, idx_d[k]);
+            put_u32(o + 16 + 4 * nd_out + 4 * k, (uint32_t)val_d[k]);
+        }
+        uint64_t k = 0;
+        for (uint64_t p = 0; p < xyz; p++)
+            if (raw[p]) {
+                uint32_t bits;
+                memcpy(&bits, &data[p], 4);
+                put_u32(o + 16 + 8 * nd_out + 4 * k, (uint32_t)p);
+                put_u32(o + 16 + 8 * nd_out + 4 * nr + 4 * k, bits);
+                k++;
+            }
+    }
+    free(q);
+    free(raw);
+    free(codes);
+    free(idx_d);
+    free(val_d);
+    return info->status = rc;
 }
 
 /* ------------------------------------------------------------------ inverse */
@@ -483,8 +642,8 @@ static inline int32_t decode(uint16_t c)
  * x_lorenzo_1d1l (256), x_lorenzo_2d1l (16x16: y then x, lorenzo_var.cuh:408-428) and
  * x_lorenzo_3d1l (8x8x8: y, x, z, :473-508), in exact int32, then one float multiply by
  * ebx2 (:357, :428, :515). */
-void fzo_reconstruct(const uint16_t* delta, float* xdata, uint32_t x, uint32_t y, uint32_t z, float ebx2,
-                     int nthreads)
+static void reconstruct_i32(const int32_t* delta, float* xdata, uint32_t x, uint32_t y, uint32_t z, float ebx2,
+                            int nthreads)
 {
     const int nd = fzo_ndim(x, y, z);
     if (nd == 1) {
@@ -493,7 +652,7 @@ void fzo_reconstruct(const uint16_t* delta, float* xdata, uint32_t x, uint32_t y
         for (int64_t b = 0; b < nblk; b++) {
             int32_t acc = 0;
             for (uint64_t id = (uint64_t)b * 256; id < (uint64_t)b * 256 + 256 && id < x; id++) {
-                acc += decode(delta[id]);
+                acc += delta[id];
                 xdata[id] = (float)acc * ebx2;
             }
         }
@@ -506,7 +665,7 @@ void fzo_reconstruct(const uint16_t* delta, float* xdata, uint32_t x, uint32_t y
             for (int iy = 0; iy < 16; iy++)
                 for (int ix = 0; ix < 16; ix++) {
                     const uint64_t gxi = bx + ix, gyi = by + iy;
-                    s[iy][ix] = (gxi < x && gyi < y) ? decode(delta[gyi * x + gxi]) : 0;
+                    s[iy][ix] = (gxi < x && gyi < y) ? delta[gyi * x + gxi] : 0;
                 }
             for (int iy = 1; iy < 16; iy++)
                 for (int ix = 0; ix < 16; ix++) s[iy][ix] += s[iy - 1][ix];
@@ -530,7 +689,7 @@ void fzo_reconstruct(const uint16_t* delta, float* xdata, uint32_t x, uint32_t y
                 for (int iy = 0; iy < 8; iy++)
                     for (int ix = 0; ix < 8; ix++) {
                         const uint64_t gxi = bx + ix, gyi = by + iy, gzi = bz + iz;
-                        s[iz][iy][ix] = (gxi < x && gyi < y && gzi < z) ? decode(delta[gzi * sz + gyi * sy + gxi]) : 0;
+                        s[iz][iy][ix] = (gxi < x && gyi < y && gzi < z) ? delta[gzi * sz + gyi * sy + gxi] : 0;
                     }
             for (int iz = 0; iz < 8; iz++)
                 for (int iy = 1; iy < 8; iy++)
@@ -551,14 +710,22 @@ void fzo_reconstruct(const uint16_t* delta, float* xdata, uint32_t x, uint32_t y
     }
 }
 
-/* Build-defined inverse of runSzs: unpack B, exclusive-scan the flags, scatter the
- * compacted blocks back into the shuffled tiles, unshuffle, append the tail trailer,
- * reconstruct. */
-int fzo_decompress(const uint8_t* payload, uint64_t payload_bytes, const uint16_t* tail, uint64_t tail_len,
-                   uint64_t len, uint32_t x, uint32_t y, uint32_t z, float ebx2, int nthreads, float* xdata,
-                   uint16_t* dbg_codes)
+void fzo_reconstruct(const uint16_t* delta, float* xdata, uint32_t x, uint32_t y, uint32_t z, float ebx2,
+                     int nthreads)
 {
-    if (len == 0 || x == 0 || y == 0 || z == 0 || (uint64_t)x * y * z > len) return FZO_ERR_INVALID_ARG;
+    const uint64_t xyz = (uint64_t)x * y * z;
+    int32_t*       d = (int32_t*)malloc((size_t)(4 * xyz));
+    for (uint64_t i = 0; i < xyz; i++) d[i] = decode(delta[i]);
+    reconstruct_i32(d, xdata, x, y, z, ebx2, nthreads);
+    free(d);
+}
+
+/* Build-defined inverse of runSzs, up to the codes: unpack B, exclusive-scan the flags,
+ * scatter the compacted blocks back into the shuffled tiles, unshuffle, append the tail
+ * trailer.  codes has room for len + 2. */
+static int decode_codes(const uint8_t* payload, uint64_t payload_bytes, const uint16_t* tail, uint64_t tail_len,
+                        uint64_t len, int nthreads, uint16_t* codes)
+{
     const uint64_t T = fzo_tiles(len), W = fzo_flag_words(len), N = 32 * W;
     if (payload_bytes < 4 * W) return FZO_ERR_CORRUPT;
     if (tail && tail_len != len - 2048 * T) return FZO_ERR_INVALID_ARG;
@@ -584,16 +751,66 @@ int fzo_decompress(const uint8_t* payload, uint64_t payload_bytes, const uint16_
     const uint8_t* blocks = payload + 4 * W;
     for (uint64_t i = 0; i < 256 * T; i++)
         if (flags[i]) memcpy((uint8_t*)shuffled + 16 * i, blocks + 16 * (uint64_t)presum[i], 16);
-    uint16_t* codes = (uint16_t*)calloc((size_t)len + 2, sizeof(uint16_t));
+    memset(codes, 0, (size_t)(2 * len + 4));
     fzo_unshuffle(shuffled, (uint32_t*)codes, T, nthreads);
     if (tail)
         for (uint64_t i = 0; i < tail_len; i++) codes[2048 * T + i] = tail[i];
-    fzo_reconstruct(codes, xdata, x, y, z, ebx2, nthreads);
-    if (dbg_codes) memcpy(dbg_codes, codes, (size_t)(2 * len));
     free(flags);
     free(presum);
     free(shuffled);
+    return FZO_SUCCESS;
+}
+
+int fzo_decompress(const uint8_t* payload, uint64_t payload_bytes, const uint16_t* tail, uint64_t tail_len,
+                   uint64_t len, uint32_t x, uint32_t y, uint32_t z, float ebx2, int nthreads, float* xdata,
+                   uint16_t* dbg_codes)
+{
+    if (len == 0 || x == 0 || y == 0 || z == 0 || (uint64_t)x * y * z > len) return FZO_ERR_INVALID_ARG;
+    uint16_t* codes = (uint16_t*)calloc((size_t)len + 2, sizeof(uint16_t));
+    const int rc = decode_codes(payload, payload_bytes, tail, tail_len, len, nthreads, codes);
+    if (rc == FZO_SUCCESS) {
+        fzo_reconstruct(codes, xdata, x, y, z, ebx2, nthreads);
+        if (dbg_codes) memcpy(dbg_codes, codes, (size_t)(2 * len));
+    }
     free(codes);
+    return rc;
+}
+
+/* Inverse of the OUTLIER mode: codes -> int32 deltas, the side channel's deltas in place of
+ * their (zero) codes, block prefix sums times ebx2 (vconst everywhere when ebx2 == 0), then
+ * the verbatim values. */
+int fzo_decompress_outlier(const uint8_t* payload, uint64_t payload_bytes, const uint16_t* tail, uint64_t tail_len,
+                           const uint8_t* side, uint64_t side_bytes, uint64_t len, uint32_t x, uint32_t y,
+                           uint32_t z, float ebx2, float vconst, float* xdata)
+{
+    if (len == 0 || x == 0 || y == 0 || z == 0 || (uint64_t)x * y * z > len) return FZO_ERR_INVALID_ARG;
+    if (side_bytes < 16) return FZO_ERR_CORRUPT;
+    const uint64_t nd = get_u64(side), nr = get_u64(side + 8), xyz = (uint64_t)x * y * z;
+    if (nd > xyz || nr > xyz || side_bytes != 16 + 8 * (nd + nr)) return FZO_ERR_CORRUPT;
+    for (uint64_t k = 0; k < nd; k++)
+        if (get_u32(side + 16 + 4 * k) >= xyz) return FZO_ERR_CORRUPT;
+    for (uint64_t k = 0; k < nr; k++)
+        if (get_u32(side + 16 + 8 * nd + 4 * k) >= xyz) return FZO_ERR_CORRUPT;
+    if (ebx2 == 0.0f) {
+        for (uint64_t i = 0; i < xyz; i++) xdata[i] = vconst;
+    } else {
+        uint16_t* codes = (uint16_t*)calloc((size_t)len + 2, sizeof(uint16_t));
+        const int rc = decode_codes(payload, payload_bytes, tail, tail_len, len, 1, codes);
+        if (rc != FZO_SUCCESS) {
+            free(codes);
+            return rc;
+        }
+        int32_t* d = (int32_t*)malloc((size_t)(4 * xyz));
+        for (uint64_t i = 0; i < xyz; i++) d[i] = decode(codes[i]);
+        for (uint64_t k = 0; k < nd; k++) d[get_u32(side + 16 + 4 * k)] = (int32_t)get_u32(side + 16 + 4 * nd + 4 * k);
+        reconstruct_i32(d, xdata, x, y, z, ebx2, 1);
+        free(d);
+        free(codes);
+    }
+    for (uint64_t k = 0; k < nr; k++) {
+        const uint32_t bits = get_u32(side + 16 + 8 * nd + 4 * nr + 4 * k);
+        memcpy(&xdata[get_u32(side + 16 + 8 * nd + 4 * k)], &bits, 4);
+    }
     return FZO_SUCCESS;
 }
 

@@ -30,6 +30,7 @@ extern "C" {
 
 #define FZO_MODE_FIXED 0
 #define FZO_MODE_LITERAL 1
+#define FZO_MODE_OUTLIER 2 /* FIXED format + side channel (this build's extension, no reference form) */
 
 /* status codes: same values as fz_status in include/fz_capi.h */
 #define FZO_SUCCESS 0
@@ -53,6 +54,8 @@ typedef struct {
     uint64_t payload_bytes; /* 4 W + 16 K                           */
     uint64_t tail_len;      /* len - 2048 T                         */
     uint64_t overflow;      /* deltas with |d| >= 32768 (code aliases the sign bit) */
+    uint64_t n_raw;         /* OUTLIER mode: values kept verbatim                    */
+    uint64_t side_bytes;    /* OUTLIER mode: 16 + 8 (overflow + n_raw)               */
 } fzo_info;
 
 /* geometry helpers (fz.cu:157-161, 173, 177-178; lorenzo_var.cuh:537-544) */
@@ -76,6 +79,16 @@ void fzo_unshuffle(const uint32_t* in, uint32_t* out, uint64_t ntiles, int nthre
 void fzo_reconstruct(const uint16_t* delta, float* xdata, uint32_t x, uint32_t y, uint32_t z, float ebx2,
                      int nthreads);
 
+/* OUTLIER mode side channel, one byte string:
+ *   u64 n_delta, u64 n_raw,
+ *   u32 idx_d[n_delta], i32 val_d[n_delta],   deltas with |d| >= 32768 (their code is 0)
+ *   u32 idx_r[n_raw],   u32 val_r[n_raw]      input bits of values kept verbatim
+ * both lists in increasing index order.  A value is kept verbatim when it is not finite or
+ * when |fl(x * ebx2_r)| >= 2^24 (its quantised integer would not be exact in binary32); its
+ * quantised integer is then 0.  Deltas are exact integers (block-local Lorenzo on the
+ * quantised integers), so the inverse is exact wherever the error bound is defined. */
+uint64_t fzo_outlier_max_bytes(uint64_t len);
+
 /* whole pipeline.  payload capacity must be >= fzo_payload_max_bytes(len); tail has room
  * for 2047 codes.  The intermediate arrays of the reference (codes, shuffled words,
  * byte flags, prefix sums) are written to the optional dbg_* pointers when non-NULL:
@@ -88,6 +101,17 @@ int fzo_compress(const float* data, uint64_t len, uint32_t x, uint32_t y, uint32
 int fzo_decompress(const uint8_t* payload, uint64_t payload_bytes, const uint16_t* tail, uint64_t tail_len,
                    uint64_t len, uint32_t x, uint32_t y, uint32_t z, float ebx2, int nthreads, float* xdata,
                    uint16_t* dbg_codes);
+
+/* OUTLIER mode: fzo_compress with mode FZO_MODE_OUTLIER also writes the side channel
+ * (capacity side_cap bytes; FZO_ERR_CAPACITY when it does not fit).  The inverse takes the
+ * side channel and vconst (the compress record's vmin), used when ebx2 == 0 (constant
+ * field: every finite value equals vmin). */
+int fzo_compress_outlier(const float* data, uint64_t len, uint32_t x, uint32_t y, uint32_t z, double eb_rel,
+                         uint8_t* payload, uint16_t* tail, uint8_t* side, uint64_t side_cap, fzo_info* info,
+                         uint16_t* dbg_codes);
+int fzo_decompress_outlier(const uint8_t* payload, uint64_t payload_bytes, const uint16_t* tail, uint64_t tail_len,
+                           const uint8_t* side, uint64_t side_bytes, uint64_t len, uint32_t x, uint32_t y,
+                           uint32_t z, float ebx2, float vconst, float* xdata);
 
 /* Quality metrics of a reconstruction (cusz_stats, type.h:185-194; size_t -> uint64_t). */
 typedef struct {

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 MODE_FIXED = 0
 MODE_LITERAL = 1
+MODE_OUTLIER = 2
 
 
 class Info(ctypes.Structure):
@@ -24,7 +25,7 @@ class Info(ctypes.Structure):
         ("eb_abs", ctypes.c_double),
         ("len", ctypes.c_uint64), ("T", ctypes.c_uint64), ("W", ctypes.c_uint64), ("N", ctypes.c_uint64),
         ("K", ctypes.c_uint64), ("payload_bytes", ctypes.c_uint64), ("tail_len", ctypes.c_uint64),
-        ("overflow", ctypes.c_uint64),
+        ("overflow", ctypes.c_uint64), ("n_raw", ctypes.c_uint64), ("side_bytes", ctypes.c_uint64),
     ]
 
     def as_dict(self):
@@ -99,6 +100,12 @@ def lib():
         L.fzo_compress.restype = i32
         L.fzo_decompress.argtypes = [vp, u64, vp, u64, u64, u32, u32, u32, f32, i32, vp, vp]
         L.fzo_decompress.restype = i32
+        L.fzo_outlier_max_bytes.argtypes = [u64]
+        L.fzo_outlier_max_bytes.restype = u64
+        L.fzo_compress_outlier.argtypes = [vp, u64, u32, u32, u32, f64, vp, vp, vp, u64, ctypes.POINTER(Info), vp]
+        L.fzo_compress_outlier.restype = i32
+        L.fzo_decompress_outlier.argtypes = [vp, u64, vp, u64, vp, u64, u64, u32, u32, u32, f32, f32, vp]
+        L.fzo_decompress_outlier.restype = i32
         L.fzo_verify.argtypes = [vp, vp, u64, ctypes.POINTER(Stats)]
         L.fzo_verify.restype = None
         _lib = L
@@ -232,3 +239,46 @@ def verify(xdata, odata):
     st = Stats()
     lib().fzo_verify(_p(x), _p(o), x.size, ctypes.byref(st))
     return st.as_dict()
+
+
+def compress_outlier(data, dims, eb_rel, side_cap=None, debug=False):
+    """OUTLIER mode: dict(status, info, payload, tail, side (the side channel bytes))."""
+    data = np.ascontiguousarray(data, dtype=np.float32).ravel()
+    n = data.size
+    payload = np.zeros(payload_max_bytes(n) + 16, dtype=np.uint8)
+    tail = np.zeros(2048, dtype=np.uint16)
+    cap = lib().fzo_outlier_max_bytes(n) if side_cap is None else int(side_cap)
+    side = np.zeros(max(cap, 16), dtype=np.uint8)
+    codes = np.zeros(n + 2, dtype=np.uint16) if debug else None
+    info = Info()
+    rc = lib().fzo_compress_outlier(_p(data), n, dims[0], dims[1], dims[2], float(eb_rel), _p(payload), _p(tail),
+                                    _p(side), cap, ctypes.byref(info), _p(codes))
+    res = dict(status=rc, info=info.as_dict())
+    if rc in (0, 7):
+        res["payload"] = payload[: info.payload_bytes].copy()
+        res["tail"] = tail[: info.tail_len].copy()
+    if rc == 0:
+        res["side"] = side[: info.side_bytes].copy()
+    if debug:
+        res["codes"] = codes[:n]
+    return res
+
+
+def decompress_outlier(payload, tail, side, n, dims, ebx2, vconst):
+    payload = np.ascontiguousarray(payload, dtype=np.uint8)
+    side = np.ascontiguousarray(side, dtype=np.uint8)
+    out = np.zeros(dims[0] * dims[1] * dims[2], dtype=np.float32)
+    t = None if tail is None else np.ascontiguousarray(tail, dtype=np.uint16)
+    rc = lib().fzo_decompress_outlier(_p(payload), payload.size, _p(t), 0 if t is None else t.size, _p(side),
+                                      side.size, n, dims[0], dims[1], dims[2], float(ebx2), float(vconst), _p(out))
+    return rc, out
+
+
+def side_lists(side):
+    """(idx_d, val_d, idx_r, val_r) of a side channel byte string."""
+    side = np.ascontiguousarray(side, dtype=np.uint8)
+    nd, nr = (int(v) for v in side[:16].view(np.uint64))
+    body = side[16:]
+    w = body.view(np.uint32)
+    return (w[:nd].copy(), w[nd:2 * nd].view(np.int32).copy(), w[2 * nd:2 * nd + nr].copy(),
+            w[2 * nd + nr:2 * nd + 2 * nr].copy())
