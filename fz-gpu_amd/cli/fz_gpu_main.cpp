@@ -3,6 +3,8 @@
 // of fz.cu:207-209.  Extra behaviour sits behind trailing options:
 //   -o <archive>   write the compressed archive (header + payload + tail)
 //   --literal      byte-identical reference mode (1D sign loss and dropped tail included)
+//   --outliers     outlier mode: FIXED payload + side channel (large deltas, NaN/Inf, values
+//                  too large to quantise, constant fields); the archive carries the channel
 //   --plain        run the plain per-stage pipeline instead of the fused kernel
 //   --json         print one JSON line of statistics after the three lines
 // and a decompress mode, with the quality metrics of the reconstruction when the original
@@ -49,7 +51,7 @@ static long file_size(const std::string& name)  // fz.cu:19-24
 static int usage()
 {
     std::fprintf(stderr,
-                 "usage: fz-gpu <file> <x> <y> <z> <eb> [-o archive] [--literal] [--plain] [--json]\n"
+                 "usage: fz-gpu <file> <x> <y> <z> <eb> [-o archive] [--literal | --outliers] [--plain] [--json]\n"
                  "       fz-gpu -d <archive> <out.f32> [original.f32]\n");
     return 1;
 }
@@ -82,7 +84,8 @@ static int decompress_main(const std::string& arc, const std::string& out, const
     fzar::Header          h;
     std::vector<uint8_t>  payload;
     std::vector<uint16_t> tail;
-    if (!fzar::read_archive(arc, h, payload, tail)) {
+    std::vector<uint8_t>  side;
+    if (!fzar::read_archive(arc, h, payload, tail, &side)) {
         std::fprintf(stderr, "fz-gpu: cannot read archive %s\n", arc.c_str());
         return 2;
     }
@@ -106,8 +109,16 @@ static int decompress_main(const std::string& arc, const std::string& out, const
     hipStream_t st;
     HIPCHK(hipStreamCreate(&st));
     FZCHK(fz_workspace_init(plan, d_ws, sz.workspace_bytes, st));
+    const bool outlier = h.mode == FZ_MODE_OUTLIER;
+    uint8_t*   d_side = nullptr;
+    if (outlier) {
+        HIPCHK(hipMalloc(&d_side, side.size()));
+        HIPCHK(hipMemcpy(d_side, side.data(), side.size(), hipMemcpyHostToDevice));
+        FZCHK(fz_decompress_outlier_f32(plan, d_payload, d_tail, h.ebx2, h.vmin, d_side, side.size(), d_ws,
+                                        sz.workspace_bytes, d_out, st));
+    }
     if (original.empty()) {
-        FZCHK(fz_decompress_f32(plan, d_payload, d_tail, h.ebx2, d_ws, sz.workspace_bytes, d_out, st));
+        if (!outlier) FZCHK(fz_decompress_f32(plan, d_payload, d_tail, h.ebx2, d_ws, sz.workspace_bytes, d_out, st));
     } else {
         if (file_size(original) != (long)(4 * n)) {
             std::fprintf(stderr, "fz-gpu: %s does not hold %llu floats\n", original.c_str(), (unsigned long long)n);
@@ -125,14 +136,17 @@ static int decompress_main(const std::string& arc, const std::string& out, const
         HIPCHK(hipMalloc(&d_orig, 4 * n + 16));
         HIPCHK(hipMalloc(&d_stats, 3 * sizeof(fz_stats)));
         HIPCHK(hipMemcpy(d_orig, orig.data(), 4 * n, hipMemcpyHostToDevice));
-        FZCHK(fz_decompress_verify_f32(plan, d_payload, d_tail, h.ebx2, d_ws, sz.workspace_bytes, d_out, d_orig,
-                                       d_stats, st));
-        // lag-1 and lag-2 autocorrelation of the original (quality_viewer.hh:105-108)
-        double lag[2] = {NAN, NAN};
         size_t tb = 0;
         FZCHK(fz_verify_f32(nullptr, &tb, nullptr, nullptr, nullptr, n, st));
         void* d_temp;
         HIPCHK(hipMalloc(&d_temp, tb));
+        if (outlier)  // already decompressed: the metrics alone
+            FZCHK(fz_verify_f32(d_temp, &tb, d_stats, d_out, d_orig, n, st));
+        else
+            FZCHK(fz_decompress_verify_f32(plan, d_payload, d_tail, h.ebx2, d_ws, sz.workspace_bytes, d_out, d_orig,
+                                           d_stats, st));
+        // lag-1 and lag-2 autocorrelation of the original (quality_viewer.hh:105-108)
+        double lag[2] = {NAN, NAN};
         for (int l = 1; l <= 2; l++)
             if (n > (uint64_t)l) FZCHK(fz_verify_f32(d_temp, &tb, d_stats + l, d_orig, d_orig + l, n - l, st));
         fz_stats hs[3];
@@ -159,6 +173,7 @@ static int decompress_main(const std::string& arc, const std::string& out, const
     (void)hipFree(d_out);
     (void)hipFree(d_ws);
     if (d_tail) (void)hipFree(d_tail);
+    if (d_side) (void)hipFree(d_side);
     (void)hipStreamDestroy(st);
     fz_plan_destroy(plan);
     return 0;
@@ -182,6 +197,8 @@ int main(int argc, char* argv[])
             out_path = argv[++i];
         else if (!std::strcmp(argv[i], "--literal"))
             mode = FZ_MODE_LITERAL;
+        else if (!std::strcmp(argv[i], "--outliers"))
+            mode = FZ_MODE_OUTLIER;
         else if (!std::strcmp(argv[i], "--plain"))
             plain = true;
         else if (!std::strcmp(argv[i], "--json"))
@@ -231,10 +248,17 @@ int main(int argc, char* argv[])
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, st));
-    if (plain)
+    uint8_t* d_side = nullptr;
+    if (mode == FZ_MODE_OUTLIER) {
+        if (plain) return usage();
+        HIPCHK(hipMalloc(&d_side, sz.side_max_bytes));
+        FZCHK(fz_compress_outlier_f32(plan, d_data, eb, d_ws, sz.workspace_bytes, d_payload, d_tail, d_side,
+                                      sz.side_max_bytes, d_res, st));
+    } else if (plain) {
         FZCHK(fz_compress_plain_f32(plan, d_data, eb, d_ws, sz.workspace_bytes, d_payload, d_tail, d_res, st));
-    else
+    } else {
         FZCHK(fz_compress_f32(plan, d_data, eb, d_ws, sz.workspace_bytes, d_payload, d_tail, d_res, st));
+    }
     HIPCHK(hipEventRecord(e1, st));
     fz_result r;
     HIPCHK(hipMemcpyAsync(&r, d_res, sizeof(r), hipMemcpyDeviceToHost, st));
@@ -279,13 +303,21 @@ int main(int argc, char* argv[])
         h.payload_bytes = r.payload_bytes;
         h.tail_len = r.tail_len;
         h.overflow = r.overflow;
-        if (!fzar::write_archive(out_path, h, payload.data(), tail.data())) {
+        std::vector<uint8_t> side;
+        if (d_side) {  // the channel's 16-byte header gives its size
+            uint64_t cnt[2];
+            HIPCHK(hipMemcpy(cnt, d_side, 16, hipMemcpyDeviceToHost));
+            side.resize(16 + 8 * (cnt[0] + cnt[1]));
+            HIPCHK(hipMemcpy(side.data(), d_side, side.size(), hipMemcpyDeviceToHost));
+        }
+        if (!fzar::write_archive(out_path, h, payload.data(), tail.data(), &side)) {
             std::fprintf(stderr, "fz-gpu: cannot write %s\n", out_path.c_str());
             return 2;
         }
     }
     (void)hipFree(d_data);
     (void)hipFree(d_payload);
+    if (d_side) (void)hipFree(d_side);
     (void)hipFree(d_tail);
     (void)hipFree(d_res);
     (void)hipFree(d_ws);

@@ -17,11 +17,12 @@ constexpr uint64_t kMaxLen = 0xFFFFF000ull;  // positions are 32-bit inside the 
 inline size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 struct Layout {
-    size_t ctr, part, tstate, sstate, dstate, codes, shuf, F, P, texcl, counts, stage, vtemp, total;
+    size_t ctr, part, tstate, sstate, dstate, codes, shuf, F, P, texcl, counts, stage, vtemp, ocnt, oofs, ochunk, dwide,
+        total;
     uint32_t n_tstate, n_sstate, n_dstate;
 };
 
-Layout make_layout(const Geom& g)
+Layout make_layout(const Geom& g, int mode)
 {
     Layout L{};
     const uint64_t N = 32ull * g.W;
@@ -42,6 +43,12 @@ Layout make_layout(const Geom& g)
     L.counts = o; o = al256(o + 4ull * (g.T + 1));
     L.stage = o;  o = al256(o + 4096ull * g.T + 64);
     L.vtemp = o;  o = al256(o + verify_workspace_bytes((uint64_t)g.xyz));
+    if (mode == FZ_MODE_OUTLIER) {  // per-unit side-channel counts and offsets; int32 deltas
+        L.ocnt = o;   o = al256(o + 8ull * (g.T + 1));
+        L.oofs = o;   o = al256(o + 8ull * (g.T + 1));
+        L.ochunk = o; o = al256(o + 8ull * ochunks(g.T));
+        L.dwide = o;  o = al256(o + 4ull * g.xyz + 64);
+    }
     L.total = o;
     return L;
 }
@@ -60,7 +67,7 @@ Geom make_geom(uint32_t x, uint32_t y, uint32_t z, uint64_t len, int mode)
     g.divX = make_fastdiv(x);
     g.divY = make_fastdiv(y);
     g.ndim = (z == 1 && y == 1) ? 1 : (z == 1 ? 2 : 3);
-    g.mode = mode;
+    g.mode = mode == FZ_MODE_LITERAL ? 1 : 0;  // kernels: 1 = the reference's 1D sign loss and no tail
     return g;
 }
 
@@ -144,11 +151,12 @@ fz_status fz_plan_create(fz_plan* plan, uint32_t x, uint32_t y, uint32_t z, uint
     if (!plan) return FZ_ERR_INVALID_ARG;
     *plan = nullptr;
     if (len == 0) len = (uint64_t)x * y * z;
-    if (!geom_ok(x, y, z, len) || (mode != FZ_MODE_FIXED && mode != FZ_MODE_LITERAL)) return FZ_ERR_INVALID_ARG;
+    if (!geom_ok(x, y, z, len) || (mode != FZ_MODE_FIXED && mode != FZ_MODE_LITERAL && mode != FZ_MODE_OUTLIER))
+        return FZ_ERR_INVALID_ARG;
     fz_plan p = new (std::nothrow) fz_plan_s;
     if (!p) return FZ_ERR_CAPACITY;
     p->g = make_geom(x, y, z, len, mode);
-    p->L = make_layout(p->g);
+    p->L = make_layout(p->g, mode);
     p->mode = mode;
     *plan = p;
     return FZ_SUCCESS;
@@ -206,6 +214,7 @@ fz_status fz_plan_sizes(const fz_plan plan, fz_sizes* s)
     s->payload_max_bytes = 4ull * g.W + 4096ull * g.T;
     s->tail_max_codes = (uint64_t)g.len - (uint64_t)kTileCodes * g.T;
     s->workspace_bytes = plan->L.total;
+    s->side_max_bytes = plan->mode == FZ_MODE_OUTLIER ? 16ull + 8ull * g.xyz : 0;
     return FZ_SUCCESS;
 }
 
@@ -218,14 +227,16 @@ fz_status fz_workspace_init(const fz_plan plan, void* d_ws, size_t ws_bytes, hip
 }
 
 static fz_status check_compress_args(const fz_plan plan, const float* d_in, double eb_rel, void* d_ws,
-                                     size_t ws_bytes, uint8_t* d_payload, uint16_t* d_tail, fz_result* d_result)
+                                     size_t ws_bytes, uint8_t* d_payload, uint16_t* d_tail, fz_result* d_result,
+                                     bool outlier = false)
 {
     if (!plan || !d_in || !d_ws || !d_payload || !d_result) return FZ_ERR_INVALID_ARG;
+    if ((plan->mode == FZ_MODE_OUTLIER) != outlier) return FZ_ERR_INVALID_ARG;  // the mode picks the entry point
     if (!(eb_rel > 0.0) || !std::isfinite(eb_rel)) return FZ_ERR_INVALID_ARG;
     if (!aligned(d_in, 16) || !aligned(d_payload, 16) || !aligned(d_ws, 256) || !aligned(d_result, 8))
         return FZ_ERR_INVALID_ARG;
     const Geom& g = plan->g;
-    if (plan->mode == FZ_MODE_FIXED && !d_tail && g.len != (uint64_t)kTileCodes * g.T) return FZ_ERR_INVALID_ARG;
+    if (plan->mode != FZ_MODE_LITERAL && !d_tail && g.len != (uint64_t)kTileCodes * g.T) return FZ_ERR_INVALID_ARG;
     if (ws_bytes < plan->L.total) return FZ_ERR_WORKSPACE_TOO_SMALL;
     return FZ_SUCCESS;
 }
@@ -281,7 +292,7 @@ fz_status fz_compress_plain_f32(const fz_plan plan, const float* d_in, double eb
     FZ_TRY(launch_encode(reinterpret_cast<const uint8_t*>(shuf), d_payload + 4ull * g.W, P, N, stream));
     FZ_TRY(prof_mark(plan, stream, false));
     const uint64_t tail_len = (uint64_t)g.len - (uint64_t)kTileCodes * g.T;
-    if (plan->mode == FZ_MODE_FIXED && tail_len)
+    if (plan->mode != FZ_MODE_LITERAL && tail_len)
         FZ_TRY(hipMemcpyAsync(d_tail, codes + (uint64_t)kTileCodes * g.T, 2 * tail_len, hipMemcpyDeviceToDevice,
                               stream));
     return FZ_SUCCESS;
@@ -290,7 +301,7 @@ fz_status fz_compress_plain_f32(const fz_plan plan, const float* d_in, double eb
 fz_status fz_decompress_f32(const fz_plan plan, const uint8_t* d_payload, const uint16_t* d_tail, float ebx2,
                             void* d_ws, size_t ws_bytes, float* d_out, hipStream_t stream)
 {
-    if (!plan || !d_payload || !d_ws || !d_out) return FZ_ERR_INVALID_ARG;
+    if (!plan || !d_payload || !d_ws || !d_out || plan->mode == FZ_MODE_OUTLIER) return FZ_ERR_INVALID_ARG;
     if (!aligned(d_payload, 16) || !aligned(d_ws, 256) || !aligned(d_out, 16)) return FZ_ERR_INVALID_ARG;
     if (!(ebx2 > 0.0f) || !std::isfinite(ebx2)) return FZ_ERR_INVALID_ARG;
     if (ws_bytes < plan->L.total) return FZ_ERR_WORKSPACE_TOO_SMALL;
@@ -301,7 +312,7 @@ fz_status fz_decompress_f32(const fz_plan plan, const uint8_t* d_payload, const 
     uint64_t*     dstate = reinterpret_cast<uint64_t*>(ws + L.dstate);
     uint32_t*     texcl = reinterpret_cast<uint32_t*>(ws + L.texcl);
     uint32_t*     codes = reinterpret_cast<uint32_t*>(ws + L.codes);
-    const uint16_t* tail = plan->mode == FZ_MODE_FIXED ? d_tail : nullptr;
+    const uint16_t* tail = plan->mode != FZ_MODE_LITERAL ? d_tail : nullptr;
     FZ_TRY(prof_mark(plan, stream, true));
     FZ_TRY(launch_scan_init(dstate, L.n_dstate, &ctr->dscan_ticket, stream));
     FZ_TRY(prof_mark(plan, stream, false));
@@ -311,6 +322,60 @@ fz_status fz_decompress_f32(const fz_plan plan, const uint8_t* d_payload, const 
     FZ_TRY(launch_decode_unshuffle(d_payload, texcl, tail, g, codes, stream));
     FZ_TRY(prof_mark(plan, stream, false));
     FZ_TRY(launch_reconstruct(reinterpret_cast<const uint16_t*>(codes), d_out, g, ebx2, stream));
+    FZ_TRY(prof_mark(plan, stream, false));
+    return FZ_SUCCESS;
+}
+
+fz_status fz_compress_outlier_f32(const fz_plan plan, const float* d_in, double eb_rel, void* d_ws,
+                                  size_t ws_bytes, uint8_t* d_payload, uint16_t* d_tail, uint8_t* d_side,
+                                  uint64_t side_capacity, fz_result* d_result, hipStream_t stream)
+{
+    fz_status st = check_compress_args(plan, d_in, eb_rel, d_ws, ws_bytes, d_payload, d_tail, d_result, true);
+    if (st != FZ_SUCCESS) return st;
+    if (!d_side || !aligned(d_side, 16) || side_capacity < 16) return FZ_ERR_INVALID_ARG;
+    const Geom&   g = plan->g;
+    const Layout& L = plan->L;
+    uint8_t*      ws = static_cast<uint8_t*>(d_ws);
+    FZ_TRY(prof_mark(plan, stream, true));
+    FZ_TRY(launch_compress_outlier(d_in, g, eb_rel, reinterpret_cast<float4*>(ws + L.part), d_result, d_payload,
+                                   d_tail, d_side, side_capacity, ws + L.stage,
+                                   reinterpret_cast<uint32_t*>(ws + L.counts), reinterpret_cast<uint32_t*>(ws + L.tstate),
+                                   L.n_tstate, reinterpret_cast<uint32_t*>(ws + L.ocnt),
+                                   reinterpret_cast<uint32_t*>(ws + L.oofs), reinterpret_cast<uint32_t*>(ws + L.ochunk),
+                                   stream));
+    FZ_TRY(prof_mark(plan, stream, false));
+    return FZ_SUCCESS;
+}
+
+fz_status fz_decompress_outlier_f32(const fz_plan plan, const uint8_t* d_payload, const uint16_t* d_tail, float ebx2,
+                                    float vconst, const uint8_t* d_side, uint64_t side_bytes, void* d_ws,
+                                    size_t ws_bytes, float* d_out, hipStream_t stream)
+{
+    if (!plan || plan->mode != FZ_MODE_OUTLIER || !d_payload || !d_side || !d_ws || !d_out) return FZ_ERR_INVALID_ARG;
+    if (!aligned(d_payload, 16) || !aligned(d_side, 16) || !aligned(d_ws, 256) || !aligned(d_out, 16))
+        return FZ_ERR_INVALID_ARG;
+    if (!(ebx2 >= 0.0f) || !std::isfinite(ebx2) || !std::isfinite(vconst) || side_bytes < 16)
+        return FZ_ERR_INVALID_ARG;
+    if (ws_bytes < plan->L.total) return FZ_ERR_WORKSPACE_TOO_SMALL;
+    const Geom&   g = plan->g;
+    const Layout& L = plan->L;
+    uint8_t*      ws = static_cast<uint8_t*>(d_ws);
+    FZ_TRY(prof_mark(plan, stream, true));
+    if (ebx2 == 0.0f) {  // constant field
+        FZ_TRY(launch_fill_outlier(d_side, side_bytes, d_out, g, vconst, stream));
+        FZ_TRY(prof_mark(plan, stream, false));
+        return FZ_SUCCESS;
+    }
+    Ctr*      ctr = reinterpret_cast<Ctr*>(ws + L.ctr);
+    uint64_t* dstate = reinterpret_cast<uint64_t*>(ws + L.dstate);
+    uint32_t* texcl = reinterpret_cast<uint32_t*>(ws + L.texcl);
+    uint32_t* codes = reinterpret_cast<uint32_t*>(ws + L.codes);
+    FZ_TRY(launch_scan_init(dstate, L.n_dstate, &ctr->dscan_ticket, stream));
+    FZ_TRY(launch_tile_scan(reinterpret_cast<const uint32_t*>(d_payload), g.T, texcl, dstate, &ctr->dscan_ticket,
+                            stream));
+    FZ_TRY(launch_decode_unshuffle(d_payload, texcl, d_tail, g, codes, stream));
+    FZ_TRY(launch_reconstruct_outlier(reinterpret_cast<const uint16_t*>(codes), reinterpret_cast<int32_t*>(ws + L.dwide),
+                                      d_side, side_bytes, d_out, g, ebx2, stream));
     FZ_TRY(prof_mark(plan, stream, false));
     return FZ_SUCCESS;
 }
@@ -876,6 +941,7 @@ fz_status fz_batch_field_sizes(const fz_batch_plan plan, uint32_t field, fz_size
         s->payload_max_bytes = 4ull * g.W + 4096ull * g.T;
         s->tail_max_codes = tail_codes(g);
         s->workspace_bytes = plan->L.total;
+        s->side_max_bytes = 0;  // batch plans have no outlier mode
         return FZ_SUCCESS;
     }
     return FZ_ERR_INVALID_ARG;

@@ -45,6 +45,10 @@ hipError_t launch_compress(const float* in, const Geom& g, double eb_rel, float4
                            uint8_t* payload, uint16_t* tail, uint8_t* staging, uint32_t* counts, uint64_t* states,
                            int wave_tile, hipEvent_t* ev, hipStream_t s);
 uint32_t   gather_grid(uint32_t T);
+uint32_t   gather_tpw(uint32_t T);
+// placement kernel of the fused compress on its own (run_sums filled by a tile kernel)
+hipError_t launch_scan_gather(const uint32_t* counts, uint32_t T, const uint8_t* staging, uint8_t* payload,
+                              const uint32_t* run_sums, fz_result* res, uint32_t W, hipStream_t s);
 // r_host / r_amax: ebx2_r and max|x| when res is null (the per-stage entry point)
 hipError_t launch_dualquant(const float* in, const Geom& g, float r_host, float r_amax, const fz_result* res,
                             uint16_t* codes, unsigned long long* ovf, hipStream_t s);
@@ -60,7 +64,24 @@ hipError_t launch_tile_scan(const uint32_t* bitflags, uint32_t T, uint32_t* tile
 hipError_t launch_decode_unshuffle(const uint8_t* payload, const uint32_t* tile_excl, const uint16_t* tail,
                                    const Geom& g, uint32_t* codes, hipStream_t s);
 hipError_t launch_reconstruct(const uint16_t* codes, float* out, const Geom& g, float ebx2, hipStream_t s);
+hipError_t launch_reconstruct_i32(const int32_t* deltas, float* out, const Geom& g, float ebx2, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+
+// Outlier mode (fz_outlier.hip).  Workspace pieces: ocnt / oofs hold 2 u32 per unit (T + 1
+// units), ochunk 2 u32 per 4096 units; dwide holds x*y*z int32 deltas for the inverse.
+constexpr uint32_t kOChunk = 4096;  // units per workgroup of the side-channel offset scan
+inline uint32_t    ochunks(uint32_t T) { return (T + 1 + kOChunk - 1) / kOChunk; }
+hipError_t launch_compress_outlier(const float* in, const Geom& g, double eb_rel, float4* part, fz_result* res,
+                                   uint8_t* payload, uint16_t* tail, uint8_t* side, uint64_t side_cap,
+                                   uint8_t* staging, uint32_t* counts, uint32_t* run_sums, uint32_t nsums,
+                                   uint32_t* ocnt, uint32_t* oofs, uint32_t* ochunk, hipStream_t s);
+// after the codes of the plain inverse are in `codes`: deltas, side-channel deltas, prefix sums,
+// verbatim values
+hipError_t launch_reconstruct_outlier(const uint16_t* codes, int32_t* dwide, const uint8_t* side,
+                                      uint64_t side_bytes, float* out, const Geom& g, float ebx2, hipStream_t s);
+// constant field: out = vconst, then the verbatim values
+hipError_t launch_fill_outlier(const uint8_t* side, uint64_t side_bytes, float* out, const Geom& g, float vconst,
+                               hipStream_t s);
 
 // Quality metrics (fz_verify.hip): scratch bytes for n elements, and the two-kernel launch.
 size_t     verify_workspace_bytes(uint64_t n);

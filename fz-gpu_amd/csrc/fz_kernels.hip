@@ -280,9 +280,18 @@ __global__ __launch_bounds__(kGatherThreads) void k_scan_gather(const uint32_t* 
     }
 }
 
-inline uint32_t gather_tpw(uint32_t T) { return (T + kGatherWG - 1) / kGatherWG; }
+uint32_t gather_tpw(uint32_t T) { return (T + kGatherWG - 1) / kGatherWG; }
 
 uint32_t gather_grid(uint32_t T) { return T == 0 ? 0u : (T + gather_tpw(T) - 1) / gather_tpw(T); }
+
+hipError_t launch_scan_gather(const uint32_t* counts, uint32_t T, const uint8_t* staging, uint8_t* payload,
+                              const uint32_t* run_sums, fz_result* res, uint32_t W, hipStream_t s)
+{
+    if (T == 0) return hipSuccess;
+    FZ_LAUNCH(k_scan_gather<<<gather_grid(T), kGatherThreads, 0, s>>>(counts, T, gather_tpw(T), staging, payload,
+                                                                        run_sums, res, W));
+    return hipGetLastError();
+}
 
 // Fused compress: range partials, tiles, scan + placement.  ev (may be null) holds start /
 // stop events for the three kernels.
@@ -622,37 +631,51 @@ hipError_t launch_decode_unshuffle(const uint8_t* payload, const uint32_t* tile_
     return hipGetLastError();
 }
 
-__global__ __launch_bounds__(kWG) void k_reconstruct_1d(const uint16_t* __restrict__ codes, float* __restrict__ out,
-                                                        Geom g, float ebx2)
+template <typename CT>
+__global__ __launch_bounds__(kWG) void k_reconstruct_1d(const CT* __restrict__ codes, float* __restrict__ out, Geom g,
+                                                        float ebx2)
 {
     recon_1d(codes, out, g, ebx2, blockIdx.x);
 }
 
-__global__ __launch_bounds__(kWG) void k_reconstruct_2d(const uint16_t* __restrict__ codes, float* __restrict__ out,
-                                                        Geom g, float ebx2)
+template <typename CT>
+__global__ __launch_bounds__(kWG) void k_reconstruct_2d(const CT* __restrict__ codes, float* __restrict__ out, Geom g,
+                                                        float ebx2)
 {
     recon_2d(codes, out, g, ebx2, blockIdx.x, blockIdx.y);
 }
 
-__global__ __launch_bounds__(kWG) void k_reconstruct_3d(const uint16_t* __restrict__ codes, float* __restrict__ out,
-                                                        Geom g, float ebx2)
+template <typename CT>
+__global__ __launch_bounds__(kWG) void k_reconstruct_3d(const CT* __restrict__ codes, float* __restrict__ out, Geom g,
+                                                        float ebx2)
 {
     recon_3d(codes, out, g, ebx2, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
-hipError_t launch_reconstruct(const uint16_t* codes, float* out, const Geom& g, float ebx2, hipStream_t s)
+template <typename CT>
+static hipError_t launch_reconstruct_t(const CT* codes, float* out, const Geom& g, float ebx2, hipStream_t s)
 {
     if (g.ndim == 1) {
         const uint32_t thr = (g.xyz + 15) / 16;
-        FZ_LAUNCH(k_reconstruct_1d<<<(thr + kWG - 1) / kWG, kWG, 0, s>>>(codes, out, g, ebx2));
+        FZ_LAUNCH(k_reconstruct_1d<CT><<<(thr + kWG - 1) / kWG, kWG, 0, s>>>(codes, out, g, ebx2));
     } else if (g.ndim == 2) {
         const dim3 grid((g.X + kRecon2DCols - 1) / kRecon2DCols, (g.Y + 15) / 16);
-        FZ_LAUNCH(k_reconstruct_2d<<<grid, kWG, 0, s>>>(codes, out, g, ebx2));
+        FZ_LAUNCH(k_reconstruct_2d<CT><<<grid, kWG, 0, s>>>(codes, out, g, ebx2));
     } else {
         const dim3 grid((g.X + kRecon3DCols - 1) / kRecon3DCols, (g.Y + 7) / 8, (g.Z + 7) / 8);
-        FZ_LAUNCH(k_reconstruct_3d<<<grid, kWG, 0, s>>>(codes, out, g, ebx2));
+        FZ_LAUNCH(k_reconstruct_3d<CT><<<grid, kWG, 0, s>>>(codes, out, g, ebx2));
     }
     return hipGetLastError();
+}
+
+hipError_t launch_reconstruct(const uint16_t* codes, float* out, const Geom& g, float ebx2, hipStream_t s)
+{
+    return launch_reconstruct_t(codes, out, g, ebx2, s);
+}
+
+hipError_t launch_reconstruct_i32(const int32_t* deltas, float* out, const Geom& g, float ebx2, hipStream_t s)
+{
+    return launch_reconstruct_t(deltas, out, g, ebx2, s);
 }
 
 }  // namespace fz

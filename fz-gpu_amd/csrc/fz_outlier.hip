@@ -1,0 +1,330 @@
+// fz_outlier.hip -- the outlier mode: the FIXED-mode payload plus a side channel that makes
+// every field compressible and every value recoverable.  It covers what the reference cannot
+// encode: deltas with |d| >= 32768 (their code aliases the sign bit), non-finite values, values
+// whose quantised integer is not exact in binary32 (|x * ebx2_r| >= 2^24) and constant fields
+// (the reference divides by a zero range).  The side channel follows cuSZ's sparse
+// (index, value) outlier vector (spcodec_vec.hh:29, launch_spv.cuh:22-58), as one byte string:
+//
+//   u64 n_delta, u64 n_raw,
+//   u32 idx_d[n_delta], i32 val_d[n_delta],   exact deltas of the positions whose code is 0
+//   u32 idx_r[n_raw],   u32 val_r[n_raw]      input bits of the values kept verbatim
+//
+// both lists in increasing index order (deterministic: per-unit counts, a scan, then each unit
+// writes its own entries).
+//
+// Compress:   k_range_part_o     min / max of the finite values, non-finite flag
+//             k_tile_compress_o  packed paths when nothing needs the side channel (level >= 3,
+//                                all finite), the integer path otherwise; per-unit counts
+//             k_scan_gather      placement, as in the fused path
+//             k_ocount_scan      per-unit offsets within 4096-unit chunks, chunk totals
+//             k_outlier_write    the units with entries recompute their deltas and write them
+// Decompress: the plain inverse's codes, then k_expand (codes -> int32 deltas), k_put_deltas,
+//             the reconstruct kernels on int32 deltas, k_put_raw.  A constant field is a fill.
+#include <hip/hip_runtime.h>
+
+#include "fz_internal.hpp"
+#include "fz_tile.hpp"
+
+namespace fz {
+
+namespace {
+
+constexpr int kORangeWG = 1024;  // threads per partial-range workgroup (as the fused path)
+
+__global__ __launch_bounds__(kORangeWG) void k_range_part_o(const float* __restrict__ in, Geom g,
+                                                            float4* __restrict__ part, uint32_t* run_sums,
+                                                            uint32_t nsums, fz_result* res)
+{
+    const uint32_t tid = threadIdx.x;
+    if (blockIdx.x == 0) {  // clear the placement run sums and the overflow count of this call
+        for (uint32_t i = tid; i < nsums; i += kORangeWG) run_sums[i] = 0;
+        if (tid == 0) res->overflow = 0;
+    }
+    const uint32_t per = (g.len + gridDim.x - 1) / gridDim.x;
+    const uint32_t f_lo = min(blockIdx.x * per, g.len), f_hi = min(f_lo + per, g.len);
+    range_span<kORangeWG, 8, true>(in, f_lo, f_hi, part + blockIdx.x);
+}
+
+template <int NDIM>
+__global__ __launch_bounds__(kWG) void k_tile_compress_o(const float* __restrict__ in, Geom g, double eb_rel,
+                                                         const float4* __restrict__ part, uint32_t nparts,
+                                                         fz_result* res, uint8_t* __restrict__ payload,
+                                                         uint8_t* __restrict__ staging, uint32_t* __restrict__ counts,
+                                                         uint16_t* __restrict__ tail, uint32_t* run_sums, uint32_t tpw,
+                                                         uint32_t* __restrict__ ocnt)
+{
+    const bool     is_tail = blockIdx.x >= g.T;
+    const uint32_t tile = is_tail ? g.T : xcd_tile(blockIdx.x, g.T);
+    const int      gp = is_tail ? 0 : geom_path<NDIM>(g, tile * (uint32_t)kTileCodes);
+    Partials       P;
+    RawTile<NDIM>  R;
+    partials_load(part, nparts, P);
+    raw_prefetch<NDIM>(in, tile * (uint32_t)kTileCodes, 8 * threadIdx.x, g, gp, R);
+    const EbParams e = partials_reduce<true>(P, eb_rel, NDIM);
+    if (blockIdx.x == 0 && threadIdx.x == 0) write_header(res, e, g);
+    if (e.status != ST_SUCCESS) return;  // uniform
+    unsigned long long* ovf = reinterpret_cast<unsigned long long*>(&res->overflow);
+    uint32_t*           bwords = reinterpret_cast<uint32_t*>(payload);
+    if (is_tail) {
+        tail_unit_o<NDIM>(in, g, e.r, bwords, tail, ovf, ocnt + 2 * g.T);
+        if (g.T == 0 && threadIdx.x == 0) {
+            res->nblocks = 0;
+            res->payload_bytes = 4ull * g.W;
+        }
+        return;
+    }
+    uint8_t* const  slot = staging + (size_t)tile * 4096;
+    uint32_t* const sum = run_sums + tile / tpw;
+    if (e.level >= 3) {
+        tile_encode<NDIM>(in, g, tile, gp, R, e.r, e.level, bwords, slot, counts + tile, ovf, sum);
+        if (threadIdx.x == 0) *reinterpret_cast<uint2*>(ocnt + 2 * tile) = make_uint2(0u, 0u);
+    } else {
+        tile_encode_o<NDIM>(in, g, tile, e.r, bwords, slot, counts + tile, ovf, sum, ocnt + 2 * tile);
+    }
+}
+
+// exclusive (overflow, verbatim) prefix of a pair over the workgroup (kWG threads), and the totals
+__device__ __forceinline__ uint2 block_excl2(uint32_t a, uint32_t b, uint2& total)
+{
+    __shared__ uint32_t sa[kWG / 64], sb[kWG / 64];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint32_t ia = wave_incl_scan(a), ib = wave_incl_scan(b);
+    if (lane == 63) {
+        sa[wv] = ia;
+        sb[wv] = ib;
+    }
+    __syncthreads();
+    uint32_t pa = 0, pb = 0, ta = 0, tb = 0;
+#pragma unroll
+    for (int i = 0; i < kWG / 64; i++) {
+        pa += (uint32_t)i < wv ? sa[i] : 0u;
+        pb += (uint32_t)i < wv ? sb[i] : 0u;
+        ta += sa[i];
+        tb += sb[i];
+    }
+    total = make_uint2(ta, tb);
+    __syncthreads();  // sa / sb are free for the next call
+    return make_uint2(pa + ia - a, pb + ib - b);
+}
+
+// Chunk c of 4096 units: thread t owns units 16t .. 16t + 15.  Writes each unit's exclusive
+// (overflow, verbatim) offsets within the chunk to oofs and the chunk's totals to ochunk.
+__global__ __launch_bounds__(kWG) void k_ocount_scan(const uint32_t* __restrict__ ocnt, uint32_t nunits,
+                                                     uint32_t* __restrict__ oofs, uint32_t* __restrict__ ochunk)
+{
+    constexpr uint32_t kPer = kOChunk / kWG;
+    const uint32_t     u0 = blockIdx.x * kOChunk + kPer * threadIdx.x;
+    uint2              c[kPer];
+    uint32_t           a = 0, b = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; j++) {
+        c[j] = u0 + j < nunits ? reinterpret_cast<const uint2*>(ocnt)[u0 + j] : make_uint2(0u, 0u);
+        a += c[j].x;
+        b += c[j].y;
+    }
+    uint2       total;
+    const uint2 ex = block_excl2(a, b, total);
+    uint32_t    ra = ex.x, rb = ex.y;
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; j++) {
+        if (u0 + j < nunits) reinterpret_cast<uint2*>(oofs)[u0 + j] = make_uint2(ra, rb);
+        ra += c[j].x;
+        rb += c[j].y;
+    }
+    if (threadIdx.x == 0) reinterpret_cast<uint2*>(ochunk)[blockIdx.x] = total;
+}
+
+__device__ __forceinline__ void put_u32(uint8_t* side, uint64_t cap, uint64_t off, uint32_t v)
+{
+    if (off + 4 <= cap) *reinterpret_cast<uint32_t*>(side + off) = v;
+}
+
+// One workgroup per unit.  Units without entries leave at once; the tail unit (the last) also
+// writes the header and, when the entries exceed the capacity, FZ_ERR_CAPACITY.
+template <int NDIM>
+__global__ __launch_bounds__(kWG) void k_outlier_write(const float* __restrict__ in, Geom g, fz_result* res,
+                                                       const uint32_t* __restrict__ ocnt,
+                                                       const uint32_t* __restrict__ oofs,
+                                                       const uint32_t* __restrict__ ochunk, uint32_t nchunks,
+                                                       uint8_t* __restrict__ side, uint64_t cap)
+{
+    __shared__ uint2 s_base, s_tot;
+    const uint32_t u = blockIdx.x, tid = threadIdx.x;
+    const bool     last = u == g.T;
+    const uint2    cnt = reinterpret_cast<const uint2*>(ocnt)[u];
+    // the only failure before this kernel: a value range that overflows (no codes were made);
+    // the tail unit may set FZ_ERR_CAPACITY meanwhile, which does not stop the other units
+    if (res->status == ST_NONFINITE) return;
+    if (cnt.x == 0 && cnt.y == 0 && !last) return;  // uniform
+    // entries before this unit's chunk, and in all
+    const uint32_t c = u / kOChunk;
+    uint32_t       ba = 0, bb = 0, ta = 0, tb = 0;
+    for (uint32_t j = tid; j < nchunks; j += kWG) {
+        const uint2 v = reinterpret_cast<const uint2*>(ochunk)[j];
+        ta += v.x;
+        tb += v.y;
+        ba += j < c ? v.x : 0u;
+        bb += j < c ? v.y : 0u;
+    }
+    uint2 tot, all;
+    (void)block_excl2(ba, bb, tot);  // tot: entries of the chunks before this one
+    (void)block_excl2(ta, tb, all);  // all: entries in all
+    if (tid == 0) {
+        const uint2 o = reinterpret_cast<const uint2*>(oofs)[u];
+        s_base = make_uint2(tot.x + o.x, tot.y + o.y);
+        s_tot = all;
+    }
+    __syncthreads();
+    const uint64_t nd = s_tot.x, nr = s_tot.y;
+    if (last && tid == 0) {
+        if (cap >= 16) {
+            reinterpret_cast<uint64_t*>(side)[0] = nd;
+            reinterpret_cast<uint64_t*>(side)[1] = nr;
+        }
+        if (16 + 8 * (nd + nr) > cap) res->status = FZ_ERR_CAPACITY;
+    }
+    if (cnt.x == 0 && cnt.y == 0) return;
+    // this unit's entries, in index order: thread t holds positions 8t .. 8t + 7
+    const uint32_t t0 = u * (uint32_t)kTileCodes;
+    const uint32_t ulen = last ? g.len - t0 : (uint32_t)kTileCodes;
+    Int8           o;
+    o.big = o.raw = 0;
+    if (8 * tid < ulen) dq8_int<NDIM>(in, t0 + 8 * tid, g, res->ebx2_r, o);
+    if (8 * tid + 8 > ulen) {  // positions past the unit (the tail unit only)
+        const uint32_t keep = 8 * tid < ulen ? (1u << (ulen - 8 * tid)) - 1u : 0u;
+        o.big &= keep;
+        o.raw &= keep;
+    }
+    uint2       unit_tot;
+    const uint2 pre = block_excl2((uint32_t)__popc(o.big), (uint32_t)__popc(o.raw), unit_tot);
+    uint64_t    kd = (uint64_t)s_base.x + pre.x, kr = (uint64_t)s_base.y + pre.y;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const uint32_t p = t0 + 8 * tid + i;
+        if ((o.big >> i) & 1u) {
+            put_u32(side, cap, 16 + 4 * kd, p);
+            put_u32(side, cap, 16 + 4 * nd + 4 * kd, (uint32_t)o.d[i]);
+            ++kd;
+        }
+        if ((o.raw >> i) & 1u) {
+            put_u32(side, cap, 16 + 8 * nd + 4 * kr, p);
+            put_u32(side, cap, 16 + 8 * nd + 4 * nr + 4 * kr, __float_as_uint(in[p]));
+            ++kr;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------- inverse
+
+// entry counts of a side channel, clamped to what side_bytes holds (so a short or damaged
+// channel never leads to an access outside it)
+__device__ __forceinline__ void side_counts(const uint8_t* side, uint64_t side_bytes, uint64_t& nd, uint64_t& nr)
+{
+    nd = nr = 0;
+    if (side_bytes < 16) return;
+    const uint64_t avail = (side_bytes - 16) / 8;
+    nd = reinterpret_cast<const uint64_t*>(side)[0];
+    nr = reinterpret_cast<const uint64_t*>(side)[1];
+    nd = nd < avail ? nd : avail;
+    nr = nr < avail - nd ? nr : avail - nd;
+}
+
+__global__ __launch_bounds__(kWG) void k_expand(const uint16_t* __restrict__ codes, int32_t* __restrict__ d, uint32_t n)
+{
+    const uint32_t i = 4 * (blockIdx.x * kWG + threadIdx.x);
+    if (i + 4 <= n) {
+        int32_t v[4];
+        delta4(codes, i, v);
+        *reinterpret_cast<int4*>(d + i) = make_int4(v[0], v[1], v[2], v[3]);
+    } else {
+        for (uint32_t j = i; j < n; j++) d[j] = decode(codes[j]);
+    }
+}
+
+__global__ __launch_bounds__(kWG) void k_put_deltas(const uint8_t* __restrict__ side, uint64_t side_bytes,
+                                                    int32_t* __restrict__ d, uint32_t n)
+{
+    uint64_t nd, nr;
+    side_counts(side, side_bytes, nd, nr);
+    const uint32_t* idx = reinterpret_cast<const uint32_t*>(side + 16);
+    const uint32_t* val = idx + nd;
+    for (uint64_t k = (uint64_t)blockIdx.x * kWG + threadIdx.x; k < nd; k += (uint64_t)gridDim.x * kWG)
+        if (idx[k] < n) d[idx[k]] = (int32_t)val[k];
+}
+
+__global__ __launch_bounds__(kWG) void k_put_raw(const uint8_t* __restrict__ side, uint64_t side_bytes,
+                                                 float* __restrict__ out, uint32_t n)
+{
+    uint64_t nd, nr;
+    side_counts(side, side_bytes, nd, nr);
+    const uint32_t* idx = reinterpret_cast<const uint32_t*>(side + 16) + 2 * nd;
+    const uint32_t* val = idx + nr;
+    for (uint64_t k = (uint64_t)blockIdx.x * kWG + threadIdx.x; k < nr; k += (uint64_t)gridDim.x * kWG)
+        if (idx[k] < n) out[idx[k]] = __uint_as_float(val[k]);
+}
+
+__global__ __launch_bounds__(kWG) void k_fill_f32(float* __restrict__ out, uint32_t n, float v)
+{
+    const uint32_t i = blockIdx.x * kWG + threadIdx.x;
+    if (i < n) out[i] = v;
+}
+
+constexpr uint32_t kScatterWG = 256;  // workgroups of the side-channel scatters (grid-stride)
+
+}  // namespace
+
+hipError_t launch_compress_outlier(const float* in, const Geom& g, double eb_rel, float4* part, fz_result* res,
+                                   uint8_t* payload, uint16_t* tail, uint8_t* side, uint64_t side_cap,
+                                   uint8_t* staging, uint32_t* counts, uint32_t* run_sums, uint32_t nsums,
+                                   uint32_t* ocnt, uint32_t* oofs, uint32_t* ochunk, hipStream_t s)
+{
+    const uint32_t nparts = range_grid(g.len);
+    (void)hipGetLastError();
+    k_range_part_o<<<nparts, kORangeWG, 0, s>>>(in, g, part, run_sums, nsums, res);
+    const uint32_t tpw = g.T ? gather_tpw(g.T) : 1u;
+    const dim3     grid(g.T + 1), blk(kWG);
+    switch (g.ndim) {
+        case 1: k_tile_compress_o<1><<<grid, blk, 0, s>>>(in, g, eb_rel, part, nparts, res, payload, staging, counts, tail, run_sums, tpw, ocnt); break;
+        case 2: k_tile_compress_o<2><<<grid, blk, 0, s>>>(in, g, eb_rel, part, nparts, res, payload, staging, counts, tail, run_sums, tpw, ocnt); break;
+        default: k_tile_compress_o<3><<<grid, blk, 0, s>>>(in, g, eb_rel, part, nparts, res, payload, staging, counts, tail, run_sums, tpw, ocnt); break;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = launch_scan_gather(counts, g.T, staging, payload, run_sums, res, g.W, s);
+    if (e != hipSuccess) return e;
+    const uint32_t nch = ochunks(g.T);
+    (void)hipGetLastError();
+    k_ocount_scan<<<nch, kWG, 0, s>>>(ocnt, g.T + 1, oofs, ochunk);
+    switch (g.ndim) {
+        case 1: k_outlier_write<1><<<grid, blk, 0, s>>>(in, g, res, ocnt, oofs, ochunk, nch, side, side_cap); break;
+        case 2: k_outlier_write<2><<<grid, blk, 0, s>>>(in, g, res, ocnt, oofs, ochunk, nch, side, side_cap); break;
+        default: k_outlier_write<3><<<grid, blk, 0, s>>>(in, g, res, ocnt, oofs, ochunk, nch, side, side_cap); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_reconstruct_outlier(const uint16_t* codes, int32_t* dwide, const uint8_t* side,
+                                      uint64_t side_bytes, float* out, const Geom& g, float ebx2, hipStream_t s)
+{
+    (void)hipGetLastError();
+    const uint32_t thr = (g.xyz + 3) / 4;
+    k_expand<<<(thr + kWG - 1) / kWG, kWG, 0, s>>>(codes, dwide, g.xyz);
+    k_put_deltas<<<kScatterWG, kWG, 0, s>>>(side, side_bytes, dwide, g.xyz);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = launch_reconstruct_i32(dwide, out, g, ebx2, s);
+    if (e != hipSuccess) return e;
+    k_put_raw<<<kScatterWG, kWG, 0, s>>>(side, side_bytes, out, g.xyz);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_outlier(const uint8_t* side, uint64_t side_bytes, float* out, const Geom& g, float vconst,
+                               hipStream_t s)
+{
+    (void)hipGetLastError();
+    k_fill_f32<<<(g.xyz + kWG - 1) / kWG, kWG, 0, s>>>(out, g.xyz, vconst);
+    k_put_raw<<<kScatterWG, kWG, 0, s>>>(side, side_bytes, out, g.xyz);
+    return hipGetLastError();
+}
+
+}  // namespace fz

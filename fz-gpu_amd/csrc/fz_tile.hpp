@@ -65,10 +65,13 @@ __device__ __forceinline__ uint32_t wave_upper(const uint32_t* __restrict__ a, u
 // min / max / non-finite flag of in[f_lo, f_hi), reduced over the NT threads of the
 // workgroup; thread 0 stores (lo, hi, bad-as-bits, 0) to *out.  float4 loads over the
 // 16-byte-aligned middle, scalar loads at the edges.
-template <int NT, uint32_t U = 4>  // U float4 loads in flight per thread
+// FIN (the outlier mode): the min / max of the finite values only (a non-finite value enters
+// as NaN, which fminf / fmaxf skip).
+template <int NT, uint32_t U = 4, bool FIN = false>  // U float4 loads in flight per thread
 __device__ __forceinline__ void range_span(const float* __restrict__ in, uint32_t f_lo, uint32_t f_hi,
                                            float4* __restrict__ out)
 {
+    auto fin = [](float v) { return (FIN && nonfinite(v)) ? __builtin_nanf("") : v; };
     __shared__ float    s_lo[NT / 64], s_hi[NT / 64];
     __shared__ uint32_t s_bad[NT / 64];
     const uint32_t tid = threadIdx.x;
@@ -83,18 +86,19 @@ __device__ __forceinline__ void range_span(const float* __restrict__ in, uint32_
             for (uint32_t j = 0; j < U; j++) v[j] = in4[min(base + j * NT, v_hi - 1)];  // re-reads are harmless
 #pragma unroll
             for (uint32_t j = 0; j < U; j++) {
-                lo = fminf(lo, fminf(fminf(v[j].x, v[j].y), fminf(v[j].z, v[j].w)));
-                hi = fmaxf(hi, fmaxf(fmaxf(v[j].x, v[j].y), fmaxf(v[j].z, v[j].w)));
                 bad |= nonfinite(v[j].x) | nonfinite(v[j].y) | nonfinite(v[j].z) | nonfinite(v[j].w);
+                const float a = fin(v[j].x), b = fin(v[j].y), c = fin(v[j].z), d = fin(v[j].w);
+                lo = fminf(lo, fminf(fminf(a, b), fminf(c, d)));
+                hi = fmaxf(hi, fmaxf(fmaxf(a, b), fmaxf(c, d)));
             }
         }
     }
     const uint32_t e0 = min(4 * v_lo, f_hi), e1 = max(4 * v_hi, e0);
     for (uint32_t k = f_lo + tid; k < e0; k += NT) {
-        lo = fminf(lo, in[k]); hi = fmaxf(hi, in[k]); bad |= nonfinite(in[k]);
+        lo = fminf(lo, fin(in[k])); hi = fmaxf(hi, fin(in[k])); bad |= nonfinite(in[k]);
     }
     for (uint32_t k = e1 + tid; k < f_hi; k += NT) {
-        lo = fminf(lo, in[k]); hi = fmaxf(hi, in[k]); bad |= nonfinite(in[k]);
+        lo = fminf(lo, fin(in[k])); hi = fmaxf(hi, fin(in[k])); bad |= nonfinite(in[k]);
     }
     wave_minmax(lo, hi);
     bad = ballot64(bad != 0) != 0;
@@ -151,6 +155,29 @@ __device__ __forceinline__ EbParams eb_params(float lo, float hi, uint32_t bad, 
     return e;
 }
 
+// The outlier mode's parameters: the range of the finite values (0, 0 when there are none);
+// a zero range gives r = ebx2 = 0 (every quantised integer is 0 and the inverse fills vmin).
+// Only a range that itself overflows is an error.  Tiles take the packed paths when no value
+// is non-finite and level >= 3 (then no delta reaches 32768 and no value is kept verbatim),
+// the integer path (kLevelOutlier) otherwise.
+constexpr int kLevelOutlier = -1;
+
+__device__ __forceinline__ EbParams eb_params_o(float lo, float hi, uint32_t bad, double eb_rel, int ndim)
+{
+    if (!(lo <= hi)) lo = hi = 0.0f;
+    const float range = hi - lo;
+    EbParams    e = eb_params(lo, hi, 0u, eb_rel, ndim);
+    if (range == 0.0f) {
+        e.eb = 0.0;
+        e.r = 0.0f;
+        e.ebx2 = 0.0f;
+        e.level = lo >= 0.0f ? 4 : 3;
+    }
+    e.status = (range - range == 0.0f) ? ST_SUCCESS : ST_NONFINITE;
+    if (bad || e.level < 3) e.level = kLevelOutlier;
+    return e;
+}
+
 // Reduces the partials part[0, nparts) inside a 256-thread workgroup and returns the eb
 // parameters to every thread.  Any nparts (a loop of 256 per round).
 __device__ __forceinline__ EbParams reduce_partials(const float4* __restrict__ part, uint32_t nparts, double eb_rel,
@@ -204,6 +231,7 @@ __device__ __forceinline__ void partials_load(const float4* __restrict__ part, u
     }
 }
 
+template <bool OUTL = false>
 __device__ __forceinline__ EbParams partials_reduce(const Partials& P, double eb_rel, int ndim)
 {
     __shared__ EbParams s_eb;
@@ -216,7 +244,7 @@ __device__ __forceinline__ EbParams partials_reduce(const Partials& P, double eb
             ballot64((__float_as_uint(p[0].z) | __float_as_uint(p[1].z) | __float_as_uint(p[2].z) |
                       __float_as_uint(p[3].z)) != 0) != 0;
         wave_minmax(lo, hi);
-        if (tid == 0) s_eb = eb_params(lo, hi, bad, eb_rel, ndim);
+        if (tid == 0) s_eb = OUTL ? eb_params_o(lo, hi, bad, eb_rel, ndim) : eb_params(lo, hi, bad, eb_rel, ndim);
     }
     __syncthreads();
     return s_eb;
@@ -336,6 +364,59 @@ __device__ __forceinline__ void pk_dispatch(const RawTile<NDIM>& R, uint32_t p0,
     else raw_compute_pk<NDIM, AL, false>(R, p0, g, r, w);
 }
 
+// Steps 2-4 of a tile (bit transpose, flags, in-tile compaction) on the thread's 4 code
+// words w, whose overflow count is ovf.  With ocnt, thread 0 also stores the tile's
+// (overflows, verbatim values) totals there: the per-unit counts of the outlier mode.
+__device__ __forceinline__ void tile_finish(uint32_t w[4], uint32_t ovf, uint32_t tile,
+                                            uint32_t* __restrict__ bwords, uint8_t* __restrict__ slot,
+                                            uint32_t* __restrict__ count_out, unsigned long long* ovf_out,
+                                            uint32_t* run_sum, uint32_t nraw = 0, uint32_t* ocnt = nullptr)
+{
+    __shared__ uint32_t sh[32 * kLdsPitch];
+    __shared__ uint32_t sh_wcnt[4], sh_wovf[4], sh_wraw[4];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+
+    // 2. bit transpose of group k = t/8 (skipped for an all-zero wave); register q then holds
+    //    o[32(4L+q) + k]
+    const uint32_t L = tid & 7, k = tid >> 3;
+    if (ballot64((w[0] | w[1] | w[2] | w[3]) != 0) != 0) transpose4(w, L);
+#pragma unroll
+    for (int q = 0; q < 4; q++) sh[kLdsPitch * (4 * L + q) + k] = w[q];
+    if (ballot64(ovf != 0) != 0) ovf = wave_sum(ovf);  // overflows are rare: skip the sum otherwise
+    if (lane == 0) sh_wovf[wv] = ovf;
+    if (ocnt) {
+        if (ballot64(nraw != 0) != 0) nraw = wave_sum(nraw);
+        if (lane == 0) sh_wraw[wv] = nraw;
+    }
+    __syncthreads();
+
+    // 3. thread i owns 16-byte block i = o[4i .. 4i+3]; flag = block != 0 (fz.cu:84-91)
+    const uint32_t c = tid >> 3, k0 = 4 * (tid & 7);
+    const uint4    blk = make_uint4(sh[kLdsPitch * c + k0], sh[kLdsPitch * c + k0 + 1], sh[kLdsPitch * c + k0 + 2],
+                                    sh[kLdsPitch * c + k0 + 3]);
+    const bool     f = (blk.x | blk.y | blk.z | blk.w) != 0;
+    const uint64_t mask = ballot64(f);
+    // bitflag word 8*tile + y holds flags 32y..32y+31: wave wv covers words 2wv (lanes 0-31)
+    // and 2wv+1 (lanes 32-63)
+    if (lane < 2) bwords[8 * tile + 2 * wv + lane] = lane ? (uint32_t)(mask >> 32) : (uint32_t)mask;
+    const uint32_t rank = rank_below(mask);
+    if (lane == 0) sh_wcnt[wv] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    const uint32_t c0 = sh_wcnt[0], c1 = sh_wcnt[1], c2 = sh_wcnt[2], c3 = sh_wcnt[3];
+    const uint32_t woff = (wv > 0 ? c0 : 0u) + (wv > 1 ? c1 : 0u) + (wv > 2 ? c2 : 0u);
+    if (tid == 0) {
+        const uint32_t total = c0 + c1 + c2 + c3;
+        *count_out = total;
+        if (run_sum && total) atomicAdd(run_sum, total);
+        const uint32_t o = sh_wovf[0] + sh_wovf[1] + sh_wovf[2] + sh_wovf[3];
+        if (o) atomicAdd(ovf_out, (unsigned long long)o);
+        if (ocnt)
+            *reinterpret_cast<uint2*>(ocnt) = make_uint2(o, sh_wraw[0] + sh_wraw[1] + sh_wraw[2] + sh_wraw[3]);
+    }
+    // 4. in-tile compaction into the tile's staging slot
+    if (f) *reinterpret_cast<uint4*>(slot + 16 * (woff + rank)) = blk;
+}
+
 // One full tile: dual-quant -> 32x32 bit transpose -> block flags -> in-tile compaction.
 //   bwords[8 tile .. 8 tile + 8)  the tile's bitflag words (fz.cu:95-99, LSB first)
 //   slot[0, 16 count)              its nonzero 16-byte blocks in order (fz.cu:56-57 order)
@@ -354,9 +435,7 @@ __device__ __forceinline__ void tile_encode(const float* __restrict__ in, const 
                                             uint32_t* __restrict__ count_out, unsigned long long* ovf_out,
                                             uint32_t* run_sum = nullptr)
 {
-    __shared__ uint32_t sh[32 * kLdsPitch];
-    __shared__ uint32_t sh_wcnt[4], sh_wovf[4];
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint32_t tid = threadIdx.x;
     const uint32_t p0 = tile * (uint32_t)kTileCodes + 8 * tid;
     uint32_t       ovf = 0;
 
@@ -384,39 +463,169 @@ __device__ __forceinline__ void tile_encode(const float* __restrict__ in, const 
         else dq8_tile<3>(in, p0, g, r, level >= 1 ? gp : 0, w, ovf);  // PATHS == 0 only
     }
 
-    // 2. bit transpose of group k = t/8 (skipped for an all-zero wave); register q then holds
-    //    o[32(4L+q) + k]
-    const uint32_t L = tid & 7, k = tid >> 3;
-    if (ballot64((w[0] | w[1] | w[2] | w[3]) != 0) != 0) transpose4(w, L);
-#pragma unroll
-    for (int q = 0; q < 4; q++) sh[kLdsPitch * (4 * L + q) + k] = w[q];
-    if (ballot64(ovf != 0) != 0) ovf = wave_sum(ovf);  // overflows are rare: skip the sum otherwise
-    if (lane == 0) sh_wovf[wv] = ovf;
-    __syncthreads();
+    tile_finish(w, ovf, tile, bwords, slot, count_out, ovf_out, run_sum);
+}
 
-    // 3. thread i owns 16-byte block i = o[4i .. 4i+3]; flag = block != 0 (fz.cu:84-91)
-    const uint32_t c = tid >> 3, k0 = 4 * (tid & 7);
-    const uint4    blk = make_uint4(sh[kLdsPitch * c + k0], sh[kLdsPitch * c + k0 + 1], sh[kLdsPitch * c + k0 + 2],
-                                    sh[kLdsPitch * c + k0 + 3]);
-    const bool     f = (blk.x | blk.y | blk.z | blk.w) != 0;
-    const uint64_t mask = ballot64(f);
-    // bitflag word 8*tile + y holds flags 32y..32y+31: wave wv covers words 2wv (lanes 0-31)
-    // and 2wv+1 (lanes 32-63)
-    if (lane < 2) bwords[8 * tile + 2 * wv + lane] = lane ? (uint32_t)(mask >> 32) : (uint32_t)mask;
-    const uint32_t rank = rank_below(mask);
-    if (lane == 0) sh_wcnt[wv] = (uint32_t)__popcll(mask);
-    __syncthreads();
-    const uint32_t c0 = sh_wcnt[0], c1 = sh_wcnt[1], c2 = sh_wcnt[2], c3 = sh_wcnt[3];
-    const uint32_t woff = (wv > 0 ? c0 : 0u) + (wv > 1 ? c1 : 0u) + (wv > 2 ? c2 : 0u);
-    if (tid == 0) {
-        const uint32_t total = c0 + c1 + c2 + c3;
-        *count_out = total;
-        if (run_sum && total) atomicAdd(run_sum, total);
-        const uint32_t o = sh_wovf[0] + sh_wovf[1] + sh_wovf[2] + sh_wovf[3];
-        if (o) atomicAdd(ovf_out, (unsigned long long)o);
+// ---------------------------------------------------------------------------- outlier mode
+//
+// Integer form of the generic dual-quant (the side-channel format of the outlier mode): a
+// value that is finite with |fl(x r)| < 2^24 quantises to s = round(x r) (exact in binary32,
+// so the inverse's float of s is exact); any other value is kept verbatim and quantises to
+// 0.  Deltas are the block-local Lorenzo deltas of these integers in exact int32 arithmetic
+// (|d| < 2^27); |d| >= 32768 takes code 0 and a side-channel entry.  Positions >= x*y*z get
+// code 0.
+
+__device__ __forceinline__ int32_t quant_i(float v, float r, bool& raw)
+{
+    const float t = __fmul_rn(v, r);
+    raw = !(__builtin_isfinite(v) && __builtin_fabsf(t) < 16777216.0f);
+    return raw ? 0 : (int32_t)__builtin_truncf(__fadd_rn(t, __builtin_copysignf(0.49999997f, t)));
+}
+
+// quantised neighbour value; 0 outside [0, xyz) and for values kept verbatim
+__device__ __forceinline__ int32_t ldq_i(const float* __restrict__ in, int64_t idx, uint32_t xyz, float r)
+{
+    bool raw;
+    return (idx >= 0 && idx < (int64_t)xyz) ? quant_i(in[idx], r, raw) : 0;
+}
+
+struct Int8 {
+    int32_t  d[8];  // deltas of p0 .. p0 + 7 (0 at positions >= x*y*z)
+    uint32_t raw;   // bit i: position p0 + i is kept verbatim
+    uint32_t big;   // bit i: |d[i]| >= 32768
+};
+
+template <int NDIM>
+__device__ __forceinline__ void dq8_int(const float* __restrict__ in, uint32_t p0, const Geom& g, float r, Int8& o)
+{
+    o.raw = o.big = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) o.d[i] = 0;
+    if (p0 >= g.xyz) return;
+    int32_t s[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        bool raw = false;
+        s[i] = p0 + i < g.xyz ? quant_i(in[p0 + i], r, raw) : 0;
+        o.raw |= raw ? 1u << i : 0u;
     }
-    // 4. in-tile compaction into the tile's staging slot
-    if (f) *reinterpret_cast<uint4*>(slot + 16 * (woff + rank)) = blk;
+    const int32_t sw = ldq_i(in, (int64_t)p0 - 1, g.xyz, r);
+    if constexpr (NDIM == 1) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int32_t left = (i == 0) ? (((p0 & 255u) != 0) ? sw : 0) : s[i - 1];
+            o.d[i] = s[i] - left;
+        }
+    } else {
+        const uint32_t q = fdiv(p0, g.divX);
+        uint32_t x = p0 - q * g.X;
+        uint32_t y, zz = 0;
+        if constexpr (NDIM == 2) {
+            y = q;
+        } else {
+            zz = fdiv(q, g.divY);
+            y = q - zz * g.Y;
+        }
+        const int64_t pN = (int64_t)p0 - g.X;
+        int32_t n[9];  // s[p - X] for p = p0-1 .. p0+7
+#pragma unroll
+        for (int j = 0; j < 9; j++) n[j] = ldq_i(in, pN - 1 + j, g.xyz, r);
+        if constexpr (NDIM == 2) {
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const bool    bx = (x & 15u) != 0, by = (y & 15u) != 0;
+                const int32_t west = (i == 0) ? sw : s[i - 1];
+                o.d[i] = s[i] - (bx ? west : 0) - (by ? n[i + 1] : 0) + ((bx && by) ? n[i] : 0);
+                if (++x == g.X) { x = 0; ++y; }
+            }
+        } else {
+            const int64_t pU = (int64_t)p0 - g.XY, pUN = pU - g.X;
+            int32_t u[9], un[9];  // s[p - XY], s[p - XY - X] for p = p0-1 .. p0+7
+#pragma unroll
+            for (int j = 0; j < 9; j++) {
+                u[j] = ldq_i(in, pU - 1 + j, g.xyz, r);
+                un[j] = ldq_i(in, pUN - 1 + j, g.xyz, r);
+            }
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const bool    bx = (x & 7u) != 0, by = (y & 7u) != 0, bz = (zz & 7u) != 0;
+                const int32_t west = (i == 0) ? sw : s[i - 1];
+                o.d[i] = s[i] - (bx ? west : 0) - (by ? n[i + 1] : 0) - (bz ? u[i + 1] : 0) + ((bx && by) ? n[i] : 0) +
+                         ((bx && bz) ? u[i] : 0) + ((by && bz) ? un[i + 1] : 0) - ((bx && by && bz) ? un[i] : 0);
+                if (++x == g.X) {
+                    x = 0;
+                    if (++y == g.Y) { y = 0; ++zz; }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        if (p0 + i >= g.xyz) o.d[i] = 0;
+        o.big |= (o.d[i] >= 32768 || o.d[i] <= -32768) ? 1u << i : 0u;
+    }
+}
+
+// sign-magnitude code of an exact integer delta; 0 for a side-channel delta
+__device__ __forceinline__ uint32_t code_i(int32_t d)
+{
+    const uint32_t a = (uint32_t)(d < 0 ? -d : d);
+    return a >= 32768u ? 0u : (a | (d < 0 ? 0x8000u : 0u));
+}
+
+// A full tile of the outlier mode's integer path: as tile_encode, plus the tile's
+// (overflows, verbatim values) counts at ocnt[0..1].
+template <int NDIM>
+__device__ __forceinline__ void tile_encode_o(const float* __restrict__ in, const Geom& g, uint32_t tile, float r,
+                                              uint32_t* __restrict__ bwords, uint8_t* __restrict__ slot,
+                                              uint32_t* __restrict__ count_out, unsigned long long* ovf_out,
+                                              uint32_t* run_sum, uint32_t* __restrict__ ocnt)
+{
+    Int8 o;
+    dq8_int<NDIM>(in, tile * (uint32_t)kTileCodes + 8 * threadIdx.x, g, r, o);
+    uint32_t w[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) w[k] = code_i(o.d[2 * k]) | (code_i(o.d[2 * k + 1]) << 16);
+    tile_finish(w, (uint32_t)__popc(o.big), tile, bwords, slot, count_out, ovf_out, run_sum, (uint32_t)__popc(o.raw),
+                ocnt);
+}
+
+// The tail unit of the outlier mode: the tail codes, the partial tile's bitflag words
+// cleared, and the unit's counts at ocnt[0..1].
+template <int NDIM>
+__device__ __forceinline__ void tail_unit_o(const float* __restrict__ in, const Geom& g, float r,
+                                            uint32_t* __restrict__ bwords, uint16_t* __restrict__ tail,
+                                            unsigned long long* ovf_out, uint32_t* __restrict__ ocnt)
+{
+    __shared__ uint32_t sh_b[4], sh_r[4];
+    const uint32_t tid = threadIdx.x, wv = tid >> 6;
+    const uint32_t t0 = (uint32_t)kTileCodes * g.T;
+    const uint32_t tail_len = g.len - t0;
+    uint32_t       nb = 0, nr = 0;
+    if (8 * tid < tail_len) {
+        Int8 o;
+        dq8_int<NDIM>(in, t0 + 8 * tid, g, r, o);
+        nb = (uint32_t)__popc(o.big);
+        nr = (uint32_t)__popc(o.raw);
+        if (tail) {
+#pragma unroll
+            for (int i = 0; i < 8; i++)
+                if (8 * tid + i < tail_len) tail[8 * tid + i] = (uint16_t)code_i(o.d[i]);
+        }
+    }
+    for (uint32_t i = 8 * g.T + tid; i < g.W; i += kWG) bwords[i] = 0;
+    nb = wave_sum(nb);
+    nr = wave_sum(nr);
+    if ((tid & 63) == 0) {
+        sh_b[wv] = nb;
+        sh_r[wv] = nr;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const uint32_t b = sh_b[0] + sh_b[1] + sh_b[2] + sh_b[3], rr = sh_r[0] + sh_r[1] + sh_r[2] + sh_r[3];
+        *reinterpret_cast<uint2*>(ocnt) = make_uint2(b, rr);
+        if (b) atomicAdd(ovf_out, (unsigned long long)b);
+    }
 }
 
 // ---------------------------------------------------------------------------- one wave per tile
@@ -588,25 +797,57 @@ __device__ __forceinline__ int32_t decode(uint32_t c)
     return (c & 0x8000u) ? -m : m;
 }
 
+// Deltas of the reconstruct kernels from either source: u16 sign-magnitude codes, or int32
+// deltas (the outlier mode, whose side-channel deltas do not fit a code).  *4 reads positions
+// [p, p + 4) with one 8- or 16-byte load (p % 4 == 0), *16 reads [p, p + 16).
+__device__ __forceinline__ int32_t delta1(const uint16_t* __restrict__ c, size_t p) { return decode(c[p]); }
+__device__ __forceinline__ int32_t delta1(const int32_t* __restrict__ c, size_t p) { return c[p]; }
+__device__ __forceinline__ void delta4(const uint16_t* __restrict__ c, size_t p, int32_t d[4])
+{
+    const uint2 v = *reinterpret_cast<const uint2*>(c + p);
+    d[0] = decode(v.x & 0xFFFFu);
+    d[1] = decode(v.x >> 16);
+    d[2] = decode(v.y & 0xFFFFu);
+    d[3] = decode(v.y >> 16);
+}
+__device__ __forceinline__ void delta4(const int32_t* __restrict__ c, size_t p, int32_t d[4])
+{
+    const int4 v = *reinterpret_cast<const int4*>(c + p);
+    d[0] = v.x;
+    d[1] = v.y;
+    d[2] = v.z;
+    d[3] = v.w;
+}
+__device__ __forceinline__ void delta16(const uint16_t* __restrict__ c, size_t p, int32_t d[16])
+{
+    const uint4 a = *reinterpret_cast<const uint4*>(c + p);
+    const uint4 b = *reinterpret_cast<const uint4*>(c + p + 8);
+    const uint32_t wd[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        d[2 * j] = decode(wd[j] & 0xffffu);
+        d[2 * j + 1] = decode(wd[j] >> 16);
+    }
+}
+__device__ __forceinline__ void delta16(const int32_t* __restrict__ c, size_t p, int32_t d[16])
+{
+#pragma unroll
+    for (int j = 0; j < 4; j++) delta4(c, p + 4 * j, d + 4 * j);
+}
+
 // 1D: workgroup wg covers codes [4096 wg, 4096 wg + 4096): 16 consecutive codes per thread,
 // 16 threads per 256-block (block-local inclusive prefix sum), x = (float)s * ebx2.
-__device__ __forceinline__ void recon_1d(const uint16_t* __restrict__ codes, float* __restrict__ out, const Geom& g,
+template <typename CT>
+__device__ __forceinline__ void recon_1d(const CT* __restrict__ codes, float* __restrict__ out, const Geom& g,
                                          float ebx2, uint32_t wg)
 {
     const uint32_t p0 = (wg * kWG + threadIdx.x) * 16;
     int32_t        s[16];
     if (p0 + 16 <= g.xyz) {
-        const uint4 a = *reinterpret_cast<const uint4*>(codes + p0);
-        const uint4 b = *reinterpret_cast<const uint4*>(codes + p0 + 8);
-        const uint32_t wd[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            s[2 * j] = decode(wd[j] & 0xffffu);
-            s[2 * j + 1] = decode(wd[j] >> 16);
-        }
+        delta16(codes, p0, s);
     } else {
 #pragma unroll
-        for (int j = 0; j < 16; j++) s[j] = (p0 + j < g.xyz) ? decode(codes[p0 + j]) : 0;
+        for (int j = 0; j < 16; j++) s[j] = (p0 + j < g.xyz) ? delta1(codes, p0 + j) : 0;
     }
 #pragma unroll
     for (int j = 1; j < 16; j++) s[j] += s[j - 1];
@@ -645,7 +886,8 @@ __device__ __forceinline__ int32_t quad_incl_scan(int32_t v)
     return v + (q >= 2 ? b : 0);
 }
 
-__device__ __forceinline__ void recon_2d(const uint16_t* __restrict__ codes, float* __restrict__ out, const Geom& g,
+template <typename CT>
+__device__ __forceinline__ void recon_2d(const CT* __restrict__ codes, float* __restrict__ out, const Geom& g,
                                          float ebx2, uint32_t bx, uint32_t by)
 {
     const uint32_t x0 = bx * kRecon2DCols + 4 * threadIdx.x, y0 = by * 16;
@@ -657,21 +899,17 @@ __device__ __forceinline__ void recon_2d(const uint16_t* __restrict__ codes, flo
         const uint32_t y = y0 + r;
         if (y >= g.Y) break;  // uniform
         const size_t row = (size_t)y * g.X;
-        uint32_t     c[4];
+        int32_t      dd[4];
         if (vec) {
-            const uint2 v = *reinterpret_cast<const uint2*>(codes + row + x0);
-            c[0] = v.x & 0xFFFFu;
-            c[1] = v.x >> 16;
-            c[2] = v.y & 0xFFFFu;
-            c[3] = v.y >> 16;
+            delta4(codes, row + x0, dd);
         } else {
 #pragma unroll
-            for (int j = 0; j < 4; j++) c[j] = x0 + j < g.X ? codes[row + x0 + j] : 0u;
+            for (int j = 0; j < 4; j++) dd[j] = x0 + j < g.X ? delta1(codes, row + x0 + j) : 0;
         }
         int32_t p[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            col[j] += decode(c[j]);
+            col[j] += dd[j];
             p[j] = col[j] + (j ? p[j - 1] : 0);
         }
         const int32_t excl = quad_incl_scan(p[3]) - p[3];
@@ -694,7 +932,8 @@ __device__ __forceinline__ void recon_2d(const uint16_t* __restrict__ codes, flo
 // and with the other half of its 8-wide Lorenzo block (the neighbour lane, one DPP step); an
 // LDS transpose (16-byte accesses) then gives thread (xq, tz) the 8 rows of its z slice for the
 // prefix over y.  Loads are 8 bytes and stores 16 bytes per lane when X % 4 == 0.
-__device__ __forceinline__ void recon_3d(const uint16_t* __restrict__ codes, float* __restrict__ out, const Geom& g,
+template <typename CT>
+__device__ __forceinline__ void recon_3d(const CT* __restrict__ codes, float* __restrict__ out, const Geom& g,
                                          float ebx2, uint32_t bx, uint32_t by, uint32_t bz)
 {
     __shared__ int4 sh[8][8][32];  // [z][y][x quad]
@@ -707,24 +946,20 @@ __device__ __forceinline__ void recon_3d(const uint16_t* __restrict__ codes, flo
         int32_t        col[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int z = 0; z < 8; z++) {
-            uint32_t c[4] = {0u, 0u, 0u, 0u};
+            int32_t dd[4] = {0, 0, 0, 0};
             if (y < g.Y && z0 + z < g.Z) {
                 const size_t row = (size_t)(z0 + z) * g.XY + (size_t)y * g.X;
                 if (vec) {
-                    const uint2 v = *reinterpret_cast<const uint2*>(codes + row + x0);
-                    c[0] = v.x & 0xFFFFu;
-                    c[1] = v.x >> 16;
-                    c[2] = v.y & 0xFFFFu;
-                    c[3] = v.y >> 16;
+                    delta4(codes, row + x0, dd);
                 } else {
 #pragma unroll
-                    for (int j = 0; j < 4; j++) c[j] = x0 + j < g.X ? codes[row + x0 + j] : 0u;
+                    for (int j = 0; j < 4; j++) dd[j] = x0 + j < g.X ? delta1(codes, row + x0 + j) : 0;
                 }
             }
             int32_t p[4];
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                col[j] += decode(c[j]);              // prefix over z
+                col[j] += dd[j];                     // prefix over z
                 p[j] = col[j] + (j ? p[j - 1] : 0);  // prefix over the thread's 4 columns
             }
             // the odd lane of a pair holds the block's upper 4 columns: add the even lane's total

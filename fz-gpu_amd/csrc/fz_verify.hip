@@ -232,7 +232,7 @@ __global__ __launch_bounds__(kVWG) void k_verify_part(const float* __restrict__ 
             ++k;
             c += gridDim.x;
         };
-        for (; c < nfull;) {
+        while (c < nfull) {  // step() advances c
             Buf A;
             load(A, c);
             step(A);

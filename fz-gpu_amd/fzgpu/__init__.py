@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libfzgpu.so")
 
 MODE_FIXED = 0
 MODE_LITERAL = 1
+MODE_OUTLIER = 2
 
 STATUS = {0: "success", 1: "invalid argument", 2: "workspace too small", 3: "HIP runtime error",
           4: "constant field", 5: "non-finite value", 6: "corrupt payload", 7: "capacity exceeded",
@@ -53,7 +54,8 @@ assert ctypes.sizeof(Result) == 64
 class Sizes(ctypes.Structure):
     _fields_ = [("len", ctypes.c_uint64), ("tiles", ctypes.c_uint64), ("flag_words", ctypes.c_uint64),
                 ("byte_flags", ctypes.c_uint64), ("payload_max_bytes", ctypes.c_uint64),
-                ("tail_max_codes", ctypes.c_uint64), ("workspace_bytes", ctypes.c_uint64)]
+                ("tail_max_codes", ctypes.c_uint64), ("workspace_bytes", ctypes.c_uint64),
+                ("side_max_bytes", ctypes.c_uint64)]
 
 
 # every symbol declared in include/fz_capi.h
@@ -66,7 +68,7 @@ EXPORTS = ["fz_version", "fz_status_string", "fz_last_error", "fz_plan_create", 
            "fz_batch_get_tuning", "fz_batch_set_tuning", "fz_batch_workspace_init", "fz_batch_compress_f32",
            "fz_batch_decompress_f32", "fz_batch_autotune", "fz_batch_profile", "fz_batch_kernel_times",
            "fz_batch_compress_graph", "fz_batch_decompress_graph", "fz_graph_launch", "fz_graph_destroy",
-           "fz_verify_f32", "fz_decompress_verify_f32"]
+           "fz_verify_f32", "fz_decompress_verify_f32", "fz_compress_outlier_f32", "fz_decompress_outlier_f32"]
 
 
 class _StatsRange(ctypes.Structure):
@@ -180,6 +182,8 @@ def lib():
     L.fz_graph_launch.argtypes = [vp, vp]
     L.fz_graph_destroy.argtypes = [vp]
     L.fz_verify_f32.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t), vp, vp, vp, u64, vp]
+    L.fz_compress_outlier_f32.argtypes = [vp, vp, f64, vp, ctypes.c_size_t, vp, vp, vp, u64, vp, vp]
+    L.fz_decompress_outlier_f32.argtypes = [vp, vp, vp, f32, f32, vp, u64, vp, ctypes.c_size_t, vp, vp]
     L.fz_decompress_verify_f32.argtypes = [vp, vp, vp, f32, vp, ctypes.c_size_t, vp, vp, vp, vp]
     for name in EXPORTS:
         if name not in ("fz_version", "fz_status_string", "fz_last_error"):
@@ -296,6 +300,35 @@ class Plan:
         t = tail if (tail is not None and tail.numel() > 0) else None
         _check(lib().fz_decompress_f32(self.handle, _ptr(src), _ptr(t), float(ebx2), _ptr(self.ws), self.ws.numel(),
                                        _ptr(out), _stream(stream)), "fz_decompress")
+        return out
+
+    def compress_outlier(self, d_in, eb_rel, side_capacity=None, stream=None):
+        """OUTLIER mode (plan made with MODE_OUTLIER): returns (payload, tail, result dict, side
+        channel uint8 tensor [16 + 8 (n_delta + n_raw)])."""
+        import torch
+        payload, tail, res = self.alloc_outputs()
+        cap = self.sizes.side_max_bytes if side_capacity is None else int(side_capacity)
+        side = torch.zeros(max(cap, 16), dtype=torch.uint8, device=self.device)
+        _check(lib().fz_compress_outlier_f32(self.handle, _ptr(d_in), float(eb_rel), _ptr(self.ws), self.ws.numel(),
+                                             _ptr(payload), _ptr(tail), _ptr(side), cap, _ptr(res),
+                                             _stream(stream)), "fz_compress_outlier")
+        r = read_result(res)
+        if r["status"] != 0:
+            raise FzError(r["status"], "compress_outlier")
+        nd, nr = (int(v) for v in side[:16].cpu().view(torch.int64).tolist())
+        return payload[: r["payload_bytes"]], tail[: r["tail_len"]], r, side[: 16 + 8 * (nd + nr)]
+
+    def decompress_outlier(self, payload, tail, side, ebx2, vconst, out=None, stream=None):
+        import torch
+        n = self.x * self.y * self.z
+        if out is None:
+            out = torch.empty(n, dtype=torch.float32, device=self.device)
+        src = payload.clone() if payload.data_ptr() % 16 else payload
+        sd = side.clone() if side.data_ptr() % 16 else side
+        t = tail if (tail is not None and tail.numel() > 0) else None
+        _check(lib().fz_decompress_outlier_f32(self.handle, _ptr(src), _ptr(t), float(ebx2), float(vconst), _ptr(sd),
+                                               sd.numel(), _ptr(self.ws), self.ws.numel(), _ptr(out),
+                                               _stream(stream)), "fz_decompress_outlier")
         return out
 
     def decompress_verify_async(self, payload, tail, ebx2, odata, out, d_stats, stream=None):

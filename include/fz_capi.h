@@ -54,8 +54,20 @@ typedef enum fz_status {
  * 3 of 4 deltas, lorenzo_var.cuh:59-63) and the len mod 2048 codes that no tile covers
  * (never encoded by the reference, fz.cu:173) are returned in the tail buffer.
  * FZ_MODE_LITERAL: byte-identical to the reference, including both losses.
- * 2D and 3D payloads are identical in the two modes. */
-typedef enum fz_mode { FZ_MODE_FIXED = 0, FZ_MODE_LITERAL = 1 } fz_mode;
+ * 2D and 3D payloads are identical in the two modes.
+ * FZ_MODE_OUTLIER (opt-in, format-changing; single-field entry points only): the FIXED
+ * payload and tail plus a side channel, modelled on cuSZ's sparse outlier vector
+ * (spcodec_vec.hh:29), so that every field compresses and every value comes back: deltas
+ * with |d| >= 32768 (code 0 in the payload, exact delta in the channel), non-finite values and
+ * values with |x * ebx2_r| >= 2^24 (kept verbatim; quantised as 0), and constant fields
+ * (ebx2 = 0, the inverse fills vmin).  Deltas are exact integers, so the inverse is exact up to
+ * the error bound.  Side channel (16-byte aligned, little-endian):
+ *   u64 n_delta, u64 n_raw, u32 idx_d[n_delta], i32 val_d[n_delta],
+ *   u32 idx_r[n_raw], u32 val_r[n_raw] (input bits),  both lists in increasing index order;
+ * its size is 16 + 8 (n_delta + n_raw) bytes, at most fz_sizes.side_max_bytes.  Where no value
+ * needs the channel and the reference's float arithmetic is exact (every |x * ebx2_r| < 2^21)
+ * the payload equals the FIXED mode's. */
+typedef enum fz_mode { FZ_MODE_FIXED = 0, FZ_MODE_LITERAL = 1, FZ_MODE_OUTLIER = 2 } fz_mode;
 
 /* Device-resident record written by the compress kernels (64 bytes). */
 typedef struct fz_result {
@@ -67,7 +79,8 @@ typedef struct fz_result {
     uint64_t nblocks;       /* K: nonzero 16-byte blocks                                 */
     uint64_t payload_bytes; /* 4 W + 16 K  (fz.cu:208)                                   */
     uint64_t tail_len;      /* codes returned in the tail buffer (0 in LITERAL mode)     */
-    uint64_t overflow;      /* deltas with |d| >= 32768 (their code aliases the sign)    */
+    uint64_t overflow;      /* deltas with |d| >= 32768 (their code aliases the sign;    */
+                            /* OUTLIER mode: n_delta of the side channel)                */
 } fz_result;
 
 /* Sizes of one field's buffers. */
@@ -79,6 +92,7 @@ typedef struct fz_sizes {
     uint64_t payload_max_bytes; /* 4 W + 4096 T: every block nonzero                      */
     uint64_t tail_max_codes;    /* len - 2048 T (< 2048)                                  */
     uint64_t workspace_bytes;   /* device scratch for every entry point of this plan      */
+    uint64_t side_max_bytes;    /* OUTLIER mode: 16 + 8 x*y*z (every value in the channel) */
 } fz_sizes;
 
 typedef struct fz_plan_s* fz_plan;
@@ -117,6 +131,23 @@ fz_status fz_compress_plain_f32(const fz_plan plan, const float* d_in, double eb
  * structure is kept with exact int32 prefix sums and one float multiply by ebx2. */
 fz_status fz_decompress_f32(const fz_plan plan, const uint8_t* d_payload, const uint16_t* d_tail, float ebx2,
                             void* d_ws, size_t ws_bytes, float* d_out, hipStream_t stream);
+
+/* OUTLIER-mode compress (a plan created with FZ_MODE_OUTLIER; the other compress entry
+ * points refuse such a plan): fz_compress_f32's payload, tail and record, and the side channel
+ * in d_side (capacity side_capacity bytes, >= 16; the record's status is FZ_ERR_CAPACITY when
+ * the channel does not fit, and the channel is then incomplete).  Read the channel's size from
+ * its first 16 bytes.  No counterpart in fz-gpu; the side channel follows cuSZ's
+ * SpcodecVec::encode (spcodec_vec.hh:47, launch_spv.cuh:22-49). */
+fz_status fz_compress_outlier_f32(const fz_plan plan, const float* d_in, double eb_rel, void* d_ws,
+                                  size_t ws_bytes, uint8_t* d_payload, uint16_t* d_tail, uint8_t* d_side,
+                                  uint64_t side_capacity, fz_result* d_result, hipStream_t stream);
+
+/* Its inverse: ebx2 and vconst are the record's ebx2 and vmin, side_bytes the channel's size
+ * (16 + 8 (n_delta + n_raw)).  Follows SpcodecVec::decode's scatter (launch_spv.cuh:52-58)
+ * into the deltas before the prefix sums, and of the verbatim values after them. */
+fz_status fz_decompress_outlier_f32(const fz_plan plan, const uint8_t* d_payload, const uint16_t* d_tail, float ebx2,
+                                    float vconst, const uint8_t* d_side, uint64_t side_bytes, void* d_ws,
+                                    size_t ws_bytes, float* d_out, hipStream_t stream);
 
 /* Convenience: runs fz_compress_f32, waits for the stream and copies *d_result to *h_result. */
 fz_status fz_compress_f32_sync(const fz_plan plan, const float* d_in, double eb_rel, void* d_ws, size_t ws_bytes,

@@ -133,3 +133,14 @@ def test_verify_size_query_and_argument_checks_need_no_device():
     assert L.fz_verify_f32(ctypes.c_void_p(256), ctypes.byref(n), None, None, None, 5, None) == 1
     p = fzgpu.Plan(100, 77, 1, workspace=False)
     assert L.fz_decompress_verify_f32(p.handle, None, None, 1.0, None, 0, None, None, None, None) == 1
+
+
+def test_outlier_mode_sizes_and_batch_refusal():
+    p = fzgpu.Plan(100, 77, 1, mode=fzgpu.MODE_OUTLIER, workspace=False)
+    assert p.sizes.side_max_bytes == 16 + 8 * 7700
+    q = fzgpu.Plan(100, 77, 1, workspace=False)
+    assert q.sizes.side_max_bytes == 0
+    assert p.sizes.workspace_bytes > q.sizes.workspace_bytes  # unit counts and int32 deltas
+    with pytest.raises(fzgpu.FzError) as e:
+        fzgpu.BatchPlan([(5000, 1, 1)], mode=fzgpu.MODE_OUTLIER, workspace=False)
+    assert e.value.status == 1

@@ -79,3 +79,44 @@ def test_cli_lines_archive_and_quality_report(tmp_path, dims, eb, literal):
     lag1 = O.verify(x[:-1], x[1:])["reduced"]["coeff"]
     lag2 = O.verify(x[:-2], x[2:])["reduced"]["coeff"]
     assert nums["auto"] == pytest.approx([lag1, lag2], abs=1e-6)
+
+
+def _run_rc(args):
+    say("case cli", " ".join(os.path.basename(a) for a in args))
+    return subprocess.run([CLI] + args, capture_output=True, text=True, timeout=120)
+
+
+@pytest.mark.parametrize("dims", [(600, 300, 1), (30, 40, 50)])
+def test_cli_outlier_mode_archive_round_trip(tmp_path, dims):
+    x = _field(dims, 8)
+    rng = np.random.default_rng(2)
+    x[rng.integers(0, x.size, 50)] = np.nan
+    x[rng.integers(0, x.size, 50)] = np.float32(5e6)
+    src = tmp_path / "f.f32"
+    x.tofile(src)
+    arc = tmp_path / "f.fzmi"
+    lines = _run([str(src), *map(str, dims), "1e-6", "-o", str(arc), "--outliers"]).splitlines()
+    ref = O.compress_outlier(x, dims, 1e-6)
+    assert ref["status"] == 0 and ref["info"]["overflow"] > 0
+    assert lines[1] == f"compressed size: {ref['info']['payload_bytes']}"
+    assert os.path.getsize(arc) == 80 + ref["info"]["payload_bytes"] + 2 * ref["info"]["tail_len"] + \
+        ref["info"]["side_bytes"]
+    out = tmp_path / "o.f32"
+    assert _run(["-d", str(arc), str(out)]) == ""
+    rc, ref_out = O.decompress_outlier(ref["payload"], ref["tail"], ref["side"], x.size, dims, ref["info"]["ebx2"],
+                                       ref["info"]["vmin"])
+    assert rc == 0
+    assert np.array_equal(np.fromfile(out, dtype=np.uint32), ref_out.view(np.uint32))
+
+
+def test_cli_constant_field_needs_the_outlier_mode(tmp_path):
+    c = np.full(20000, 7.5, np.float32)
+    src = tmp_path / "c.f32"
+    c.tofile(src)
+    p = _run_rc([str(src), "200", "100", "1", "1e-3"])
+    assert p.returncode != 0 and "constant" in p.stderr
+    arc = tmp_path / "c.fzmi"
+    _run([str(src), "200", "100", "1", "1e-3", "-o", str(arc), "--outliers"])
+    out = tmp_path / "c.out"
+    _run(["-d", str(arc), str(out)])
+    assert np.array_equal(np.fromfile(out, dtype=np.float32), c)
