@@ -264,6 +264,71 @@ def bench_verify(reps=20, sizes=(3600 * 1800, 1 << 24, 1 << 26, 1 << 28)):
     return res
 
 
+def bench_staged(x, dims, eb, nreq=24, depth=3, nbuf=4):
+    """Host-resident fields (PCIe-inclusive): the reference's pattern (blocking pageable copy in,
+    compress, blocking copy of the result out, one field after the other) against the stager
+    (pinned inputs, depth requests in flight).  Input GB/s over nreq requests rotating over nbuf
+    host copies, every payload checked against the device path."""
+    import numpy as np
+    import torch
+
+    import fzgpu
+    n = x.size
+    variants = [(x * np.float32(1 + 0.01 * k)).astype(np.float32) for k in range(nbuf)]
+    plan = fzgpu.Plan(*dims)
+    ref = []
+    for v in variants:
+        p, t, r = plan.compress(torch.from_numpy(v).to("cuda"), eb)
+        ref.append(p.cpu().numpy())
+    # reference pattern: pageable numpy -> device (blocking), compress, record + payload back
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(nreq):
+        d = torch.from_numpy(variants[i % nbuf]).to("cuda")
+        p, t, r = plan.compress(d, eb)
+        out = p.cpu().numpy()
+    torch.cuda.synchronize()
+    blocking_s = time.perf_counter() - t0
+    assert np.array_equal(out, ref[(nreq - 1) % nbuf])
+    # stager: pinned inputs, depth in flight
+    bufs = []
+    for v in variants:
+        b = fzgpu.HostBuffer(n, np.float32)
+        b.array[:] = v
+        bufs.append(b)
+    st = fzgpu.Stager(*dims, depth=depth)
+    for k in range(depth):  # warm-up
+        st.wait(st.compress(bufs[k % nbuf].array, eb))
+    t0 = time.perf_counter()
+    tickets, mism = [], 0
+    for i in range(nreq):
+        tickets.append(st.compress(bufs[i % nbuf].array, eb))
+        if len(tickets) >= depth:
+            k = tickets[len(tickets) - depth]
+            res, payload, _ = st.wait(k)
+            mism += 0 if np.array_equal(payload, ref[(k - depth) % nbuf]) else 1
+    for k in tickets[len(tickets) - depth + 1:]:
+        res, payload, _ = st.wait(k)
+        mism += 0 if np.array_equal(payload, ref[(k - depth) % nbuf]) else 1
+    staged_s = time.perf_counter() - t0
+    # the copy-in alone: pinned H2D of the same bytes, back to back
+    dev = torch.empty(n, dtype=torch.float32, device="cuda")
+    pin = torch.from_numpy(bufs[0].array)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(8):
+        dev.copy_(pin, non_blocking=True)
+    torch.cuda.synchronize()
+    h2d_gbps = 8 * 4 * n / (time.perf_counter() - t0) / 1e9
+    st.close()
+    res = {"requests": nreq, "depth": depth, "field_MB": round(4 * n / 1e6, 1),
+           "blocking_pageable_GBps": round(nreq * 4 * n / blocking_s / 1e9, 2),
+           "staged_pinned_GBps": round(nreq * 4 * n / staged_s / 1e9, 2),
+           "pinned_h2d_copy_GBps": round(h2d_gbps, 2), "payload_mismatches": mism}
+    log(f"staged: {res}")
+    return res
+
+
 def cpu_baseline(x, dims, eb, seconds):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -348,6 +413,9 @@ def main():
                     help="profiling target: the 256-field uneven batch, default tuning (one group), 3 compress calls")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-serving leg")
     ap.add_argument("--serve-only", action="store_true", help="only the request-serving leg (prints its JSON)")
+    ap.add_argument("--no-staged", action="store_true", help="skip the host-resident (PCIe-inclusive) leg")
+    ap.add_argument("--staged-only", action="store_true",
+                    help="only the host-resident (PCIe-inclusive) leg: blocking pageable vs stager (prints its JSON)")
     ap.add_argument("--verify-only", action="store_true",
                     help="only the quality-metric reduction at four sizes (prints its JSON)")
     ap.add_argument("--pmc-target", action="store_true",
@@ -374,6 +442,10 @@ def main():
         return
     if args.verify_only:
         print(json.dumps(bench_verify(max(args.steps, 5))), flush=True)
+        return
+    if args.staged_only:
+        x, dims, eb = D.config_field(args.workload)
+        print(json.dumps(bench_staged(x.ravel(), dims, eb)), flush=True)
         return
     if args.serve_only:
         print(json.dumps(bench_serving()), flush=True)
@@ -428,6 +500,9 @@ def main():
     if not args.no_serve:
         log("serving loop")
         extra["serving"] = bench_serving()
+    if not args.no_staged:
+        log("host-resident fields (PCIe-inclusive)")
+        extra["host_staged"] = bench_staged(x, dims, eb)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")

@@ -68,7 +68,9 @@ EXPORTS = ["fz_version", "fz_status_string", "fz_last_error", "fz_plan_create", 
            "fz_batch_get_tuning", "fz_batch_set_tuning", "fz_batch_workspace_init", "fz_batch_compress_f32",
            "fz_batch_decompress_f32", "fz_batch_autotune", "fz_batch_profile", "fz_batch_kernel_times",
            "fz_batch_compress_graph", "fz_batch_decompress_graph", "fz_graph_launch", "fz_graph_destroy",
-           "fz_verify_f32", "fz_decompress_verify_f32", "fz_compress_outlier_f32", "fz_decompress_outlier_f32"]
+           "fz_verify_f32", "fz_decompress_verify_f32", "fz_compress_outlier_f32", "fz_decompress_outlier_f32",
+           "fz_host_alloc", "fz_host_free", "fz_stager_create", "fz_stager_destroy", "fz_stager_sizes",
+           "fz_stager_compress", "fz_stager_decompress", "fz_stager_wait"]
 
 
 class _StatsRange(ctypes.Structure):
@@ -183,6 +185,14 @@ def lib():
     L.fz_graph_destroy.argtypes = [vp]
     L.fz_verify_f32.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t), vp, vp, vp, u64, vp]
     L.fz_compress_outlier_f32.argtypes = [vp, vp, f64, vp, ctypes.c_size_t, vp, vp, vp, u64, vp, vp]
+    L.fz_host_alloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t]
+    L.fz_host_free.argtypes = [vp]
+    L.fz_stager_create.argtypes = [ctypes.POINTER(vp), u32, u32, u32, u64, i32]
+    L.fz_stager_destroy.argtypes = [vp]
+    L.fz_stager_sizes.argtypes = [vp, ctypes.POINTER(Sizes)]
+    L.fz_stager_compress.argtypes = [vp, vp, f64, ctypes.POINTER(u64)]
+    L.fz_stager_decompress.argtypes = [vp, vp, u64, vp, f32, vp, ctypes.POINTER(u64)]
+    L.fz_stager_wait.argtypes = [vp, u64, ctypes.POINTER(Result), ctypes.POINTER(vp), ctypes.POINTER(vp)]
     L.fz_decompress_outlier_f32.argtypes = [vp, vp, vp, f32, f32, vp, u64, vp, ctypes.c_size_t, vp, vp]
     L.fz_decompress_verify_f32.argtypes = [vp, vp, vp, f32, vp, ctypes.c_size_t, vp, vp, vp, vp]
     for name in EXPORTS:
@@ -352,6 +362,93 @@ class Plan:
         d_stats = torch.zeros(ctypes.sizeof(Stats), dtype=torch.uint8, device=self.device)
         self.decompress_verify_async(src, tail, ebx2, odata, out, d_stats, stream)
         return out, read_stats(d_stats)
+
+
+class HostBuffer:
+    """Pinned host memory from fz_host_alloc, viewed as a numpy array (freed with the object)."""
+
+    def __init__(self, n, dtype):
+        import numpy as np
+        self.dtype = np.dtype(dtype)
+        self.nbytes = max(int(n) * self.dtype.itemsize, 16)
+        p = ctypes.c_void_p()
+        _check(lib().fz_host_alloc(ctypes.byref(p), self.nbytes), "fz_host_alloc")
+        self.ptr = p
+        buf = (ctypes.c_uint8 * self.nbytes).from_address(p.value)
+        self.array = np.frombuffer(buf, dtype=np.uint8)[: int(n) * self.dtype.itemsize].view(self.dtype)
+
+    def close(self):
+        if self.ptr is not None and _lib is not None:
+            _lib.fz_host_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Stager:
+    """Host-resident requests of one geometry (fz_stager_*): depth requests in flight, the
+    copy in, kernels and copy out of consecutive requests overlapped."""
+
+    def __init__(self, x, y, z, length=0, depth=3):
+        h = ctypes.c_void_p()
+        _check(lib().fz_stager_create(ctypes.byref(h), int(x), int(y), int(z), int(length), int(depth)),
+               "fz_stager_create")
+        self.handle = h
+        self.depth = depth
+        s = Sizes()
+        _check(lib().fz_stager_sizes(h, ctypes.byref(s)), "fz_stager_sizes")
+        self.sizes = s
+        self.keep = {}  # host buffers of requests in flight
+
+    def compress(self, h_in, eb_rel):
+        """Enqueue; h_in is a numpy float32 array (pinned, e.g. a HostBuffer's, for an
+        asynchronous copy).  Returns the ticket."""
+        t = ctypes.c_uint64()
+        _check(lib().fz_stager_compress(self.handle, ctypes.c_void_p(h_in.ctypes.data), float(eb_rel),
+                                        ctypes.byref(t)), "fz_stager_compress")
+        self.keep[t.value] = h_in
+        return t.value
+
+    def decompress(self, h_payload, h_tail, ebx2, h_out):
+        t = ctypes.c_uint64()
+        tail_p = None if h_tail is None or h_tail.size == 0 else ctypes.c_void_p(h_tail.ctypes.data)
+        _check(lib().fz_stager_decompress(self.handle, ctypes.c_void_p(h_payload.ctypes.data), h_payload.nbytes,
+                                          tail_p, float(ebx2), ctypes.c_void_p(h_out.ctypes.data),
+                                          ctypes.byref(t)), "fz_stager_decompress")
+        self.keep[t.value] = (h_payload, h_tail, h_out)
+        return t.value
+
+    def wait(self, ticket):
+        """Compress: (result dict, payload bytes copy, tail u16 copy); decompress: None."""
+        import numpy as np
+        r = Result()
+        pp, tp = ctypes.c_void_p(), ctypes.c_void_p()
+        st = lib().fz_stager_wait(self.handle, int(ticket), ctypes.byref(r), ctypes.byref(pp), ctypes.byref(tp))
+        self.keep.pop(int(ticket), None)
+        if st == 1:
+            raise FzError(st, "fz_stager_wait")
+        if not pp.value:
+            _check(st, "fz_stager_wait")
+            return None
+        res = r.as_dict()
+        payload = np.ctypeslib.as_array((ctypes.c_uint8 * max(int(r.payload_bytes), 1)).from_address(pp.value))
+        tail = np.ctypeslib.as_array((ctypes.c_uint16 * max(int(r.tail_len), 1)).from_address(tp.value))
+        return res, payload[: r.payload_bytes].copy(), tail[: r.tail_len].copy()
+
+    def close(self):
+        if self.handle is not None and _lib is not None:
+            _lib.fz_stager_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def read_stats(stats_tensor):

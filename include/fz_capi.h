@@ -167,6 +167,42 @@ fz_status fz_plan_set_tile_layout(fz_plan plan, int wave_tile);
 fz_status fz_plan_profile(fz_plan plan, int enable);
 fz_status fz_plan_kernel_times(const fz_plan plan, float* ms, int capacity, int* count);
 
+/* ---- host-resident requests: pinned, asynchronous staging (fz.cu:143, 195-201) ---- */
+
+/* Pinned host memory (hipHostMalloc): a buffer the stager can copy from or to asynchronously. */
+fz_status fz_host_alloc(void** p, size_t bytes);
+fz_status fz_host_free(void* p);
+
+#define FZ_STAGER_MAX_DEPTH 8
+typedef struct fz_stager_s* fz_stager;
+
+/* A stager compresses (FIXED mode) and decompresses fields of one geometry held in host
+ * memory, replacing the reference's blocking pageable copies around its kernels (fz.cu:143,
+ * :195-201).  It keeps `depth` requests (1..8) in flight on three streams, so the copy in of
+ * one request, the kernels of the one before and the copy out of the one before that
+ * overlap.  It owns its device buffers, one workspace and, per slot, a pinned output buffer. */
+fz_status fz_stager_create(fz_stager* stager, uint32_t x, uint32_t y, uint32_t z, uint64_t len, int depth);
+fz_status fz_stager_destroy(fz_stager stager);
+fz_status fz_stager_sizes(const fz_stager stager, fz_sizes* sizes);
+
+/* Enqueue a compress of h_in (len floats; pinned memory, e.g. from fz_host_alloc, gives an
+ * asynchronous copy, pageable memory a blocking one).  h_in must stay unchanged until the
+ * request completes.  Returns at once unless all `depth` slots are in flight, in which case
+ * it first waits for the oldest.  The results of a request stay readable until `depth` newer
+ * requests have been made. */
+fz_status fz_stager_compress(fz_stager stager, const float* h_in, double eb_rel, uint64_t* ticket);
+
+/* Enqueue a decompress of a payload (and the tail codes, or NULL) into h_out (x*y*z floats;
+ * pinned for an asynchronous copy). */
+fz_status fz_stager_decompress(fz_stager stager, const uint8_t* h_payload, uint64_t payload_bytes,
+                               const uint16_t* h_tail, float ebx2, float* h_out, uint64_t* ticket);
+
+/* Wait for a request.  For a compress: *result is its record, *payload and *tail point into the
+ * slot's pinned buffer (payload_bytes and tail_len from the record).  Returns the record's
+ * status (FZ_SUCCESS for a decompress). */
+fz_status fz_stager_wait(fz_stager stager, uint64_t ticket, fz_result* result, const uint8_t** payload,
+                         const uint16_t** tail);
+
 /* ---- quality metrics of a reconstruction (verify.hh:25-83, quality_viewer.hh:97-110) ---- */
 
 /* cusz_stats (type.h:185-194), size_t as uint64_t. */
