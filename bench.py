@@ -241,6 +241,42 @@ def bench_serving(rates=(50, 100, 200, 500, 1000, 2000), duration_s=1.0, pool_fi
     return out
 
 
+def bench_outlier(x, dims, eb, steps, rotate_bytes=1.2e9):
+    """Outlier mode (FIXED payload + side channel) on the same field, inputs rotated as in
+    bench_field: input GB/s and the channel's size."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    import fzgpu
+    n = x.size
+    ncopies = max(2, min(64, math.ceil(rotate_bytes / (4 * n))))
+    base = torch.from_numpy(np.ascontiguousarray(x)).to("cuda")
+    copies = [base] + [base.clone() for _ in range(ncopies - 1)]
+    plan = fzgpu.Plan(*dims, mode=fzgpu.MODE_OUTLIER)
+    payload, tail, res = plan.alloc_outputs()
+    cap = plan.sizes.side_max_bytes
+    side = torch.zeros(cap, dtype=torch.uint8, device="cuda")
+    L = fzgpu.lib()
+
+    def step(k):
+        fzgpu._check(L.fz_compress_outlier_f32(plan.handle, fzgpu._ptr(copies[k % ncopies]), float(eb),
+                                               fzgpu._ptr(plan.ws), plan.ws.numel(), fzgpu._ptr(payload),
+                                               fzgpu._ptr(tail), fzgpu._ptr(side), cap, fzgpu._ptr(res),
+                                               fzgpu._stream()), "fz_compress_outlier")
+
+    for k in range(3):
+        step(k)
+    ms, _ = time_steps(step, steps)
+    r = fzgpu.read_result(res)
+    nd, nr = (int(v) for v in side[:16].cpu().view(torch.int64).tolist())
+    del copies, base
+    torch.cuda.empty_cache()
+    return {"GBps": round(4 * n / (ms * 1e-3) / 1e9, 2), "ms": round(ms, 5), "status": r["status"],
+            "payload_bytes": r["payload_bytes"], "side_bytes": 16 + 8 * (nd + nr), "n_delta": nd, "n_raw": nr}
+
+
 def bench_verify(reps=20, sizes=(3600 * 1800, 1 << 24, 1 << 26, 1 << 28)):
     """fz_verify_f32 (verify.hh:25-83 on the device) on reconstruction-like pairs of several
     sizes: microseconds per call and GB/s of the 8n bytes it reads."""
@@ -476,6 +512,8 @@ def main():
         "verify_max_rel_err": main_r["verify_stats"]["max_err"]["rel"],
         "input_copies": main_r["copies"],
     }
+    log("timing the outlier mode")
+    extra["outlier_mode"] = bench_outlier(x, dims, eb, args.steps)
     log("timing plain pipeline")
     plain = bench_field(x, dims, eb, args.steps, args.warmup, fused=False, label="plain")
     extra["plain_pipeline_GBps"] = round(plain["gbps"], 2)
@@ -493,7 +531,8 @@ def main():
                                           "fused_kernel_frac": round((4 * xe.size + r["payload_bytes"]) /
                                                                      (r["kernel_ms"][1] * 1e-3) / 1e9 /
                                                                      HBM_PEAK_GBPS, 4),
-                                          "decompress_GBps": round(r["decompress_gbps"], 2)}
+                                          "decompress_GBps": round(r["decompress_gbps"], 2),
+                                          "outlier_mode_GBps": bench_outlier(xe.ravel(), de, e, args.steps)["GBps"]}
     if not args.no_batch:
         log(f"timing the {args.batch_fields}-field uneven batch")
         extra["uneven_batch"] = bench_batch(min(args.steps, 20), min(args.warmup, 3), args.batch_fields)

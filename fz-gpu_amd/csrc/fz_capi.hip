@@ -17,8 +17,7 @@ constexpr uint64_t kMaxLen = 0xFFFFF000ull;  // positions are 32-bit inside the 
 inline size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 struct Layout {
-    size_t ctr, part, tstate, sstate, dstate, codes, shuf, F, P, texcl, counts, stage, vtemp, ocnt, oofs, ochunk, dwide,
-        total;
+    size_t ctr, part, tstate, sstate, dstate, codes, shuf, F, P, texcl, counts, stage, vtemp, ocnt, orun, dwide, total;
     uint32_t n_tstate, n_sstate, n_dstate;
 };
 
@@ -45,8 +44,7 @@ Layout make_layout(const Geom& g, int mode)
     L.vtemp = o;  o = al256(o + verify_workspace_bytes((uint64_t)g.xyz));
     if (mode == FZ_MODE_OUTLIER) {  // per-unit side-channel counts and offsets; int32 deltas
         L.ocnt = o;   o = al256(o + 8ull * (g.T + 1));
-        L.oofs = o;   o = al256(o + 8ull * (g.T + 1));
-        L.ochunk = o; o = al256(o + 8ull * ochunks(g.T));
+        L.orun = o;   o = al256(o + 8ull * outlier_runs(g.T));
         L.dwide = o;  o = al256(o + 4ull * g.xyz + 64);
     }
     L.total = o;
@@ -341,8 +339,7 @@ fz_status fz_compress_outlier_f32(const fz_plan plan, const float* d_in, double 
                                    d_tail, d_side, side_capacity, ws + L.stage,
                                    reinterpret_cast<uint32_t*>(ws + L.counts), reinterpret_cast<uint32_t*>(ws + L.tstate),
                                    L.n_tstate, reinterpret_cast<uint32_t*>(ws + L.ocnt),
-                                   reinterpret_cast<uint32_t*>(ws + L.oofs), reinterpret_cast<uint32_t*>(ws + L.ochunk),
-                                   stream));
+                                   reinterpret_cast<uint32_t*>(ws + L.orun), stream));
     FZ_TRY(prof_mark(plan, stream, false));
     return FZ_SUCCESS;
 }

@@ -67,14 +67,13 @@ hipError_t launch_reconstruct(const uint16_t* codes, float* out, const Geom& g, 
 hipError_t launch_reconstruct_i32(const int32_t* deltas, float* out, const Geom& g, float ebx2, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 
-// Outlier mode (fz_outlier.hip).  Workspace pieces: ocnt / oofs hold 2 u32 per unit (T + 1
-// units), ochunk 2 u32 per 4096 units; dwide holds x*y*z int32 deltas for the inverse.
-constexpr uint32_t kOChunk = 4096;  // units per workgroup of the side-channel offset scan
-inline uint32_t    ochunks(uint32_t T) { return (T + 1 + kOChunk - 1) / kOChunk; }
+// Outlier mode (fz_outlier.hip).  Workspace pieces: ocnt holds 2 u32 per unit (T + 1 units),
+// orun 2 u32 per placement run; dwide holds x*y*z int32 deltas for the inverse.
+inline uint32_t outlier_runs(uint32_t T) { return T ? gather_grid(T) : 1u; }
 hipError_t launch_compress_outlier(const float* in, const Geom& g, double eb_rel, float4* part, fz_result* res,
                                    uint8_t* payload, uint16_t* tail, uint8_t* side, uint64_t side_cap,
                                    uint8_t* staging, uint32_t* counts, uint32_t* run_sums, uint32_t nsums,
-                                   uint32_t* ocnt, uint32_t* oofs, uint32_t* ochunk, hipStream_t s);
+                                   uint32_t* ocnt, uint32_t* orun, hipStream_t s);
 // after the codes of the plain inverse are in `codes`: deltas, side-channel deltas, prefix sums,
 // verbatim values
 hipError_t launch_reconstruct_outlier(const uint16_t* codes, int32_t* dwide, const uint8_t* side,

@@ -9,15 +9,19 @@
 //   u32 idx_d[n_delta], i32 val_d[n_delta],   exact deltas of the positions whose code is 0
 //   u32 idx_r[n_raw],   u32 val_r[n_raw]      input bits of the values kept verbatim
 //
-// both lists in increasing index order (deterministic: per-unit counts, a scan, then each unit
-// writes its own entries).
+// both lists in increasing index order (deterministic: per-unit counts, per-run totals, then
+// each run writes its units' entries in order).
 //
 // Compress:   k_range_part_o     min / max of the finite values, non-finite flag
-//             k_tile_compress_o  packed paths when nothing needs the side channel (level >= 3,
-//                                all finite), the integer path otherwise; per-unit counts
+//             k_tile_compress_o  the fixed mode's paths when no value is kept verbatim (all
+//                                finite, |s| < 2^23), redoing a tile with an overflowing delta
+//                                on the integer path; the integer path otherwise; per-unit counts
 //             k_scan_gather      placement, as in the fused path
-//             k_ocount_scan      per-unit offsets within 4096-unit chunks, chunk totals
-//             k_outlier_write    the units with entries recompute their deltas and write them
+//             k_outlier_place    one workgroup per placement run (the tile kernel adds each
+//                                unit's counts to its run's pair): the run's offset from the
+//                                runs before it, then its units with entries recompute their
+//                                deltas and write them; with no entry anywhere it only writes
+//                                the header
 // Decompress: the plain inverse's codes, then k_expand (codes -> int32 deltas), k_put_deltas,
 //             the reconstruct kernels on int32 deltas, k_put_raw.  A constant field is a fill.
 #include <hip/hip_runtime.h>
@@ -33,11 +37,13 @@ constexpr int kORangeWG = 1024;  // threads per partial-range workgroup (as the 
 
 __global__ __launch_bounds__(kORangeWG) void k_range_part_o(const float* __restrict__ in, Geom g,
                                                             float4* __restrict__ part, uint32_t* run_sums,
-                                                            uint32_t nsums, fz_result* res)
+                                                            uint32_t nsums, uint32_t* orun, uint32_t norun,
+                                                            fz_result* res)
 {
     const uint32_t tid = threadIdx.x;
-    if (blockIdx.x == 0) {  // clear the placement run sums and the overflow count of this call
+    if (blockIdx.x == 0) {  // clear the run sums, the runs' outlier pairs and the overflow count
         for (uint32_t i = tid; i < nsums; i += kORangeWG) run_sums[i] = 0;
+        for (uint32_t i = tid; i < norun; i += kORangeWG) orun[i] = 0;
         if (tid == 0) res->overflow = 0;
     }
     const uint32_t per = (g.len + gridDim.x - 1) / gridDim.x;
@@ -51,22 +57,28 @@ __global__ __launch_bounds__(kWG) void k_tile_compress_o(const float* __restrict
                                                          fz_result* res, uint8_t* __restrict__ payload,
                                                          uint8_t* __restrict__ staging, uint32_t* __restrict__ counts,
                                                          uint16_t* __restrict__ tail, uint32_t* run_sums, uint32_t tpw,
-                                                         uint32_t* __restrict__ ocnt)
+                                                         uint32_t* __restrict__ ocnt, uint32_t* orun, uint32_t nruns)
 {
     const bool     is_tail = blockIdx.x >= g.T;
     const uint32_t tile = is_tail ? g.T : xcd_tile(blockIdx.x, g.T);
     const int      gp = is_tail ? 0 : geom_path<NDIM>(g, tile * (uint32_t)kTileCodes);
     Partials       P;
     RawTile<NDIM>  R;
-    partials_load(part, nparts, P);
-    raw_prefetch<NDIM>(in, tile * (uint32_t)kTileCodes, 8 * threadIdx.x, g, gp, R);
+    if constexpr (NDIM < 3) {  // the fused kernel's load order (partials first, then the tile)
+        partials_load(part, nparts, P);
+        __builtin_amdgcn_sched_barrier(0);
+        raw_prefetch<NDIM>(in, tile * (uint32_t)kTileCodes, 8 * threadIdx.x, g, gp, R);
+    } else {
+        raw_prefetch<NDIM>(in, tile * (uint32_t)kTileCodes, 8 * threadIdx.x, g, gp, R);
+        partials_load(part, nparts, P);
+    }
     const EbParams e = partials_reduce<true>(P, eb_rel, NDIM);
     if (blockIdx.x == 0 && threadIdx.x == 0) write_header(res, e, g);
     if (e.status != ST_SUCCESS) return;  // uniform
     unsigned long long* ovf = reinterpret_cast<unsigned long long*>(&res->overflow);
     uint32_t*           bwords = reinterpret_cast<uint32_t*>(payload);
     if (is_tail) {
-        tail_unit_o<NDIM>(in, g, e.r, bwords, tail, ovf, ocnt + 2 * g.T);
+        tail_unit_o<NDIM>(in, g, e.r, bwords, tail, ovf, ocnt + 2 * g.T, orun + 2 * (nruns - 1));
         if (g.T == 0 && threadIdx.x == 0) {
             res->nblocks = 0;
             res->payload_bytes = 4ull * g.W;
@@ -75,12 +87,8 @@ __global__ __launch_bounds__(kWG) void k_tile_compress_o(const float* __restrict
     }
     uint8_t* const  slot = staging + (size_t)tile * 4096;
     uint32_t* const sum = run_sums + tile / tpw;
-    if (e.level >= 3) {
-        tile_encode<NDIM>(in, g, tile, gp, R, e.r, e.level, bwords, slot, counts + tile, ovf, sum);
-        if (threadIdx.x == 0) *reinterpret_cast<uint2*>(ocnt + 2 * tile) = make_uint2(0u, 0u);
-    } else {
-        tile_encode_o<NDIM>(in, g, tile, e.r, bwords, slot, counts + tile, ovf, sum, ocnt + 2 * tile);
-    }
+    tile_encode_o<NDIM>(in, g, tile, gp, R, e.r, e.level, bwords, slot, counts + tile, ovf, sum, ocnt + 2 * tile,
+                        orun + 2 * (tile / tpw));
 }
 
 // exclusive (overflow, verbatim) prefix of a pair over the workgroup (kWG threads), and the totals
@@ -107,97 +115,33 @@ __device__ __forceinline__ uint2 block_excl2(uint32_t a, uint32_t b, uint2& tota
     return make_uint2(pa + ia - a, pb + ib - b);
 }
 
-// Chunk c of 4096 units: thread t owns units 16t .. 16t + 15.  Writes each unit's exclusive
-// (overflow, verbatim) offsets within the chunk to oofs and the chunk's totals to ochunk.
-__global__ __launch_bounds__(kWG) void k_ocount_scan(const uint32_t* __restrict__ ocnt, uint32_t nunits,
-                                                     uint32_t* __restrict__ oofs, uint32_t* __restrict__ ochunk)
-{
-    constexpr uint32_t kPer = kOChunk / kWG;
-    const uint32_t     u0 = blockIdx.x * kOChunk + kPer * threadIdx.x;
-    uint2              c[kPer];
-    uint32_t           a = 0, b = 0;
-#pragma unroll
-    for (uint32_t j = 0; j < kPer; j++) {
-        c[j] = u0 + j < nunits ? reinterpret_cast<const uint2*>(ocnt)[u0 + j] : make_uint2(0u, 0u);
-        a += c[j].x;
-        b += c[j].y;
-    }
-    uint2       total;
-    const uint2 ex = block_excl2(a, b, total);
-    uint32_t    ra = ex.x, rb = ex.y;
-#pragma unroll
-    for (uint32_t j = 0; j < kPer; j++) {
-        if (u0 + j < nunits) reinterpret_cast<uint2*>(oofs)[u0 + j] = make_uint2(ra, rb);
-        ra += c[j].x;
-        rb += c[j].y;
-    }
-    if (threadIdx.x == 0) reinterpret_cast<uint2*>(ochunk)[blockIdx.x] = total;
-}
-
 __device__ __forceinline__ void put_u32(uint8_t* side, uint64_t cap, uint64_t off, uint32_t v)
 {
     if (off + 4 <= cap) *reinterpret_cast<uint32_t*>(side + off) = v;
 }
 
-// One workgroup per unit.  Units without entries leave at once; the tail unit (the last) also
-// writes the header and, when the entries exceed the capacity, FZ_ERR_CAPACITY.
+// Recompute unit u's deltas (all threads) and write its entries from the list positions kd
+// (deltas) and kr (verbatim values).
 template <int NDIM>
-__global__ __launch_bounds__(kWG) void k_outlier_write(const float* __restrict__ in, Geom g, fz_result* res,
-                                                       const uint32_t* __restrict__ ocnt,
-                                                       const uint32_t* __restrict__ oofs,
-                                                       const uint32_t* __restrict__ ochunk, uint32_t nchunks,
-                                                       uint8_t* __restrict__ side, uint64_t cap)
+__device__ __forceinline__ void write_unit(const float* __restrict__ in, const Geom& g, float r, uint32_t u,
+                                           uint64_t kd, uint64_t kr, uint64_t nd, uint64_t nr, uint8_t* side,
+                                           uint64_t cap)
 {
-    __shared__ uint2 s_base, s_tot;
-    const uint32_t u = blockIdx.x, tid = threadIdx.x;
-    const bool     last = u == g.T;
-    const uint2    cnt = reinterpret_cast<const uint2*>(ocnt)[u];
-    // the only failure before this kernel: a value range that overflows (no codes were made);
-    // the tail unit may set FZ_ERR_CAPACITY meanwhile, which does not stop the other units
-    if (res->status == ST_NONFINITE) return;
-    if (cnt.x == 0 && cnt.y == 0 && !last) return;  // uniform
-    // entries before this unit's chunk, and in all
-    const uint32_t c = u / kOChunk;
-    uint32_t       ba = 0, bb = 0, ta = 0, tb = 0;
-    for (uint32_t j = tid; j < nchunks; j += kWG) {
-        const uint2 v = reinterpret_cast<const uint2*>(ochunk)[j];
-        ta += v.x;
-        tb += v.y;
-        ba += j < c ? v.x : 0u;
-        bb += j < c ? v.y : 0u;
-    }
-    uint2 tot, all;
-    (void)block_excl2(ba, bb, tot);  // tot: entries of the chunks before this one
-    (void)block_excl2(ta, tb, all);  // all: entries in all
-    if (tid == 0) {
-        const uint2 o = reinterpret_cast<const uint2*>(oofs)[u];
-        s_base = make_uint2(tot.x + o.x, tot.y + o.y);
-        s_tot = all;
-    }
-    __syncthreads();
-    const uint64_t nd = s_tot.x, nr = s_tot.y;
-    if (last && tid == 0) {
-        if (cap >= 16) {
-            reinterpret_cast<uint64_t*>(side)[0] = nd;
-            reinterpret_cast<uint64_t*>(side)[1] = nr;
-        }
-        if (16 + 8 * (nd + nr) > cap) res->status = FZ_ERR_CAPACITY;
-    }
-    if (cnt.x == 0 && cnt.y == 0) return;
-    // this unit's entries, in index order: thread t holds positions 8t .. 8t + 7
+    const uint32_t tid = threadIdx.x;
     const uint32_t t0 = u * (uint32_t)kTileCodes;
-    const uint32_t ulen = last ? g.len - t0 : (uint32_t)kTileCodes;
+    const uint32_t ulen = u == g.T ? g.len - t0 : (uint32_t)kTileCodes;
     Int8           o;
     o.big = o.raw = 0;
-    if (8 * tid < ulen) dq8_int<NDIM>(in, t0 + 8 * tid, g, res->ebx2_r, o);
+    if (8 * tid < ulen) dq8_int<NDIM>(in, t0 + 8 * tid, g, r, o);
     if (8 * tid + 8 > ulen) {  // positions past the unit (the tail unit only)
         const uint32_t keep = 8 * tid < ulen ? (1u << (ulen - 8 * tid)) - 1u : 0u;
         o.big &= keep;
         o.raw &= keep;
     }
-    uint2       unit_tot;
-    const uint2 pre = block_excl2((uint32_t)__popc(o.big), (uint32_t)__popc(o.raw), unit_tot);
-    uint64_t    kd = (uint64_t)s_base.x + pre.x, kr = (uint64_t)s_base.y + pre.y;
+    uint2       tot;
+    const uint2 pre = block_excl2((uint32_t)__popc(o.big), (uint32_t)__popc(o.raw), tot);
+    kd += pre.x;
+    kr += pre.y;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
         const uint32_t p = t0 + 8 * tid + i;
@@ -211,6 +155,70 @@ __global__ __launch_bounds__(kWG) void k_outlier_write(const float* __restrict__
             put_u32(side, cap, 16 + 8 * nd + 4 * nr + 4 * kr, __float_as_uint(in[p]));
             ++kr;
         }
+    }
+}
+
+// One workgroup per placement run r (units [r tpw, (r+1) tpw), the tail unit in the last run).
+// The runs' (overflow, verbatim) pairs give the run's first list positions and the totals;
+// workgroup 0 writes the header and, when the entries exceed the capacity, FZ_ERR_CAPACITY.
+// Only runs with entries go on: slices of 256 units are scanned, and each unit with entries is
+// recomputed and written in order.
+template <int NDIM>
+__global__ __launch_bounds__(kWG) void k_outlier_place(const float* __restrict__ in, Geom g, fz_result* res,
+                                                       const uint32_t* __restrict__ ocnt,
+                                                       const uint32_t* __restrict__ orun, uint32_t nruns, uint32_t tpw,
+                                                       uint8_t* __restrict__ side, uint64_t cap)
+{
+    __shared__ uint32_t sh_u[kWG];
+    __shared__ uint2    sh_k[kWG];
+    __shared__ uint32_t sh_n;
+    const uint32_t r = blockIdx.x, tid = threadIdx.x;
+    // the only failure before this kernel: a value range that overflows (no codes were made)
+    if (res->status == ST_NONFINITE) return;
+    uint32_t bd = 0, br = 0, td = 0, tr = 0;
+    for (uint32_t i = tid; i < nruns; i += kWG) {
+        const uint2 v = reinterpret_cast<const uint2*>(orun)[i];
+        td += v.x;
+        tr += v.y;
+        bd += i < r ? v.x : 0u;
+        br += i < r ? v.y : 0u;
+    }
+    uint2 before, all;
+    (void)block_excl2(bd, br, before);
+    (void)block_excl2(td, tr, all);
+    const uint64_t nd = all.x, nr = all.y;
+    if (r == 0 && tid == 0) {
+        if (cap >= 16) {
+            reinterpret_cast<uint64_t*>(side)[0] = nd;
+            reinterpret_cast<uint64_t*>(side)[1] = nr;
+        }
+        if (16 + 8 * (nd + nr) > cap) res->status = FZ_ERR_CAPACITY;
+    }
+    const uint2 mine = reinterpret_cast<const uint2*>(orun)[r];
+    if (mine.x == 0 && mine.y == 0) return;  // uniform: no entry in this run
+    const float    rq = res->ebx2_r;
+    const uint32_t u_lo = r * tpw, u_hi = r + 1 == nruns ? g.T + 1 : min((r + 1) * tpw, g.T + 1);
+    uint64_t       kd = before.x, kr = before.y;
+    for (uint32_t s0 = u_lo; s0 < u_hi; s0 += kWG) {
+        const uint32_t u = s0 + tid;
+        const uint2    c = u < u_hi ? reinterpret_cast<const uint2*>(ocnt)[u] : make_uint2(0u, 0u);
+        uint2          slice;
+        const uint2    ex = block_excl2(c.x, c.y, slice);
+        // compact the units with entries (in order) with their first list positions
+        const bool     has = (c.x | c.y) != 0;
+        uint2          nunits;
+        const uint2    at = block_excl2(has ? 1u : 0u, 0u, nunits);
+        if (has) {
+            sh_u[at.x] = u;
+            sh_k[at.x] = ex;
+        }
+        if (tid == 0) sh_n = nunits.x;
+        __syncthreads();
+        for (uint32_t k = 0; k < sh_n; k++)
+            write_unit<NDIM>(in, g, rq, sh_u[k], kd + sh_k[k].x, kr + sh_k[k].y, nd, nr, side, cap);
+        kd += slice.x;
+        kr += slice.y;
+        __syncthreads();  // sh_u / sh_k / sh_n are read before the next slice overwrites them
     }
 }
 
@@ -276,29 +284,28 @@ constexpr uint32_t kScatterWG = 256;  // workgroups of the side-channel scatters
 hipError_t launch_compress_outlier(const float* in, const Geom& g, double eb_rel, float4* part, fz_result* res,
                                    uint8_t* payload, uint16_t* tail, uint8_t* side, uint64_t side_cap,
                                    uint8_t* staging, uint32_t* counts, uint32_t* run_sums, uint32_t nsums,
-                                   uint32_t* ocnt, uint32_t* oofs, uint32_t* ochunk, hipStream_t s)
+                                   uint32_t* ocnt, uint32_t* orun, hipStream_t s)
 {
     const uint32_t nparts = range_grid(g.len);
-    (void)hipGetLastError();
-    k_range_part_o<<<nparts, kORangeWG, 0, s>>>(in, g, part, run_sums, nsums, res);
     const uint32_t tpw = g.T ? gather_tpw(g.T) : 1u;
-    const dim3     grid(g.T + 1), blk(kWG);
+    const uint32_t nruns = outlier_runs(g.T);
+    (void)hipGetLastError();
+    k_range_part_o<<<nparts, kORangeWG, 0, s>>>(in, g, part, run_sums, nsums, orun, 2 * nruns, res);
+    const dim3 grid(g.T + 1), blk(kWG);
     switch (g.ndim) {
-        case 1: k_tile_compress_o<1><<<grid, blk, 0, s>>>(in, g, eb_rel, part, nparts, res, payload, staging, counts, tail, run_sums, tpw, ocnt); break;
-        case 2: k_tile_compress_o<2><<<grid, blk, 0, s>>>(in, g, eb_rel, part, nparts, res, payload, staging, counts, tail, run_sums, tpw, ocnt); break;
-        default: k_tile_compress_o<3><<<grid, blk, 0, s>>>(in, g, eb_rel, part, nparts, res, payload, staging, counts, tail, run_sums, tpw, ocnt); break;
+        case 1: k_tile_compress_o<1><<<grid, blk, 0, s>>>(in, g, eb_rel, part, nparts, res, payload, staging, counts, tail, run_sums, tpw, ocnt, orun, nruns); break;
+        case 2: k_tile_compress_o<2><<<grid, blk, 0, s>>>(in, g, eb_rel, part, nparts, res, payload, staging, counts, tail, run_sums, tpw, ocnt, orun, nruns); break;
+        default: k_tile_compress_o<3><<<grid, blk, 0, s>>>(in, g, eb_rel, part, nparts, res, payload, staging, counts, tail, run_sums, tpw, ocnt, orun, nruns); break;
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     e = launch_scan_gather(counts, g.T, staging, payload, run_sums, res, g.W, s);
     if (e != hipSuccess) return e;
-    const uint32_t nch = ochunks(g.T);
     (void)hipGetLastError();
-    k_ocount_scan<<<nch, kWG, 0, s>>>(ocnt, g.T + 1, oofs, ochunk);
     switch (g.ndim) {
-        case 1: k_outlier_write<1><<<grid, blk, 0, s>>>(in, g, res, ocnt, oofs, ochunk, nch, side, side_cap); break;
-        case 2: k_outlier_write<2><<<grid, blk, 0, s>>>(in, g, res, ocnt, oofs, ochunk, nch, side, side_cap); break;
-        default: k_outlier_write<3><<<grid, blk, 0, s>>>(in, g, res, ocnt, oofs, ochunk, nch, side, side_cap); break;
+        case 1: k_outlier_place<1><<<nruns, blk, 0, s>>>(in, g, res, ocnt, orun, nruns, tpw, side, side_cap); break;
+        case 2: k_outlier_place<2><<<nruns, blk, 0, s>>>(in, g, res, ocnt, orun, nruns, tpw, side, side_cap); break;
+        default: k_outlier_place<3><<<nruns, blk, 0, s>>>(in, g, res, ocnt, orun, nruns, tpw, side, side_cap); break;
     }
     return hipGetLastError();
 }

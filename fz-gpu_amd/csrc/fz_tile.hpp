@@ -157,9 +157,11 @@ __device__ __forceinline__ EbParams eb_params(float lo, float hi, uint32_t bad, 
 
 // The outlier mode's parameters: the range of the finite values (0, 0 when there are none);
 // a zero range gives r = ebx2 = 0 (every quantised integer is 0 and the inverse fills vmin).
-// Only a range that itself overflows is an error.  Tiles take the packed paths when no value
-// is non-finite and level >= 3 (then no delta reaches 32768 and no value is kept verbatim),
-// the integer path (kLevelOutlier) otherwise.
+// Only a range that itself overflows is an error.  With no non-finite value and every
+// |s| < 2^23 (level >= 1) nothing is kept verbatim and the fixed mode's float paths give the
+// exact integer deltas wherever |d| < 32768 (1D and 2D: every partial sum is below 2^24 in
+// magnitude; 3D needs level >= 2 for that), so tiles take them and redo only a tile with an
+// overflowing delta on the integer path; otherwise (kLevelOutlier) every tile takes it.
 constexpr int kLevelOutlier = -1;
 
 __device__ __forceinline__ EbParams eb_params_o(float lo, float hi, uint32_t bad, double eb_rel, int ndim)
@@ -174,7 +176,7 @@ __device__ __forceinline__ EbParams eb_params_o(float lo, float hi, uint32_t bad
         e.level = lo >= 0.0f ? 4 : 3;
     }
     e.status = (range - range == 0.0f) ? ST_SUCCESS : ST_NONFINITE;
-    if (bad || e.level < 3) e.level = kLevelOutlier;
+    if (bad || e.level < (ndim == 3 ? 2 : 1)) e.level = kLevelOutlier;
     return e;
 }
 
@@ -366,11 +368,13 @@ __device__ __forceinline__ void pk_dispatch(const RawTile<NDIM>& R, uint32_t p0,
 
 // Steps 2-4 of a tile (bit transpose, flags, in-tile compaction) on the thread's 4 code
 // words w, whose overflow count is ovf.  With ocnt, thread 0 also stores the tile's
-// (overflows, verbatim values) totals there: the per-unit counts of the outlier mode.
+// (overflows, verbatim values) totals there and adds them to its run's pair at orun: the
+// per-unit counts of the outlier mode.
 __device__ __forceinline__ void tile_finish(uint32_t w[4], uint32_t ovf, uint32_t tile,
                                             uint32_t* __restrict__ bwords, uint8_t* __restrict__ slot,
                                             uint32_t* __restrict__ count_out, unsigned long long* ovf_out,
-                                            uint32_t* run_sum, uint32_t nraw = 0, uint32_t* ocnt = nullptr)
+                                            uint32_t* run_sum, uint32_t nraw = 0, uint32_t* ocnt = nullptr,
+                                            uint32_t* orun = nullptr)
 {
     __shared__ uint32_t sh[32 * kLdsPitch];
     __shared__ uint32_t sh_wcnt[4], sh_wovf[4], sh_wraw[4];
@@ -410,37 +414,25 @@ __device__ __forceinline__ void tile_finish(uint32_t w[4], uint32_t ovf, uint32_
         if (run_sum && total) atomicAdd(run_sum, total);
         const uint32_t o = sh_wovf[0] + sh_wovf[1] + sh_wovf[2] + sh_wovf[3];
         if (o) atomicAdd(ovf_out, (unsigned long long)o);
-        if (ocnt)
-            *reinterpret_cast<uint2*>(ocnt) = make_uint2(o, sh_wraw[0] + sh_wraw[1] + sh_wraw[2] + sh_wraw[3]);
+        if (ocnt) {
+            const uint32_t rw = sh_wraw[0] + sh_wraw[1] + sh_wraw[2] + sh_wraw[3];
+            *reinterpret_cast<uint2*>(ocnt) = make_uint2(o, rw);
+            if (o) atomicAdd(orun, o);  // the run's (overflow, verbatim) totals
+            if (rw) atomicAdd(orun + 1, rw);
+        }
     }
     // 4. in-tile compaction into the tile's staging slot
     if (f) *reinterpret_cast<uint4*>(slot + 16 * (woff + rank)) = blk;
 }
 
-// One full tile: dual-quant -> 32x32 bit transpose -> block flags -> in-tile compaction.
-//   bwords[8 tile .. 8 tile + 8)  the tile's bitflag words (fz.cu:95-99, LSB first)
-//   slot[0, 16 count)              its nonzero 16-byte blocks in order (fz.cu:56-57 order)
-//   *count_out                     count
-//   *run_sum                       += count (when run_sum is not null; relaxed atomic, read
-//                                  only after the kernel ends)
-// `R` holds the tile's raw loads when gp is 2 or 3 (raw_prefetch, issued by the caller before
-// its other latency-bound work).  PATHS (3D): 1 compiles only the paths of gp 2 / 3 tiles, 2
-// only the generic path with its own loads (gp 0 / 1); a kernel that branches between the two
-// at its top keeps the register count of the larger one instead of their union (69 against
-// 87 VGPRs: 7 waves per SIMD instead of 5).
+// Step 1 of a tile: dual-quant of the thread's 8 codes into its 4 words w (overflows added
+// to ovf), by the path the tile's geometry (gp) and the field's level allow.
 template <int NDIM, int PATHS = 0>
-__device__ __forceinline__ void tile_encode(const float* __restrict__ in, const Geom& g, uint32_t tile, int gp,
-                                            const RawTile<NDIM>& R, float r, int level,
-                                            uint32_t* __restrict__ bwords, uint8_t* __restrict__ slot,
-                                            uint32_t* __restrict__ count_out, unsigned long long* ovf_out,
-                                            uint32_t* run_sum = nullptr)
+__device__ __forceinline__ void tile_dualquant(const float* __restrict__ in, const Geom& g, uint32_t tile, int gp,
+                                               const RawTile<NDIM>& R, float r, int level, uint32_t w[4],
+                                               uint32_t& ovf)
 {
-    const uint32_t tid = threadIdx.x;
-    const uint32_t p0 = tile * (uint32_t)kTileCodes + 8 * tid;
-    uint32_t       ovf = 0;
-
-    // 1. dual-quant: words 4t..4t+3 of the tile
-    uint32_t w[4];
+    const uint32_t p0 = tile * (uint32_t)kTileCodes + 8 * threadIdx.x;
     if constexpr (NDIM == 1) {
         if (gp == 2 && level >= 3) pk_dispatch<1, true>(R, p0, g, r, level, w);
         else if (gp == 2 && level >= 1) raw_compute<1>(R, p0, g, r, w, ovf);
@@ -462,7 +454,30 @@ __device__ __forceinline__ void tile_encode(const float* __restrict__ in, const 
         else if (gp >= 2) generic3_raw(p0, g, r, R, w, ovf);
         else dq8_tile<3>(in, p0, g, r, level >= 1 ? gp : 0, w, ovf);  // PATHS == 0 only
     }
+}
 
+// One full tile: dual-quant -> 32x32 bit transpose -> block flags -> in-tile compaction.
+//   bwords[8 tile .. 8 tile + 8)  the tile's bitflag words (fz.cu:95-99, LSB first)
+//   slot[0, 16 count)              its nonzero 16-byte blocks in order (fz.cu:56-57 order)
+//   *count_out                     count
+//   *run_sum                       += count (when run_sum is not null; relaxed atomic, read
+//                                  only after the kernel ends)
+// `R` holds the tile's raw loads when gp is 2 or 3 (raw_prefetch, issued by the caller before
+// its other latency-bound work).  PATHS (3D): 1 compiles only the paths of gp 2 / 3 tiles, 2
+// only the generic path with its own loads (gp 0 / 1); a kernel that branches between the two
+// at its top keeps the register count of the larger one instead of their union (69 against
+// 87 VGPRs: 7 waves per SIMD instead of 5).
+template <int NDIM, int PATHS = 0>
+__device__ __forceinline__ void tile_encode(const float* __restrict__ in, const Geom& g, uint32_t tile, int gp,
+                                            const RawTile<NDIM>& R, float r, int level,
+                                            uint32_t* __restrict__ bwords, uint8_t* __restrict__ slot,
+                                            uint32_t* __restrict__ count_out, unsigned long long* ovf_out,
+                                            uint32_t* run_sum = nullptr)
+{
+    // 1. dual-quant: words 4t..4t+3 of the tile
+    uint32_t w[4];
+    uint32_t ovf = 0;
+    tile_dualquant<NDIM, PATHS>(in, g, tile, gp, R, r, level, w, ovf);
     tile_finish(w, ovf, tile, bwords, slot, count_out, ovf_out, run_sum);
 }
 
@@ -573,29 +588,40 @@ __device__ __forceinline__ uint32_t code_i(int32_t d)
     return a >= 32768u ? 0u : (a | (d < 0 ? 0x8000u : 0u));
 }
 
-// A full tile of the outlier mode's integer path: as tile_encode, plus the tile's
-// (overflows, verbatim values) counts at ocnt[0..1].
+// A full tile of the outlier mode: as tile_encode, plus the tile's (overflows, verbatim
+// values) counts at ocnt[0..1].  At level >= 0 the fixed mode's paths run first and the
+// integer path only redoes a tile in which some delta overflowed (uniform decision).
 template <int NDIM>
-__device__ __forceinline__ void tile_encode_o(const float* __restrict__ in, const Geom& g, uint32_t tile, float r,
+__device__ __forceinline__ void tile_encode_o(const float* __restrict__ in, const Geom& g, uint32_t tile, int gp,
+                                              const RawTile<NDIM>& R, float r, int level,
                                               uint32_t* __restrict__ bwords, uint8_t* __restrict__ slot,
                                               uint32_t* __restrict__ count_out, unsigned long long* ovf_out,
-                                              uint32_t* run_sum, uint32_t* __restrict__ ocnt)
+                                              uint32_t* run_sum, uint32_t* __restrict__ ocnt, uint32_t* orun)
 {
-    Int8 o;
-    dq8_int<NDIM>(in, tile * (uint32_t)kTileCodes + 8 * threadIdx.x, g, r, o);
     uint32_t w[4];
+    uint32_t ovf = 0, nraw = 0;
+    bool     exact = level == kLevelOutlier;
+    if (!exact) {
+        tile_dualquant<NDIM>(in, g, tile, gp, R, r, level, w, ovf);
+        exact = __syncthreads_or(ovf != 0) != 0;
+    }
+    if (exact) {
+        Int8 o;
+        dq8_int<NDIM>(in, tile * (uint32_t)kTileCodes + 8 * threadIdx.x, g, r, o);
 #pragma unroll
-    for (int k = 0; k < 4; k++) w[k] = code_i(o.d[2 * k]) | (code_i(o.d[2 * k + 1]) << 16);
-    tile_finish(w, (uint32_t)__popc(o.big), tile, bwords, slot, count_out, ovf_out, run_sum, (uint32_t)__popc(o.raw),
-                ocnt);
+        for (int k = 0; k < 4; k++) w[k] = code_i(o.d[2 * k]) | (code_i(o.d[2 * k + 1]) << 16);
+        ovf = (uint32_t)__popc(o.big);
+        nraw = (uint32_t)__popc(o.raw);
+    }
+    tile_finish(w, ovf, tile, bwords, slot, count_out, ovf_out, run_sum, nraw, ocnt, orun);
 }
 
 // The tail unit of the outlier mode: the tail codes, the partial tile's bitflag words
-// cleared, and the unit's counts at ocnt[0..1].
+// cleared, and the unit's counts at ocnt[0..1], added to its run's pair at orun.
 template <int NDIM>
 __device__ __forceinline__ void tail_unit_o(const float* __restrict__ in, const Geom& g, float r,
                                             uint32_t* __restrict__ bwords, uint16_t* __restrict__ tail,
-                                            unsigned long long* ovf_out, uint32_t* __restrict__ ocnt)
+                                            unsigned long long* ovf_out, uint32_t* __restrict__ ocnt, uint32_t* orun)
 {
     __shared__ uint32_t sh_b[4], sh_r[4];
     const uint32_t tid = threadIdx.x, wv = tid >> 6;
@@ -625,6 +651,8 @@ __device__ __forceinline__ void tail_unit_o(const float* __restrict__ in, const 
         const uint32_t b = sh_b[0] + sh_b[1] + sh_b[2] + sh_b[3], rr = sh_r[0] + sh_r[1] + sh_r[2] + sh_r[3];
         *reinterpret_cast<uint2*>(ocnt) = make_uint2(b, rr);
         if (b) atomicAdd(ovf_out, (unsigned long long)b);
+        if (b) atomicAdd(orun, b);
+        if (rr) atomicAdd(orun + 1, rr);
     }
 }
 

@@ -76,6 +76,16 @@ def test_outlier_mode_matches_oracle(dims, eb):
     _check(x, dims, eb)
 
 
+@pytest.mark.parametrize("dims", SHAPES)
+@pytest.mark.parametrize("eb", [1e-4, 1e-5])
+def test_finite_fields_with_spikes(dims, eb):
+    # all finite: the fixed mode's float paths run and only the tiles with a spike's
+    # overflowing delta are redone on the integer path
+    x = _field(dims, 3 * sum(dims), spikes=5)
+    r, ref, _ = _check(x, dims, eb)
+    assert ref["info"]["n_raw"] == 0
+
+
 @pytest.mark.parametrize("dims", [(5000, 1, 1), (100, 77, 1), (64, 64, 64)])
 def test_fields_needing_no_side_channel_take_the_fixed_payload(dims):
     x = _field(dims, 9)
