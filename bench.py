@@ -405,6 +405,36 @@ def pmc_target(D):
         del d
 
 
+def inverse_target(D, workload):
+    """Profiling workload for the inverse and the outlier mode: the calibration copy, then 5
+    decompress calls, 5 outlier-mode compress calls and 5 outlier-mode decompress calls on
+    one workload (inputs device-resident)."""
+    import torch
+
+    import fzgpu
+    a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    x, dims, eb = D.config_field(workload)
+    d = torch.from_numpy(x.ravel()).cuda()
+    plan = fzgpu.Plan(*dims)
+    payload, tail, r = plan.compress(d, eb)
+    out = torch.empty_like(d)
+    for _ in range(5):
+        plan.decompress(payload, tail, r["ebx2"], out=out)
+    oplan = fzgpu.Plan(*dims, mode=fzgpu.MODE_OUTLIER)
+    for _ in range(5):
+        op, ot, orr, side = oplan.compress_outlier(d, eb)
+    for _ in range(5):
+        oplan.decompress_outlier(op, ot, side, orr["ebx2"], orr["vmin"], out=out)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": workload, "n": int(x.size), "payload_bytes": r["payload_bytes"],
+                      "outlier_payload_bytes": orr["payload_bytes"], "side_bytes": int(side.numel())}), flush=True)
+
+
 def batch_target(D):
     """Counter / trace target for the batched kernels: 3 compress calls of the 256-field
     uneven batch at the default tuning (one group on one stream)."""
@@ -450,6 +480,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the request-serving leg")
     ap.add_argument("--serve-only", action="store_true", help="only the request-serving leg (prints its JSON)")
     ap.add_argument("--no-staged", action="store_true", help="skip the host-resident (PCIe-inclusive) leg")
+    ap.add_argument("--inverse-target", action="store_true",
+                    help="profiling workload: decompress and the outlier mode on --workload")
     ap.add_argument("--staged-only", action="store_true",
                     help="only the host-resident (PCIe-inclusive) leg: blocking pageable vs stager (prints its JSON)")
     ap.add_argument("--verify-only", action="store_true",
@@ -478,6 +510,9 @@ def main():
         return
     if args.verify_only:
         print(json.dumps(bench_verify(max(args.steps, 5))), flush=True)
+        return
+    if args.inverse_target:
+        inverse_target(D, args.workload)
         return
     if args.staged_only:
         x, dims, eb = D.config_field(args.workload)
