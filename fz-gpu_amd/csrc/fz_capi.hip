@@ -17,7 +17,8 @@ constexpr uint64_t kMaxLen = 0xFFFFF000ull;  // positions are 32-bit inside the 
 inline size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 struct Layout {
-    size_t ctr, part, tstate, sstate, dstate, codes, shuf, F, P, texcl, counts, stage, vtemp, ocnt, orun, dwide, total;
+    size_t ctr, part, tstate, sstate, dstate, obits, codes, shuf, F, P, texcl, counts, stage, vtemp, ocnt, orun, over,
+        total;
     uint32_t n_tstate, n_sstate, n_dstate;
 };
 
@@ -34,6 +35,9 @@ Layout make_layout(const Geom& g, int mode)
     L.tstate = o; o = al256(o + 8ull * L.n_tstate);
     L.sstate = o; o = al256(o + 8ull * L.n_sstate);
     L.dstate = o; o = al256(o + 8ull * L.n_dstate);
+    if (mode == FZ_MODE_OUTLIER) {  // inside the zero-filled prefix: no bit set between calls
+        L.obits = o; o = al256(o + 4ull * ((g.xyz + 31) / 32));
+    }
     L.codes = o;  o = al256(o + 2ull * g.len + 64);
     L.shuf = o;   o = al256(o + 2ull * g.len + 64);
     L.F = o;      o = al256(o + 4ull * N + 64);
@@ -45,7 +49,7 @@ Layout make_layout(const Geom& g, int mode)
     if (mode == FZ_MODE_OUTLIER) {  // per-unit side-channel counts and offsets; int32 deltas
         L.ocnt = o;   o = al256(o + 8ull * (g.T + 1));
         L.orun = o;   o = al256(o + 8ull * outlier_runs(g.T));
-        L.dwide = o;  o = al256(o + 4ull * g.xyz + 64);
+        L.over = o;   o = al256(o + 4ull * g.xyz + 64);
     }
     L.total = o;
     return L;
@@ -220,7 +224,7 @@ fz_status fz_workspace_init(const fz_plan plan, void* d_ws, size_t ws_bytes, hip
 {
     if (!plan || !d_ws || !aligned(d_ws, 256)) return FZ_ERR_INVALID_ARG;
     if (ws_bytes < plan->L.total) return FZ_ERR_WORKSPACE_TOO_SMALL;
-    // counters, partials and all look-back states sit before the code buffers
+    // counters, partials, all look-back states and the outlier bitmap sit before the code buffers
     return hip_st(launch_fill_u32(static_cast<uint32_t*>(d_ws), plan->L.codes / 4, 0u, stream));
 }
 
@@ -359,7 +363,8 @@ fz_status fz_decompress_outlier_f32(const fz_plan plan, const uint8_t* d_payload
     uint8_t*      ws = static_cast<uint8_t*>(d_ws);
     FZ_TRY(prof_mark(plan, stream, true));
     if (ebx2 == 0.0f) {  // constant field
-        FZ_TRY(launch_fill_outlier(d_side, side_bytes, d_out, g, vconst, stream));
+        FZ_TRY(launch_fill_outlier(d_side, side_bytes, reinterpret_cast<uint32_t*>(ws + L.obits), d_out, g, vconst,
+                                   stream));
         FZ_TRY(prof_mark(plan, stream, false));
         return FZ_SUCCESS;
     }
@@ -371,8 +376,9 @@ fz_status fz_decompress_outlier_f32(const fz_plan plan, const uint8_t* d_payload
     FZ_TRY(launch_tile_scan(reinterpret_cast<const uint32_t*>(d_payload), g.T, texcl, dstate, &ctr->dscan_ticket,
                             stream));
     FZ_TRY(launch_decode_unshuffle(d_payload, texcl, d_tail, g, codes, stream));
-    FZ_TRY(launch_reconstruct_outlier(reinterpret_cast<const uint16_t*>(codes), reinterpret_cast<int32_t*>(ws + L.dwide),
-                                      d_side, side_bytes, d_out, g, ebx2, stream));
+    FZ_TRY(launch_reconstruct_outlier(reinterpret_cast<const uint16_t*>(codes), reinterpret_cast<uint32_t*>(ws + L.obits),
+                                      reinterpret_cast<int32_t*>(ws + L.over), d_side, side_bytes, d_out, g, ebx2,
+                                      stream));
     FZ_TRY(prof_mark(plan, stream, false));
     return FZ_SUCCESS;
 }

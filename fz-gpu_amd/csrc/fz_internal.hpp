@@ -64,23 +64,26 @@ hipError_t launch_tile_scan(const uint32_t* bitflags, uint32_t T, uint32_t* tile
 hipError_t launch_decode_unshuffle(const uint8_t* payload, const uint32_t* tile_excl, const uint16_t* tail,
                                    const Geom& g, uint32_t* codes, hipStream_t s);
 hipError_t launch_reconstruct(const uint16_t* codes, float* out, const Geom& g, float ebx2, hipStream_t s);
-hipError_t launch_reconstruct_i32(const int32_t* deltas, float* out, const Geom& g, float ebx2, hipStream_t s);
+// reconstruct from u16 codes whose positions flagged in `bits` take their delta from `over`
+hipError_t launch_reconstruct_outlier_src(const uint16_t* codes, const uint32_t* bits, const int32_t* over,
+                                          float* out, const Geom& g, float ebx2, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 
 // Outlier mode (fz_outlier.hip).  Workspace pieces: ocnt holds 2 u32 per unit (T + 1 units),
-// orun 2 u32 per placement run; dwide holds x*y*z int32 deltas for the inverse.
+// orun 2 u32 per placement run; for the inverse, obits one bit per position (zero between
+// calls) and over the side channel's deltas at their positions.
 inline uint32_t outlier_runs(uint32_t T) { return T ? gather_grid(T) : 1u; }
 hipError_t launch_compress_outlier(const float* in, const Geom& g, double eb_rel, float4* part, fz_result* res,
                                    uint8_t* payload, uint16_t* tail, uint8_t* side, uint64_t side_cap,
                                    uint8_t* staging, uint32_t* counts, uint32_t* run_sums, uint32_t nsums,
                                    uint32_t* ocnt, uint32_t* orun, hipStream_t s);
-// after the codes of the plain inverse are in `codes`: deltas, side-channel deltas, prefix sums,
-// verbatim values
-hipError_t launch_reconstruct_outlier(const uint16_t* codes, int32_t* dwide, const uint8_t* side,
+// after the codes of the plain inverse are in `codes`: the side channel's deltas flagged in
+// obits, prefix sums, the verbatim values, obits cleared again
+hipError_t launch_reconstruct_outlier(const uint16_t* codes, uint32_t* obits, int32_t* over, const uint8_t* side,
                                       uint64_t side_bytes, float* out, const Geom& g, float ebx2, hipStream_t s);
 // constant field: out = vconst, then the verbatim values
-hipError_t launch_fill_outlier(const uint8_t* side, uint64_t side_bytes, float* out, const Geom& g, float vconst,
-                               hipStream_t s);
+hipError_t launch_fill_outlier(const uint8_t* side, uint64_t side_bytes, uint32_t* obits, float* out, const Geom& g,
+                               float vconst, hipStream_t s);
 
 // Quality metrics (fz_verify.hip): scratch bytes for n elements, and the two-kernel launch.
 size_t     verify_workspace_bytes(uint64_t n);

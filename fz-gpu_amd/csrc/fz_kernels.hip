@@ -631,39 +631,36 @@ hipError_t launch_decode_unshuffle(const uint8_t* payload, const uint32_t* tile_
     return hipGetLastError();
 }
 
-template <typename CT>
-__global__ __launch_bounds__(kWG) void k_reconstruct_1d(const CT* __restrict__ codes, float* __restrict__ out, Geom g,
-                                                        float ebx2)
+template <typename SRC>
+__global__ __launch_bounds__(kWG) void k_reconstruct_1d(const SRC codes, float* __restrict__ out, Geom g, float ebx2)
 {
     recon_1d(codes, out, g, ebx2, blockIdx.x);
 }
 
-template <typename CT>
-__global__ __launch_bounds__(kWG) void k_reconstruct_2d(const CT* __restrict__ codes, float* __restrict__ out, Geom g,
-                                                        float ebx2)
+template <typename SRC>
+__global__ __launch_bounds__(kWG) void k_reconstruct_2d(const SRC codes, float* __restrict__ out, Geom g, float ebx2)
 {
     recon_2d(codes, out, g, ebx2, blockIdx.x, blockIdx.y);
 }
 
-template <typename CT>
-__global__ __launch_bounds__(kWG) void k_reconstruct_3d(const CT* __restrict__ codes, float* __restrict__ out, Geom g,
-                                                        float ebx2)
+template <typename SRC>
+__global__ __launch_bounds__(kWG) void k_reconstruct_3d(const SRC codes, float* __restrict__ out, Geom g, float ebx2)
 {
     recon_3d(codes, out, g, ebx2, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
-template <typename CT>
-static hipError_t launch_reconstruct_t(const CT* codes, float* out, const Geom& g, float ebx2, hipStream_t s)
+template <typename SRC>
+static hipError_t launch_reconstruct_t(const SRC codes, float* out, const Geom& g, float ebx2, hipStream_t s)
 {
     if (g.ndim == 1) {
         const uint32_t thr = (g.xyz + 15) / 16;
-        FZ_LAUNCH(k_reconstruct_1d<CT><<<(thr + kWG - 1) / kWG, kWG, 0, s>>>(codes, out, g, ebx2));
+        FZ_LAUNCH(k_reconstruct_1d<SRC><<<(thr + kWG - 1) / kWG, kWG, 0, s>>>(codes, out, g, ebx2));
     } else if (g.ndim == 2) {
         const dim3 grid((g.X + kRecon2DCols - 1) / kRecon2DCols, (g.Y + 15) / 16);
-        FZ_LAUNCH(k_reconstruct_2d<CT><<<grid, kWG, 0, s>>>(codes, out, g, ebx2));
+        FZ_LAUNCH(k_reconstruct_2d<SRC><<<grid, kWG, 0, s>>>(codes, out, g, ebx2));
     } else {
         const dim3 grid((g.X + kRecon3DCols - 1) / kRecon3DCols, (g.Y + 7) / 8, (g.Z + 7) / 8);
-        FZ_LAUNCH(k_reconstruct_3d<CT><<<grid, kWG, 0, s>>>(codes, out, g, ebx2));
+        FZ_LAUNCH(k_reconstruct_3d<SRC><<<grid, kWG, 0, s>>>(codes, out, g, ebx2));
     }
     return hipGetLastError();
 }
@@ -673,9 +670,10 @@ hipError_t launch_reconstruct(const uint16_t* codes, float* out, const Geom& g, 
     return launch_reconstruct_t(codes, out, g, ebx2, s);
 }
 
-hipError_t launch_reconstruct_i32(const int32_t* deltas, float* out, const Geom& g, float ebx2, hipStream_t s)
+hipError_t launch_reconstruct_outlier_src(const uint16_t* codes, const uint32_t* bits, const int32_t* over,
+                                          float* out, const Geom& g, float ebx2, hipStream_t s)
 {
-    return launch_reconstruct_t(deltas, out, g, ebx2, s);
+    return launch_reconstruct_t(OutlierSrc{codes, bits, over}, out, g, ebx2, s);
 }
 
 }  // namespace fz

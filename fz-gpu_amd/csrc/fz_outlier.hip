@@ -22,8 +22,10 @@
 //                                runs before it, then its units with entries recompute their
 //                                deltas and write them; with no entry anywhere it only writes
 //                                the header
-// Decompress: the plain inverse's codes, then k_expand (codes -> int32 deltas), k_put_deltas,
-//             the reconstruct kernels on int32 deltas, k_put_raw.  A constant field is a fill.
+// Decompress: the plain inverse's codes, then k_put_deltas (the side channel's deltas at their
+//             positions, flagged in a bitmap), the reconstruct kernels reading codes and the
+//             flagged deltas, k_put_raw (the verbatim values; the bitmap cleared again).  A
+//             constant field is a fill.
 #include <hip/hip_runtime.h>
 
 #include "fz_internal.hpp"
@@ -237,38 +239,38 @@ __device__ __forceinline__ void side_counts(const uint8_t* side, uint64_t side_b
     nr = nr < avail - nd ? nr : avail - nd;
 }
 
-__global__ __launch_bounds__(kWG) void k_expand(const uint16_t* __restrict__ codes, int32_t* __restrict__ d, uint32_t n)
-{
-    const uint32_t i = 4 * (blockIdx.x * kWG + threadIdx.x);
-    if (i + 4 <= n) {
-        int32_t v[4];
-        delta4(codes, i, v);
-        *reinterpret_cast<int4*>(d + i) = make_int4(v[0], v[1], v[2], v[3]);
-    } else {
-        for (uint32_t j = i; j < n; j++) d[j] = decode(codes[j]);
-    }
-}
-
 __global__ __launch_bounds__(kWG) void k_put_deltas(const uint8_t* __restrict__ side, uint64_t side_bytes,
-                                                    int32_t* __restrict__ d, uint32_t n)
+                                                    uint32_t* __restrict__ bits, int32_t* __restrict__ over,
+                                                    uint32_t n)
 {
     uint64_t nd, nr;
     side_counts(side, side_bytes, nd, nr);
     const uint32_t* idx = reinterpret_cast<const uint32_t*>(side + 16);
     const uint32_t* val = idx + nd;
-    for (uint64_t k = (uint64_t)blockIdx.x * kWG + threadIdx.x; k < nd; k += (uint64_t)gridDim.x * kWG)
-        if (idx[k] < n) d[idx[k]] = (int32_t)val[k];
+    for (uint64_t k = (uint64_t)blockIdx.x * kWG + threadIdx.x; k < nd; k += (uint64_t)gridDim.x * kWG) {
+        const uint32_t p = idx[k];
+        if (p < n) {
+            over[p] = (int32_t)val[k];
+            atomicOr(&bits[p >> 5], 1u << (p & 31));
+        }
+    }
 }
 
+// the verbatim values, and the bitmap bits of the deltas cleared (the workspace's state
+// between calls)
 __global__ __launch_bounds__(kWG) void k_put_raw(const uint8_t* __restrict__ side, uint64_t side_bytes,
-                                                 float* __restrict__ out, uint32_t n)
+                                                 uint32_t* __restrict__ bits, float* __restrict__ out, uint32_t n)
 {
     uint64_t nd, nr;
     side_counts(side, side_bytes, nd, nr);
-    const uint32_t* idx = reinterpret_cast<const uint32_t*>(side + 16) + 2 * nd;
+    const uint32_t* idx_d = reinterpret_cast<const uint32_t*>(side + 16);
+    const uint32_t* idx = idx_d + 2 * nd;
     const uint32_t* val = idx + nr;
-    for (uint64_t k = (uint64_t)blockIdx.x * kWG + threadIdx.x; k < nr; k += (uint64_t)gridDim.x * kWG)
+    const uint64_t  step = (uint64_t)gridDim.x * kWG;
+    for (uint64_t k = (uint64_t)blockIdx.x * kWG + threadIdx.x; k < nr; k += step)
         if (idx[k] < n) out[idx[k]] = __uint_as_float(val[k]);
+    for (uint64_t k = (uint64_t)blockIdx.x * kWG + threadIdx.x; k < nd; k += step)
+        if (idx_d[k] < n) atomicAnd(&bits[idx_d[k] >> 5], ~(1u << (idx_d[k] & 31)));
 }
 
 __global__ __launch_bounds__(kWG) void k_fill_f32(float* __restrict__ out, uint32_t n, float v)
@@ -310,27 +312,25 @@ hipError_t launch_compress_outlier(const float* in, const Geom& g, double eb_rel
     return hipGetLastError();
 }
 
-hipError_t launch_reconstruct_outlier(const uint16_t* codes, int32_t* dwide, const uint8_t* side,
+hipError_t launch_reconstruct_outlier(const uint16_t* codes, uint32_t* obits, int32_t* over, const uint8_t* side,
                                       uint64_t side_bytes, float* out, const Geom& g, float ebx2, hipStream_t s)
 {
     (void)hipGetLastError();
-    const uint32_t thr = (g.xyz + 3) / 4;
-    k_expand<<<(thr + kWG - 1) / kWG, kWG, 0, s>>>(codes, dwide, g.xyz);
-    k_put_deltas<<<kScatterWG, kWG, 0, s>>>(side, side_bytes, dwide, g.xyz);
+    k_put_deltas<<<kScatterWG, kWG, 0, s>>>(side, side_bytes, obits, over, g.xyz);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    e = launch_reconstruct_i32(dwide, out, g, ebx2, s);
+    e = launch_reconstruct_outlier_src(codes, obits, over, out, g, ebx2, s);
     if (e != hipSuccess) return e;
-    k_put_raw<<<kScatterWG, kWG, 0, s>>>(side, side_bytes, out, g.xyz);
+    k_put_raw<<<kScatterWG, kWG, 0, s>>>(side, side_bytes, obits, out, g.xyz);
     return hipGetLastError();
 }
 
-hipError_t launch_fill_outlier(const uint8_t* side, uint64_t side_bytes, float* out, const Geom& g, float vconst,
-                               hipStream_t s)
+hipError_t launch_fill_outlier(const uint8_t* side, uint64_t side_bytes, uint32_t* obits, float* out, const Geom& g,
+                               float vconst, hipStream_t s)
 {
     (void)hipGetLastError();
     k_fill_f32<<<(g.xyz + kWG - 1) / kWG, kWG, 0, s>>>(out, g.xyz, vconst);
-    k_put_raw<<<kScatterWG, kWG, 0, s>>>(side, side_bytes, out, g.xyz);
+    k_put_raw<<<kScatterWG, kWG, 0, s>>>(side, side_bytes, obits, out, g.xyz);
     return hipGetLastError();
 }
 

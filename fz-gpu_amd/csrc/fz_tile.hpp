@@ -825,8 +825,8 @@ __device__ __forceinline__ int32_t decode(uint32_t c)
     return (c & 0x8000u) ? -m : m;
 }
 
-// Deltas of the reconstruct kernels from either source: u16 sign-magnitude codes, or int32
-// deltas (the outlier mode, whose side-channel deltas do not fit a code).  *4 reads positions
+// Deltas of the reconstruct kernels (the source SRC of recon_*): u16 sign-magnitude codes, or
+// int32 deltas, or the outlier mode's codes with side-channel overrides.  *4 reads positions
 // [p, p + 4) with one 8- or 16-byte load (p % 4 == 0), *16 reads [p, p + 16).
 __device__ __forceinline__ int32_t delta1(const uint16_t* __restrict__ c, size_t p) { return decode(c[p]); }
 __device__ __forceinline__ int32_t delta1(const int32_t* __restrict__ c, size_t p) { return c[p]; }
@@ -863,11 +863,43 @@ __device__ __forceinline__ void delta16(const int32_t* __restrict__ c, size_t p,
     for (int j = 0; j < 4; j++) delta4(c, p + 4 * j, d + 4 * j);
 }
 
+// The outlier mode's source: u16 codes, a bitmap of the positions whose delta comes from the
+// side channel instead (their code is 0), and those deltas at their positions.
+struct OutlierSrc {
+    const uint16_t* __restrict__ codes;
+    const uint32_t* __restrict__ bits;
+    const int32_t* __restrict__ over;
+};
+__device__ __forceinline__ int32_t delta1(const OutlierSrc& s, size_t p)
+{
+    return ((s.bits[p >> 5] >> (p & 31)) & 1u) ? s.over[p] : decode(s.codes[p]);
+}
+__device__ __forceinline__ void delta4(const OutlierSrc& s, size_t p, int32_t d[4])
+{
+    delta4(s.codes, p, d);
+    const uint32_t m = (s.bits[p >> 5] >> (p & 31)) & 0xFu;  // p % 4 == 0: one word
+    if (m) {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if ((m >> j) & 1u) d[j] = s.over[p + j];
+    }
+}
+__device__ __forceinline__ void delta16(const OutlierSrc& s, size_t p, int32_t d[16])
+{
+    delta16(s.codes, p, d);
+    const uint32_t m = (s.bits[p >> 5] >> (p & 31)) & 0xFFFFu;  // p % 16 == 0: one word
+    if (m) {
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+            if ((m >> j) & 1u) d[j] = s.over[p + j];
+    }
+}
+
 // 1D: workgroup wg covers codes [4096 wg, 4096 wg + 4096): 16 consecutive codes per thread,
 // 16 threads per 256-block (block-local inclusive prefix sum), x = (float)s * ebx2.
-template <typename CT>
-__device__ __forceinline__ void recon_1d(const CT* __restrict__ codes, float* __restrict__ out, const Geom& g,
-                                         float ebx2, uint32_t wg)
+template <typename SRC>
+__device__ __forceinline__ void recon_1d(const SRC codes, float* __restrict__ out, const Geom& g, float ebx2,
+                                         uint32_t wg)
 {
     const uint32_t p0 = (wg * kWG + threadIdx.x) * 16;
     int32_t        s[16];
@@ -914,9 +946,9 @@ __device__ __forceinline__ int32_t quad_incl_scan(int32_t v)
     return v + (q >= 2 ? b : 0);
 }
 
-template <typename CT>
-__device__ __forceinline__ void recon_2d(const CT* __restrict__ codes, float* __restrict__ out, const Geom& g,
-                                         float ebx2, uint32_t bx, uint32_t by)
+template <typename SRC>
+__device__ __forceinline__ void recon_2d(const SRC codes, float* __restrict__ out, const Geom& g, float ebx2,
+                                         uint32_t bx, uint32_t by)
 {
     const uint32_t x0 = bx * kRecon2DCols + 4 * threadIdx.x, y0 = by * 16;
     const bool     full = x0 + 4 <= g.X;
@@ -960,9 +992,9 @@ __device__ __forceinline__ void recon_2d(const CT* __restrict__ codes, float* __
 // and with the other half of its 8-wide Lorenzo block (the neighbour lane, one DPP step); an
 // LDS transpose (16-byte accesses) then gives thread (xq, tz) the 8 rows of its z slice for the
 // prefix over y.  Loads are 8 bytes and stores 16 bytes per lane when X % 4 == 0.
-template <typename CT>
-__device__ __forceinline__ void recon_3d(const CT* __restrict__ codes, float* __restrict__ out, const Geom& g,
-                                         float ebx2, uint32_t bx, uint32_t by, uint32_t bz)
+template <typename SRC>
+__device__ __forceinline__ void recon_3d(const SRC codes, float* __restrict__ out, const Geom& g, float ebx2,
+                                         uint32_t bx, uint32_t by, uint32_t bz)
 {
     __shared__ int4 sh[8][8][32];  // [z][y][x quad]
     const uint32_t xq = threadIdx.x & 31, ty = threadIdx.x >> 5;
