@@ -156,3 +156,70 @@ def test_decompress_verify_matches_oracle(dims, eb):
     # the same numbers as a separate verify call, and the compressor's error bound holds
     assert g == fzgpu.verify(out, d_o)
     assert g["max_err"]["abs"] <= r["eb_abs"] * (1 + 1e-6) + 4 * 2.0 ** -24 * float(np.abs(o).max())
+
+
+def test_single_field_entry_points_replay_in_a_graph():
+    """fz_compress_f32, fz_decompress_verify_f32 and the outlier pair launch kernels only, so
+    they capture into a graph; every replay (with eager work and syncs between replays) gives
+    the eager bytes and numbers."""
+    import ctypes
+
+    import torch
+    dims = (3600, 180, 1)
+    n = dims[0] * dims[1]
+    say("case graph capture of the single-field entry points")
+    x, o = _pair(n, 21)
+    o[::997] = np.nan  # the outlier plan keeps these verbatim
+    d_o = _to_dev(np.nan_to_num(o, nan=0.0))
+    d_on = _to_dev(o)
+    plan = fzgpu.Plan(*dims)
+    payload, tail, res = plan.alloc_outputs()
+    out = torch.empty(n, dtype=torch.float32, device="cuda")
+    st = torch.zeros(160, dtype=torch.uint8, device="cuda")
+    oplan = fzgpu.Plan(*dims, mode=fzgpu.MODE_OUTLIER)
+    opay, otail, ores = oplan.alloc_outputs()
+    cap = oplan.sizes.side_max_bytes
+    side = torch.zeros(cap, dtype=torch.uint8, device="cuda")
+    oout = torch.empty(n, dtype=torch.float32, device="cuda")
+    L = fzgpu.lib()
+    P = fzgpu._ptr
+
+    # eager pass fixes ebx2 / vmin / side size, the values the captured calls are given
+    plan.compress_async(d_o, 1e-3, payload, tail, res)
+    r = fzgpu.read_result(res)
+    L.fz_compress_outlier_f32(oplan.handle, P(d_on), 1e-3, P(oplan.ws), oplan.ws.numel(), P(opay), P(otail), P(side),
+                              cap, P(ores), fzgpu._stream())
+    orr = fzgpu.read_result(ores)
+    nd, nr = (int(v) for v in side[:16].cpu().view(torch.int64).tolist())
+    sbytes = 16 + 8 * (nd + nr)
+    assert nr == int(np.isnan(o).sum())
+
+    def calls():
+        plan.compress_async(d_o, 1e-3, payload, tail, res)
+        plan.decompress_verify_async(payload, tail, r["ebx2"], d_o, out, st)
+        fzgpu._check(L.fz_compress_outlier_f32(oplan.handle, P(d_on), 1e-3, P(oplan.ws), oplan.ws.numel(), P(opay),
+                                               P(otail), P(side), cap, P(ores), fzgpu._stream()), "outlier")
+        fzgpu._check(L.fz_decompress_outlier_f32(oplan.handle, P(opay), P(otail), float(orr["ebx2"]),
+                                                 float(orr["vmin"]), P(side), sbytes, P(oplan.ws), oplan.ws.numel(),
+                                                 P(oout), fzgpu._stream()), "outlier inverse")
+
+    calls()
+    torch.cuda.synchronize()
+    ref = [t.clone() for t in (payload, tail, res, out, st, opay, side[:sbytes], oout)]
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        calls()
+    torch.cuda.current_stream().wait_stream(s)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        calls()
+    for rep in range(4):
+        for t in (payload, out, st, opay, side, oout):
+            t.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        got = (payload, tail, res, out, st, opay, side[:sbytes], oout)
+        for a, b in zip(got, ref):
+            assert torch.equal(a.view(torch.uint8), b.view(torch.uint8)), rep
