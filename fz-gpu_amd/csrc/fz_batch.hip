@@ -388,9 +388,9 @@ __global__ __launch_bounds__(kBScanWG) void k_batch_dscan(BatchDev d)
     }
 }
 
-// One workgroup per unit: the tile's codes (or the tail codes) into the unit's 4096-byte slot;
+// One wave per unit: the tile's codes (or the tail codes) into the unit's 4096-byte slot;
 // a field's slots are consecutive, so its codes form one array.
-__global__ __launch_bounds__(kWG) void k_batch_decode(BatchDev d, uint32_t nf)
+__global__ __launch_bounds__(64) void k_batch_decode(BatchDev d, uint32_t nf)
 {
     const uint32_t gu = blockIdx.x;
     const uint32_t f = d.ufield[gu];
@@ -401,8 +401,8 @@ __global__ __launch_bounds__(kWG) void k_batch_decode(BatchDev d, uint32_t nf)
         decode_tail(g.mode == 0 ? q.tail : nullptr, g, reinterpret_cast<uint16_t*>(d.staging + (size_t)u0 * 4096));
         return;
     }
-    decode_tile(static_cast<const uint8_t*>(q.src), g.W, tile, d.texcl[gu],
-                reinterpret_cast<uint32_t*>(d.staging + (size_t)gu * 4096));
+    decode_tile_w(static_cast<const uint8_t*>(q.src), g.W, tile, d.texcl[gu],
+                  reinterpret_cast<uint32_t*>(d.staging + (size_t)gu * 4096));
 }
 
 template <int NDIM>
@@ -432,7 +432,7 @@ __global__ __launch_bounds__(kWG) void k_batch_recon(BatchDev d, uint32_t r_lo, 
 hipError_t launch_batch_decompress(const BatchDev& d, const BatchShape& b, hipEvent_t* ev, hipStream_t s)
 {
     hipError_t e = launch_pair(k_batch_dscan, dim3(b.nf), dim3(kBScanWG), s, ev, 0, d);
-    if (e == hipSuccess) e = launch_pair(k_batch_decode, dim3(b.U), dim3(kWG), s, ev, 1, d, b.nf);
+    if (e == hipSuccess) e = launch_pair(k_batch_decode, dim3(b.U), dim3(64), s, ev, 1, d, b.nf);
     for (int nd = 1; nd <= 3 && e == hipSuccess; nd++) {
         const uint32_t n = b.r_hi[nd] - b.r_lo[nd];
         if (n == 0) {

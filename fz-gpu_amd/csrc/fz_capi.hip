@@ -316,10 +316,8 @@ fz_status fz_decompress_f32(const fz_plan plan, const uint8_t* d_payload, const 
     uint32_t*     codes = reinterpret_cast<uint32_t*>(ws + L.codes);
     const uint16_t* tail = plan->mode != FZ_MODE_LITERAL ? d_tail : nullptr;
     FZ_TRY(prof_mark(plan, stream, true));
-    FZ_TRY(launch_scan_init(dstate, L.n_dstate, &ctr->dscan_ticket, stream));
-    FZ_TRY(prof_mark(plan, stream, false));
-    FZ_TRY(launch_tile_scan(reinterpret_cast<const uint32_t*>(d_payload), g.T, texcl, dstate, &ctr->dscan_ticket,
-                            stream));
+    FZ_TRY(launch_tile_scan(reinterpret_cast<const uint32_t*>(d_payload), g.T, texcl, dstate, L.n_dstate,
+                            &ctr->dscan_ticket, stream));
     FZ_TRY(prof_mark(plan, stream, false));
     FZ_TRY(launch_decode_unshuffle(d_payload, texcl, tail, g, codes, stream));
     FZ_TRY(prof_mark(plan, stream, false));
@@ -372,9 +370,8 @@ fz_status fz_decompress_outlier_f32(const fz_plan plan, const uint8_t* d_payload
     uint64_t* dstate = reinterpret_cast<uint64_t*>(ws + L.dstate);
     uint32_t* texcl = reinterpret_cast<uint32_t*>(ws + L.texcl);
     uint32_t* codes = reinterpret_cast<uint32_t*>(ws + L.codes);
-    FZ_TRY(launch_scan_init(dstate, L.n_dstate, &ctr->dscan_ticket, stream));
-    FZ_TRY(launch_tile_scan(reinterpret_cast<const uint32_t*>(d_payload), g.T, texcl, dstate, &ctr->dscan_ticket,
-                            stream));
+    FZ_TRY(launch_tile_scan(reinterpret_cast<const uint32_t*>(d_payload), g.T, texcl, dstate, L.n_dstate,
+                            &ctr->dscan_ticket, stream));
     FZ_TRY(launch_decode_unshuffle(d_payload, texcl, d_tail, g, codes, stream));
     FZ_TRY(launch_reconstruct_outlier(reinterpret_cast<const uint16_t*>(codes), reinterpret_cast<uint32_t*>(ws + L.obits),
                                       reinterpret_cast<int32_t*>(ws + L.over), d_side, side_bytes, d_out, g, ebx2,

@@ -605,29 +605,32 @@ __global__ __launch_bounds__(kWG) void k_tile_scan(const uint32_t* __restrict__ 
 }
 
 hipError_t launch_tile_scan(const uint32_t* bitflags, uint32_t T, uint32_t* tile_excl, uint64_t* states,
-                            uint32_t* ticket, hipStream_t s)
+                            uint32_t nstates, uint32_t* ticket, hipStream_t s)
 {
     if (T == 0) return hipSuccess;
+    const hipError_t e = launch_scan_init(states, nstates, ticket, s);
+    if (e != hipSuccess) return e;
     FZ_LAUNCH(k_tile_scan<<<(T + kWG - 1) / kWG, kWG, 0, s>>>(bitflags, T, tile_excl, states, ticket));
     return hipGetLastError();
 }
 
 // Inverse of the tile compress: the extra workgroup writes the tail codes (or zeros when
-// there is no tail buffer).
-__global__ __launch_bounds__(kWG) void k_decode_unshuffle(const uint8_t* __restrict__ payload,
-                                                          const uint32_t* __restrict__ tile_excl,
-                                                          const uint16_t* __restrict__ tail, Geom g,
-                                                          uint32_t* __restrict__ codes)
+// there is no tail buffer); one wave per tile, so a CU holds four times as many tiles in
+// flight as with a workgroup per tile, and all-zero tiles only store zeros.
+__global__ __launch_bounds__(64) void k_decode_unshuffle(const uint8_t* __restrict__ payload,
+                                                         const uint32_t* __restrict__ tile_excl,
+                                                         const uint16_t* __restrict__ tail, Geom g,
+                                                         uint32_t* __restrict__ codes)
 {
     const uint32_t tile = blockIdx.x;
     if (tile >= g.T) decode_tail(tail, g, reinterpret_cast<uint16_t*>(codes));
-    else decode_tile(payload, g.W, tile, tile_excl[tile], codes + (size_t)tile * kTileWords);
+    else decode_tile_w(payload, g.W, tile, tile_excl[tile], codes + (size_t)tile * kTileWords);
 }
 
 hipError_t launch_decode_unshuffle(const uint8_t* payload, const uint32_t* tile_excl, const uint16_t* tail,
                                    const Geom& g, uint32_t* codes, hipStream_t s)
 {
-    FZ_LAUNCH(k_decode_unshuffle<<<g.T + 1, kWG, 0, s>>>(payload, tile_excl, tail, g, codes));
+    FZ_LAUNCH(k_decode_unshuffle<<<g.T + 1, 64, 0, s>>>(payload, tile_excl, tail, g, codes));
     return hipGetLastError();
 }
 

@@ -780,35 +780,54 @@ __device__ __forceinline__ void tail_unit_w(const float* __restrict__ in, const 
 
 // ---------------------------------------------------------------------------- decompress tiles
 
-// Scatter a tile's nonzero blocks back (block i at payload + 4W + 16 (excl + rank)),
-// unshuffle, store its 1024 code words to out_words.
-__device__ __forceinline__ void decode_tile(const uint8_t* __restrict__ payload, uint32_t W, uint32_t tile,
-                                            uint32_t excl, uint32_t* __restrict__ out_words)
+// Scatter a tile's nonzero blocks back (block i at payload + 4W + 16 (excl + rank)), unshuffle,
+// store its 1024 code words to out_words; one wave (64 threads).  Lane l owns blocks 4l .. 4l + 3
+// (flag bits 4(l % 8) .. + 3 of word l / 8), puts them in LDS as rows of bit planes, then
+// unshuffles the words of threads l + 64v, v < 4, of a 256-thread layout (8 lanes per 32×32 bit
+// transpose).  A tile with no nonzero block stores zeros without touching LDS (wave-uniform).
+__device__ __forceinline__ void decode_tile_w(const uint8_t* __restrict__ payload, uint32_t W, uint32_t tile,
+                                              uint32_t excl, uint32_t* __restrict__ out_words)
 {
     __shared__ uint32_t sh[32 * kLdsPitch];
-    __shared__ uint32_t sh_w[4];
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const uint32_t* bwords = reinterpret_cast<const uint32_t*>(payload);
-    const uint32_t  word = bwords[8 * tile + (tid >> 5)];
-    const bool      f = (word >> (tid & 31)) & 1u;
-    const uint64_t  mask = ballot64(f);
-    if (lane == 0) sh_w[wv] = (uint32_t)__popcll(mask);
-    __syncthreads();
-    const uint32_t woff = (wv > 0 ? sh_w[0] : 0u) + (wv > 1 ? sh_w[1] : 0u) + (wv > 2 ? sh_w[2] : 0u);
-    uint4          blk = make_uint4(0, 0, 0, 0);
-    if (f) blk = ld16(payload + 4ull * W + 16ull * (excl + woff + rank_below(mask)));
-    const uint32_t c = tid >> 3, k0 = 4 * (tid & 7);
-    sh[kLdsPitch * c + k0] = blk.x;
-    sh[kLdsPitch * c + k0 + 1] = blk.y;
-    sh[kLdsPitch * c + k0 + 2] = blk.z;
-    sh[kLdsPitch * c + k0 + 3] = blk.w;
-    __syncthreads();
-    const uint32_t L = tid & 7, k = tid >> 3;
-    uint32_t       w[4];
+    const uint32_t l = threadIdx.x;
+    const uint32_t word = reinterpret_cast<const uint32_t*>(payload)[8 * tile + (l >> 3)];
+    const uint32_t nib = (word >> (4 * (l & 7))) & 0xFu;
+    uint4* const   out4 = reinterpret_cast<uint4*>(out_words);
+    if (ballot64(nib != 0) == 0) {
 #pragma unroll
-    for (int q = 0; q < 4; q++) w[q] = sh[kLdsPitch * (4 * L + q) + k];
-    transpose4(w, L);
-    *reinterpret_cast<uint4*>(out_words + 4 * tid) = make_uint4(w[0], w[1], w[2], w[3]);
+        for (int v = 0; v < 4; v++) out4[l + 64 * v] = make_uint4(0, 0, 0, 0);
+        return;
+    }
+    const uint32_t cnt = __popc(nib);
+    uint32_t       idx = excl + wave_incl_scan(cnt) - cnt;  // rank of block 4l among the tile's nonzero blocks
+    const uint8_t* blocks = payload + 4ull * W;
+    uint4          blk[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const bool f = (nib >> i) & 1u;
+        blk[i] = f ? ld16(blocks + 16ull * idx) : make_uint4(0, 0, 0, 0);
+        idx += f ? 1u : 0u;
+    }
+    // block b = 4l + i holds words 4 (b % 8) .. + 3 of bit plane b / 8 = l / 2
+    uint32_t* row = sh + kLdsPitch * (l >> 1) + 16 * (l & 1);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        row[4 * i] = blk[i].x;
+        row[4 * i + 1] = blk[i].y;
+        row[4 * i + 2] = blk[i].z;
+        row[4 * i + 3] = blk[i].w;
+    }
+    __syncthreads();
+    const uint32_t L = l & 7;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const uint32_t k = (l >> 3) + 8 * v;
+        uint32_t       w[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) w[q] = sh[kLdsPitch * (4 * L + q) + k];
+        transpose4(w, L);
+        out4[l + 64 * v] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
 }
 
 // Codes of the positions after the last full tile: the trailer, or 0 without one.
@@ -816,7 +835,7 @@ __device__ __forceinline__ void decode_tail(const uint16_t* __restrict__ tail, c
                                             uint16_t* __restrict__ codes)
 {
     const uint32_t t0 = kTileCodes * g.T;
-    for (uint32_t i = threadIdx.x; t0 + i < g.len; i += kWG) codes[t0 + i] = tail ? tail[i] : (uint16_t)0;
+    for (uint32_t i = threadIdx.x; t0 + i < g.len; i += blockDim.x) codes[t0 + i] = tail ? tail[i] : (uint16_t)0;
 }
 
 __device__ __forceinline__ int32_t decode(uint32_t c)
