@@ -370,12 +370,14 @@ fz_status fz_decompress_outlier_f32(const fz_plan plan, const uint8_t* d_payload
     uint64_t* dstate = reinterpret_cast<uint64_t*>(ws + L.dstate);
     uint32_t* texcl = reinterpret_cast<uint32_t*>(ws + L.texcl);
     uint32_t* codes = reinterpret_cast<uint32_t*>(ws + L.codes);
+    uint32_t* obits = reinterpret_cast<uint32_t*>(ws + L.obits);
+    int32_t*  over = reinterpret_cast<int32_t*>(ws + L.over);
+    FZ_TRY(launch_put_deltas(d_side, side_bytes, obits, over, g, dstate, L.n_dstate, &ctr->dscan_ticket, stream));
     FZ_TRY(launch_tile_scan(reinterpret_cast<const uint32_t*>(d_payload), g.T, texcl, dstate, L.n_dstate,
-                            &ctr->dscan_ticket, stream));
+                            &ctr->dscan_ticket, stream, false));
     FZ_TRY(launch_decode_unshuffle(d_payload, texcl, d_tail, g, codes, stream));
-    FZ_TRY(launch_reconstruct_outlier(reinterpret_cast<const uint16_t*>(codes), reinterpret_cast<uint32_t*>(ws + L.obits),
-                                      reinterpret_cast<int32_t*>(ws + L.over), d_side, side_bytes, d_out, g, ebx2,
-                                      stream));
+    FZ_TRY(launch_reconstruct_outlier(reinterpret_cast<const uint16_t*>(codes), obits, over, d_side, side_bytes, d_out,
+                                      g, ebx2, stream));
     FZ_TRY(prof_mark(plan, stream, false));
     return FZ_SUCCESS;
 }

@@ -59,10 +59,10 @@ hipError_t launch_scan_init(uint64_t* states, uint32_t nstates, uint32_t* ticket
 hipError_t launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* states, uint32_t* ticket,
                        bool write_total, fz_result* res, uint32_t W, hipStream_t s);
 hipError_t launch_encode(const uint8_t* in, uint8_t* out, const uint32_t* presum, uint32_t nblocks, hipStream_t s);
-// tile_excl[t] = nonzero blocks of tiles < t (look-back scan; clears its nstates states and the
-// ticket first)
+// tile_excl[t] = nonzero blocks of tiles < t (look-back scan; with `clear`, a launch first clears
+// its nstates states and the ticket, else the caller has)
 hipError_t launch_tile_scan(const uint32_t* bitflags, uint32_t T, uint32_t* tile_excl, uint64_t* states,
-                            uint32_t nstates, uint32_t* ticket, hipStream_t s);
+                            uint32_t nstates, uint32_t* ticket, hipStream_t s, bool clear = true);
 hipError_t launch_decode_unshuffle(const uint8_t* payload, const uint32_t* tile_excl, const uint16_t* tail,
                                    const Geom& g, uint32_t* codes, hipStream_t s);
 hipError_t launch_reconstruct(const uint16_t* codes, float* out, const Geom& g, float ebx2, hipStream_t s);
@@ -79,8 +79,12 @@ hipError_t launch_compress_outlier(const float* in, const Geom& g, double eb_rel
                                    uint8_t* payload, uint16_t* tail, uint8_t* side, uint64_t side_cap,
                                    uint8_t* staging, uint32_t* counts, uint32_t* run_sums, uint32_t nsums,
                                    uint32_t* ocnt, uint32_t* orun, hipStream_t s);
-// after the codes of the plain inverse are in `codes`: the side channel's deltas flagged in
-// obits, prefix sums, the verbatim values, obits cleared again
+// before the tile scan (launched with clear = false): the side channel's deltas into `over`,
+// flagged in obits, and the scan's look-back states and ticket cleared
+hipError_t launch_put_deltas(const uint8_t* side, uint64_t side_bytes, uint32_t* obits, int32_t* over, const Geom& g,
+                             uint64_t* states, uint32_t nstates, uint32_t* ticket, hipStream_t s);
+// after the codes of the plain inverse are in `codes`: prefix sums with the deltas flagged in
+// obits, the verbatim values, obits cleared again
 hipError_t launch_reconstruct_outlier(const uint16_t* codes, uint32_t* obits, int32_t* over, const uint8_t* side,
                                       uint64_t side_bytes, float* out, const Geom& g, float ebx2, hipStream_t s);
 // constant field: out = vconst, then the verbatim values

@@ -605,10 +605,10 @@ __global__ __launch_bounds__(kWG) void k_tile_scan(const uint32_t* __restrict__ 
 }
 
 hipError_t launch_tile_scan(const uint32_t* bitflags, uint32_t T, uint32_t* tile_excl, uint64_t* states,
-                            uint32_t nstates, uint32_t* ticket, hipStream_t s)
+                            uint32_t nstates, uint32_t* ticket, hipStream_t s, bool clear)
 {
     if (T == 0) return hipSuccess;
-    const hipError_t e = launch_scan_init(states, nstates, ticket, s);
+    const hipError_t e = clear ? launch_scan_init(states, nstates, ticket, s) : hipSuccess;
     if (e != hipSuccess) return e;
     FZ_LAUNCH(k_tile_scan<<<(T + kWG - 1) / kWG, kWG, 0, s>>>(bitflags, T, tile_excl, states, ticket));
     return hipGetLastError();

@@ -239,15 +239,22 @@ __device__ __forceinline__ void side_counts(const uint8_t* side, uint64_t side_b
     nr = nr < avail - nd ? nr : avail - nd;
 }
 
+// The side channel's deltas into `over` with their bitmap bits set, in the same launch as the
+// clearing of the tile scan's look-back states and ticket (both must precede the scan and the
+// reconstruct, and neither depends on the other).
 __global__ __launch_bounds__(kWG) void k_put_deltas(const uint8_t* __restrict__ side, uint64_t side_bytes,
                                                     uint32_t* __restrict__ bits, int32_t* __restrict__ over,
-                                                    uint32_t n)
+                                                    uint32_t n, uint64_t* __restrict__ states, uint32_t nstates,
+                                                    uint32_t* __restrict__ ticket)
 {
+    const uint32_t gtid = blockIdx.x * kWG + threadIdx.x;
+    for (uint32_t i = gtid; i < nstates; i += gridDim.x * kWG) states[i] = 0;
+    if (gtid == 0) *ticket = 0;
     uint64_t nd, nr;
     side_counts(side, side_bytes, nd, nr);
     const uint32_t* idx = reinterpret_cast<const uint32_t*>(side + 16);
     const uint32_t* val = idx + nd;
-    for (uint64_t k = (uint64_t)blockIdx.x * kWG + threadIdx.x; k < nd; k += (uint64_t)gridDim.x * kWG) {
+    for (uint64_t k = gtid; k < nd; k += (uint64_t)gridDim.x * kWG) {
         const uint32_t p = idx[k];
         if (p < n) {
             over[p] = (int32_t)val[k];
@@ -312,15 +319,19 @@ hipError_t launch_compress_outlier(const float* in, const Geom& g, double eb_rel
     return hipGetLastError();
 }
 
+hipError_t launch_put_deltas(const uint8_t* side, uint64_t side_bytes, uint32_t* obits, int32_t* over, const Geom& g,
+                             uint64_t* states, uint32_t nstates, uint32_t* ticket, hipStream_t s)
+{
+    (void)hipGetLastError();
+    k_put_deltas<<<kScatterWG, kWG, 0, s>>>(side, side_bytes, obits, over, g.xyz, states, nstates, ticket);
+    return hipGetLastError();
+}
+
 hipError_t launch_reconstruct_outlier(const uint16_t* codes, uint32_t* obits, int32_t* over, const uint8_t* side,
                                       uint64_t side_bytes, float* out, const Geom& g, float ebx2, hipStream_t s)
 {
-    (void)hipGetLastError();
-    k_put_deltas<<<kScatterWG, kWG, 0, s>>>(side, side_bytes, obits, over, g.xyz);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    e = launch_reconstruct_outlier_src(codes, obits, over, out, g, ebx2, s);
-    if (e != hipSuccess) return e;
+    hipError_t e = launch_reconstruct_outlier_src(codes, obits, over, out, g, ebx2, s);
+    if (e != hipSuccess || side_bytes < 24) return e;  // a channel of 16 header bytes has no entry
     k_put_raw<<<kScatterWG, kWG, 0, s>>>(side, side_bytes, obits, out, g.xyz);
     return hipGetLastError();
 }
