@@ -190,6 +190,20 @@ def bench_batch(steps, warmup, nfields=256):
     ms_fused_host, _ = time_steps(lambda k: per_field(True), pf_steps)
     g_fused = _graph_of(lambda: per_field(True))
     ms_fused_graph, _ = time_steps(lambda k: g_fused.replay(), pf_steps)
+    # batched decompress of the same archives (eager launches), every 16th field's floats
+    # compared with its single-field decompress
+    dec_out = [torch.empty(max(t.numel(), 4), dtype=torch.float32, device="cuda") for t in d]
+    ebx2 = [x["ebx2"] for x in res]
+    plan.decompress_async(out.payloads, out.tails, ebx2, dec_out)
+    ms_dec, _ = time_steps(lambda k: plan.decompress_async(out.payloads, out.tails, ebx2, dec_out),
+                           max(3, steps // 2))
+    dec_mismatch = 0
+    for i in range(0, nfields, 16):
+        r_i = res[i]
+        one = plans[i].decompress(out.payloads[i][: r_i["payload_bytes"]], out.tails[i][: r_i["tail_len"]],
+                                  r_i["ebx2"])
+        dec_mismatch += 0 if torch.equal(one.view(torch.int32), dec_out[i][: one.numel()].view(torch.int32)) else 1
+    del dec_out
     gbps = lambda ms: round(in_bytes / (ms * 1e-3) / 1e9, 2)  # noqa: E731
     names = ["range", "range_final", "tile_1d", "tile_2d", "tile_3d", "scan", "gather"]
     r = {
@@ -203,6 +217,8 @@ def bench_batch(steps, warmup, nfields=256):
         "speedup_vs_fused_graph": round(ms_fused_graph / ms_batch, 3),
         "kernel_ms": {k: round(v, 5) for k, v in zip(names, kt)},
         "tuning": tune, "groups_sweep": sweep, "bytes_mismatched_fields": mismatch,
+        "batched_decompress_ms": round(ms_dec, 4), "batched_decompress_GBps": gbps(ms_dec),
+        "decompress_mismatched_fields": dec_mismatch,
     }
     del g, g_plain, g_fused, plans, outs, out, plan, d
     torch.cuda.empty_cache()
