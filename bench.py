@@ -238,22 +238,26 @@ def bench_serving(rates=(50, 100, 200, 500, 1000, 2000), duration_s=1.0, pool_fi
     log(f"  pool of {pool_fields} fields ({sum(4 * x.numel() for _, x in pool) / 1e9:.2f} GB) in "
         f"{time.perf_counter() - t0:.1f} s")
     out = {"pool_GB": round(sum(4 * x.numel() for _, x in pool) / 1e9, 3), "duration_s": duration_s}
-    # warm-up: every pool field compressed and decompressed alone, so that the caching
-    # allocator holds blocks of every request size before the timed streams
-    warm = S.Server(S.DeviceBackend(), hbm_budget_bytes=200e9, slots=3, max_fields=1, max_archives=len(pool))
+    # one long-lived device backend (its plan pool holds up to 16 GB of workspaces, as a
+    # caching allocator would); warm-up: every pool field compressed and decompressed alone,
+    # so that the caching allocator and the plan pool hold every request size before the
+    # timed streams
+    be = S.DeviceBackend(plan_pool_bytes=16 << 30)
+    warm = S.Server(be, hbm_budget_bytes=200e9, slots=3, max_fields=1, max_archives=len(pool))
     warm.run([S.Request(i, "compress", d, 1e-3, x, 0.0) for i, (d, x) in enumerate(pool)], realtime=False)
     arch = [r.result for r, _ in warm.archives]
-    warm2 = S.Server(S.DeviceBackend(), hbm_budget_bytes=200e9, slots=3, max_fields=1)
+    warm2 = S.Server(be, hbm_budget_bytes=200e9, slots=3, max_fields=1)
     warm2.run([S.Request(i, "decompress", a.dims, 1e-3, a, 0.0) for i, a in enumerate(arch)], realtime=False)
     del warm, warm2, arch
     for k, rate in enumerate(rates):
         reqs = S.make_requests(rate, duration_s, pool, seed=3000 + k)
-        srv = S.Server(S.DeviceBackend(), hbm_budget_bytes=200e9, slots=3, max_fields=256)
+        srv = S.Server(be, hbm_budget_bytes=200e9, slots=3, max_fields=256)
         srv.run(reqs, realtime=True, on_arrival=S.pick_archive(4000 + k))
         out[f"rate_{rate}"] = S.summarize(srv, reqs)
         log(f"  rate {rate}/s: {out[f'rate_{rate}']['latency']}")
         del srv, reqs
         torch.cuda.empty_cache()
+    out["plan_pool_GB"] = round(be.pool_bytes / 1e9, 3)
     return out
 
 

@@ -528,7 +528,8 @@ def read_result(res_tensor):
 
 
 def read_results(res_tensor, n):
-    """n consecutive 64-byte device result records -> list of dicts (synchronises)."""
+    """n consecutive 64-byte result records -> list of dicts (a device tensor is copied to the
+    host, which synchronises; a host tensor is read as it is)."""
     raw = bytes(res_tensor[: 64 * n].cpu().numpy().tobytes())
     return [Result.from_buffer_copy(raw[64 * i: 64 * i + 64]).as_dict() for i in range(n)]
 
