@@ -252,14 +252,54 @@ def poisson_arrivals(rate, duration_s, rng):
 
 # ---------------------------------------------------------------- the device backend
 
+class PlanPool:
+    """Free plans of finished batches keyed by their field shapes: take() hands out a free plan
+    of those shapes (or None), give() returns one, dropping the least recently used keys' plans
+    while more than max_bytes of workspace would be held (a plan larger than that is dropped
+    at once).  nbytes(plan) gives a plan's workspace size and close(plan) releases it."""
+
+    def __init__(self, max_bytes, nbytes=lambda p: p.ws.numel(), close=lambda p: p.close()):
+        self.max_bytes = int(max_bytes)
+        self.nbytes = nbytes
+        self.close = close
+        self.free = collections.OrderedDict()  # shapes -> [plans], least recently used first
+        self.bytes = 0
+
+    def take(self, shapes):
+        lst = self.free.get(shapes)
+        if not lst:
+            return None
+        plan = lst.pop()
+        self.bytes -= self.nbytes(plan)
+        if not lst:
+            del self.free[shapes]
+        return plan
+
+    def give(self, shapes, plan):
+        nb = self.nbytes(plan)
+        if nb > self.max_bytes:
+            self.close(plan)
+            return
+        while self.free and self.bytes + nb > self.max_bytes:
+            old_shapes, lst = next(iter(self.free.items()))
+            old = lst.pop(0)
+            self.bytes -= self.nbytes(old)
+            self.close(old)
+            if not lst:
+                del self.free[old_shapes]
+        self.free.setdefault(shapes, []).append(plan)
+        self.free.move_to_end(shapes)
+        self.bytes += nb
+
+
 class DeviceBackend:
     """Batches through the batched C ABI, one stream per batch from a small stream pool.
 
-    The plans of finished batches are kept for reuse, keyed by their field shapes (least
-    recently used first out, at most `plan_pool_bytes` of workspace held, like blocks kept by
-    a caching allocator), so a batch of a shape seen before skips plan creation and workspace
-    set-up.  A compress batch's result records are copied into pinned host memory behind its
-    kernels, so finishing the batch reads host memory and waits for nothing."""
+    The plans of finished batches are kept for reuse in a PlanPool (at most `plan_pool_bytes`
+    of workspace held, like blocks kept by a caching allocator), so a batch of a shape seen
+    before skips plan creation and workspace set-up.  A compress batch's result records are
+    copied into pinned host memory behind its kernels, so finishing the batch reads host
+    memory and waits for nothing."""
 
     def __init__(self, mode=0, nstreams=3, device="cuda", plan_pool_bytes=4 << 30):
         import torch
@@ -268,36 +308,17 @@ class DeviceBackend:
         self.device = device
         self.streams = [torch.cuda.Stream(device=device) for _ in range(nstreams)]
         self.busy = [False] * nstreams
-        self.pool = collections.OrderedDict()  # shapes -> free plans
-        self.pool_bytes = 0
-        self.plan_pool_bytes = int(plan_pool_bytes)
+        self.pool = PlanPool(plan_pool_bytes)
+
+    @property
+    def pool_bytes(self):
+        return self.pool.bytes
 
     def _take_plan(self, shapes):
         import fzgpu
-        free = self.pool.get(shapes)
-        if free:
-            plan = free.pop()
-            self.pool_bytes -= plan.ws.numel()
-            if not free:
-                del self.pool[shapes]
-            return plan
-        return fzgpu.BatchPlan(list(shapes), mode=self.mode)  # workspace set up on the current stream
-
-    def _give_plan(self, shapes, plan):
-        nbytes = plan.ws.numel()
-        if nbytes > self.plan_pool_bytes:
-            plan.close()
-            return
-        while self.pool and self.pool_bytes + nbytes > self.plan_pool_bytes:
-            old_shapes, free = next(iter(self.pool.items()))
-            old = free.pop(0)
-            self.pool_bytes -= old.ws.numel()
-            old.close()
-            if not free:
-                del self.pool[old_shapes]
-        self.pool.setdefault(shapes, []).append(plan)
-        self.pool.move_to_end(shapes)
-        self.pool_bytes += nbytes
+        plan = self.pool.take(shapes)
+        # a new plan sets up its workspace on the current stream
+        return plan if plan is not None else fzgpu.BatchPlan(list(shapes), mode=self.mode)
 
     def launch(self, kind, reqs):
         torch = self.torch
@@ -330,7 +351,7 @@ class DeviceBackend:
     def finish(self, h, reqs):
         import fzgpu
         self.busy[h["si"]] = False
-        self._give_plan(h["shapes"], h["plan"])
+        self.pool.give(h["shapes"], h["plan"])
         out = h["keep"]
         if h["kind"] == "compress":
             recs = fzgpu.read_results(h["res"], len(reqs))

@@ -118,3 +118,21 @@ def test_latency_stats_and_poisson():
     reqs = [S.Request(i, "compress", (10, 1, 1), arrival=0.0, done=(i + 1) * 1e-3) for i in range(100)]
     st = S.latency_stats(reqs)
     assert st["n"] == 100 and math.isclose(st["p50_ms"], 50.5, rel_tol=1e-6) and st["p99_ms"] <= 100.0
+
+
+def test_plan_pool_reuses_by_shape_and_drops_least_recent():
+    closed = []
+    pool = S.PlanPool(100, nbytes=lambda p: p[1], close=closed.append)
+    a, b, c, big = ("a", 40), ("b", 40), ("c", 40), ("big", 101)
+    assert pool.take(((8, 1, 1),)) is None
+    pool.give(((8, 1, 1),), a)
+    pool.give(((9, 1, 1),), b)
+    assert pool.bytes == 80
+    assert pool.take(((8, 1, 1),)) is a and pool.bytes == 40  # handed out: no longer held
+    pool.give(((8, 1, 1),), a)                                 # (9,) is now the least recent
+    pool.give(((7, 1, 1),), c)                                 # 120 > 100: drops b
+    assert closed == [b] and pool.bytes == 80
+    assert pool.take(((9, 1, 1),)) is None
+    pool.give(((6, 1, 1),), big)                               # larger than the whole pool
+    assert closed == [b, big] and pool.bytes == 80
+    assert pool.take(((7, 1, 1),)) is c and pool.take(((8, 1, 1),)) is a and pool.bytes == 0
