@@ -253,7 +253,7 @@ __global__ __launch_bounds__(kGatherThreads) void k_scan_gather(const uint32_t* 
         __syncthreads();
         uint8_t* out = payload + 4ull * W + 16ull * ((uint64_t)excl + base);
         for (uint32_t j0 = 0; j0 < run; j0 += 4 * kGatherThreads) {
-            uint32_t src[4];
+            uint64_t src[4];  // staging byte offsets pass 2^32 beyond 2^20 tiles
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const uint32_t j = j0 + tid + u * kGatherThreads;
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(kGatherThreads) void k_scan_gather(const uint32_t* 
                     if (sh_off[mid] <= j) lo = mid;
                     else hi = mid - 1;
                 }
-                src[u] = (r0 + lo) * 4096u + 16u * (j - sh_off[lo]);
+                src[u] = (uint64_t)(r0 + lo) * 4096u + 16u * (j - sh_off[lo]);
             }
             uint4 v[4];
 #pragma unroll

@@ -245,6 +245,34 @@ def test_field_beyond_one_gather_run():
     say(f"  CR {4 * n / rf['payload_bytes']:.3f} maxerr/eb {err / rf['eb_abs']:.6f}")
 
 
+# ---------------------------------------------------------------- the largest plans
+
+@pytest.mark.parametrize("dims", [(0xFFFFF000 - 100, 1, 1), (65535, 65535, 1)])
+def test_maximum_length_fields(dims):
+    """The largest fields a plan accepts (17.2 GB; len = 2^32 - 4096 is the limit, positions
+    stay 32-bit): over 2^20 tiles, so staging offsets pass 2^32 bytes.  The fused archive
+    must equal the plain pipeline's byte for byte, and decompress within the error bound."""
+    import torch
+    n = dims[0] * dims[1] * dims[2]
+    torch.cuda.empty_cache()
+    say(f"maximum-length field dims={dims} n={n}")
+    x = torch.arange(n, dtype=torch.float32, device="cuda")
+    x.mul_(1e-3).sin_()
+    plan = fzgpu.Plan(*dims)
+    assert plan.sizes.len == n and n // 2048 > 1 << 20
+    pf, tf, rf = plan.compress(x, 1e-3, fused=True)
+    pp, tp, rp = plan.compress(x, 1e-3, fused=False)
+    assert rf == rp
+    assert torch.equal(pf, pp) and torch.equal(tf, tp)
+    del pp, tp
+    xr = plan.decompress(pf, tf, rf["ebx2"])
+    err = float(xr.sub_(x).abs_().max())
+    assert err <= rf["eb_abs"] + 4 * 2.0**-24
+    say(f"  CR {4 * n / rf['payload_bytes']:.3f} maxerr/eb {err / rf['eb_abs']:.6f}")
+    del x, xr, pf, tf, plan
+    torch.cuda.empty_cache()
+
+
 # ---------------------------------------------------------------- rounding ties
 
 @pytest.mark.parametrize("dims", [(40000, 1, 1), (64, 300, 1), (101, 300, 1), (32, 32, 24), (37, 30, 20)])
