@@ -181,3 +181,31 @@ def test_full_size_round_trip_with_planted_values(dims):
     # the payload is deterministic: a second call gives the same bytes
     p2, t2, r2, s2 = plan.compress_outlier(x, 1e-5)
     assert torch.equal(p2, payload) and torch.equal(s2, side) and torch.equal(t2, tail)
+
+
+def test_maximum_length_field_with_planted_values():
+    """The largest field a plan accepts (len = 2^32 - 4196, 17.2 GB, over 2^20 tiles) in the
+    outlier mode, with NaN, +-Inf and spikes planted up to the last positions: finite values
+    within the bound, the planted values bit for bit, the channel's counts as planted."""
+    import torch
+    n = 0xFFFFF000 - 100
+    torch.cuda.empty_cache()
+    say(f"case outlier maximum length n={n}")
+    x = torch.arange(n, dtype=torch.float32, device="cuda")
+    x.mul_(1e-3).sin_()
+    pos = torch.tensor([0, 5, 1 << 31, (1 << 32) - 9000, n - 2049, n - 2, n - 1], dtype=torch.int64, device="cuda")
+    vals = torch.tensor([float("nan"), float("inf"), 1e7, -float("inf"), float("nan"), 1e7, float("nan")],
+                        device="cuda")
+    x[pos] = vals
+    plan = fzgpu.Plan(n, 1, 1, mode=fzgpu.MODE_OUTLIER)
+    # eb 1e-6 of a 1e7 range: the spikes' deltas overflow the 16-bit codes
+    payload, tail, r, side = plan.compress_outlier(x, 1e-6, side_capacity=1 << 20)
+    nd, nr = (int(v) for v in side[:16].cpu().view(torch.int64).tolist())
+    assert nr == 5 and nd == r["overflow"] and nd > 0
+    out = plan.decompress_outlier(payload, tail, side, r["ebx2"], r["vmin"])
+    fin = torch.isfinite(x)
+    assert torch.equal(out[pos].view(torch.int32)[~torch.isfinite(vals)], vals.view(torch.int32)[~torch.isfinite(vals)])
+    err = float(out.sub_(x).masked_fill_(~fin, 0).abs_().max())
+    assert err <= r["eb_abs"] + 4 * 2.0**-24 * 1e7
+    del x, out, payload, tail, side, plan
+    torch.cuda.empty_cache()
