@@ -857,7 +857,7 @@ fz_status fz_batch_plan_create(fz_batch_plan* plan, const fz_field_shape* shapes
         p->rstart[k] = (uint32_t)r;
         for (int c = 0; c < kNumChunks; c++) {
             p->pstart[c][k] = (uint32_t)parts[c];
-            parts[c] += (g.len + kRangeChunks[c] - 1) / kRangeChunks[c];
+            parts[c] += ((uint64_t)g.len + kRangeChunks[c] - 1) / kRangeChunks[c];  // len + chunk passes 2^32
         }
         u += (uint64_t)g.T + 1;
         r += (uint64_t)F.rdims[0] * F.rdims[1] * F.rdims[2];

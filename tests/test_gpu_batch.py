@@ -240,3 +240,37 @@ def test_uneven_batch_full_size_equals_per_field_plain():
         ref = O.compress(x, shapes[i], ebs[i], O.MODE_FIXED)
         p, t = _field_bytes(out, res, i)
         assert np.array_equal(p, ref["payload"]) and np.array_equal(t, ref["tail"]), i
+
+
+def test_batch_with_the_largest_field():
+    """A batch holding the largest field a plan accepts (17.2 GB, over 2^20 units) between two
+    small ones: every field's archive equals the single-field fused path's, and the batched
+    decompress equals the single-field decompress bit for bit."""
+    import torch
+    big = (0xFFFFF000 - 100, 1, 1)
+    shapes = [(1000, 999, 1), big, (33, 20, 17)]
+    torch.cuda.empty_cache()
+    say(f"batch with the largest field {big}")
+    xs = [_to_dev(_field(shapes[0], 41)), torch.arange(big[0], dtype=torch.float32, device="cuda"),
+          _to_dev(_field(shapes[2], 43))]
+    xs[1].mul_(1e-3).sin_()
+    ebs = [1e-3, 1e-4, 1e-2]
+    plan = fzgpu.BatchPlan(shapes)
+    out, res = plan.compress(xs, ebs)
+    singles = []
+    for i, s in enumerate(shapes):
+        single = fzgpu.Plan(*s)
+        payload, tail, r = single.compress(xs[i], ebs[i])
+        assert r == res[i], (i, s)
+        assert torch.equal(out.payloads[i][: r["payload_bytes"]], payload), (i, s)
+        assert torch.equal(out.tails[i][: r["tail_len"]], tail), (i, s)
+        singles.append((single, payload, tail, r))
+    pays = [out.payloads[i] for i in range(3)]
+    tails = [out.tails[i][: res[i]["tail_len"]] for i in range(3)]
+    outs = plan.decompress(pays, tails, [r["ebx2"] for r in res])
+    for i, (single, payload, tail, r) in enumerate(singles):
+        one = single.decompress(payload, tail, r["ebx2"])
+        assert torch.equal(outs[i].view(torch.int32), one.view(torch.int32)), i
+        del one
+    del xs, out, outs, singles, plan
+    torch.cuda.empty_cache()
