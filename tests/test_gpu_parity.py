@@ -247,7 +247,8 @@ def test_field_beyond_one_gather_run():
 
 # ---------------------------------------------------------------- the largest plans
 
-@pytest.mark.parametrize("dims", [(0xFFFFF000 - 100, 1, 1), (65535, 65535, 1)])
+@pytest.mark.parametrize("dims", [(0xFFFFF000 - 100, 1, 1), (65535, 65535, 1), (2147481548, 2, 1),
+                                  (1625, 1625, 1626)])
 def test_maximum_length_fields(dims):
     """The largest fields a plan accepts (17.2 GB; len = 2^32 - 4096 is the limit, positions
     stay 32-bit): over 2^20 tiles, so staging offsets pass 2^32 bytes.  The fused archive
