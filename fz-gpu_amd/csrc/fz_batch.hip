@@ -406,10 +406,10 @@ __global__ __launch_bounds__(64) void k_batch_decode(BatchDev d, uint32_t nf)
 }
 
 template <int NDIM>
-__global__ __launch_bounds__(kWG) void k_batch_recon(BatchDev d, uint32_t r_lo, uint32_t f_lo, uint32_t f_hi)
+__global__ __launch_bounds__(kWG) void k_batch_recon(BatchDev d, uint32_t r_lo)
 {
     const uint32_t wg = r_lo + blockIdx.x;
-    const uint32_t f = wave_upper(d.rstart, f_lo, f_hi, wg);
+    const uint32_t f = d.rfield[wg];  // uniform: one scalar load (a search would chain two)
     const BField&  F = d.fields[f];
     const Geom     g = F.g;
     const uint32_t local = wg - d.rstart[f];
@@ -440,11 +440,11 @@ hipError_t launch_batch_decompress(const BatchDev& d, const BatchShape& b, hipEv
             continue;
         }
         if (nd == 1)
-            e = launch_pair(k_batch_recon<1>, dim3(n), dim3(kWG), s, ev, 2, d, b.r_lo[1], b.f_lo[1], b.f_hi[1]);
+            e = launch_pair(k_batch_recon<1>, dim3(n), dim3(kWG), s, ev, 2, d, b.r_lo[1]);
         else if (nd == 2)
-            e = launch_pair(k_batch_recon<2>, dim3(n), dim3(kWG), s, ev, 3, d, b.r_lo[2], b.f_lo[2], b.f_hi[2]);
+            e = launch_pair(k_batch_recon<2>, dim3(n), dim3(kWG), s, ev, 3, d, b.r_lo[2]);
         else
-            e = launch_pair(k_batch_recon<3>, dim3(n), dim3(kWG), s, ev, 4, d, b.r_lo[3], b.f_lo[3], b.f_hi[3]);
+            e = launch_pair(k_batch_recon<3>, dim3(n), dim3(kWG), s, ev, 4, d, b.r_lo[3]);
     }
     return e;
 }

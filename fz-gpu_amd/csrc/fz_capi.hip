@@ -552,7 +552,7 @@ struct fz_batch_plan_s {
     BatchShape            shape{};
     fz_batch_tuning       tune{};
     struct {
-        size_t fields, ustart, ufield, pstart[kNumChunks], rstart, tables_end, ptrs, prm, part, counts, dsts, texcl,
+        size_t fields, ustart, ufield, pstart[kNumChunks], rstart, rfield, tables_end, ptrs, prm, part, counts, dsts, texcl,
             staging, total;
     } L{};
     std::vector<BPtrs>      host_ptrs;  // staging of the call's pointer records (sorted order)
@@ -684,6 +684,7 @@ BatchDev batch_dev(const fz_batch_plan p, void* d_ws, fz_result* res)
     d.ufield = reinterpret_cast<const uint32_t*>(ws + p->L.ufield);
     d.pstart = reinterpret_cast<const uint32_t*>(ws + p->L.pstart[chunk_index(p)]);
     d.rstart = reinterpret_cast<const uint32_t*>(ws + p->L.rstart);
+    d.rfield = reinterpret_cast<const uint32_t*>(ws + p->L.rfield);
     d.ptrs = reinterpret_cast<BPtrs*>(ws + p->L.ptrs);
     d.prm = reinterpret_cast<BParams*>(ws + p->L.prm);
     d.part = reinterpret_cast<float4*>(ws + p->L.part);
@@ -889,6 +890,7 @@ fz_status fz_batch_plan_create(fz_batch_plan* plan, const fz_field_shape* shapes
         o = al256(o + 4ull * (nfields + 1));
     }
     L.rstart = o; o = al256(o + 4ull * (nfields + 1));
+    L.rfield = o; o = al256(o + 4ull * r);
     L.tables_end = o;
     L.ptrs = o;    o = al256(o + sizeof(BPtrs) * nfields);
     L.prm = o;     o = al256(o + sizeof(BParams) * nfields);
@@ -980,6 +982,11 @@ fz_status fz_batch_workspace_init(const fz_batch_plan plan, void* d_ws, size_t w
     for (int c = 0; c < kNumChunks; c++)
         std::memcpy(t.data() + L.pstart[c], plan->pstart[c].data(), 4ull * (plan->nf + 1));
     std::memcpy(t.data() + L.rstart, plan->rstart.data(), 4ull * (plan->nf + 1));
+    {
+        uint32_t* rf = reinterpret_cast<uint32_t*>(t.data() + L.rfield);
+        for (uint32_t f = 0; f < plan->nf; f++)
+            for (uint32_t w = plan->rstart[f]; w < plan->rstart[f + 1]; w++) rf[w] = f;
+    }
     FZ_TRY(hipMemcpyAsync(d_ws, t.data(), t.size(), hipMemcpyHostToDevice, stream));
     FZ_TRY(hipStreamSynchronize(stream));
     return FZ_SUCCESS;

@@ -142,6 +142,7 @@ struct BatchDev {
     const uint32_t* ufield;  // per unit: its field (sorted order)
     const uint32_t* pstart;  // nf + 1, first range partial of each field (selected chunk)
     const uint32_t* rstart;  // nf + 1, first reconstruct workgroup of each field
+    const uint32_t* rfield;  // per reconstruct workgroup: its field (sorted order)
     BPtrs*          ptrs;
     BParams*        prm;
     float4*         part;
