@@ -423,9 +423,7 @@ __global__ __launch_bounds__(kWG) void k_batch_recon(BatchDev d, uint32_t r_lo)
         const uint32_t rx = F.rdims[0];
         recon_2d(codes, out, g, ebx2, local % rx, local / rx);
     } else {
-        const uint32_t rx = F.rdims[0], ry = F.rdims[1];
-        const uint32_t t = local / rx;
-        recon_3d(codes, out, g, ebx2, local % rx, t % ry, t / ry);
+        recon_3d_any((int)F.rq, codes, out, g, ebx2, local, F.rdims[0], F.rdims[1]);
     }
 }
 

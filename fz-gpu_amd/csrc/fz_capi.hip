@@ -844,8 +844,9 @@ fz_status fz_batch_plan_create(fz_batch_plan* plan, const fz_field_shape* shapes
             F.rdims[1] = (g.Y + 15) / 16;
             F.rdims[2] = 1;
         } else {
-            F.rdims[0] = (g.X + kRecon3DCols - 1) / kRecon3DCols;
-            F.rdims[1] = (g.Y + 7) / 8;
+            F.rq = recon3d_q(g.X, g.Y);
+            F.rdims[0] = (g.X + 4 * F.rq - 1) / (4 * F.rq);
+            F.rdims[1] = (g.Y + kWG / F.rq - 1) / (kWG / F.rq);
             F.rdims[2] = (g.Z + 7) / 8;
         }
         const int nd = g.ndim;

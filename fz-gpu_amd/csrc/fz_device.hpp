@@ -14,7 +14,6 @@ namespace fz {
 
 constexpr int kWG = 256;           // threads per workgroup
 constexpr uint32_t kRecon2DCols = 4 * kWG;  // columns per 2D reconstruct workgroup (4 per thread)
-constexpr uint32_t kRecon3DCols = 128;      // columns per 3D reconstruct workgroup (4 per thread)
 constexpr int kTileCodes = 2048;   // codes per bitshuffle tile
 constexpr int kTileWords = 1024;   // u32 words per tile
 constexpr int kTileBlocks = 256;   // 16-byte blocks per tile

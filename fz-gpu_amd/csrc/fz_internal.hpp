@@ -36,6 +36,24 @@ inline uint32_t range_grid(uint32_t len)
     return (uint32_t)g;
 }
 
+// Quads per row Q of the 3D reconstruct workgroup (4Q columns x 256/Q rows x 8 planes) for an
+// X x Y plane: the Q in {32, 16, 8, 4} whose workgroups leave the fewest lanes past the
+// field's edges (ties go to the larger Q).
+inline uint32_t recon3d_q(uint32_t X, uint32_t Y)
+{
+    uint32_t best = 32;
+    uint64_t best_cells = ~0ull;  // lanes the grid spans for the plane, per 4 columns
+    for (uint32_t q = 32; q >= 4; q /= 2) {
+        const uint64_t c = 4ull * q, r = kWG / q;
+        const uint64_t cells = ((X + c - 1) / c) * c * ((Y + r - 1) / r) * r;
+        if (cells < best_cells) {
+            best_cells = cells;
+            best = q;
+        }
+    }
+    return best;
+}
+
 // -------- launchers (return hipError_t of the launch)
 hipError_t launch_range(const float* in, const Geom& g, double eb_rel, float4* part, fz_result* res,
                         uint64_t* zero_states, uint32_t nzero, uint32_t* zero_ticket, hipStream_t s);
@@ -112,7 +130,8 @@ struct BField {        // per field, uploaded once per workspace (128 B)
     uint32_t caller;   // index of the field in the caller's arrays (its result record)
     uint32_t ustart;   // first global unit
     uint32_t rdims[3]; // reconstruct workgroups along x, y, z
-    uint32_t pad[9];
+    uint32_t rq;       // 3D: quads per row of its reconstruct workgroups (recon3d_q)
+    uint32_t pad[8];
 };
 static_assert(sizeof(BField) == 128, "BField layout");
 

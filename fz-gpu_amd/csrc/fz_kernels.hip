@@ -640,16 +640,20 @@ __global__ __launch_bounds__(kWG) void k_reconstruct_1d(const SRC codes, float* 
     recon_1d(codes, out, g, ebx2, blockIdx.x);
 }
 
+// 2D and 3D grids are one-dimensional (workgroup -> (bx, by[, bz]) by division), so tall
+// fields are not bound by the y / z grid limits
 template <typename SRC>
-__global__ __launch_bounds__(kWG) void k_reconstruct_2d(const SRC codes, float* __restrict__ out, Geom g, float ebx2)
+__global__ __launch_bounds__(kWG) void k_reconstruct_2d(const SRC codes, float* __restrict__ out, Geom g, float ebx2,
+                                                        uint32_t rx)
 {
-    recon_2d(codes, out, g, ebx2, blockIdx.x, blockIdx.y);
+    recon_2d(codes, out, g, ebx2, blockIdx.x % rx, blockIdx.x / rx);
 }
 
 template <typename SRC>
-__global__ __launch_bounds__(kWG) void k_reconstruct_3d(const SRC codes, float* __restrict__ out, Geom g, float ebx2)
+__global__ __launch_bounds__(kWG) void k_reconstruct_3d(const SRC codes, float* __restrict__ out, Geom g, float ebx2,
+                                                        int q, uint32_t rx, uint32_t ry)
 {
-    recon_3d(codes, out, g, ebx2, blockIdx.x, blockIdx.y, blockIdx.z);
+    recon_3d_any(q, codes, out, g, ebx2, blockIdx.x, rx, ry);
 }
 
 template <typename SRC>
@@ -659,11 +663,16 @@ static hipError_t launch_reconstruct_t(const SRC codes, float* out, const Geom& 
         const uint32_t thr = (g.xyz + 15) / 16;
         FZ_LAUNCH(k_reconstruct_1d<SRC><<<(thr + kWG - 1) / kWG, kWG, 0, s>>>(codes, out, g, ebx2));
     } else if (g.ndim == 2) {
-        const dim3 grid((g.X + kRecon2DCols - 1) / kRecon2DCols, (g.Y + 15) / 16);
-        FZ_LAUNCH(k_reconstruct_2d<SRC><<<grid, kWG, 0, s>>>(codes, out, g, ebx2));
+        const uint32_t rx = (g.X + kRecon2DCols - 1) / kRecon2DCols;
+        const uint64_t n = (uint64_t)rx * ((g.Y + 15) / 16);
+        if (n > 0xFFFFFFFFull) return hipErrorInvalidConfiguration;
+        FZ_LAUNCH(k_reconstruct_2d<SRC><<<(uint32_t)n, kWG, 0, s>>>(codes, out, g, ebx2, rx));
     } else {
-        const dim3 grid((g.X + kRecon3DCols - 1) / kRecon3DCols, (g.Y + 7) / 8, (g.Z + 7) / 8);
-        FZ_LAUNCH(k_reconstruct_3d<SRC><<<grid, kWG, 0, s>>>(codes, out, g, ebx2));
+        const uint32_t q = recon3d_q(g.X, g.Y);
+        const uint32_t rx = (g.X + 4 * q - 1) / (4 * q), ry = (g.Y + kWG / q - 1) / (kWG / q);
+        const uint64_t n = (uint64_t)rx * ry * ((g.Z + 7) / 8);
+        if (n > 0xFFFFFFFFull) return hipErrorInvalidConfiguration;
+        FZ_LAUNCH(k_reconstruct_3d<SRC><<<(uint32_t)n, kWG, 0, s>>>(codes, out, g, ebx2, (int)q, rx, ry));
     }
     return hipGetLastError();
 }

@@ -1006,22 +1006,27 @@ __device__ __forceinline__ void recon_2d(const SRC codes, float* __restrict__ ou
     }
 }
 
-// 3D: workgroup (bx, by, bz) = 128 (x) x 8 (y) x 8 (z); thread (xq, ty) = 4 adjacent columns
-// of row y0 + ty over the 8 z of the strip.  Prefix over z in registers, over x in the thread
-// and with the other half of its 8-wide Lorenzo block (the neighbour lane, one DPP step); an
-// LDS transpose (16-byte accesses) then gives thread (xq, tz) the 8 rows of its z slice for the
-// prefix over y.  Loads are 8 bytes and stores 16 bytes per lane when X % 4 == 0.
-template <typename SRC>
+// 3D: workgroup (bx, by, bz) = 4Q columns (x) x 256/Q rows (y) x 8 planes (z), Q in {32, 16,
+// 8, 4} quads per row (narrow fields take a small Q so that fewer lanes sit past the row's
+// end; the y extent is then several 8-row Lorenzo blocks).  Phase 1: thread (xq, yy) = 4
+// adjacent columns of row y0 + yy over the 8 z of the strip; prefix over z in registers, over
+// x in the thread and with the other half of its 8-wide Lorenzo block (the neighbour lane,
+// one DPP step).  Phase 2, after an LDS transpose (16-byte accesses): thread (xq, tz, yb) runs
+// the prefix over the 8 rows of y block yb in plane tz.  Loads are 8 bytes and stores 16
+// bytes per lane when X % 4 == 0.
+template <int Q, typename SRC>
 __device__ __forceinline__ void recon_3d(const SRC codes, float* __restrict__ out, const Geom& g, float ebx2,
-                                         uint32_t bx, uint32_t by, uint32_t bz)
+                                         uint32_t bx, uint32_t by, uint32_t bz, int4* sh)  // sh: 8 * kWG int4
 {
-    __shared__ int4 sh[8][8][32];  // [z][y][x quad]
-    const uint32_t xq = threadIdx.x & 31, ty = threadIdx.x >> 5;
-    const uint32_t x0 = bx * kRecon3DCols + 4 * xq, y0 = by * 8, z0 = bz * 8;
+    static_assert(Q == 32 || Q == 16 || Q == 8 || Q == 4, "quads per row");
+    constexpr uint32_t R = kWG / Q;  // rows per workgroup (a multiple of 8)
+    auto at = [&](uint32_t z, uint32_t y, uint32_t x) -> int4& { return sh[(z * R + y) * Q + x]; };  // [z][y][x quad]
+    const uint32_t xq = threadIdx.x % Q, yy = threadIdx.x / Q;
+    const uint32_t x0 = bx * (4 * Q) + 4 * xq, y0 = by * R, z0 = bz * 8;
     const bool     full = x0 + 4 <= g.X;
     const bool     vec = full && (g.X & 3u) == 0;
     {
-        const uint32_t y = y0 + ty;
+        const uint32_t y = y0 + yy;
         int32_t        col[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int z = 0; z < 8; z++) {
@@ -1042,24 +1047,26 @@ __device__ __forceinline__ void recon_3d(const SRC codes, float* __restrict__ ou
                 p[j] = col[j] + (j ? p[j - 1] : 0);  // prefix over the thread's 4 columns
             }
             // the odd lane of a pair holds the block's upper 4 columns: add the even lane's total
+            // (xq and the lane index have the same parity, Q being even)
             const int32_t lower = __builtin_amdgcn_mov_dpp(p[3], 0xA0, 0xF, 0xF, false);  // quad_perm [0,0,2,2]
             const int32_t add = (xq & 1u) ? lower : 0;
-            sh[z][ty][xq] = make_int4(p[0] + add, p[1] + add, p[2] + add, p[3] + add);
+            at(z, yy, xq) = make_int4(p[0] + add, p[1] + add, p[2] + add, p[3] + add);
         }
     }
     __syncthreads();
-    const uint32_t tz = ty;
+    const uint32_t rest = threadIdx.x / Q, tz = rest & 7u, yb = rest >> 3;
     if (z0 + tz >= g.Z) return;
     int4 acc = make_int4(0, 0, 0, 0);
 #pragma unroll
     for (int y = 0; y < 8; y++) {
-        if (y0 + y >= g.Y) break;  // uniform
-        const int4 v = sh[tz][y][xq];
+        const uint32_t yr = yb * 8 + y;
+        if (y0 + yr >= g.Y) break;  // uniform over the lanes of one (tz, yb)
+        const int4 v = at(tz, yr, xq);
         acc.x += v.x;
         acc.y += v.y;
         acc.z += v.z;
         acc.w += v.w;
-        const size_t row = (size_t)(z0 + tz) * g.XY + (size_t)(y0 + y) * g.X;
+        const size_t row = (size_t)(z0 + tz) * g.XY + (size_t)(y0 + yr) * g.X;
         const float  o[4] = {(float)acc.x * ebx2, (float)acc.y * ebx2, (float)acc.z * ebx2, (float)acc.w * ebx2};
         if (full) {
             typedef float f32x4a4_t __attribute__((ext_vector_type(4), aligned(4)));
@@ -1069,6 +1076,21 @@ __device__ __forceinline__ void recon_3d(const SRC codes, float* __restrict__ ou
             for (int j = 0; j < 4; j++)
                 if (x0 + j < g.X) out[row + x0 + j] = o[j];
         }
+    }
+}
+
+// recon_3d of workgroup `wg` of a field whose grid is rx x ry x (Z / 8), for the field's Q
+template <typename SRC>
+__device__ __forceinline__ void recon_3d_any(int q, const SRC codes, float* __restrict__ out, const Geom& g,
+                                             float ebx2, uint32_t wg, uint32_t rx, uint32_t ry)
+{
+    __shared__ int4 sh[8 * kWG];  // one 32 KB array for every Q (a static array per variant would add up)
+    const uint32_t  bx = wg % rx, t = wg / rx, by = t % ry, bz = t / ry;
+    switch (q) {
+    case 4: recon_3d<4>(codes, out, g, ebx2, bx, by, bz, sh); break;
+    case 8: recon_3d<8>(codes, out, g, ebx2, bx, by, bz, sh); break;
+    case 16: recon_3d<16>(codes, out, g, ebx2, bx, by, bz, sh); break;
+    default: recon_3d<32>(codes, out, g, ebx2, bx, by, bz, sh); break;
     }
 }
 
