@@ -93,6 +93,17 @@ def test_decompress_bit_exact_and_error_bound(dims):
     assert float(np.abs(xr - x).max()) <= tol
 
 
+@pytest.mark.parametrize("dims", [(4, 1100000, 1), (3, 3, 600000)])
+def test_tall_fields_decompress(dims):
+    """More than 65535 reconstruct workgroups along y (2D) or z (3D): the reconstruct grids
+    are one-dimensional, so such fields decompress like any other."""
+    x = _field(dims, sum(dims) + 5)
+    plan, payload, tail, r, ref = _check_against_oracle(x, dims, 1e-3, O.MODE_FIXED, fused=True)
+    xr = _np(plan.decompress(payload, tail, r["ebx2"]))
+    rc, xo = O.decompress(ref["payload"], ref["tail"], x.size, dims, ref["info"]["ebx2"])
+    assert rc == 0 and np.array_equal(xr.view(np.uint32), xo.view(np.uint32))
+
+
 def test_len_longer_than_dims():
     x = _field((6000, 1, 1), 21)
     _check_against_oracle(x, (1000, 3, 1), 1e-3, O.MODE_FIXED, fused=True)
