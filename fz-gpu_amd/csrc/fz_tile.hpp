@@ -603,7 +603,9 @@ __device__ __forceinline__ void tile_encode_o(const float* __restrict__ in, cons
     bool     exact = level == kLevelOutlier;
     if (!exact) {
         tile_dualquant<NDIM>(in, g, tile, gp, R, r, level, w, ovf);
-        exact = __syncthreads_or(ovf != 0) != 0;
+        // levels 3-4 (packed paths) bound |d| by 2^(ndim-1) * max|s| < 32768: no delta can
+        // overflow, so only the float paths need the workgroup's vote (uniform branch)
+        if (level < 3) exact = __syncthreads_or(ovf != 0) != 0;
     }
     if (exact) {
         Int8 o;
