@@ -89,8 +89,17 @@ __global__ __launch_bounds__(kWG) void k_tile_compress_o(const float* __restrict
     }
     uint8_t* const  slot = staging + (size_t)tile * 4096;
     uint32_t* const sum = run_sums + tile / tpw;
-    tile_encode_o<NDIM>(in, g, tile, gp, R, e.r, e.level, bwords, slot, counts + tile, ovf, sum, ocnt + 2 * tile,
-                        orun + 2 * (tile / tpw));
+    if constexpr (NDIM == 3) {
+        if (gp < 2)
+            tile_encode_o<3, 2>(in, g, tile, gp, R, e.r, e.level, bwords, slot, counts + tile, ovf, sum,
+                                ocnt + 2 * tile, orun + 2 * (tile / tpw));
+        else
+            tile_encode_o<3, 1>(in, g, tile, gp, R, e.r, e.level, bwords, slot, counts + tile, ovf, sum,
+                                ocnt + 2 * tile, orun + 2 * (tile / tpw));
+    } else {
+        tile_encode_o<NDIM>(in, g, tile, gp, R, e.r, e.level, bwords, slot, counts + tile, ovf, sum, ocnt + 2 * tile,
+                            orun + 2 * (tile / tpw));
+    }
 }
 
 // exclusive (overflow, verbatim) prefix of a pair over the workgroup (kWG threads), and the totals

@@ -591,7 +591,7 @@ __device__ __forceinline__ uint32_t code_i(int32_t d)
 // A full tile of the outlier mode: as tile_encode, plus the tile's (overflows, verbatim
 // values) counts at ocnt[0..1].  At level >= 0 the fixed mode's paths run first and the
 // integer path only redoes a tile in which some delta overflowed (uniform decision).
-template <int NDIM>
+template <int NDIM, int PATHS = 0>
 __device__ __forceinline__ void tile_encode_o(const float* __restrict__ in, const Geom& g, uint32_t tile, int gp,
                                               const RawTile<NDIM>& R, float r, int level,
                                               uint32_t* __restrict__ bwords, uint8_t* __restrict__ slot,
@@ -602,7 +602,7 @@ __device__ __forceinline__ void tile_encode_o(const float* __restrict__ in, cons
     uint32_t ovf = 0, nraw = 0;
     bool     exact = level == kLevelOutlier;
     if (!exact) {
-        tile_dualquant<NDIM>(in, g, tile, gp, R, r, level, w, ovf);
+        tile_dualquant<NDIM, PATHS>(in, g, tile, gp, R, r, level, w, ovf);
         // levels 3-4 (packed paths) bound |d| by 2^(ndim-1) * max|s| < 32768: no delta can
         // overflow, so only the float paths need the workgroup's vote (uniform branch)
         if (level < 3) exact = __syncthreads_or(ovf != 0) != 0;
