@@ -30,7 +30,7 @@ extern "C" {
 
 #define FZO_MODE_FIXED 0
 #define FZO_MODE_LITERAL 1
-#define FZO_MODE_OUTLIER 2 /* FIXED format + side channel (this build's extension, no reference form) */
+#define FZO_MODE_OUTLIER 2 /* FIXED format + side channel (an extension of the format; no reference form) */
 
 /* status codes: same values as fz_status in include/fz_capi.h */
 #define FZO_SUCCESS 0
