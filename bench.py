@@ -457,7 +457,8 @@ def inverse_target(D, workload):
 
 def batch_target(D):
     """Counter / trace target for the batched kernels: 3 compress calls of the 256-field
-    uneven batch at the default tuning (one group on one stream)."""
+    uneven batch at the default tuning (one group on one stream), then 3 batched decompress
+    calls of its archives."""
     import torch
 
     import fzgpu
@@ -469,6 +470,10 @@ def batch_target(D):
         plan.compress_async(d, [f["eb"] for f in spec], out)
     torch.cuda.synchronize()
     res = fzgpu.read_results(out.res, len(spec))
+    outs = [torch.empty(max(t.numel(), 4), dtype=torch.float32, device="cuda") for t in d]
+    for _ in range(3):
+        plan.decompress_async(out.payloads, out.tails, [r["ebx2"] for r in res], outs)
+    torch.cuda.synchronize()
     print(json.dumps({"workload": "uneven-batch-256", "input_bytes": sum(4 * t.numel() for t in d),
                       "payload_bytes": sum(r["payload_bytes"] for r in res)}), flush=True)
 
@@ -496,7 +501,8 @@ def main():
     ap.add_argument("--batch-only", action="store_true", help="only the 256-field uneven batch (prints its JSON)")
     ap.add_argument("--batch-fields", type=int, default=256)
     ap.add_argument("--batch-target", action="store_true",
-                    help="profiling target: the 256-field uneven batch, default tuning (one group), 3 compress calls")
+                    help="profiling target: the 256-field uneven batch, default tuning (one group), 3 compress "
+                         "calls, then 3 batched decompress calls")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-serving leg")
     ap.add_argument("--serve-only", action="store_true", help="only the request-serving leg (prints its JSON)")
     ap.add_argument("--no-staged", action="store_true", help="skip the host-resident (PCIe-inclusive) leg")
