@@ -140,7 +140,7 @@ def bench_batch(steps, warmup, nfields=256):
     in_bytes = sum(4 * t.numel() for t in d)
     plan = fzgpu.BatchPlan(shapes)
     out = plan.alloc_outputs()
-    tune = plan.autotune(d, ebs, out, reps=2)
+    tune = plan.autotune(d, ebs, out, reps=3)
     # group-count sweep: eager launches and graph replays
     sweep = {}
     for ng in (1, 2, 4, 8, 16):
