@@ -17,9 +17,10 @@
 //                        payload address of each tile's first block; payload size
 //   k_batch_gather       one wave per unit: copy the staged blocks to their address
 //
-// Decompress: k_batch_dscan (per-field scan of the flag popcounts), k_batch_decode (one
-// workgroup per unit: scatter, unshuffle, codes into the unit's slot), k_batch_recon<NDIM>
-// (Lorenzo prefix sums over each field's codes).
+// Decompress: k_batch_dscan (per-field scan of the flag popcounts), k_batch_decode (one wave
+// per unit: scatter, unshuffle, codes into the unit's slot; a 1D field's unit is also
+// reconstructed there, straight to floats), k_batch_recon<2|3> (Lorenzo prefix sums over each
+// 2D / 3D field's codes).
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -389,7 +390,9 @@ __global__ __launch_bounds__(kBScanWG) void k_batch_dscan(BatchDev d)
 }
 
 // One wave per unit: the tile's codes (or the tail codes) into the unit's 4096-byte slot;
-// a field's slots are consecutive, so its codes form one array.
+// a field's slots are consecutive, so its codes form one array.  A 1D field's unit is
+// reconstructed here as well, straight to floats (its Lorenzo blocks lie inside the unit), so
+// no reconstruct kernel runs for 1D fields.
 __global__ __launch_bounds__(64) void k_batch_decode(BatchDev d, uint32_t nf)
 {
     const uint32_t gu = blockIdx.x;
@@ -397,6 +400,11 @@ __global__ __launch_bounds__(64) void k_batch_decode(BatchDev d, uint32_t nf)
     const Geom     g = d.fields[f].g;
     const uint32_t u0 = d.ustart[f], tile = gu - u0;
     const BPtrs    q = d.ptrs[f];
+    if (g.ndim == 1) {  // uniform
+        decode_recon_1d_w(static_cast<const uint8_t*>(q.src), g.mode == 0 ? q.tail : nullptr, g, tile,
+                          tile < g.T ? d.texcl[gu] : 0u, (float)q.param, static_cast<float*>(q.dst));
+        return;
+    }
     if (tile >= g.T) {
         decode_tail(g.mode == 0 ? q.tail : nullptr, g, reinterpret_cast<uint16_t*>(d.staging + (size_t)u0 * 4096));
         return;
@@ -417,9 +425,8 @@ __global__ __launch_bounds__(kWG) void k_batch_recon(BatchDev d, uint32_t r_lo)
     const uint16_t* codes = reinterpret_cast<const uint16_t*>(d.staging + (size_t)F.ustart * 4096);
     float*          out = static_cast<float*>(q.dst);
     const float     ebx2 = (float)q.param;
-    if constexpr (NDIM == 1) {
-        recon_1d(codes, out, g, ebx2, local);
-    } else if constexpr (NDIM == 2) {
+    static_assert(NDIM == 2 || NDIM == 3, "1D fields are reconstructed by k_batch_decode");
+    if constexpr (NDIM == 2) {
         const uint32_t rx = F.rdims[0];
         recon_2d(codes, out, g, ebx2, local % rx, local / rx);
     } else {
@@ -438,7 +445,7 @@ hipError_t launch_batch_decompress(const BatchDev& d, const BatchShape& b, hipEv
             continue;
         }
         if (nd == 1)
-            e = launch_pair(k_batch_recon<1>, dim3(n), dim3(kWG), s, ev, 2, d, b.r_lo[1]);
+            e = skip_pair(s, ev, 2);  // reconstructed by k_batch_decode
         else if (nd == 2)
             e = launch_pair(k_batch_recon<2>, dim3(n), dim3(kWG), s, ev, 3, d, b.r_lo[2]);
         else

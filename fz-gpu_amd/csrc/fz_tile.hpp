@@ -782,22 +782,31 @@ __device__ __forceinline__ void tail_unit_w(const float* __restrict__ in, const 
 
 // ---------------------------------------------------------------------------- decompress tiles
 
-// Scatter a tile's nonzero blocks back (block i at payload + 4W + 16 (excl + rank)), unshuffle,
-// store its 1024 code words to out_words; one wave (64 threads).  Lane l owns blocks 4l .. 4l + 3
+// sign-magnitude code -> delta
+__device__ __forceinline__ int32_t decode(uint32_t c)
+{
+    const int32_t m = (int32_t)(c & 0x7fffu);
+    return (c & 0x8000u) ? -m : m;
+}
+
+// Scatter a tile's nonzero blocks back (block i at payload + 4W + 16 (excl + rank)) and
+// unshuffle, in one wave (64 threads): lane l ends with words 4 (l + 64 v) .. + 3 of the tile in
+// words[v], v < 4.  Lane l owns blocks 4l .. 4l + 3
 // (flag bits 4(l % 8) .. + 3 of word l / 8), puts them in LDS as rows of bit planes, then
 // unshuffles the words of threads l + 64v, v < 4, of a 256-thread layout (8 lanes per 32×32 bit
-// transpose).  A tile with no nonzero block stores zeros without touching LDS (wave-uniform).
-__device__ __forceinline__ void decode_tile_w(const uint8_t* __restrict__ payload, uint32_t W, uint32_t tile,
-                                              uint32_t excl, uint32_t* __restrict__ out_words)
+// transpose).  A tile with no nonzero block gives zeros without touching LDS (wave-uniform).
+__device__ __forceinline__ void decode_tile_w_regs(const uint8_t* __restrict__ payload, uint32_t W, uint32_t tile,
+                                                   uint32_t excl, uint32_t words[4][4])
 {
     __shared__ uint32_t sh[32 * kLdsPitch];
     const uint32_t l = threadIdx.x;
     const uint32_t word = reinterpret_cast<const uint32_t*>(payload)[8 * tile + (l >> 3)];
     const uint32_t nib = (word >> (4 * (l & 7))) & 0xFu;
-    uint4* const   out4 = reinterpret_cast<uint4*>(out_words);
     if (ballot64(nib != 0) == 0) {
 #pragma unroll
-        for (int v = 0; v < 4; v++) out4[l + 64 * v] = make_uint4(0, 0, 0, 0);
+        for (int v = 0; v < 4; v++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) words[v][q] = 0;
         return;
     }
     const uint32_t cnt = __popc(nib);
@@ -824,11 +833,76 @@ __device__ __forceinline__ void decode_tile_w(const uint8_t* __restrict__ payloa
 #pragma unroll
     for (int v = 0; v < 4; v++) {
         const uint32_t k = (l >> 3) + 8 * v;
-        uint32_t       w[4];
 #pragma unroll
-        for (int q = 0; q < 4; q++) w[q] = sh[kLdsPitch * (4 * L + q) + k];
-        transpose4(w, L);
-        out4[l + 64 * v] = make_uint4(w[0], w[1], w[2], w[3]);
+        for (int q = 0; q < 4; q++) words[v][q] = sh[kLdsPitch * (4 * L + q) + k];
+        transpose4(words[v], L);
+    }
+}
+
+// The tile's 1024 code words to out_words: words[v] of lane l are words 4 (l + 64 v) .. + 3.
+__device__ __forceinline__ void decode_tile_w(const uint8_t* __restrict__ payload, uint32_t W, uint32_t tile,
+                                              uint32_t excl, uint32_t* __restrict__ out_words)
+{
+    uint32_t words[4][4];
+    decode_tile_w_regs(payload, W, tile, excl, words);
+    uint4* const out4 = reinterpret_cast<uint4*>(out_words);
+#pragma unroll
+    for (int v = 0; v < 4; v++) out4[threadIdx.x + 64 * v] = make_uint4(words[v][0], words[v][1], words[v][2], words[v][3]);
+}
+
+// 1D fields: decode and reconstruct a unit in one wave, straight to floats (the 256-code
+// Lorenzo blocks of 1D lie inside one 2048-code tile, lorenzo_var.cuh:47-108 inverse).  Lane l
+// holds codes 8 (l + 64 v) .. + 7 of the unit for v < 4, so lanes 0-31 and 32-63 each cover
+// one block: prefix in the thread, exclusive scan over the 32 lanes, x' = s * ebx2 for the
+// positions below x*y*z.  A full tile decodes from the payload; the unit after the last full
+// tile takes the trailer's codes (0 without one).
+__device__ __forceinline__ void decode_recon_1d_w(const uint8_t* __restrict__ payload, const uint16_t* __restrict__ tail,
+                                                  const Geom& g, uint32_t tile, uint32_t excl, float ebx2,
+                                                  float* __restrict__ out)
+{
+    const uint32_t l = threadIdx.x, t0 = (uint32_t)kTileCodes * tile;
+    uint32_t       words[4][4];
+    if (tile < g.T) {
+        decode_tile_w_regs(payload, g.W, tile, excl, words);
+    } else {
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                uint32_t wv = 0;
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const uint32_t p = t0 + 8 * (l + 64 * v) + 2 * q + h;  // < 2^32: len <= 2^32 - 4096
+                    const uint32_t c = (tail && p < g.len) ? tail[p - t0] : 0u;
+                    wv |= c << (16 * h);
+                }
+                words[v][q] = wv;
+            }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        int32_t s[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) s[i] = decode((words[v][i >> 1] >> (16 * (i & 1))) & 0xFFFFu) + (i ? s[i - 1] : 0);
+        int32_t t = s[7];
+#pragma unroll
+        for (int dd = 1; dd < 32; dd <<= 1) {
+            const int32_t o = __shfl_up(t, dd, 32);
+            if ((l & 31u) >= (uint32_t)dd) t += o;
+        }
+        const int32_t  ex = t - s[7];
+        const uint32_t p0 = t0 + 8 * (l + 64 * v);
+        if (p0 + 8 <= g.xyz) {
+            float4* o4 = reinterpret_cast<float4*>(out + p0);
+            o4[0] = make_float4((float)(ex + s[0]) * ebx2, (float)(ex + s[1]) * ebx2, (float)(ex + s[2]) * ebx2,
+                                (float)(ex + s[3]) * ebx2);
+            o4[1] = make_float4((float)(ex + s[4]) * ebx2, (float)(ex + s[5]) * ebx2, (float)(ex + s[6]) * ebx2,
+                                (float)(ex + s[7]) * ebx2);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; i++)
+                if (p0 + i < g.xyz) out[p0 + i] = (float)(ex + s[i]) * ebx2;
+        }
     }
 }
 
@@ -840,11 +914,6 @@ __device__ __forceinline__ void decode_tail(const uint16_t* __restrict__ tail, c
     for (uint32_t i = threadIdx.x; t0 + i < g.len; i += blockDim.x) codes[t0 + i] = tail ? tail[i] : (uint16_t)0;
 }
 
-__device__ __forceinline__ int32_t decode(uint32_t c)
-{
-    const int32_t m = (int32_t)(c & 0x7fffu);
-    return (c & 0x8000u) ? -m : m;
-}
 
 // Deltas of the reconstruct kernels (the source SRC of recon_*): u16 sign-magnitude codes, or
 // int32 deltas, or the outlier mode's codes with side-channel overrides.  *4 reads positions
