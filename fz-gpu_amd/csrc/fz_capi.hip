@@ -319,6 +319,11 @@ fz_status fz_decompress_f32(const fz_plan plan, const uint8_t* d_payload, const 
     FZ_TRY(launch_tile_scan(reinterpret_cast<const uint32_t*>(d_payload), g.T, texcl, dstate, L.n_dstate,
                             &ctr->dscan_ticket, stream));
     FZ_TRY(prof_mark(plan, stream, false));
+    if (g.ndim == 1) {  // 1D Lorenzo blocks lie inside a tile: one fused launch
+        FZ_TRY(launch_decode_recon_1d(d_payload, texcl, tail, g, ebx2, d_out, stream));
+        FZ_TRY(prof_mark(plan, stream, false));
+        return FZ_SUCCESS;
+    }
     FZ_TRY(launch_decode_unshuffle(d_payload, texcl, tail, g, codes, stream));
     FZ_TRY(prof_mark(plan, stream, false));
     FZ_TRY(launch_reconstruct(reinterpret_cast<const uint16_t*>(codes), d_out, g, ebx2, stream));

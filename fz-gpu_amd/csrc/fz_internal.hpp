@@ -83,6 +83,9 @@ hipError_t launch_tile_scan(const uint32_t* bitflags, uint32_t T, uint32_t* tile
                             uint32_t nstates, uint32_t* ticket, hipStream_t s, bool clear = true);
 hipError_t launch_decode_unshuffle(const uint8_t* payload, const uint32_t* tile_excl, const uint16_t* tail,
                                    const Geom& g, uint32_t* codes, hipStream_t s);
+// 1D fields: decode + reconstruct to floats in one launch (replaces the two above)
+hipError_t launch_decode_recon_1d(const uint8_t* payload, const uint32_t* tile_excl, const uint16_t* tail,
+                                  const Geom& g, float ebx2, float* out, hipStream_t s);
 hipError_t launch_reconstruct(const uint16_t* codes, float* out, const Geom& g, float ebx2, hipStream_t s);
 // reconstruct from u16 codes whose positions flagged in `bits` take their delta from `over`
 hipError_t launch_reconstruct_outlier_src(const uint16_t* codes, const uint32_t* bits, const int32_t* over,

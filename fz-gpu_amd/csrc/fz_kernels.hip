@@ -634,6 +634,24 @@ hipError_t launch_decode_unshuffle(const uint8_t* payload, const uint32_t* tile_
     return hipGetLastError();
 }
 
+// 1D fields: decode and reconstruct in one wave per tile (the unit after the last full tile
+// takes the trailer), straight to floats; no code array in between.
+__global__ __launch_bounds__(64) void k_decode_recon_1d(const uint8_t* __restrict__ payload,
+                                                        const uint32_t* __restrict__ tile_excl,
+                                                        const uint16_t* __restrict__ tail, Geom g, float ebx2,
+                                                        float* __restrict__ out)
+{
+    const uint32_t tile = blockIdx.x;
+    decode_recon_1d_w(payload, tail, g, tile, tile < g.T ? tile_excl[tile] : 0u, ebx2, out);
+}
+
+hipError_t launch_decode_recon_1d(const uint8_t* payload, const uint32_t* tile_excl, const uint16_t* tail,
+                                  const Geom& g, float ebx2, float* out, hipStream_t s)
+{
+    FZ_LAUNCH(k_decode_recon_1d<<<g.T + 1, 64, 0, s>>>(payload, tile_excl, tail, g, ebx2, out));
+    return hipGetLastError();
+}
+
 template <typename SRC>
 __global__ __launch_bounds__(kWG) void k_reconstruct_1d(const SRC codes, float* __restrict__ out, Geom g, float ebx2)
 {
