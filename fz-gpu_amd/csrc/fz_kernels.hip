@@ -604,10 +604,76 @@ __global__ __launch_bounds__(kWG) void k_tile_scan(const uint32_t* __restrict__ 
     if (t < T) tile_excl[t] = sh_excl + woff + incl - cnt;
 }
 
+// The same scan without look-back for up to 256 chunks of 256 tiles: k_tile_sums writes each
+// chunk's block count, k_tile_scan2 adds the counts of the chunks before its own (one per
+// thread, at most 256) to an in-chunk scan.  Nothing to clear and no waiting between
+// workgroups.
+constexpr uint32_t kTileScan2MaxChunks = kWG;
+
+__device__ __forceinline__ uint32_t tile_popcount(const uint32_t* __restrict__ bitflags, uint32_t t)
+{
+    const uint4 a = *reinterpret_cast<const uint4*>(bitflags + 8 * (size_t)t);
+    const uint4 b = *reinterpret_cast<const uint4*>(bitflags + 8 * (size_t)t + 4);
+    return __popc(a.x) + __popc(a.y) + __popc(a.z) + __popc(a.w) + __popc(b.x) + __popc(b.y) + __popc(b.z) +
+           __popc(b.w);
+}
+
+// exclusive prefix of v over the kWG threads of the workgroup, and the total
+__device__ __forceinline__ uint32_t block_excl(uint32_t v, uint32_t& total)
+{
+    __shared__ uint32_t sh_w[kWG / 64];
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t incl = wave_incl_scan(v);
+    if (lane == 63) sh_w[wv] = incl;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kWG / 64; i++) {
+        pre += (uint32_t)i < wv ? sh_w[i] : 0u;
+        tot += sh_w[i];
+    }
+    total = tot;
+    return pre + incl - v;
+}
+
+__global__ __launch_bounds__(kWG) void k_tile_sums(const uint32_t* __restrict__ bitflags, uint32_t T,
+                                                   uint32_t* __restrict__ chunk_sums)
+{
+    const uint32_t t = blockIdx.x * kWG + threadIdx.x;
+    uint32_t       total;
+    (void)block_excl(t < T ? tile_popcount(bitflags, t) : 0u, total);
+    if (threadIdx.x == 0) chunk_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kWG) void k_tile_scan2(const uint32_t* __restrict__ bitflags, uint32_t T,
+                                                    const uint32_t* __restrict__ chunk_sums,
+                                                    uint32_t* __restrict__ tile_excl)
+{
+    __shared__ uint32_t sh_base;
+    const uint32_t t = blockIdx.x * kWG + threadIdx.x;
+    const uint32_t cnt = t < T ? tile_popcount(bitflags, t) : 0u;
+    uint32_t       base;
+    (void)block_excl(threadIdx.x < blockIdx.x ? chunk_sums[threadIdx.x] : 0u, base);
+    if (threadIdx.x == 0) sh_base = base;
+    __syncthreads();  // block_excl's LDS is reused below
+    uint32_t       total;
+    const uint32_t excl = block_excl(cnt, total);
+    if (t < T) tile_excl[t] = sh_base + excl;
+}
+
 hipError_t launch_tile_scan(const uint32_t* bitflags, uint32_t T, uint32_t* tile_excl, uint64_t* states,
                             uint32_t nstates, uint32_t* ticket, hipStream_t s, bool clear)
 {
     if (T == 0) return hipSuccess;
+    const uint32_t nchunks = (T + kWG - 1) / kWG;
+    if (nchunks <= kTileScan2MaxChunks) {  // the state words hold the chunk counts
+        uint32_t* sums = reinterpret_cast<uint32_t*>(states);
+        FZ_LAUNCH(k_tile_sums<<<nchunks, kWG, 0, s>>>(bitflags, T, sums));
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        FZ_LAUNCH(k_tile_scan2<<<nchunks, kWG, 0, s>>>(bitflags, T, sums, tile_excl));
+        return hipGetLastError();
+    }
     const hipError_t e = clear ? launch_scan_init(states, nstates, ticket, s) : hipSuccess;
     if (e != hipSuccess) return e;
     FZ_LAUNCH(k_tile_scan<<<(T + kWG - 1) / kWG, kWG, 0, s>>>(bitflags, T, tile_excl, states, ticket));
