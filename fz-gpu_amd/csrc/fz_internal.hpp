@@ -77,8 +77,10 @@ hipError_t launch_scan_init(uint64_t* states, uint32_t nstates, uint32_t* ticket
 hipError_t launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* states, uint32_t* ticket,
                        bool write_total, fz_result* res, uint32_t W, hipStream_t s);
 hipError_t launch_encode(const uint8_t* in, uint8_t* out, const uint32_t* presum, uint32_t nblocks, hipStream_t s);
-// tile_excl[t] = nonzero blocks of tiles < t (look-back scan; with `clear`, a launch first clears
-// its nstates states and the ticket, else the caller has)
+// tile_excl[t] = nonzero blocks of tiles < t.  With `clear`: up to 65536 tiles two kernels
+// (chunk counts, then chunk scans; the states hold the counts), beyond that a clearing launch
+// and the look-back scan.  Without `clear` the caller has cleared the nstates states and the
+// ticket: the look-back scan alone.
 hipError_t launch_tile_scan(const uint32_t* bitflags, uint32_t T, uint32_t* tile_excl, uint64_t* states,
                             uint32_t nstates, uint32_t* ticket, hipStream_t s, bool clear = true);
 hipError_t launch_decode_unshuffle(const uint8_t* payload, const uint32_t* tile_excl, const uint16_t* tail,

@@ -666,7 +666,9 @@ hipError_t launch_tile_scan(const uint32_t* bitflags, uint32_t T, uint32_t* tile
 {
     if (T == 0) return hipSuccess;
     const uint32_t nchunks = (T + kWG - 1) / kWG;
-    if (nchunks <= kTileScan2MaxChunks) {  // the state words hold the chunk counts
+    // with the states already cleared by the caller (a launch it needed anyway) the look-back
+    // scan is one launch; otherwise the two-kernel scan saves the clearing launch
+    if (clear && nchunks <= kTileScan2MaxChunks) {  // the state words hold the chunk counts
         uint32_t* sums = reinterpret_cast<uint32_t*>(states);
         FZ_LAUNCH(k_tile_sums<<<nchunks, kWG, 0, s>>>(bitflags, T, sums));
         hipError_t e = hipGetLastError();
