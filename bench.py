@@ -3,7 +3,7 @@
 
 A step is one compress of one field through the fused HIP path (device value range ->
 fused dual-quant / bitshuffle / flags / look-back scan / encode), with the input already
-resident in HBM.  The default workload is the 39200x388 detector-like field at eb = 1e-3
+resident in HBM.  The default workload is the 3600x1800 CESM-like 2D field at eb = 1e-3
 (value-range relative).  Inputs rotate over enough copies (>= 1.2 GB) that each step reads
 its field from HBM, not from the 256 MB MALL.
 
@@ -21,8 +21,16 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "fz-gpu_amd"))
+# thread placement of the CPU baseline's OpenMP runtime, fixed before any library starts one
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 HBM_PEAK_GBPS = 8000.0
+WORKLOAD_TEXT = {
+    "cesm-like-3600x1800": "CESM-like 2D field: 32 plane waves, 1/k amplitudes, noise, 15% exact zeros in blobs",
+    "detector-39200x388": "detector-like field, 97% exact zeros, Gaussian peaks + noisy rows",
+    "blobs3d-500x500x100": "3D Gaussian blobs, 70% exact zeros",
+}
 METRIC = "compress GB/s + ratio at eb=1e-3 rel; batched vs per-field; p99 latency"
 
 
@@ -121,7 +129,7 @@ def _graph_of(fn):
     return g
 
 
-def bench_batch(steps, warmup, nfields=256):
+def bench_batch(steps, warmup, nfields=256, host_out=None):
     """The uneven batch: nfields fields (1D/2D/3D, 64K-64M floats, eb in {1e-2,1e-3,1e-4}),
     generated on the device.  Batched = one captured graph of the batched kernels; baseline =
     the plain per-field pipeline, launched field by field from the host and also captured as
@@ -220,15 +228,20 @@ def bench_batch(steps, warmup, nfields=256):
         "batched_decompress_ms": round(ms_dec, 4), "batched_decompress_GBps": gbps(ms_dec),
         "decompress_mismatched_fields": dec_mismatch,
     }
+    if host_out is not None:  # host copies of the fields for the CPU baseline's batch loop
+        host_out.extend((t.cpu().numpy(), s, e) for t, s, e in zip(d, shapes, ebs))
     del g, g_plain, g_fused, plans, outs, out, plan, d
     torch.cuda.empty_cache()
     return r
 
 
-def bench_serving(rates=(50, 100, 200, 500, 1000, 2000), duration_s=1.0, pool_fields=48):
+def bench_serving(rates=(50, 100, 200, 500, 1000, 2000, 20000), duration_s=1.0, pool_fields=48, min_requests=150):
     """Poisson request streams (80% compress / 20% decompress of archives already served)
     over a device-resident pool of fields log-uniform in 256 KB - 1 GB, under a 200 GB HBM
-    budget; latency is arrival to completion."""
+    budget; latency is arrival to completion.  Each rate runs for max(duration_s,
+    min_requests / rate) seconds of arrivals, so every p99 rests on >= 100 samples; the last
+    rate offers more work than the device serves (~2 TB/s of input at a ~130 MB mean field),
+    so requests queue and share batches."""
     import torch
 
     from fzgpu import serve as S
@@ -250,11 +263,12 @@ def bench_serving(rates=(50, 100, 200, 500, 1000, 2000), duration_s=1.0, pool_fi
     warm2.run([S.Request(i, "decompress", a.dims, 1e-3, a, 0.0) for i, a in enumerate(arch)], realtime=False)
     del warm, warm2, arch
     for k, rate in enumerate(rates):
-        reqs = S.make_requests(rate, duration_s, pool, seed=3000 + k)
+        reqs = S.make_requests(rate, max(duration_s, min_requests / rate) if rate < 10000 else 0.25, pool,
+                               seed=3000 + k)
         srv = S.Server(be, hbm_budget_bytes=200e9, slots=3, max_fields=256)
         srv.run(reqs, realtime=True, on_arrival=S.pick_archive(4000 + k))
         out[f"rate_{rate}"] = S.summarize(srv, reqs)
-        log(f"  rate {rate}/s: {out[f'rate_{rate}']['latency']}")
+        log(f"  rate {rate}/s: {out[f'rate_{rate}']['latency']} batches {out[f'rate_{rate}']['batch_size_hist']}")
         del srv, reqs
         torch.cuda.empty_cache()
     out["plan_pool_GB"] = round(be.pool_bytes / 1e9, 3)
@@ -385,20 +399,106 @@ def bench_staged(x, dims, eb, nreq=24, depth=3, nbuf=4):
     return res
 
 
-def cpu_baseline(x, dims, eb, seconds):
+def _host_info():
+    import platform
+    import subprocess
+    model = platform.processor() or ""
+    try:
+        for ln in subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout.splitlines():
+            if ln.lower().startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)), "lscpu_model": model,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"), "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"),
+            "OMP_PLACES": os.environ.get("OMP_PLACES")}
+
+
+def _median_time(fn, reps, warmup=1):
+    for _ in range(warmup):
+        fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    return ts[len(ts) // 2]
+
+
+def cpu_baseline(workload, batch_fields=None, reps=5):
+    """The CPU side of the comparison: the oracle (a single-threaded restatement of the
+    reference kernels) and the same code under OpenMP on the host's cores, compress (range +
+    dual-quant + bitshuffle / flags + scan + encode) and decompress, no file I/O; the median of
+    `reps` timed runs after one warm-up, on the three single-field workloads (3D at both error
+    bounds) and on the uneven batch run as a loop over its fields.  batch_fields: host arrays
+    [(x, dims, eb)] of the batch (None skips it)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+
     import oracle as O
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        r = O.compress(x, dims, eb, O.MODE_FIXED, nthreads=threads)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or reps >= 200:
-            break
-    assert r["status"] == 0
-    return dict(value=round(4 * x.size * reps / el / 1e9, 4), unit="GB/s", cores=threads, kind="port",
-                sample=f"{reps} full-field OpenMP oracle compress passes over the same field ({el:.1f} s)")
+    from fzgpu import datagen as D
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    table = {}
+
+    def one(x, dims, eb, nt):
+        r = O.compress(x, dims, eb, O.MODE_FIXED, nthreads=nt)
+        assert r["status"] == 0
+        return r
+
+    for name in D.CONFIGS:
+        x, dims, eb0 = D.config_field(name)
+        x = np.ascontiguousarray(x.ravel())
+        for eb in ([eb0, 1e-4] if dims[2] > 1 else [eb0]):
+            row = {}
+            ref = one(x, dims, eb, threads)
+            rc, xo = O.decompress(ref["payload"], ref["tail"], x.size, dims, ref["info"]["ebx2"], nthreads=threads)
+            assert rc == 0
+            row["CR"] = round(4 * x.size / ref["info"]["payload_bytes"], 4)
+            row["max_err_over_range"] = float(np.abs(xo.astype(np.float64) - x).max() /
+                                              (float(x.max()) - float(x.min())))
+            for nt, tag in ((1, "1t"), (threads, "omp")):
+                tc = _median_time(lambda: one(x, dims, eb, nt), reps)
+                td = _median_time(lambda: O.decompress(ref["payload"], ref["tail"], x.size, dims,
+                                                       ref["info"]["ebx2"], nthreads=nt), reps)
+                row[f"compress_{tag}_GBps"] = round(4 * x.size / tc / 1e9, 4)
+                row[f"decompress_{tag}_GBps"] = round(4 * x.size / td / 1e9, 4)
+            table[f"{name}@{eb:g}"] = row
+            log(f"  cpu {name}@{eb:g}: {row}")
+    if batch_fields:
+        arch = [one(x, dims, eb, threads) for x, dims, eb in batch_fields]
+        nbytes = sum(4 * x.size for x, _, _ in batch_fields)
+
+        def loop_c(nt, fields):
+            for x, dims, eb in fields:
+                one(x, dims, eb, nt)
+
+        def loop_d(nt, idx):
+            for i in idx:
+                x, dims, _ = batch_fields[i]
+                a = arch[i]
+                O.decompress(a["payload"], a["tail"], x.size, dims, a["info"]["ebx2"], nthreads=nt)
+
+        breps = max(1, reps // 2)
+        row = {"fields": len(batch_fields), "input_GB": round(nbytes / 1e9, 3),
+               "compress_omp_GBps": round(nbytes / _median_time(lambda: loop_c(threads, batch_fields), breps) / 1e9, 4),
+               "decompress_omp_GBps": round(nbytes / _median_time(lambda: loop_d(threads, range(len(arch))), breps) /
+                                            1e9, 4)}
+        sub = list(range(0, len(batch_fields), 16))  # 1-thread: every 16th field, one timed pass
+        sb = sum(4 * batch_fields[i][0].size for i in sub)
+        row["compress_1t_GBps"] = round(sb / _median_time(lambda: loop_c(1, [batch_fields[i] for i in sub]), 1, 0) /
+                                        1e9, 4)
+        row["decompress_1t_GBps"] = round(sb / _median_time(lambda: loop_d(1, sub), 1, 0) / 1e9, 4)
+        row["sample"] = (f"OpenMP: all fields, median of {breps} after 1 warm-up; 1 thread: every 16th field "
+                         f"({len(sub)} fields, {sb / 1e9:.2f} GB), one pass")
+        table["uneven-batch-256"] = row
+        log(f"  cpu batch: {row}")
+    head = table[f"{workload}@{D.config_field(workload)[2]:g}"]
+    return dict(value=head["compress_omp_GBps"], unit="GB/s", cores=threads, kind="port",
+                sample=(f"oracle compress of the whole {workload} field under OpenMP, {threads} threads, median "
+                        f"of {reps} after 1 warm-up; per-workload 1-thread / OpenMP compress and decompress in "
+                        f"'table'"),
+                host=_host_info(), table=table)
 
 
 def pmc_target(D):
@@ -493,8 +593,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="detector-39200x388")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--workload", default="cesm-like-3600x1800")
+    ap.add_argument("--cpu-reps", type=int, default=5, help="timed runs per CPU-baseline figure (median)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-field uneven batch")
@@ -594,9 +694,11 @@ def main():
                                                                      HBM_PEAK_GBPS, 4),
                                           "decompress_GBps": round(r["decompress_gbps"], 2),
                                           "outlier_mode_GBps": bench_outlier(xe.ravel(), de, e, args.steps)["GBps"]}
+    batch_host = [] if not args.no_cpu else None
     if not args.no_batch:
         log(f"timing the {args.batch_fields}-field uneven batch")
-        extra["uneven_batch"] = bench_batch(min(args.steps, 20), min(args.warmup, 3), args.batch_fields)
+        extra["uneven_batch"] = bench_batch(min(args.steps, 20), min(args.warmup, 3), args.batch_fields,
+                                            host_out=batch_host)
     if not args.no_serve:
         log("serving loop")
         extra["serving"] = bench_serving()
@@ -606,7 +708,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
-        cpu = cpu_baseline(x, dims, eb, args.cpu_seconds)
+        cpu = cpu_baseline(args.workload, batch_host or None, args.cpu_reps)
+        batch_host = None
     line = {
         "metric": METRIC,
         "value": round(main_r["gbps"], 3),
@@ -619,7 +722,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded; detector-like field, 97% exact zeros, Gaussian peaks + noisy rows)",
+        "data": "synthetic (seeded; " + WORKLOAD_TEXT.get(args.workload, args.workload) + ")",
         "config": {"workload": args.workload, "dims_xyz": list(dims), "eb_rel": eb, "mode": "fixed",
                    "pipeline": "fused (range + tile_compress + scan_gather kernels)",
                    "inputs": "HBM-resident, rotated over copies (MALL-cold)"},

@@ -68,8 +68,14 @@ inline bool read_archive(const std::string& path, Header& h, std::vector<uint8_t
 {
     FILE* f = std::fopen(path.c_str(), "rb");
     if (!f) return false;
-    bool ok = std::fread(&h, sizeof(h), 1, f) == 1 && std::memcmp(h.magic, "FZMI", 4) == 0 && h.version == 1 &&
-              h.mode <= kModeOutlier;
+    long fsize = -1;
+    if (std::fseek(f, 0, SEEK_END) == 0) fsize = std::ftell(f);
+    std::rewind(f);
+    bool ok = fsize >= (long)sizeof(h) && std::fread(&h, sizeof(h), 1, f) == 1 &&
+              std::memcmp(h.magic, "FZMI", 4) == 0 && h.version == 1 && h.mode <= kModeOutlier;
+    // sizes the file cannot hold are refused before anything is allocated for them
+    const uint64_t rest = ok ? (uint64_t)fsize - sizeof(h) : 0;
+    ok = ok && h.payload_bytes <= rest && h.tail_len <= (rest - h.payload_bytes) / 2;
     if (ok) {
         payload.resize(h.payload_bytes);
         tail.resize(h.tail_len);
@@ -79,11 +85,13 @@ inline bool read_archive(const std::string& path, Header& h, std::vector<uint8_t
     if (ok && h.mode == kModeOutlier) {  // side channel: its 16-byte header gives its size
         uint64_t n[2];
         ok = side && std::fread(n, 8, 2, f) == 2 && n[0] <= h.len && n[1] <= h.len;
+        const long at = ok ? std::ftell(f) : -1;
+        ok = ok && at >= 0 && 8 * (n[0] + n[1]) <= (uint64_t)(fsize - at);
         if (ok) {
             side->resize(16 + 8 * (n[0] + n[1]));
             std::memcpy(side->data(), n, 16);
-            const size_t rest = side->size() - 16;
-            ok = rest == 0 || std::fread(side->data() + 16, 1, rest, f) == rest;
+            const size_t lists = side->size() - 16;
+            ok = lists == 0 || std::fread(side->data() + 16, 1, lists, f) == lists;
         }
     }
     std::fclose(f);

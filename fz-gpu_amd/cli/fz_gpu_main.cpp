@@ -93,6 +93,9 @@ static int decompress_main(const std::string& arc, const std::string& out, const
     FZCHK(fz_plan_create(&plan, h.x, h.y, h.z, h.len, h.mode));
     fz_sizes sz;
     FZCHK(fz_plan_sizes(plan, &sz));
+    // the header's sizes and the payload's bitflags must agree before anything reaches the GPU
+    FZCHK(fz_payload_check(plan, payload.data(), h.payload_bytes, h.tail_len));
+    if (h.mode == FZ_MODE_OUTLIER) FZCHK(fz_side_check(plan, side.data(), side.size()));
     const uint64_t n = (uint64_t)h.x * h.y * h.z;
     uint8_t*       d_payload;
     uint16_t*      d_tail = nullptr;

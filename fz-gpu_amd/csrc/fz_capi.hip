@@ -22,6 +22,13 @@ struct Layout {
     uint32_t n_tstate, n_sstate, n_dstate;
 };
 
+// Workspace regions, 256-byte aligned.  State words are shared between kernels of one call:
+//   tstate  fused compress: per-placement-run block sums (u32), cleared by k_range_part;
+//   sstate  u32 flag scan of the plain pipeline: look-back states, cleared by the range kernel;
+//   dstate  decompress tile scan: look-back states (u64 flag|value), or with the two-kernel
+//           form (<= 65536 tiles) raw u32 chunk counts.  Its contents are undefined after any
+//           decompress, so every look-back user clears it first (launch_tile_scan with clear,
+//           or k_put_deltas in the outlier inverse).
 Layout make_layout(const Geom& g, int mode)
 {
     Layout L{};
@@ -308,6 +315,7 @@ fz_status fz_decompress_f32(const fz_plan plan, const uint8_t* d_payload, const 
     if (!(ebx2 > 0.0f) || !std::isfinite(ebx2)) return FZ_ERR_INVALID_ARG;
     if (ws_bytes < plan->L.total) return FZ_ERR_WORKSPACE_TOO_SMALL;
     const Geom&   g = plan->g;
+    if (plan->mode != FZ_MODE_LITERAL && !d_tail && g.len != (uint64_t)kTileCodes * g.T) return FZ_ERR_INVALID_ARG;
     const Layout& L = plan->L;
     uint8_t*      ws = static_cast<uint8_t*>(d_ws);
     Ctr*          ctr = reinterpret_cast<Ctr*>(ws + L.ctr);
@@ -362,6 +370,7 @@ fz_status fz_decompress_outlier_f32(const fz_plan plan, const uint8_t* d_payload
         return FZ_ERR_INVALID_ARG;
     if (ws_bytes < plan->L.total) return FZ_ERR_WORKSPACE_TOO_SMALL;
     const Geom&   g = plan->g;
+    if (ebx2 != 0.0f && !d_tail && g.len != (uint64_t)kTileCodes * g.T) return FZ_ERR_INVALID_ARG;
     const Layout& L = plan->L;
     uint8_t*      ws = static_cast<uint8_t*>(d_ws);
     FZ_TRY(prof_mark(plan, stream, true));
@@ -384,6 +393,45 @@ fz_status fz_decompress_outlier_f32(const fz_plan plan, const uint8_t* d_payload
     FZ_TRY(launch_reconstruct_outlier(reinterpret_cast<const uint16_t*>(codes), obits, over, d_side, side_bytes, d_out,
                                       g, ebx2, stream));
     FZ_TRY(prof_mark(plan, stream, false));
+    return FZ_SUCCESS;
+}
+
+// Host checks of an archive's payload and side channel: the oracle's corrupt-input rules
+// (fz_oracle.c decode_codes / fzo_decompress_outlier), run before anything is copied to the
+// device.
+fz_status fz_payload_check(const fz_plan plan, const uint8_t* h_payload, uint64_t payload_bytes, uint64_t tail_len)
+{
+    if (!plan || (!h_payload && payload_bytes)) return FZ_ERR_INVALID_ARG;
+    const Geom&    g = plan->g;
+    const uint64_t W = g.W, T = g.T;
+    if (payload_bytes < 4 * W) return FZ_ERR_CORRUPT;
+    if (payload_bytes > 4 * W + 4096 * T) return FZ_ERR_CORRUPT;
+    if (plan->mode != FZ_MODE_LITERAL && tail_len != (uint64_t)g.len - (uint64_t)kTileCodes * T) return FZ_ERR_CORRUPT;
+    uint64_t K = 0;
+    for (uint64_t i = 0; i < W; i++) {
+        uint32_t v;
+        std::memcpy(&v, h_payload + 4 * i, 4);  // little-endian host
+        if (i >= 8 * T && v) return FZ_ERR_CORRUPT;  // flags of the partial tile are never set
+        K += (uint64_t)__builtin_popcount(v);
+    }
+    return payload_bytes == 4 * W + 16 * K ? FZ_SUCCESS : FZ_ERR_CORRUPT;
+}
+
+fz_status fz_side_check(const fz_plan plan, const uint8_t* h_side, uint64_t side_bytes)
+{
+    if (!plan || plan->mode != FZ_MODE_OUTLIER || (!h_side && side_bytes)) return FZ_ERR_INVALID_ARG;
+    if (side_bytes < 16) return FZ_ERR_CORRUPT;
+    uint64_t n[2];
+    std::memcpy(n, h_side, 16);
+    const uint64_t xyz = plan->g.xyz;
+    if (n[0] > xyz || n[1] > xyz || side_bytes != 16 + 8 * (n[0] + n[1])) return FZ_ERR_CORRUPT;
+    const uint8_t* idx_d = h_side + 16;
+    const uint8_t* idx_r = h_side + 16 + 8 * n[0];
+    for (uint64_t k = 0; k < n[0] + n[1]; k++) {
+        uint32_t v;
+        std::memcpy(&v, (k < n[0] ? idx_d + 4 * k : idx_r + 4 * (k - n[0])), 4);
+        if (v >= xyz) return FZ_ERR_CORRUPT;
+    }
     return FZ_SUCCESS;
 }
 

@@ -70,7 +70,7 @@ EXPORTS = ["fz_version", "fz_status_string", "fz_last_error", "fz_plan_create", 
            "fz_batch_compress_graph", "fz_batch_decompress_graph", "fz_graph_launch", "fz_graph_destroy",
            "fz_verify_f32", "fz_decompress_verify_f32", "fz_compress_outlier_f32", "fz_decompress_outlier_f32",
            "fz_host_alloc", "fz_host_free", "fz_stager_create", "fz_stager_destroy", "fz_stager_sizes",
-           "fz_stager_compress", "fz_stager_decompress", "fz_stager_wait"]
+           "fz_stager_compress", "fz_stager_decompress", "fz_stager_wait", "fz_payload_check", "fz_side_check"]
 
 
 class _StatsRange(ctypes.Structure):
@@ -195,6 +195,8 @@ def lib():
     L.fz_stager_wait.argtypes = [vp, u64, ctypes.POINTER(Result), ctypes.POINTER(vp), ctypes.POINTER(vp)]
     L.fz_decompress_outlier_f32.argtypes = [vp, vp, vp, f32, f32, vp, u64, vp, ctypes.c_size_t, vp, vp]
     L.fz_decompress_verify_f32.argtypes = [vp, vp, vp, f32, vp, ctypes.c_size_t, vp, vp, vp, vp]
+    L.fz_payload_check.argtypes = [vp, vp, u64, u64]
+    L.fz_side_check.argtypes = [vp, vp, u64]
     for name in EXPORTS:
         if name not in ("fz_version", "fz_status_string", "fz_last_error"):
             getattr(L, name).restype = i32
@@ -298,6 +300,22 @@ class Plan:
         if r["status"] != 0:
             raise FzError(r["status"], "compress")
         return payload[: r["payload_bytes"]], tail[: r["tail_len"]], r
+
+    def check_payload(self, payload, tail_len):
+        """fz_payload_check on host bytes (numpy uint8 array or bytes): raises FzError(6) when the
+        payload's size, bitflags or the tail length disagree with this plan."""
+        import numpy as np
+        b = np.ascontiguousarray(np.frombuffer(payload, dtype=np.uint8) if isinstance(payload, (bytes, bytearray))
+                                 else np.asarray(payload, dtype=np.uint8))
+        _check(lib().fz_payload_check(self.handle, b.ctypes.data_as(ctypes.c_void_p) if b.size else None, b.size,
+                                      int(tail_len)), "fz_payload_check")
+
+    def check_side(self, side):
+        """fz_side_check on a host side channel (OUTLIER plans)."""
+        import numpy as np
+        b = np.ascontiguousarray(np.asarray(side, dtype=np.uint8))
+        _check(lib().fz_side_check(self.handle, b.ctypes.data_as(ctypes.c_void_p) if b.size else None, b.size),
+               "fz_side_check")
 
     def decompress(self, payload, tail, ebx2, out=None, stream=None):
         import torch

@@ -54,10 +54,21 @@ class Archive:
     tail: object
     record: dict
     keep: object = None       # the batch buffers the views above live in
+    buffer: object = None     # the server's OutputBuffer accounting for `keep`
+    pins: int = 0             # waiting or running decompress requests that read this archive
 
     @property
     def nbytes(self):
         return int(self.record["payload_bytes"]) + 2 * int(self.record["tail_len"])
+
+
+class OutputBuffer:
+    """Device outputs of one finished compress batch.  Its archives are views into one
+    allocation, so its bytes stay in the budget until the last of them is dropped."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.refs = 0
 
 
 class HbmBudget:
@@ -108,7 +119,7 @@ class Server:
         self.backend = backend
         self.budget = HbmBudget(hbm_budget_bytes)
         self.max_archives = int(max_archives)
-        self.archives = collections.deque()  # (request, bytes) of compress results kept on the device
+        self.archives = collections.deque()  # (request, OutputBuffer) of compress results kept on the device
         self.slots = int(slots)
         self.max_fields = int(max_fields)
         self.clock = clock
@@ -119,6 +130,7 @@ class Server:
         self.completed = []
         self.batches = []         # (batch id, kind, n requests, input bytes, launch time, finish time)
         self.max_inflight = 0
+        self.budget_limited = 0   # batches cut short because the next request did not fit the budget
 
     def now(self):
         return self.clock() - self.t0
@@ -126,6 +138,8 @@ class Server:
     def submit(self, req):
         if sum(request_bytes(req)) > self.budget.total:
             raise ValueError(f"request {req.rid} needs more device memory than the whole budget")
+        if req.kind == "decompress" and isinstance(req.source, Archive):
+            req.source.pins += 1  # not evicted while this request waits or runs
         self.waiting[req.kind].append(req)
 
     def _form(self, kind):
@@ -135,20 +149,31 @@ class Server:
         take, nbytes = [], 0
         while q and len(take) < self.max_fields:
             need = sum(request_bytes(q[0]))
-            while not self.budget.fits(nbytes + need) and self.archives:
-                self._evict()
+            while not self.budget.fits(nbytes + need) and self._evict():
+                pass
             if not self.budget.fits(nbytes + need):
+                self.budget_limited += 1
                 break
             take.append(q.popleft())
             nbytes += need
         return take, nbytes
 
+    def _drop_ref(self, buf):
+        buf.refs -= 1
+        if buf.refs == 0:
+            self.budget.give(buf.nbytes)
+
     def _evict(self):
-        # a decompress request that already picked this archive keeps its buffers alive until
-        # it finishes (they are counted once, here, until eviction)
-        req, nbytes = self.archives.popleft()
-        req.result = None
-        self.budget.give(nbytes)
+        """Drops the oldest archive no decompress request is reading; its batch's output
+        buffer returns to the budget with its last archive.  False when nothing can go."""
+        for i, (req, buf) in enumerate(self.archives):
+            if req.result is not None and req.result.pins > 0:
+                continue
+            del self.archives[i]
+            req.result = None
+            self._drop_ref(buf)
+            return True
+        return False
 
     def archive_pool(self):
         """Compress results currently kept on the device."""
@@ -188,15 +213,25 @@ class Server:
                 for r in reqs:
                     r.done = t
                 self.completed.extend(reqs)
-                self.budget.give(nbytes)
-                if kind == "compress" and self.max_archives > 0:  # archives stay on the device, in the budget
+                out_bytes = sum(request_bytes(r)[1] for r in reqs) if kind == "compress" else 0
+                self.budget.give(nbytes - out_bytes)  # the workspace share; decompress outputs go to the caller
+                if kind == "decompress":
                     for r in reqs:
-                        out = request_bytes(r)[1]
-                        while self.archives and (len(self.archives) >= self.max_archives or not self.budget.fits(out)):
-                            self._evict()
-                        if self.budget.fits(out):
-                            self.budget.take(out)
-                            self.archives.append((r, out))
+                        if isinstance(r.source, Archive):
+                            r.source.pins -= 1
+                elif self.max_archives > 0:  # archives stay on the device, their buffer in the budget
+                    buf = OutputBuffer(out_bytes)
+                    buf.refs = 1  # held until every archive of the batch is registered
+                    for r in reqs:
+                        if isinstance(r.result, Archive):
+                            r.result.buffer = buf
+                            buf.refs += 1
+                            self.archives.append((r, buf))
+                    self._drop_ref(buf)
+                    while len(self.archives) > self.max_archives and self._evict():
+                        pass
+                else:
+                    self.budget.give(out_bytes)
                 self.batches[bid][5] = t
                 done_any = True
             else:
@@ -235,8 +270,10 @@ def latency_stats(reqs):
     lat = np.array([r.latency for r in reqs], dtype=np.float64) * 1e3
     if lat.size == 0:
         return {"n": 0}
+    # a 99th percentile of fewer than 100 samples is only their maximum: not reported
+    p99 = round(float(np.percentile(lat, 99)), 4) if lat.size >= 100 else None
     return {"n": int(lat.size), "p50_ms": round(float(np.percentile(lat, 50)), 4),
-            "p99_ms": round(float(np.percentile(lat, 99)), 4), "mean_ms": round(float(lat.mean()), 4),
+            "p99_ms": p99, "mean_ms": round(float(lat.mean()), 4),
             "max_ms": round(float(lat.max()), 4)}
 
 
@@ -430,6 +467,7 @@ def summarize(server, reqs):
     t_start = min(r.arrival for r in done) if done else 0.0
     nbytes = sum(r.input_bytes for r in done)
     sizes = [b[2] for b in server.batches]
+    hist = collections.Counter(sizes)
     return {
         "requests": len(done), "compress": sum(r.kind == "compress" for r in done),
         "decompress": sum(r.kind == "decompress" for r in done),
@@ -438,5 +476,8 @@ def summarize(server, reqs):
         "latency_decompress": latency_stats([r for r in done if r.kind == "decompress"]),
         "GBps": round(nbytes / max(t_end - t_start, 1e-9) / 1e9, 2),
         "batches": len(server.batches), "mean_batch": round(float(np.mean(sizes)), 2) if sizes else 0,
+        "max_batch": max(sizes) if sizes else 0,
+        "batch_size_hist": {str(k): hist[k] for k in sorted(hist)},
+        "budget_limited_batches": server.budget_limited,
         "max_inflight_batches": server.max_inflight, "hbm_peak_GB": round(server.budget.peak / 1e9, 3),
     }

@@ -114,9 +114,12 @@ fz_status fz_plan_sizes(const fz_plan plan, fz_sizes* sizes);
 fz_status fz_workspace_init(const fz_plan plan, void* d_ws, size_t ws_bytes, hipStream_t stream);
 
 /* Fused single-field compress (replaces runSzs, fz.cu:113-209, without its file I/O and
- * host round trips): device value range -> eb -> one fused dual-quant/bitshuffle/flag/
- * look-back-scan/encode kernel.  Writes payload (capacity payload_max_bytes), the tail
- * codes (FIXED mode; capacity tail_max_codes) and *d_result.  d_in must be 16-byte aligned. */
+ * host round trips), three kernels: device value range partials -> one kernel per
+ * 2048-code tile doing dual-quant, bitshuffle, flags and in-tile compaction (each tile's
+ * workgroup reduces the range partials to the eb itself) -> scan of the per-tile block counts
+ * and placement of the compacted blocks.  Writes payload (capacity payload_max_bytes), the
+ * tail codes (FIXED mode; capacity tail_max_codes) and *d_result.  d_in must be 16-byte
+ * aligned. */
 fz_status fz_compress_f32(const fz_plan plan, const float* d_in, double eb_rel, void* d_ws, size_t ws_bytes,
                           uint8_t* d_payload, uint16_t* d_tail, fz_result* d_result, hipStream_t stream);
 
@@ -125,7 +128,9 @@ fz_status fz_compress_f32(const fz_plan plan, const float* d_in, double eb_rel, 
 fz_status fz_compress_plain_f32(const fz_plan plan, const float* d_in, double eb_rel, void* d_ws, size_t ws_bytes,
                                 uint8_t* d_payload, uint16_t* d_tail, fz_result* d_result, hipStream_t stream);
 
-/* Inverse: payload (+ tail codes in FIXED mode, NULL otherwise) -> x*y*z floats.
+/* Inverse: payload (+ tail codes in FIXED mode, NULL otherwise; NULL in FIXED mode is
+ * refused when the plan has tail codes) -> x*y*z floats.  The payload is trusted: check an
+ * archive read from outside with fz_payload_check first.
  * ebx2 = fz_result.ebx2 of the compress call.  The inverse of the reference's Lorenzo
  * (x_lorenzo_*, lorenzo_var.cuh:297-518) cannot decode sign-magnitude codes, so its block
  * structure is kept with exact int32 prefix sums and one float multiply by ebx2. */
@@ -148,6 +153,19 @@ fz_status fz_compress_outlier_f32(const fz_plan plan, const float* d_in, double 
 fz_status fz_decompress_outlier_f32(const fz_plan plan, const uint8_t* d_payload, const uint16_t* d_tail, float ebx2,
                                     float vconst, const uint8_t* d_side, uint64_t side_bytes, void* d_ws,
                                     size_t ws_bytes, float* d_out, hipStream_t stream);
+
+/* Host-side checks of an archive's pieces before any of them reaches the device (the
+ * reference never reads an archive back, fz.cu:215, so these have no counterpart there).
+ * fz_payload_check: h_payload (host memory, payload_bytes bytes) must hold at least the W
+ * bitflag words, no flag may be set beyond the full tiles (the reference leaves those words
+ * zero, fz.cu:167), payload_bytes must equal 4 W + 16 popcount(bitflags) (fz.cu:208) and so
+ * be <= payload_max_bytes, and in FIXED and OUTLIER mode tail_len must equal
+ * tail_max_codes (LITERAL mode ignores it).  fz_side_check: an OUTLIER-mode side channel of
+ * side_bytes = 16 + 8 (n_delta + n_raw) bytes with both counts and every index below x*y*z.
+ * Both return FZ_ERR_CORRUPT for an inconsistent input and read nothing past the bytes
+ * given. */
+fz_status fz_payload_check(const fz_plan plan, const uint8_t* h_payload, uint64_t payload_bytes, uint64_t tail_len);
+fz_status fz_side_check(const fz_plan plan, const uint8_t* h_side, uint64_t side_bytes);
 
 /* Convenience: runs fz_compress_f32, waits for the stream and copies *d_result to *h_result. */
 fz_status fz_compress_f32_sync(const fz_plan plan, const float* d_in, double eb_rel, void* d_ws, size_t ws_bytes,

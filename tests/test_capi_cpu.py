@@ -144,3 +144,133 @@ def test_outlier_mode_sizes_and_batch_refusal():
     with pytest.raises(fzgpu.FzError) as e:
         fzgpu.BatchPlan([(5000, 1, 1)], mode=fzgpu.MODE_OUTLIER, workspace=False)
     assert e.value.status == 1
+
+
+# ---------------------------------------------------------------- archive input checks (host only)
+
+def _oracle_payload(dims, mode=0, seed=5):
+    import numpy as np
+
+    import oracle as O
+    from fzgpu import datagen as D
+    x = D.waves(dims, seed).ravel().astype(np.float32)
+    return O.compress(x, dims, 1e-3, mode)
+
+
+@pytest.mark.parametrize("dims", [(1000, 21, 1), (6000, 1, 1), (20, 20, 20)])
+def test_payload_check_accepts_oracle_payloads(dims):
+    ref = _oracle_payload(dims)
+    p = fzgpu.Plan(*dims, workspace=False)
+    p.check_payload(ref["payload"], len(ref["tail"]))
+
+
+def test_payload_check_refuses_inconsistent_payloads():
+    import numpy as np
+    dims = (1000, 21, 1)
+    ref = _oracle_payload(dims)
+    p = fzgpu.Plan(*dims, workspace=False)
+    good, tl = ref["payload"].copy(), len(ref["tail"])
+    W, T = p.sizes.flag_words, p.sizes.tiles
+    assert tl == p.sizes.tail_max_codes > 0
+
+    def refused(payload, tail_len=tl):
+        with pytest.raises(fzgpu.FzError) as e:
+            p.check_payload(payload, tail_len)
+        assert e.value.status == 6
+
+    refused(good[:-16])                    # one block short
+    refused(good[: 4 * W - 4])             # shorter than the bitflags
+    refused(np.concatenate([good, np.zeros(16, np.uint8)]))  # one block too many
+    flipped = good.copy()
+    b = int(np.flatnonzero(np.unpackbits(good[: 4 * W], bitorder="little") == 0)[0])
+    flipped[b // 8] |= np.uint8(1 << (b % 8))  # a set flag with no block behind it
+    refused(flipped)
+    if W > 8 * T:  # a flag of the partial tile, which the reference never sets
+        tail_flag = good.copy()
+        tail_flag[4 * 8 * T] |= 1
+        refused(np.concatenate([tail_flag, np.zeros(16, np.uint8)]))
+    refused(good, tl - 1)                  # a short tail would make the decode read past it
+    refused(good, 0)
+    # LITERAL mode ignores the tail length
+    lit = fzgpu.Plan(*dims, mode=fzgpu.MODE_LITERAL, workspace=False)
+    ref_l = _oracle_payload(dims, mode=1)
+    lit.check_payload(ref_l["payload"], 0)
+
+
+def test_side_check_follows_the_oracle_rules():
+    import numpy as np
+    dims = (100, 77, 1)
+    p = fzgpu.Plan(*dims, mode=fzgpu.MODE_OUTLIER, workspace=False)
+    xyz = 7700
+
+    def side(idx_d, idx_r):
+        h = np.array([len(idx_d), len(idx_r)], np.uint64).view(np.uint8)
+        body = [np.asarray(idx_d, np.uint32).view(np.uint8), np.zeros(4 * len(idx_d), np.uint8),
+                np.asarray(idx_r, np.uint32).view(np.uint8), np.zeros(4 * len(idx_r), np.uint8)]
+        return np.concatenate([h] + body)
+
+    p.check_side(side([], []))
+    p.check_side(side([0, 5, xyz - 1], [7]))
+    for bad in (side([xyz], []), side([], [xyz + 3]), side([1], [2])[:-4], np.zeros(8, np.uint8)):
+        with pytest.raises(fzgpu.FzError) as e:
+            p.check_side(bad)
+        assert e.value.status == 6
+    with pytest.raises(fzgpu.FzError) as e:  # only OUTLIER plans have a side channel
+        fzgpu.Plan(*dims, workspace=False).check_side(side([], []))
+    assert e.value.status == 1
+
+
+def test_decompress_refuses_a_missing_tail_before_any_launch():
+    L = fzgpu.lib()
+    p = fzgpu.Plan(1000, 21, 1, workspace=False)  # 21000 floats: 10 tiles + 520 tail codes
+    assert p.sizes.tail_max_codes == 520
+    # aligned non-null stand-ins: the argument checks return before anything is dereferenced
+    fake = ctypes.c_void_p(1 << 20)
+    ws = ctypes.c_size_t(p.sizes.workspace_bytes)
+    assert L.fz_decompress_f32(p.handle, fake, None, ctypes.c_float(0.5), fake, ws, fake, None) == 1
+
+
+def _cli():
+    path = os.path.join(ROOT, "fz-gpu_amd", "bin", "fz-gpu")
+    if not os.path.exists(path):
+        pytest.skip("fz-gpu CLI not built")
+    return path
+
+
+def test_cli_refuses_corrupt_archives_before_touching_the_gpu(tmp_path):
+    """`fz-gpu -d` checks the header's sizes against the file and the payload's bitflags against
+    its size on the host, so a damaged archive exits with "corrupt payload" (status 3) before any
+    HIP call: this runs on a machine without a GPU."""
+    import struct
+    import subprocess
+
+    import numpy as np
+    dims = (1000, 21, 1)
+    ref = _oracle_payload(dims)
+    payload, tail = ref["payload"], ref["tail"]
+    n = 21000
+
+    def archive(pb=None, tl=None, body=None):
+        pb = len(payload) if pb is None else pb
+        tl = len(tail) if tl is None else tl
+        h = struct.pack("<4sHBB3II QdfffI QQQ", b"FZMI", 1, 0, 2, *dims, 0, n, 1e-3,
+                        float(ref["info"]["vmin"]), float(ref["info"]["vmax"]), float(ref["info"]["ebx2"]), 0,
+                        pb, tl, 0)
+        assert len(h) == 80
+        body = payload.tobytes() + tail.astype(np.uint16).tobytes() if body is None else body
+        f = tmp_path / "a.fz"
+        f.write_bytes(h + body)
+        return str(f)
+
+    def run(a):
+        return subprocess.run([_cli(), "-d", a, str(tmp_path / "out.f32")], capture_output=True, text=True,
+                              timeout=60)
+
+    cases = [archive(tl=len(tail) - 1, body=payload.tobytes() + tail[:-1].astype(np.uint16).tobytes()),
+             archive(pb=len(payload) - 16, body=payload[:-16].tobytes() + tail.astype(np.uint16).tobytes())]
+    for a in cases:
+        r = run(a)
+        assert r.returncode == 3 and "corrupt payload" in r.stderr, (r.returncode, r.stderr)
+    # sizes the file cannot hold: refused by the reader
+    r = run(archive(pb=1 << 40))
+    assert r.returncode == 2 and "cannot read archive" in r.stderr, (r.returncode, r.stderr)

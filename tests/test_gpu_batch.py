@@ -233,6 +233,20 @@ def test_uneven_batch_full_size_equals_per_field_plain():
         assert torch.equal(out.payloads[i][:n], payload), (i, f)
         assert torch.equal(out.tails[i][: r["tail_len"]], tail), (i, f)
         del single, payload, tail
+    # batched decompress of every field: the error bound, strict where the reference's binary32
+    # arithmetic allows it (slack 4 * 2^-24 * max|x|, as for the single-field workloads)
+    outs = plan.decompress([out.payloads[i][: res[i]["payload_bytes"]] for i in range(len(spec))],
+                           [out.tails[i][: res[i]["tail_len"]] for i in range(len(spec))],
+                           [res[i]["ebx2"] for i in range(len(spec))])
+    strict = 0
+    for i in range(len(spec)):
+        n = shapes[i][0] * shapes[i][1] * shapes[i][2]
+        err = float((outs[i][:n].double() - d[i].double()).abs().max())
+        amax = float(d[i].abs().max())
+        assert err <= res[i]["eb_abs"] + 4 * 2.0 ** -24 * amax, (i, spec[i])
+        strict += err <= res[i]["eb_abs"]
+    say(f"error bound: {strict} of {len(spec)} fields within eb (max - min) exactly")
+    del outs
     small = sorted(range(len(spec)), key=lambda i: shapes[i][0] * shapes[i][1] * shapes[i][2])[:3]
     for i in small:
         say(f"oracle check of field {i} {shapes[i]}")

@@ -52,3 +52,59 @@ def test_served_results_equal_alone():
             x = src_of[id(a)].source.float()
             tol = a.record["eb_abs"] + 4 * 2.0 ** -24 * float(x.abs().max())
             assert float((r.result - x).abs().max()) <= tol, r.rid
+
+
+def _check_alone(done):
+    import torch
+    src_of = {id(r.result): r for r in done if r.kind == "compress"}
+    for r in done:
+        plan = fzgpu.Plan(*r.dims)
+        if r.kind == "compress":
+            payload, tail, rec = plan.compress(r.source, r.eb)
+            a = r.result
+            assert a.record == rec, r.rid
+            assert torch.equal(a.payload, payload) and torch.equal(a.tail, tail), r.rid
+        else:
+            a = r.source
+            alone = plan.decompress(a.payload, a.tail, a.record["ebx2"])
+            assert torch.equal(r.result, alone), r.rid
+            x = src_of[id(a)].source.float()
+            tol = a.record["eb_abs"] + 4 * 2.0 ** -24 * float(x.abs().max())
+            assert float((r.result - x).abs().max()) <= tol, r.rid
+
+
+def test_saturating_stream_shares_batches_under_the_budget():
+    """Arrivals far faster than the device serves them (one every 20 us), over a pool whose
+    largest field is 268 MB, and a 1.5 GB budget smaller than what the waiting requests need
+    together: waiting requests must share batches, several batches must be on the device at
+    once, and admission must stop at the budget.  Every request's result still equals the same
+    request served alone, byte for byte."""
+    big = (8192, 8192, 1)  # 67.1M floats, 268 MB
+    shapes = [big, (1 << 20, 1, 1), (300, 301, 1), (64, 64, 64), (2001, 333, 1), (123457, 1, 1),
+              (97, 55, 33), (4096, 64, 1)]
+    pool = [(d, D.generate_gpu(D.BATCH_KINDS[i % 3], d, 70 + i)) for i, d in enumerate(shapes)]
+    rng = np.random.default_rng(5)
+    reqs = []
+    for i in range(160):
+        dims, x = pool[0] if i % 16 == 0 else pool[1 + int(rng.integers(0, len(pool) - 1))]
+        kind = "decompress" if i % 5 == 4 else "compress"
+        reqs.append(S.Request(i, kind, dims, S.SERVE_EBS[i % 3], x, 2e-5 * i))
+    budget = 1.5e9
+    need_big = sum(S.request_bytes(S.Request(0, "compress", big)))
+    assert 2 * need_big < budget < sum(sum(S.request_bytes(r)) for r in reqs)
+    say(f"saturating stream: {len(reqs)} requests, budget {budget / 1e9} GB, big request {need_big / 1e6:.0f} MB")
+    srv = S.Server(S.DeviceBackend(nstreams=3), hbm_budget_bytes=budget, slots=3, max_fields=16, max_archives=24)
+    done = srv.run(reqs, realtime=True, on_arrival=S.pick_archive(3))
+    assert len(done) == len(reqs)
+    sizes = [b[2] for b in srv.batches]
+    st = S.summarize(srv, done)
+    say(f"batches {len(sizes)}, max batch {max(sizes)}, mean {st['mean_batch']}, max in flight {srv.max_inflight}, "
+        f"budget-limited {srv.budget_limited}, peak {srv.budget.peak / 1e9:.3f} GB, "
+        f"decompress {st['decompress']}, hist {st['batch_size_hist']}")
+    assert max(sizes) >= 2                      # requests shared a batch
+    assert srv.max_inflight >= 2                # batches overlapped on the device
+    assert srv.budget_limited >= 1              # admission stopped at the budget
+    assert srv.budget.peak <= srv.budget.total
+    assert sum(r.dims == big for r in done) >= 5
+    assert st["decompress"] >= 1
+    _check_alone(done)

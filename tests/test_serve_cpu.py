@@ -93,7 +93,40 @@ def test_archives_are_evicted_to_make_room():
     srv.run(_reqs(30), realtime=False)
     assert len(srv.completed) == 30
     assert srv.budget.peak <= srv.budget.total
-    assert len(srv.archives) >= 1 and sum(b for _, b in srv.archives) <= srv.budget.total
+    assert len(srv.archives) >= 1
+    held = {id(b): b.nbytes for _, b in srv.archives}
+    assert sum(held.values()) == srv.budget.used <= srv.budget.total
+
+
+def test_archive_bytes_return_with_the_last_archive_of_a_batch():
+    """The archives of one compress batch share one output buffer: evicting some of them frees
+    nothing, evicting the last gives the whole buffer back."""
+    out = S.request_bytes(S.Request(0, "compress", (1 << 20, 1, 1)))[1]
+    srv, be, _ = _server(slots=1, max_fields=4, max_archives=100)
+    reqs = [S.Request(i, "compress", (1 << 20, 1, 1), 1e-3, None, 0.0) for i in range(4)]
+    srv.run(reqs, realtime=False)
+    assert len(be.launched) == 1 and len(srv.archives) == 4
+    assert srv.budget.used == 4 * out
+    for k in range(3):
+        assert srv._evict()
+        assert srv.budget.used == 4 * out, k
+    assert srv._evict() and srv.budget.used == 0 and not srv.archives
+    assert not srv._evict()
+
+
+def test_archives_read_by_waiting_decompress_requests_are_not_evicted():
+    one = sum(S.request_bytes(S.Request(0, "compress", (1 << 20, 1, 1))))
+    srv, be, _ = _server(budget=3 * one, slots=1, max_fields=1, max_archives=100)
+    srv.run([S.Request(0, "compress", (1 << 20, 1, 1), 1e-3, None, 0.0)], realtime=False)
+    a = srv.archives[0][0].result
+    dec = S.Request(1, "decompress", (1 << 20, 1, 1), 1e-3, a, 0.0)
+    srv.submit(dec)
+    assert a.pins == 1 and not srv._evict()  # the only archive is pinned
+    # compress requests that need the room must wait for the decompress instead of evicting
+    more = [S.Request(2 + i, "compress", (1 << 20, 1, 1), 1e-3, None, 0.0) for i in range(3)]
+    srv.run(more, realtime=False)
+    assert dec.result == "floats" and a.pins == 0
+    assert srv.budget.peak <= srv.budget.total
 
 
 def test_decompress_requests_pick_archives():
