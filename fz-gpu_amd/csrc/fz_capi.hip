@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "fz_internal.hpp"
@@ -462,6 +463,103 @@ fz_status fz_decompress_verify_f32(const fz_plan plan, const uint8_t* d_payload,
     uint8_t* ws = static_cast<uint8_t*>(d_ws);
     FZ_TRY(launch_verify(d_out, d_odata, plan->g.xyz, ws + plan->L.vtemp, d_stats, stream));
     FZ_TRY(prof_mark(plan, stream, false));
+    return FZ_SUCCESS;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- compress without a workspace
+//
+// fz_compress_f32_noalloc runs fz_compress_f32's kernels on a workspace that is a file-scope
+// device array instead of a caller's buffer, so the call allocates nothing.  The array is one
+// per device, allocated when the library's code object is loaded on that device and held until
+// the process ends; its device address is looked up once per device (hipGetSymbolAddress, with
+// that device current) and kept, since host code that names the array gets a host-side stand-in,
+// not a device address.  The fused compress needs nothing zeroed on entry: its range kernel
+// clears the per-run block sums the tile kernel adds to, and every other region is written
+// before it is read.  Calls that overlap on one device share the array and race.
+namespace {
+
+constexpr uint32_t kNoallocMaxTiles = FZ_NOALLOC_MAX_TILES;
+
+struct ScratchLayout {
+    size_t part, tstate, counts, stage, total;
+};
+
+ScratchLayout scratch_layout(uint32_t T)
+{
+    ScratchLayout S{};
+    size_t o = 0;
+    S.part = o;   o = al256(o + 16ull * kRangeMaxWG);
+    S.tstate = o; o = al256(o + 8ull * (gather_grid(T) + 2));
+    S.counts = o; o = al256(o + 4ull * (T + 1));
+    S.stage = o;  o = al256(o + 4096ull * T + 64);
+    S.total = o;
+    return S;
+}
+
+// room for the largest field the entry point accepts (gather_grid(T) <= 256)
+constexpr size_t kScratchBytes = 256 + 16ull * kRangeMaxWG + 8ull * 258 + 256 + 4ull * (kNoallocMaxTiles + 1) + 256 +
+                                 4096ull * kNoallocMaxTiles + 64 + 256;
+
+}  // namespace
+
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex    g_scratch_mu;
+void*         g_scratch_dev[kMaxDevices] = {};
+
+// device address of the current device's copy of `scratch`
+hipError_t scratch_address(void** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint64_t fz_noalloc_scratch_bytes(void) { return kScratchBytes; }
+
+fz_status fz_compress_f32_noalloc(const fz_plan plan, const float* d_in, double eb_rel, uint8_t* d_payload,
+                                  uint16_t* d_tail, fz_result* d_result, hipStream_t stream)
+{
+    if (!plan || !d_in || !d_payload || !d_result || plan->mode == FZ_MODE_OUTLIER) return FZ_ERR_INVALID_ARG;
+    if (!(eb_rel > 0.0) || !std::isfinite(eb_rel)) return FZ_ERR_INVALID_ARG;
+    if (!aligned(d_in, 16) || !aligned(d_payload, 16) || !aligned(d_result, 8)) return FZ_ERR_INVALID_ARG;
+    const Geom& g = plan->g;
+    if (plan->mode != FZ_MODE_LITERAL && !d_tail && g.len != (uint64_t)kTileCodes * g.T) return FZ_ERR_INVALID_ARG;
+    if (g.T > kNoallocMaxTiles) return FZ_ERR_CAPACITY;
+    const ScratchLayout S = scratch_layout(g.T);
+    if (S.total > kScratchBytes) return FZ_ERR_CAPACITY;
+    void* base = nullptr;
+    FZ_TRY(scratch_address(&base));
+    uint8_t*    ws = static_cast<uint8_t*>(base);
+    hipEvent_t* ev = nullptr;
+    if (plan->prof) {
+        ev = plan->ev;
+        plan->nev = g.T == 0 ? 4 : 6;
+        plan->paired = 1;
+    }
+    FZ_TRY(launch_compress(d_in, g, eb_rel, reinterpret_cast<float4*>(ws + S.part), d_result, d_payload, d_tail,
+                           ws + S.stage, reinterpret_cast<uint32_t*>(ws + S.counts),
+                           reinterpret_cast<uint64_t*>(ws + S.tstate), plan->wave_tile, ev, stream));
     return FZ_SUCCESS;
 }
 

@@ -70,7 +70,8 @@ EXPORTS = ["fz_version", "fz_status_string", "fz_last_error", "fz_plan_create", 
            "fz_batch_compress_graph", "fz_batch_decompress_graph", "fz_graph_launch", "fz_graph_destroy",
            "fz_verify_f32", "fz_decompress_verify_f32", "fz_compress_outlier_f32", "fz_decompress_outlier_f32",
            "fz_host_alloc", "fz_host_free", "fz_stager_create", "fz_stager_destroy", "fz_stager_sizes",
-           "fz_stager_compress", "fz_stager_decompress", "fz_stager_wait", "fz_payload_check", "fz_side_check"]
+           "fz_stager_compress", "fz_stager_decompress", "fz_stager_wait", "fz_payload_check", "fz_side_check",
+           "fz_compress_f32_noalloc", "fz_noalloc_scratch_bytes"]
 
 
 class _StatsRange(ctypes.Structure):
@@ -197,8 +198,10 @@ def lib():
     L.fz_decompress_verify_f32.argtypes = [vp, vp, vp, f32, vp, ctypes.c_size_t, vp, vp, vp, vp]
     L.fz_payload_check.argtypes = [vp, vp, u64, u64]
     L.fz_side_check.argtypes = [vp, vp, u64]
+    L.fz_compress_f32_noalloc.argtypes = [vp, vp, f64, vp, vp, vp, vp]
+    L.fz_noalloc_scratch_bytes.restype = u64
     for name in EXPORTS:
-        if name not in ("fz_version", "fz_status_string", "fz_last_error"):
+        if name not in ("fz_version", "fz_status_string", "fz_last_error", "fz_noalloc_scratch_bytes"):
             getattr(L, name).restype = i32
     _lib = L
     return L

@@ -130,17 +130,24 @@ class Server:
         self.completed = []
         self.batches = []         # (batch id, kind, n requests, input bytes, launch time, finish time)
         self.max_inflight = 0
-        self.budget_limited = 0   # batches cut short because the next request did not fit the budget
+        self.budget_limited = 0   # admission attempts stopped because the next request did not fit the budget
+        self.max_need = 0         # the largest request submitted so far (device bytes)
 
     def now(self):
         return self.clock() - self.t0
 
     def submit(self, req):
-        if sum(request_bytes(req)) > self.budget.total:
+        need = sum(request_bytes(req))
+        if need > self.budget.total:
             raise ValueError(f"request {req.rid} needs more device memory than the whole budget")
+        self.max_need = max(self.max_need, need)
         if req.kind == "decompress" and isinstance(req.source, Archive):
             req.source.pins += 1  # not evicted while this request waits or runs
         self.waiting[req.kind].append(req)
+
+    def _pinned_bytes(self):
+        return sum({id(buf): buf.nbytes for req, buf in self.archives
+                    if req.result is not None and req.result.pins > 0}.values())
 
     def _form(self, kind):
         """FIFO batch of waiting requests of `kind` that fits the budget (a request that does
@@ -149,14 +156,23 @@ class Server:
         take, nbytes = [], 0
         while q and len(take) < self.max_fields:
             need = sum(request_bytes(q[0]))
-            while not self.budget.fits(nbytes + need) and self._evict():
-                pass
+            # archives are dropped only when dropping them makes the request fit; memory held
+            # by running batches is not theirs to free
+            if not self.budget.fits(nbytes + need) and \
+                    self.budget.used - self._freeable() + nbytes + need <= self.budget.total:
+                while not self.budget.fits(nbytes + need) and self._evict():
+                    pass
             if not self.budget.fits(nbytes + need):
                 self.budget_limited += 1
                 break
             take.append(q.popleft())
             nbytes += need
         return take, nbytes
+
+    def _freeable(self):
+        """Bytes that evicting every archive no decompress request is reading would return."""
+        blocked = {id(buf) for req, buf in self.archives if req.result is not None and req.result.pins > 0}
+        return sum({id(buf): buf.nbytes for _, buf in self.archives if id(buf) not in blocked}.values())
 
     def _drop_ref(self, buf):
         buf.refs -= 1
@@ -176,8 +192,21 @@ class Server:
         return False
 
     def archive_pool(self):
-        """Compress results currently kept on the device."""
-        return [r for r, _ in self.archives]
+        """Compress results currently kept on the device that a new decompress request may read.
+        Pinned archives (read by waiting decompress requests) cannot be evicted, so an archive
+        whose buffer is not pinned yet is offered only while the pinned bytes leave room for the
+        largest request: the head of every queue then fits once the running batches finish, and
+        admission cannot deadlock on memory held for requests that are still waiting."""
+        pinned = {id(buf) for req, buf in self.archives if req.result is not None and req.result.pins > 0}
+        held = self._pinned_bytes()
+        out = []
+        for r, buf in self.archives:
+            if r.result is None:
+                continue
+            need = max(self.max_need, sum(request_bytes(Request(-1, "decompress", r.dims))))
+            if id(buf) in pinned or held + buf.nbytes + need <= self.budget.total:
+                out.append(r)
+        return out
 
     def _launch_ready(self):
         launched = False
@@ -255,7 +284,10 @@ class Server:
                 self.submit(pending[i])
                 i += 1
             self._poll()
-            self._launch_ready()
+            launched = self._launch_ready()
+            if not launched and not self.inflight and any(self.waiting.values()) and i >= len(pending):
+                # nothing runs, nothing arrives and nothing can be admitted: no later step changes that
+                raise RuntimeError("admission deadlock: waiting requests can never fit the budget")
             if not self.inflight and not any(self.waiting.values()) and i < len(pending) and realtime:
                 # idle until shortly before the next arrival, then poll: a sleeping thread
                 # wakes late by tens to hundreds of microseconds, which would land on the
@@ -478,6 +510,6 @@ def summarize(server, reqs):
         "batches": len(server.batches), "mean_batch": round(float(np.mean(sizes)), 2) if sizes else 0,
         "max_batch": max(sizes) if sizes else 0,
         "batch_size_hist": {str(k): hist[k] for k in sorted(hist)},
-        "budget_limited_batches": server.budget_limited,
+        "budget_blocked_admissions": server.budget_limited,
         "max_inflight_batches": server.max_inflight, "hbm_peak_GB": round(server.budget.peak / 1e9, 3),
     }

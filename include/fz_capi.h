@@ -167,6 +167,21 @@ fz_status fz_decompress_outlier_f32(const fz_plan plan, const uint8_t* d_payload
 fz_status fz_payload_check(const fz_plan plan, const uint8_t* h_payload, uint64_t payload_bytes, uint64_t tail_len);
 fz_status fz_side_check(const fz_plan plan, const uint8_t* h_side, uint64_t side_bytes);
 
+/* fz_compress_f32 without a workspace argument: no counterpart in the reference; for callers
+ * that cannot allocate device memory (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The workspace is a fixed device array inside the library,
+ * one per GPU, of fz_noalloc_scratch_bytes() bytes (about 63 MiB), held from the library's load
+ * on that GPU until the process ends.  Fields of at most FZ_NOALLOC_MAX_TILES full tiles
+ * (len < 2048 * (FZ_NOALLOC_MAX_TILES + 1) floats) are accepted; larger ones return
+ * FZ_ERR_CAPACITY.  The call allocates and frees nothing; the caller passes the input and the
+ * outputs as for fz_compress_f32, whose bytes it writes.  Every call on one GPU uses that GPU's
+ * array: calls that overlap on one GPU (other streams, other threads) race and must not be
+ * issued. */
+#define FZ_NOALLOC_MAX_TILES 16000
+fz_status fz_compress_f32_noalloc(const fz_plan plan, const float* d_in, double eb_rel, uint8_t* d_payload,
+                                  uint16_t* d_tail, fz_result* d_result, hipStream_t stream);
+uint64_t  fz_noalloc_scratch_bytes(void);
+
 /* Convenience: runs fz_compress_f32, waits for the stream and copies *d_result to *h_result. */
 fz_status fz_compress_f32_sync(const fz_plan plan, const float* d_in, double eb_rel, void* d_ws, size_t ws_bytes,
                                uint8_t* d_payload, uint16_t* d_tail, fz_result* d_result, fz_result* h_result,

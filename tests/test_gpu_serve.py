@@ -74,8 +74,8 @@ def _check_alone(done):
 
 
 def test_saturating_stream_shares_batches_under_the_budget():
-    """Arrivals far faster than the device serves them (one every 20 us), over a pool whose
-    largest field is 268 MB, and a 1.5 GB budget smaller than what the waiting requests need
+    """Arrivals far faster than the device serves them (one every 50 us), over a pool whose
+    largest field is 268 MB (every 8th request), and a 0.6 GB budget smaller than what the waiting requests need
     together: waiting requests must share batches, several batches must be on the device at
     once, and admission must stop at the budget.  Every request's result still equals the same
     request served alone, byte for byte."""
@@ -85,15 +85,21 @@ def test_saturating_stream_shares_batches_under_the_budget():
     pool = [(d, D.generate_gpu(D.BATCH_KINDS[i % 3], d, 70 + i)) for i, d in enumerate(shapes)]
     rng = np.random.default_rng(5)
     reqs = []
-    for i in range(160):
-        dims, x = pool[0] if i % 16 == 0 else pool[1 + int(rng.integers(0, len(pool) - 1))]
+    for i in range(240):
+        dims, x = pool[0] if i % 8 == 0 else pool[1 + int(rng.integers(0, len(pool) - 1))]
         kind = "decompress" if i % 5 == 4 else "compress"
-        reqs.append(S.Request(i, kind, dims, S.SERVE_EBS[i % 3], x, 2e-5 * i))
-    budget = 1.5e9
+        reqs.append(S.Request(i, kind, dims, S.SERVE_EBS[i % 3], x, 5e-5 * i))
+    budget = 0.6e9
     need_big = sum(S.request_bytes(S.Request(0, "compress", big)))
     assert 2 * need_big < budget < sum(sum(S.request_bytes(r)) for r in reqs)
     say(f"saturating stream: {len(reqs)} requests, budget {budget / 1e9} GB, big request {need_big / 1e6:.0f} MB")
     srv = S.Server(S.DeviceBackend(nstreams=3), hbm_budget_bytes=budget, slots=3, max_fields=16, max_archives=24)
+    # archives to decompress from the first arrival on: every pool field served once beforehand
+    warm = [S.Request(1000 + i, "compress", d, 1e-3, x, 0.0) for i, (d, x) in enumerate(pool)]
+    srv.run(warm, realtime=False)
+    srv.completed.clear()
+    srv.batches.clear()
+    srv.max_inflight = srv.budget_limited = 0
     done = srv.run(reqs, realtime=True, on_arrival=S.pick_archive(3))
     assert len(done) == len(reqs)
     sizes = [b[2] for b in srv.batches]
@@ -107,4 +113,4 @@ def test_saturating_stream_shares_batches_under_the_budget():
     assert srv.budget.peak <= srv.budget.total
     assert sum(r.dims == big for r in done) >= 5
     assert st["decompress"] >= 1
-    _check_alone(done)
+    _check_alone(warm + done)

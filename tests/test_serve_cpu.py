@@ -169,3 +169,36 @@ def test_plan_pool_reuses_by_shape_and_drops_least_recent():
     pool.give(((6, 1, 1),), big)                               # larger than the whole pool
     assert closed == [b, big] and pool.bytes == 80
     assert pool.take(((7, 1, 1),)) is c and pool.take(((8, 1, 1),)) is a and pool.bytes == 0
+
+
+def test_pinned_archives_never_block_admission_for_good():
+    """A saturating stream over a pool with a field near half the budget, decompress requests
+    reading archives that are pinned until they run: the archives offered to new decompress
+    requests leave room for the largest request, so every request is served (a stall would
+    raise) and the budget holds."""
+    big = (8192, 8192, 1)
+    shapes = [big, (1 << 20, 1, 1), (300, 301, 1), (64, 64, 64), (2001, 333, 1), (123457, 1, 1)]
+    for seed in range(2):
+        rng = np.random.default_rng(seed)
+        reqs = []
+        for i in range(240):
+            dims = big if i % 8 == 0 else shapes[1 + int(rng.integers(0, len(shapes) - 1))]
+            reqs.append(S.Request(i, "decompress" if i % 5 == 4 else "compress", dims, 1e-3, None, 5e-5 * i))
+        srv, be, clock = _server(budget=0.6e9, slots=3, max_fields=16, max_archives=24)
+        srv.run([S.Request(1000 + i, "compress", d, 1e-3, None, 0.0) for i, d in enumerate(shapes)], realtime=False)
+        srv.completed.clear()
+        done = srv.run(reqs, realtime=True, on_arrival=S.pick_archive(3))
+        assert len(done) == len(reqs)
+        assert sum(r.kind == "decompress" for r in done) > 0
+        assert srv.budget.peak <= srv.budget.total
+
+
+def test_a_request_that_can_never_fit_raises_instead_of_spinning():
+    one = sum(S.request_bytes(S.Request(0, "compress", (1 << 20, 1, 1))))
+    srv, _, _ = _server(budget=1.5 * one, slots=1, max_fields=1, max_archives=100)
+    srv.run([S.Request(0, "compress", (1 << 20, 1, 1), 1e-3, None, 0.0)], realtime=False)
+    a = srv.archives[0][0].result
+    srv.submit(S.Request(1, "decompress", (1 << 20, 1, 1), 1e-3, a, 0.0))  # pins the archive
+    srv.budget.take(srv.budget.total - srv.budget.used)  # memory held outside the server's requests
+    with pytest.raises(RuntimeError):
+        srv.run([], realtime=False)
