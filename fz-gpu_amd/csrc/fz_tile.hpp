@@ -323,12 +323,10 @@ __device__ __forceinline__ void dq8_tile(const float* __restrict__ in, uint32_t 
 // The unit after the last full tile: codes of the len - 2048 T positions no tile covers,
 // returned in `tail` in FIXED mode (the reference never encodes them, fz.cu:173), and the
 // bitflag words of the partial tile cleared (fz.cu:167 memsets them).
-// With a null ovf_out the unit's overflow count is returned to every thread instead (one
-// workgroup barrier).
 template <int NDIM>
-__device__ __forceinline__ uint32_t tail_unit(const float* __restrict__ in, const Geom& g, float r,
-                                              uint32_t* __restrict__ bwords, uint16_t* __restrict__ tail,
-                                              unsigned long long* ovf_out)
+__device__ __forceinline__ void tail_unit(const float* __restrict__ in, const Geom& g, float r,
+                                          uint32_t* __restrict__ bwords, uint16_t* __restrict__ tail,
+                                          unsigned long long* ovf_out)
 {
     const uint32_t tid = threadIdx.x;
     const uint32_t t0 = (uint32_t)kTileCodes * g.T;
@@ -345,14 +343,7 @@ __device__ __forceinline__ uint32_t tail_unit(const float* __restrict__ in, cons
     }
     for (uint32_t i = 8 * g.T + tid; i < g.W; i += kWG) bwords[i] = 0;
     ovf = wave_sum(ovf);
-    if (ovf_out) {
-        if ((tid & 63) == 0 && ovf) atomicAdd(ovf_out, (unsigned long long)ovf);
-        return 0;
-    }
-    __shared__ uint32_t s_ovf[kWG / 64];
-    if ((tid & 63) == 0) s_ovf[tid >> 6] = ovf;
-    __syncthreads();
-    return s_ovf[0] + s_ovf[1] + s_ovf[2] + s_ovf[3];
+    if ((tid & 63) == 0 && ovf) atomicAdd(ovf_out, (unsigned long long)ovf);
 }
 
 // The 3D generic path of a fast-geometry tile (gp 2 or 3, whose raw loads R holds): level 0,
@@ -378,9 +369,8 @@ __device__ __forceinline__ void pk_dispatch(const RawTile<NDIM>& R, uint32_t p0,
 // Steps 2-4 of a tile (bit transpose, flags, in-tile compaction) on the thread's 4 code
 // words w, whose overflow count is ovf.  With ocnt, thread 0 also stores the tile's
 // (overflows, verbatim values) totals there and adds them to its run's pair at orun: the
-// per-unit counts of the outlier mode.  Returns the tile's (nonzero blocks, overflows) to
-// every thread; a null ovf_out adds the overflows nowhere.
-__device__ __forceinline__ uint2 tile_finish(uint32_t w[4], uint32_t ovf, uint32_t tile,
+// per-unit counts of the outlier mode.
+__device__ __forceinline__ void tile_finish(uint32_t w[4], uint32_t ovf, uint32_t tile,
                                             uint32_t* __restrict__ bwords, uint8_t* __restrict__ slot,
                                             uint32_t* __restrict__ count_out, unsigned long long* ovf_out,
                                             uint32_t* run_sum, uint32_t nraw = 0, uint32_t* ocnt = nullptr,
@@ -418,12 +408,12 @@ __device__ __forceinline__ uint2 tile_finish(uint32_t w[4], uint32_t ovf, uint32
     __syncthreads();
     const uint32_t c0 = sh_wcnt[0], c1 = sh_wcnt[1], c2 = sh_wcnt[2], c3 = sh_wcnt[3];
     const uint32_t woff = (wv > 0 ? c0 : 0u) + (wv > 1 ? c1 : 0u) + (wv > 2 ? c2 : 0u);
-    const uint32_t total = c0 + c1 + c2 + c3;
-    const uint32_t o = sh_wovf[0] + sh_wovf[1] + sh_wovf[2] + sh_wovf[3];
     if (tid == 0) {
-        if (count_out) *count_out = total;
+        const uint32_t total = c0 + c1 + c2 + c3;
+        *count_out = total;
         if (run_sum && total) atomicAdd(run_sum, total);
-        if (o && ovf_out) atomicAdd(ovf_out, (unsigned long long)o);
+        const uint32_t o = sh_wovf[0] + sh_wovf[1] + sh_wovf[2] + sh_wovf[3];
+        if (o) atomicAdd(ovf_out, (unsigned long long)o);
         if (ocnt) {
             const uint32_t rw = sh_wraw[0] + sh_wraw[1] + sh_wraw[2] + sh_wraw[3];
             *reinterpret_cast<uint2*>(ocnt) = make_uint2(o, rw);
@@ -433,7 +423,6 @@ __device__ __forceinline__ uint2 tile_finish(uint32_t w[4], uint32_t ovf, uint32
     }
     // 4. in-tile compaction into the tile's staging slot
     if (f) *reinterpret_cast<uint4*>(slot + 16 * (woff + rank)) = blk;
-    return make_uint2(total, o);
 }
 
 // Step 1 of a tile: dual-quant of the thread's 8 codes into its 4 words w (overflows added
@@ -479,7 +468,7 @@ __device__ __forceinline__ void tile_dualquant(const float* __restrict__ in, con
 // at its top keeps the register count of the larger one instead of their union (69 against
 // 87 VGPRs: 7 waves per SIMD instead of 5).
 template <int NDIM, int PATHS = 0>
-__device__ __forceinline__ uint2 tile_encode(const float* __restrict__ in, const Geom& g, uint32_t tile, int gp,
+__device__ __forceinline__ void tile_encode(const float* __restrict__ in, const Geom& g, uint32_t tile, int gp,
                                             const RawTile<NDIM>& R, float r, int level,
                                             uint32_t* __restrict__ bwords, uint8_t* __restrict__ slot,
                                             uint32_t* __restrict__ count_out, unsigned long long* ovf_out,
@@ -489,7 +478,7 @@ __device__ __forceinline__ uint2 tile_encode(const float* __restrict__ in, const
     uint32_t w[4];
     uint32_t ovf = 0;
     tile_dualquant<NDIM, PATHS>(in, g, tile, gp, R, r, level, w, ovf);
-    return tile_finish(w, ovf, tile, bwords, slot, count_out, ovf_out, run_sum);
+    tile_finish(w, ovf, tile, bwords, slot, count_out, ovf_out, run_sum);
 }
 
 // ---------------------------------------------------------------------------- outlier mode

@@ -132,6 +132,7 @@ class Server:
         self.max_inflight = 0
         self.budget_limited = 0   # admission attempts stopped because the next request did not fit the budget
         self.max_need = 0         # the largest request submitted so far (device bytes)
+        self.on_complete = None
 
     def now(self):
         return self.clock() - self.t0
@@ -241,6 +242,8 @@ class Server:
                 self.backend.finish(handle, reqs)
                 for r in reqs:
                     r.done = t
+                    if self.on_complete:
+                        self.on_complete(r)
                 self.completed.extend(reqs)
                 out_bytes = sum(request_bytes(r)[1] for r in reqs) if kind == "compress" else 0
                 self.budget.give(nbytes - out_bytes)  # the workspace share; decompress outputs go to the caller
@@ -268,11 +271,13 @@ class Server:
         self.inflight = still
         return done_any
 
-    def run(self, requests, realtime=True, on_arrival=None):
+    def run(self, requests, realtime=True, on_arrival=None, on_complete=None):
         """Serve `requests` (arrival times in seconds from the start).  With realtime, a
         request becomes visible when the clock passes its arrival; otherwise all are
         visible at once.  on_arrival(server, req) may complete the request before it queues
-        (a decompress request picks its archive there)."""
+        (a decompress request picks its archive there); on_complete(req) sees each finished
+        request before its archive can be evicted."""
+        self.on_complete = on_complete
         pending = sorted(requests, key=lambda r: r.arrival)
         self.t0 = self.clock()
         i = 0

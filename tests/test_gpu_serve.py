@@ -54,14 +54,16 @@ def test_served_results_equal_alone():
             assert float((r.result - x).abs().max()) <= tol, r.rid
 
 
-def _check_alone(done):
+def _check_alone(done, kept):
+    """kept: rid -> the Archive a compress request produced (copied when it finished, since the
+    server may evict it later)."""
     import torch
-    src_of = {id(r.result): r for r in done if r.kind == "compress"}
+    src_of = {id(kept[r.rid]): r for r in done if r.kind == "compress"}
     for r in done:
         plan = fzgpu.Plan(*r.dims)
         if r.kind == "compress":
             payload, tail, rec = plan.compress(r.source, r.eb)
-            a = r.result
+            a = kept[r.rid]
             assert a.record == rec, r.rid
             assert torch.equal(a.payload, payload) and torch.equal(a.tail, tail), r.rid
         else:
@@ -95,12 +97,18 @@ def test_saturating_stream_shares_batches_under_the_budget():
     say(f"saturating stream: {len(reqs)} requests, budget {budget / 1e9} GB, big request {need_big / 1e6:.0f} MB")
     srv = S.Server(S.DeviceBackend(nstreams=3), hbm_budget_bytes=budget, slots=3, max_fields=16, max_archives=24)
     # archives to decompress from the first arrival on: every pool field served once beforehand
+    kept = {}
+
+    def keep(r):  # the archive as served, before any eviction can drop it
+        if r.kind == "compress":
+            kept[r.rid] = r.result
+
     warm = [S.Request(1000 + i, "compress", d, 1e-3, x, 0.0) for i, (d, x) in enumerate(pool)]
-    srv.run(warm, realtime=False)
+    srv.run(warm, realtime=False, on_complete=keep)
     srv.completed.clear()
     srv.batches.clear()
     srv.max_inflight = srv.budget_limited = 0
-    done = srv.run(reqs, realtime=True, on_arrival=S.pick_archive(3))
+    done = srv.run(reqs, realtime=True, on_arrival=S.pick_archive(3), on_complete=keep)
     assert len(done) == len(reqs)
     sizes = [b[2] for b in srv.batches]
     st = S.summarize(srv, done)
@@ -113,4 +121,4 @@ def test_saturating_stream_shares_batches_under_the_budget():
     assert srv.budget.peak <= srv.budget.total
     assert sum(r.dims == big for r in done) >= 5
     assert st["decompress"] >= 1
-    _check_alone(warm + done)
+    _check_alone(warm + done, kept)
