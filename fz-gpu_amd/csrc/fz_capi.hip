@@ -714,6 +714,11 @@ struct fz_batch_plan_s {
     int                prof = 0;
     int                nslots = 0;  // event pairs recorded by the last call
     hipEvent_t         ev[2 * kBatchSlots] = {};
+    // OUTLIER mode: the single-field outlier launches field after field in one shared
+    // workspace, laid out per field (caller order) as a single-field plan's
+    std::vector<Geom>   ogeo;
+    std::vector<Layout> olay;
+    size_t              o_prefix = 0;  // largest zero-filled prefix over the fields
 };
 
 struct fz_graph_s {
@@ -854,6 +859,7 @@ fz_status batch_compress_impl(const fz_batch_plan plan, const float* const* d_in
                               fz_result* d_results, hipStream_t stream, bool timed)
 {
     if (!plan || !d_in || !eb_rel || !d_ws || !d_payload || !d_results) return FZ_ERR_INVALID_ARG;
+    if (plan->mode == FZ_MODE_OUTLIER) return FZ_ERR_INVALID_ARG;  // fz_batch_compress_outlier_f32
     if (!aligned(d_ws, 256) || !aligned(d_results, 8)) return FZ_ERR_INVALID_ARG;
     if (ws_bytes < plan->L.total) return FZ_ERR_WORKSPACE_TOO_SMALL;
     for (uint32_t k = 0; k < plan->nf; k++) {
@@ -888,6 +894,7 @@ fz_status batch_decompress_impl(const fz_batch_plan plan, const uint8_t* const* 
                                 float* const* d_out, hipStream_t stream, bool timed)
 {
     if (!plan || !d_payload || !ebx2 || !d_ws || !d_out) return FZ_ERR_INVALID_ARG;
+    if (plan->mode == FZ_MODE_OUTLIER) return FZ_ERR_INVALID_ARG;  // fz_batch_decompress_outlier_f32
     if (!aligned(d_ws, 256)) return FZ_ERR_INVALID_ARG;
     if (ws_bytes < plan->L.total) return FZ_ERR_WORKSPACE_TOO_SMALL;
     for (uint32_t k = 0; k < plan->nf; k++) {
@@ -959,7 +966,7 @@ fz_status fz_batch_plan_create(fz_batch_plan* plan, const fz_field_shape* shapes
     if (!plan) return FZ_ERR_INVALID_ARG;
     *plan = nullptr;
     if (!shapes || nfields == 0 || nfields > kBatchMaxFields) return FZ_ERR_INVALID_ARG;
-    if (mode != FZ_MODE_FIXED && mode != FZ_MODE_LITERAL) return FZ_ERR_INVALID_ARG;
+    if (mode != FZ_MODE_FIXED && mode != FZ_MODE_LITERAL && mode != FZ_MODE_OUTLIER) return FZ_ERR_INVALID_ARG;
     std::vector<Geom> geo(nfields);
     for (uint32_t i = 0; i < nfields; i++) {
         const fz_field_shape& s = shapes[i];
@@ -1052,6 +1059,17 @@ fz_status fz_batch_plan_create(fz_batch_plan* plan, const fz_field_shape* shapes
     L.texcl = o;   o = al256(o + 4ull * u);
     L.staging = o; o = al256(o + 4096ull * u);
     L.total = o;
+    if (mode == FZ_MODE_OUTLIER) {  // one single-field workspace, sized for the largest field
+        p->ogeo = geo;
+        p->olay.resize(nfields);
+        size_t big = 0;
+        for (uint32_t i = 0; i < nfields; i++) {
+            p->olay[i] = make_layout(geo[i], mode);
+            big = std::max(big, p->olay[i].total);
+            p->o_prefix = std::max(p->o_prefix, p->olay[i].codes);
+        }
+        L.total = big;
+    }
     p->host_ptrs.resize(nfields);
     p->tune.range_chunk = 65536;
     p->tune.xcd_order[0] = p->tune.xcd_order[1] = p->tune.xcd_order[2] = 1;
@@ -1097,7 +1115,7 @@ fz_status fz_batch_field_sizes(const fz_batch_plan plan, uint32_t field, fz_size
         s->payload_max_bytes = 4ull * g.W + 4096ull * g.T;
         s->tail_max_codes = tail_codes(g);
         s->workspace_bytes = plan->L.total;
-        s->side_max_bytes = 0;  // batch plans have no outlier mode
+        s->side_max_bytes = plan->mode == FZ_MODE_OUTLIER ? 16ull + 8ull * g.xyz : 0;
         return FZ_SUCCESS;
     }
     return FZ_ERR_INVALID_ARG;
@@ -1122,6 +1140,8 @@ fz_status fz_batch_workspace_init(const fz_batch_plan plan, void* d_ws, size_t w
 {
     if (!plan || !d_ws || !aligned(d_ws, 256)) return FZ_ERR_INVALID_ARG;
     if (ws_bytes < plan->L.total) return FZ_ERR_WORKSPACE_TOO_SMALL;
+    if (plan->mode == FZ_MODE_OUTLIER)  // every field's zero-filled prefix
+        return hip_st(launch_fill_u32(static_cast<uint32_t*>(d_ws), plan->o_prefix / 4, 0u, stream));
     const auto&          L = plan->L;
     std::vector<uint8_t> t(L.tables_end, 0);
     std::memcpy(t.data() + L.fields, plan->fields.data(), sizeof(BField) * plan->nf);
@@ -1158,6 +1178,84 @@ fz_status fz_batch_decompress_f32(const fz_batch_plan plan, const uint8_t* const
     return batch_decompress_impl(plan, d_payload, d_tail, ebx2, d_ws, ws_bytes, d_out, stream, true);
 }
 
+// OUTLIER-mode batches: the single-field outlier launches (fz_compress_outlier_f32 /
+// fz_decompress_outlier_f32) field after field on the caller's stream, one shared workspace
+// laid out for each field in turn.  Compress needs nothing zeroed on entry; the inverse needs
+// its field's position bitmap clear, which a previous field's other regions may have written,
+// so each field's bitmap is cleared first.
+fz_status fz_batch_compress_outlier_f32(const fz_batch_plan plan, const float* const* d_in, const double* eb_rel,
+                                        void* d_ws, size_t ws_bytes, uint8_t* const* d_payload,
+                                        uint16_t* const* d_tail, uint8_t* const* d_side,
+                                        const uint64_t* side_capacity, fz_result* d_results, hipStream_t stream)
+{
+    if (!plan || plan->mode != FZ_MODE_OUTLIER) return FZ_ERR_INVALID_ARG;
+    if (!d_in || !eb_rel || !d_ws || !d_payload || !d_side || !side_capacity || !d_results) return FZ_ERR_INVALID_ARG;
+    if (!aligned(d_ws, 256) || !aligned(d_results, 8)) return FZ_ERR_INVALID_ARG;
+    if (ws_bytes < plan->L.total) return FZ_ERR_WORKSPACE_TOO_SMALL;
+    for (uint32_t i = 0; i < plan->nf; i++) {
+        const Geom& g = plan->ogeo[i];
+        if (!d_in[i] || !aligned(d_in[i], 16) || !d_payload[i] || !aligned(d_payload[i], 16)) return FZ_ERR_INVALID_ARG;
+        if (!(eb_rel[i] > 0.0) || !std::isfinite(eb_rel[i])) return FZ_ERR_INVALID_ARG;
+        if (tail_codes(g) > 0 && (!d_tail || !d_tail[i])) return FZ_ERR_INVALID_ARG;
+        if (!d_side[i] || !aligned(d_side[i], 16) || side_capacity[i] < 16) return FZ_ERR_INVALID_ARG;
+    }
+    uint8_t* ws = static_cast<uint8_t*>(d_ws);
+    for (uint32_t i = 0; i < plan->nf; i++) {
+        const Geom&   g = plan->ogeo[i];
+        const Layout& L = plan->olay[i];
+        FZ_TRY(launch_compress_outlier(d_in[i], g, eb_rel[i], reinterpret_cast<float4*>(ws + L.part), d_results + i,
+                                       d_payload[i], d_tail ? d_tail[i] : nullptr, d_side[i], side_capacity[i],
+                                       ws + L.stage, reinterpret_cast<uint32_t*>(ws + L.counts),
+                                       reinterpret_cast<uint32_t*>(ws + L.tstate), L.n_tstate,
+                                       reinterpret_cast<uint32_t*>(ws + L.ocnt), reinterpret_cast<uint32_t*>(ws + L.orun),
+                                       stream));
+    }
+    return FZ_SUCCESS;
+}
+
+fz_status fz_batch_decompress_outlier_f32(const fz_batch_plan plan, const uint8_t* const* d_payload,
+                                          const uint16_t* const* d_tail, const float* ebx2, const float* vconst,
+                                          const uint8_t* const* d_side, const uint64_t* side_bytes, void* d_ws,
+                                          size_t ws_bytes, float* const* d_out, hipStream_t stream)
+{
+    if (!plan || plan->mode != FZ_MODE_OUTLIER) return FZ_ERR_INVALID_ARG;
+    if (!d_payload || !ebx2 || !vconst || !d_side || !side_bytes || !d_ws || !d_out) return FZ_ERR_INVALID_ARG;
+    if (!aligned(d_ws, 256)) return FZ_ERR_INVALID_ARG;
+    if (ws_bytes < plan->L.total) return FZ_ERR_WORKSPACE_TOO_SMALL;
+    for (uint32_t i = 0; i < plan->nf; i++) {
+        const Geom& g = plan->ogeo[i];
+        if (!d_payload[i] || !aligned(d_payload[i], 16) || !d_out[i] || !aligned(d_out[i], 16)) return FZ_ERR_INVALID_ARG;
+        if (!d_side[i] || !aligned(d_side[i], 16) || side_bytes[i] < 16) return FZ_ERR_INVALID_ARG;
+        if (!(ebx2[i] >= 0.0f) || !std::isfinite(ebx2[i]) || !std::isfinite(vconst[i])) return FZ_ERR_INVALID_ARG;
+        if (ebx2[i] != 0.0f && tail_codes(g) > 0 && (!d_tail || !d_tail[i])) return FZ_ERR_INVALID_ARG;
+    }
+    uint8_t* ws = static_cast<uint8_t*>(d_ws);
+    for (uint32_t i = 0; i < plan->nf; i++) {
+        const Geom&   g = plan->ogeo[i];
+        const Layout& L = plan->olay[i];
+        uint32_t*     obits = reinterpret_cast<uint32_t*>(ws + L.obits);
+        FZ_TRY(launch_fill_u32(obits, (g.xyz + 31) / 32, 0u, stream));
+        if (ebx2[i] == 0.0f) {  // constant field
+            FZ_TRY(launch_fill_outlier(d_side[i], side_bytes[i], obits, d_out[i], g, vconst[i], stream));
+            continue;
+        }
+        Ctr*      ctr = reinterpret_cast<Ctr*>(ws + L.ctr);
+        uint64_t* dstate = reinterpret_cast<uint64_t*>(ws + L.dstate);
+        uint32_t* texcl = reinterpret_cast<uint32_t*>(ws + L.texcl);
+        uint32_t* codes = reinterpret_cast<uint32_t*>(ws + L.codes);
+        int32_t*  over = reinterpret_cast<int32_t*>(ws + L.over);
+        const uint16_t* tail = d_tail ? d_tail[i] : nullptr;
+        FZ_TRY(launch_put_deltas(d_side[i], side_bytes[i], obits, over, g, dstate, L.n_dstate, &ctr->dscan_ticket,
+                                 stream));
+        FZ_TRY(launch_tile_scan(reinterpret_cast<const uint32_t*>(d_payload[i]), g.T, texcl, dstate, L.n_dstate,
+                                &ctr->dscan_ticket, stream, false));
+        FZ_TRY(launch_decode_unshuffle(d_payload[i], texcl, tail, g, codes, stream));
+        FZ_TRY(launch_reconstruct_outlier(reinterpret_cast<const uint16_t*>(codes), obits, over, d_side[i],
+                                          side_bytes[i], d_out[i], g, ebx2[i], stream));
+    }
+    return FZ_SUCCESS;
+}
+
 fz_status fz_batch_profile(fz_batch_plan plan, int enable)
 {
     if (!plan) return FZ_ERR_INVALID_ARG;
@@ -1184,7 +1282,7 @@ fz_status fz_batch_autotune(fz_batch_plan plan, const float* const* d_in, const 
                             size_t ws_bytes, uint8_t* const* d_payload, uint16_t* const* d_tail, fz_result* d_results,
                             int reps, fz_batch_tuning* chosen, hipStream_t stream)
 {
-    if (!plan || reps < 1) return FZ_ERR_INVALID_ARG;
+    if (!plan || reps < 1 || plan->mode == FZ_MODE_OUTLIER) return FZ_ERR_INVALID_ARG;
     const int             prof0 = plan->prof;
     const fz_batch_tuning start = plan->tune;
     fz_status             st = fz_batch_profile(plan, 1);

@@ -15,6 +15,7 @@ Mirrors of the reference's surface:
     quality_report(xdata, odata, ...)   -> quality_viewer.hh:97 echo_metric_gpu
 """
 import ctypes
+import math
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -71,7 +72,8 @@ EXPORTS = ["fz_version", "fz_status_string", "fz_last_error", "fz_plan_create", 
            "fz_verify_f32", "fz_decompress_verify_f32", "fz_compress_outlier_f32", "fz_decompress_outlier_f32",
            "fz_host_alloc", "fz_host_free", "fz_stager_create", "fz_stager_destroy", "fz_stager_sizes",
            "fz_stager_compress", "fz_stager_decompress", "fz_stager_wait", "fz_payload_check", "fz_side_check",
-           "fz_compress_f32_noalloc", "fz_noalloc_scratch_bytes"]
+           "fz_compress_f32_noalloc", "fz_noalloc_scratch_bytes", "fz_batch_compress_outlier_f32",
+           "fz_batch_decompress_outlier_f32"]
 
 
 class _StatsRange(ctypes.Structure):
@@ -200,6 +202,8 @@ def lib():
     L.fz_side_check.argtypes = [vp, vp, u64]
     L.fz_compress_f32_noalloc.argtypes = [vp, vp, f64, vp, vp, vp, vp]
     L.fz_noalloc_scratch_bytes.restype = u64
+    L.fz_batch_compress_outlier_f32.argtypes = [vp, vp, vp, vp, ctypes.c_size_t, vp, vp, vp, vp, vp, vp]
+    L.fz_batch_decompress_outlier_f32.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_size_t, vp, vp]
     for name in EXPORTS:
         if name not in ("fz_version", "fz_status_string", "fz_last_error", "fz_noalloc_scratch_bytes"):
             getattr(L, name).restype = i32
@@ -607,16 +611,37 @@ class BatchOutputs:
         self.payloads = [self.payload_arena[o: o + n + 16] for o, n in self.payload_off]
         self.tails = [self.tail_arena[o: o + 2 * n + 2].view(torch.int16) for o, n in self.tail_off]
         self.res = torch.zeros(64 * plan.nf, dtype=torch.uint8, device=plan.device)
+        # OUTLIER mode: one side-channel arena, a slice of side_capacity(i) bytes per field
+        self.sides, self.side_caps = [], []
+        if plan.mode == MODE_OUTLIER:
+            caps = [plan.side_capacity(i) for i in range(plan.nf)]
+            offs, so = [], 0
+            for c in caps:
+                offs.append(so)
+                so += _al(c)
+            self.side_arena = torch.zeros(max(so, 256), dtype=torch.uint8, device=plan.device)
+            self.sides = [self.side_arena[o: o + c] for o, c in zip(offs, caps)]
+            self.side_caps = caps
+
+    def side_channels(self):
+        """Each field's side channel (device uint8 views of 16 + 8 (n_delta + n_raw) bytes)."""
+        import torch
+        out = []
+        for sd in self.sides:
+            nd, nr = (int(v) for v in sd[:16].cpu().view(torch.int64).tolist())
+            out.append(sd[: 16 + 8 * (nd + nr)])
+        return out
 
 
 class BatchPlan:
     """Many fields of uneven shapes (list of (x, y, z) or (x, y, z, len)) compressed by one
     launch sequence; each field's bytes equal those of Plan on that field alone."""
 
-    def __init__(self, shapes, mode=MODE_FIXED, device="cuda", workspace=True):
+    def __init__(self, shapes, mode=MODE_FIXED, device="cuda", workspace=True, side_fraction=1.0):
         self.shapes = [tuple(int(v) for v in s) for s in shapes]
         self.nf = len(self.shapes)
         self.mode = mode
+        self.side_fraction = float(side_fraction)
         self.device = device
         arr = (FieldShape * max(self.nf, 1))()
         for i, s in enumerate(self.shapes):
@@ -722,6 +747,47 @@ class BatchPlan:
         outs = [torch.empty(max(s[0] * s[1] * s[2], 4), dtype=torch.float32, device=self.device)
                 for s in self.shapes]
         self.decompress_async(payloads, tails, ebx2s, outs, stream)
+        return [o[: s[0] * s[1] * s[2]] for o, s in zip(outs, self.shapes)]
+
+    def side_capacity(self, i):
+        """Side-channel bytes a batched OUTLIER compress reserves for field i: the largest
+        channel the field can need, times the plan's side_fraction (1 by default)."""
+        full = self.field_sizes(i).side_max_bytes
+        return int(max(16, min(full, 16 + 8 * math.ceil((full - 16) / 8 * self.side_fraction))))
+
+    def compress_outlier_async(self, inputs, ebs, out, stream=None):
+        """OUTLIER-mode batch (plan made with MODE_OUTLIER): payloads, tails, side channels
+        (out.sides) and records; launch only."""
+        assert self.mode == MODE_OUTLIER
+        ins, eb, pays, tails = self._compress_args(inputs, ebs, out)
+        sides = _ptr_array(out.sides)
+        caps = (ctypes.c_uint64 * self.nf)(*out.side_caps)
+        _check(lib().fz_batch_compress_outlier_f32(self.handle, ins, eb, _ptr(self.ws), self.ws.numel(), pays, tails,
+                                                   sides, caps, _ptr(out.res), _stream(stream)),
+               "fz_batch_compress_outlier_f32")
+
+    def compress_outlier(self, inputs, ebs, stream=None):
+        out = self.alloc_outputs()
+        self.compress_outlier_async(inputs, ebs, out, stream)
+        return out, read_results(out.res, self.nf)
+
+    def decompress_outlier_async(self, payloads, tails, sides, ebx2s, vconsts, outs, stream=None):
+        assert self.mode == MODE_OUTLIER
+        pays, tl, e, o = self._decompress_args(payloads, tails, ebx2s, outs)
+        src = [sd.clone() if sd.data_ptr() % 16 else sd for sd in sides]
+        sp = _ptr_array(src)
+        sb = (ctypes.c_uint64 * self.nf)(*[int(sd.numel()) for sd in src])
+        vc = (ctypes.c_float * self.nf)(*[float(v) for v in vconsts])
+        _check(lib().fz_batch_decompress_outlier_f32(self.handle, pays, tl, e, vc, sp, sb, _ptr(self.ws),
+                                                     self.ws.numel(), o, _stream(stream)),
+               "fz_batch_decompress_outlier_f32")
+        return src
+
+    def decompress_outlier(self, payloads, tails, sides, ebx2s, vconsts, stream=None):
+        import torch
+        outs = [torch.empty(max(s[0] * s[1] * s[2], 4), dtype=torch.float32, device=self.device)
+                for s in self.shapes]
+        self.decompress_outlier_async(payloads, tails, sides, ebx2s, vconsts, outs, stream)
         return [o[: s[0] * s[1] * s[2]] for o, s in zip(outs, self.shapes)]
 
     def decompress_graph(self, payloads, tails, ebx2s, outs):

@@ -55,6 +55,7 @@ class Archive:
     record: dict
     keep: object = None       # the batch buffers the views above live in
     buffer: object = None     # the server's OutputBuffer accounting for `keep`
+    side: object = None       # outlier mode: the side channel (its capacity slice)
     pins: int = 0             # waiting or running decompress requests that read this archive
 
     @property
@@ -92,15 +93,21 @@ class HbmBudget:
         assert self.used >= 0
 
 
-def request_bytes(req):
+def request_bytes(req, side_fraction=0.0):
     """(workspace, output) device bytes of a request, upper bounds: its share of the batch
     workspace (staging / codes, 4096 B per tile, plus counters) and its output (the archive
-    slot of a compress request, the floats of a decompress request)."""
+    slot of a compress request, the floats of a decompress request).  In the outlier mode
+    (side_fraction > 0) an archive also reserves that fraction of its largest side channel
+    (16 + 8 n bytes), and the workspace is the single-field outlier one (about 18 n bytes)."""
     n = req.n
     tiles = n // 2048 + 1
     w = (n + 255) // 256
     ws = 4096 * tiles + 32 * tiles + 1024
     out = (4 * w + 4096 * tiles + 4096 + 64) if req.kind == "compress" else 4 * n + 64
+    if side_fraction > 0:
+        ws = 18 * n + 16 * tiles + 65536
+        if req.kind == "compress":
+            out += 16 + 8 * math.ceil(n * side_fraction) + 256
     return ws, out
 
 
@@ -117,6 +124,8 @@ class Server:
     def __init__(self, backend, hbm_budget_bytes=200e9, slots=3, max_fields=256, max_archives=64,
                  clock=time.perf_counter, sleep=time.sleep):
         self.backend = backend
+        # the outlier mode reserves side-channel room per archive (DeviceBackend(mode=OUTLIER))
+        self.side_fraction = float(getattr(backend, "side_fraction", 0.0))
         self.budget = HbmBudget(hbm_budget_bytes)
         self.max_archives = int(max_archives)
         self.archives = collections.deque()  # (request, OutputBuffer) of compress results kept on the device
@@ -138,13 +147,16 @@ class Server:
         return self.clock() - self.t0
 
     def submit(self, req):
-        need = sum(request_bytes(req))
+        need = sum(self._bytes(req))
         if need > self.budget.total:
             raise ValueError(f"request {req.rid} needs more device memory than the whole budget")
         self.max_need = max(self.max_need, need)
         if req.kind == "decompress" and isinstance(req.source, Archive):
             req.source.pins += 1  # not evicted while this request waits or runs
         self.waiting[req.kind].append(req)
+
+    def _bytes(self, req):
+        return request_bytes(req, self.side_fraction)
 
     def _pinned_bytes(self):
         return sum({id(buf): buf.nbytes for req, buf in self.archives
@@ -156,7 +168,7 @@ class Server:
         q = self.waiting[kind]
         take, nbytes = [], 0
         while q and len(take) < self.max_fields:
-            need = sum(request_bytes(q[0]))
+            need = sum(self._bytes(q[0]))
             # archives are dropped only when dropping them makes the request fit; memory held
             # by running batches is not theirs to free
             if not self.budget.fits(nbytes + need) and \
@@ -204,7 +216,7 @@ class Server:
         for r, buf in self.archives:
             if r.result is None:
                 continue
-            need = max(self.max_need, sum(request_bytes(Request(-1, "decompress", r.dims))))
+            need = max(self.max_need, sum(self._bytes(Request(-1, "decompress", r.dims))))
             if id(buf) in pinned or held + buf.nbytes + need <= self.budget.total:
                 out.append(r)
         return out
@@ -245,7 +257,7 @@ class Server:
                     if self.on_complete:
                         self.on_complete(r)
                 self.completed.extend(reqs)
-                out_bytes = sum(request_bytes(r)[1] for r in reqs) if kind == "compress" else 0
+                out_bytes = sum(self._bytes(r)[1] for r in reqs) if kind == "compress" else 0
                 self.budget.give(nbytes - out_bytes)  # the workspace share; decompress outputs go to the caller
                 if kind == "decompress":
                     for r in reqs:
@@ -375,10 +387,13 @@ class DeviceBackend:
     copied into pinned host memory behind its kernels, so finishing the batch reads host
     memory and waits for nothing."""
 
-    def __init__(self, mode=0, nstreams=3, device="cuda", plan_pool_bytes=4 << 30):
+    def __init__(self, mode=0, nstreams=3, device="cuda", plan_pool_bytes=4 << 30, side_fraction=1.0):
         import torch
+        import fzgpu
         self.torch = torch
         self.mode = mode
+        # outlier mode: archives carry a side channel with room for this fraction of values
+        self.side_fraction = float(side_fraction) if mode == fzgpu.MODE_OUTLIER else 0.0
         self.device = device
         self.streams = [torch.cuda.Stream(device=device) for _ in range(nstreams)]
         self.busy = [False] * nstreams
@@ -392,7 +407,9 @@ class DeviceBackend:
         import fzgpu
         plan = self.pool.take(shapes)
         # a new plan sets up its workspace on the current stream
-        return plan if plan is not None else fzgpu.BatchPlan(list(shapes), mode=self.mode)
+        if plan is None:
+            plan = fzgpu.BatchPlan(list(shapes), mode=self.mode, side_fraction=self.side_fraction or 1.0)
+        return plan
 
     def launch(self, kind, reqs):
         torch = self.torch
@@ -405,7 +422,10 @@ class DeviceBackend:
             plan = self._take_plan(shapes)
             if kind == "compress":
                 out = plan.alloc_outputs()
-                plan.compress_async([r.source for r in reqs], [r.eb for r in reqs], out, stream=s)
+                if self.side_fraction:
+                    plan.compress_outlier_async([r.source for r in reqs], [r.eb for r in reqs], out, stream=s)
+                else:
+                    plan.compress_async([r.source for r in reqs], [r.eb for r in reqs], out, stream=s)
                 res_host = torch.empty(out.res.numel(), dtype=torch.uint8, pin_memory=True)
                 res_host.copy_(out.res, non_blocking=True)
                 keep = out
@@ -413,7 +433,12 @@ class DeviceBackend:
                 pays = [r.source.payload for r in reqs]
                 tails = [r.source.tail for r in reqs]
                 outs = [torch.empty(max(r.n, 4), dtype=torch.float32, device=self.device) for r in reqs]
-                plan.decompress_async(pays, tails, [r.source.record["ebx2"] for r in reqs], outs, stream=s)
+                ebx2 = [r.source.record["ebx2"] for r in reqs]
+                if self.side_fraction:  # side channels by capacity: the kernels read their counts
+                    plan.decompress_outlier_async(pays, tails, [r.source.side for r in reqs], ebx2,
+                                                  [r.source.record["vmin"] for r in reqs], outs, stream=s)
+                else:
+                    plan.decompress_async(pays, tails, ebx2, outs, stream=s)
                 keep = outs
             ev = torch.cuda.Event()
             ev.record(s)
@@ -432,7 +457,7 @@ class DeviceBackend:
             for i, r in enumerate(reqs):
                 rec = recs[i]
                 r.result = Archive(r.dims, out.payloads[i][: rec["payload_bytes"]], out.tails[i][: rec["tail_len"]],
-                                   rec, keep=out)
+                                   rec, keep=out, side=out.sides[i] if out.sides else None)
         else:
             for i, r in enumerate(reqs):
                 r.result = out[i][: r.n]

@@ -55,7 +55,7 @@ typedef enum fz_status {
  * (never encoded by the reference, fz.cu:173) are returned in the tail buffer.
  * FZ_MODE_LITERAL: byte-identical to the reference, including both losses.
  * 2D and 3D payloads are identical in the two modes.
- * FZ_MODE_OUTLIER (opt-in, format-changing; single-field entry points only): the FIXED
+ * FZ_MODE_OUTLIER (opt-in, format-changing; the outlier entry points only): the FIXED
  * payload and tail plus a side channel, modelled on cuSZ's sparse outlier vector
  * (spcodec_vec.hh:29), so that every field compresses and every value comes back: deltas
  * with |d| >= 32768 (code 0 in the payload, exact delta in the channel), non-finite values and
@@ -370,6 +370,26 @@ fz_status fz_batch_compress_f32(const fz_batch_plan plan, const float* const* d_
 fz_status fz_batch_decompress_f32(const fz_batch_plan plan, const uint8_t* const* d_payload,
                                   const uint16_t* const* d_tail, const float* ebx2, void* d_ws, size_t ws_bytes,
                                   float* const* d_out, hipStream_t stream);
+
+/* OUTLIER-mode batches (a batch plan created with FZ_MODE_OUTLIER; the two calls above refuse
+ * such a plan): each field's payload, tail, side channel and record equal
+ * fz_compress_outlier_f32 on that field alone (fz_sizes.side_max_bytes from
+ * fz_batch_field_sizes bounds its channel; a smaller capacity may give that field
+ * FZ_ERR_CAPACITY).  No counterpart in the reference; the side channel follows cuSZ's
+ * SpcodecVec (spcodec_vec.hh:29).  The fields run one after another on `stream`, in one
+ * workspace sized for the largest field, through the single-field outlier kernels, so a batch
+ * of fields with large deltas, constant values or NaN / Inf compresses where the FIXED batch
+ * reports them as failed.  Kernels only: the calls can be captured into a hipGraph.  For the
+ * inverse, side_bytes[i] is the channel's size (16 + 8 (n_delta + n_raw)) or any capacity at
+ * least that large; ebx2 and vconst come from each field's record (ebx2, vmin). */
+fz_status fz_batch_compress_outlier_f32(const fz_batch_plan plan, const float* const* d_in, const double* eb_rel,
+                                        void* d_ws, size_t ws_bytes, uint8_t* const* d_payload,
+                                        uint16_t* const* d_tail, uint8_t* const* d_side,
+                                        const uint64_t* side_capacity, fz_result* d_results, hipStream_t stream);
+fz_status fz_batch_decompress_outlier_f32(const fz_batch_plan plan, const uint8_t* const* d_payload,
+                                          const uint16_t* const* d_tail, const float* ebx2, const float* vconst,
+                                          const uint8_t* const* d_side, const uint64_t* side_bytes, void* d_ws,
+                                          size_t ws_bytes, float* const* d_out, hipStream_t stream);
 
 /* Times the tuning candidates on these buffers (reps runs each, per-kernel events), keeps
  * the fastest in the plan and returns it in *chosen (may be NULL).  Waits for the stream. */

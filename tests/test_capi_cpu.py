@@ -135,15 +135,34 @@ def test_verify_size_query_and_argument_checks_need_no_device():
     assert L.fz_decompress_verify_f32(p.handle, None, None, 1.0, None, 0, None, None, None, None) == 1
 
 
-def test_outlier_mode_sizes_and_batch_refusal():
+def test_outlier_mode_sizes_single_and_batched():
     p = fzgpu.Plan(100, 77, 1, mode=fzgpu.MODE_OUTLIER, workspace=False)
     assert p.sizes.side_max_bytes == 16 + 8 * 7700
     q = fzgpu.Plan(100, 77, 1, workspace=False)
     assert q.sizes.side_max_bytes == 0
     assert p.sizes.workspace_bytes > q.sizes.workspace_bytes  # unit counts and int32 deltas
-    with pytest.raises(fzgpu.FzError) as e:
-        fzgpu.BatchPlan([(5000, 1, 1)], mode=fzgpu.MODE_OUTLIER, workspace=False)
-    assert e.value.status == 1
+    # an OUTLIER batch runs the single-field outlier launches in one workspace sized for its
+    # largest field; each field's sizes are those of its single-field plan
+    shapes = [(5000, 1, 1), (100, 77, 1), (33, 20, 17)]
+    bp = fzgpu.BatchPlan(shapes, mode=fzgpu.MODE_OUTLIER, workspace=False)
+    singles = [fzgpu.Plan(*s, mode=fzgpu.MODE_OUTLIER, workspace=False).sizes for s in shapes]
+    assert bp.sizes.workspace_bytes == max(t.workspace_bytes for t in singles)
+    for i, t in enumerate(singles):
+        fs = bp.field_sizes(i)
+        assert (fs.side_max_bytes, fs.payload_max_bytes, fs.tail_max_codes) == \
+            (t.side_max_bytes, t.payload_max_bytes, t.tail_max_codes)
+    half = fzgpu.BatchPlan(shapes, mode=fzgpu.MODE_OUTLIER, workspace=False, side_fraction=0.5)
+    assert half.side_capacity(1) == 16 + 8 * 3850
+    # the FIXED batch entry points refuse an OUTLIER plan, the OUTLIER ones a FIXED plan
+    L = fzgpu.lib()
+    fake = ctypes.c_void_p(1 << 20)
+    arr = (ctypes.c_void_p * 3)(fake, fake, fake)
+    ebs = (ctypes.c_double * 3)(1e-3, 1e-3, 1e-3)
+    assert L.fz_batch_compress_f32(bp.handle, arr, ebs, fake, 1 << 40, arr, arr, fake, None) == 1
+    fixed = fzgpu.BatchPlan(shapes, workspace=False)
+    caps = (ctypes.c_uint64 * 3)(64, 64, 64)
+    assert L.fz_batch_compress_outlier_f32(fixed.handle, arr, ebs, fake, 1 << 40, arr, arr, arr, caps, fake,
+                                           None) == 1
 
 
 # ---------------------------------------------------------------- archive input checks (host only)

@@ -122,3 +122,51 @@ def test_saturating_stream_shares_batches_under_the_budget():
     assert sum(r.dims == big for r in done) >= 5
     assert st["decompress"] >= 1
     _check_alone(warm + done, kept)
+
+
+def test_outlier_mode_serving_handles_fields_the_fixed_mode_refuses():
+    """A served stream in the outlier mode over a pool with a constant field, a field with NaN
+    and +-Inf and a field with deltas beyond the u16 code: every request completes (status 0)
+    and equals the single-field outlier entry point on the same input; decompress requests
+    restore the non-finite values bit for bit."""
+    import torch
+    pool = []
+    rng = np.random.default_rng(9)
+    for i, d in enumerate([(300, 301, 1), (65536, 1, 1), (41, 37, 29), (2001, 333, 1)]):
+        x = D.generate_gpu(D.BATCH_KINDS[i % 3], d, 90 + i)
+        if i == 0:
+            x.fill_(1.75)
+        elif i == 1:
+            x[5] = float("nan")
+            x[9] = float("inf")
+            x[11] = float("-inf")
+        elif i == 2:
+            x[rng.integers(0, x.numel(), 4)] = 1e7
+        pool.append((d, x))
+    reqs = S.make_requests(4000.0, 0.02, pool, seed=21)
+    srv = S.Server(S.DeviceBackend(mode=fzgpu.MODE_OUTLIER, nstreams=3), slots=3, max_fields=8, max_archives=10_000)
+    kept = {}
+    done = srv.run(reqs, realtime=True, on_arrival=S.pick_archive(22),
+                   on_complete=lambda r: kept.__setitem__(r.rid, r.result) if r.kind == "compress" else None)
+    assert len(done) == len(reqs)
+    say(f"outlier serving: {len(done)} requests, {len(srv.batches)} batches, max in flight {srv.max_inflight}")
+    src_of = {id(kept[r.rid]): r for r in done if r.kind == "compress"}
+    for r in done:
+        plan = fzgpu.Plan(*r.dims, mode=fzgpu.MODE_OUTLIER)
+        if r.kind == "compress":
+            a = kept[r.rid]
+            assert a.record["status"] == 0, r.rid
+            p, t, rec, side = plan.compress_outlier(r.source, r.eb)
+            assert a.record == rec, r.rid
+            assert torch.equal(a.payload, p) and torch.equal(a.tail, t), r.rid
+            assert torch.equal(a.side[: side.numel()], side), r.rid
+        else:
+            a = r.source
+            x = src_of[id(a)].source
+            fin = torch.isfinite(x)
+            assert torch.equal(r.result[~fin].view(torch.int32), x[~fin].view(torch.int32)), r.rid
+            if a.record["ebx2"] == 0.0:
+                assert torch.equal(r.result, x), r.rid
+            else:
+                err = float((r.result[fin].double() - x[fin].double()).abs().max())
+                assert err <= a.record["eb_abs"] + 4 * 2.0 ** -24 * float(x[fin].abs().max()), r.rid
