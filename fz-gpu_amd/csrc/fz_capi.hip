@@ -26,8 +26,8 @@ struct Layout {
 // Workspace regions, 256-byte aligned.  State words are shared between kernels of one call:
 //   tstate  fused compress: per-placement-run block sums (u32), cleared by k_range_part;
 //   sstate  u32 flag scan of the plain pipeline: look-back states, cleared by the range kernel;
-//   dstate  decompress tile scan: look-back states (u64 flag|value), or with the two-kernel
-//           form (<= 65536 tiles) raw u32 chunk counts.  Its contents are undefined after any
+//   dstate  decompress tile offsets: look-back states (u64 flag|value), or (<= 65536 tiles,
+//           launch_tile_offsets) raw u32 chunk counts.  Its contents are undefined after any
 //           decompress, so every look-back user clears it first (launch_tile_scan with clear,
 //           or k_put_deltas in the outlier inverse).
 Layout make_layout(const Geom& g, int mode)
@@ -325,15 +325,16 @@ fz_status fz_decompress_f32(const fz_plan plan, const uint8_t* d_payload, const 
     uint32_t*     codes = reinterpret_cast<uint32_t*>(ws + L.codes);
     const uint16_t* tail = plan->mode != FZ_MODE_LITERAL ? d_tail : nullptr;
     FZ_TRY(prof_mark(plan, stream, true));
-    FZ_TRY(launch_tile_scan(reinterpret_cast<const uint32_t*>(d_payload), g.T, texcl, dstate, L.n_dstate,
-                            &ctr->dscan_ticket, stream));
+    const uint32_t* csums = nullptr;
+    FZ_TRY(launch_tile_offsets(reinterpret_cast<const uint32_t*>(d_payload), g.T, texcl, dstate, L.n_dstate,
+                               &ctr->dscan_ticket, stream, &csums));
     FZ_TRY(prof_mark(plan, stream, false));
     if (g.ndim == 1) {  // 1D Lorenzo blocks lie inside a tile: one fused launch
-        FZ_TRY(launch_decode_recon_1d(d_payload, texcl, tail, g, ebx2, d_out, stream));
+        FZ_TRY(launch_decode_recon_1d(d_payload, texcl, csums, tail, g, ebx2, d_out, stream));
         FZ_TRY(prof_mark(plan, stream, false));
         return FZ_SUCCESS;
     }
-    FZ_TRY(launch_decode_unshuffle(d_payload, texcl, tail, g, codes, stream));
+    FZ_TRY(launch_decode_unshuffle(d_payload, texcl, csums, tail, g, codes, stream));
     FZ_TRY(prof_mark(plan, stream, false));
     FZ_TRY(launch_reconstruct(reinterpret_cast<const uint16_t*>(codes), d_out, g, ebx2, stream));
     FZ_TRY(prof_mark(plan, stream, false));
@@ -390,7 +391,7 @@ fz_status fz_decompress_outlier_f32(const fz_plan plan, const uint8_t* d_payload
     FZ_TRY(launch_put_deltas(d_side, side_bytes, obits, over, g, dstate, L.n_dstate, &ctr->dscan_ticket, stream));
     FZ_TRY(launch_tile_scan(reinterpret_cast<const uint32_t*>(d_payload), g.T, texcl, dstate, L.n_dstate,
                             &ctr->dscan_ticket, stream, false));
-    FZ_TRY(launch_decode_unshuffle(d_payload, texcl, d_tail, g, codes, stream));
+    FZ_TRY(launch_decode_unshuffle(d_payload, texcl, nullptr, d_tail, g, codes, stream));
     FZ_TRY(launch_reconstruct_outlier(reinterpret_cast<const uint16_t*>(codes), obits, over, d_side, side_bytes, d_out,
                                       g, ebx2, stream));
     FZ_TRY(prof_mark(plan, stream, false));
@@ -1249,7 +1250,7 @@ fz_status fz_batch_decompress_outlier_f32(const fz_batch_plan plan, const uint8_
                                  stream));
         FZ_TRY(launch_tile_scan(reinterpret_cast<const uint32_t*>(d_payload[i]), g.T, texcl, dstate, L.n_dstate,
                                 &ctr->dscan_ticket, stream, false));
-        FZ_TRY(launch_decode_unshuffle(d_payload[i], texcl, tail, g, codes, stream));
+        FZ_TRY(launch_decode_unshuffle(d_payload[i], texcl, nullptr, tail, g, codes, stream));
         FZ_TRY(launch_reconstruct_outlier(reinterpret_cast<const uint16_t*>(codes), obits, over, d_side[i],
                                           side_bytes[i], d_out[i], g, ebx2[i], stream));
     }

@@ -77,17 +77,22 @@ hipError_t launch_scan_init(uint64_t* states, uint32_t nstates, uint32_t* ticket
 hipError_t launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* states, uint32_t* ticket,
                        bool write_total, fz_result* res, uint32_t W, hipStream_t s);
 hipError_t launch_encode(const uint8_t* in, uint8_t* out, const uint32_t* presum, uint32_t nblocks, hipStream_t s);
-// tile_excl[t] = nonzero blocks of tiles < t.  With `clear`: up to 65536 tiles two kernels
-// (chunk counts, then chunk scans; the states hold the counts), beyond that a clearing launch
-// and the look-back scan.  Without `clear` the caller has cleared the nstates states and the
-// ticket: the look-back scan alone.
+// tile_excl[t] = nonzero blocks of tiles < t by the look-back scan; with `clear` after a
+// launch that clears the nstates states and the ticket, else the caller has cleared them.
 hipError_t launch_tile_scan(const uint32_t* bitflags, uint32_t T, uint32_t* tile_excl, uint64_t* states,
                             uint32_t nstates, uint32_t* ticket, hipStream_t s, bool clear = true);
-hipError_t launch_decode_unshuffle(const uint8_t* payload, const uint32_t* tile_excl, const uint16_t* tail,
-                                   const Geom& g, uint32_t* codes, hipStream_t s);
+// The tile offsets the decode kernels take, in one launch up to 65536 tiles: tile_excl[t] =
+// nonzero blocks of the tiles before t in its 256-tile chunk, *chunk_sums = the chunk counts
+// (in the states' memory); beyond that the look-back scan's global offsets and *chunk_sums =
+// nullptr.
+hipError_t launch_tile_offsets(const uint32_t* bitflags, uint32_t T, uint32_t* tile_excl, uint64_t* states,
+                               uint32_t nstates, uint32_t* ticket, hipStream_t s, const uint32_t** chunk_sums);
+// chunk_sums: as launch_tile_offsets gives it (nullptr: tile_excl holds global offsets)
+hipError_t launch_decode_unshuffle(const uint8_t* payload, const uint32_t* tile_excl, const uint32_t* chunk_sums,
+                                   const uint16_t* tail, const Geom& g, uint32_t* codes, hipStream_t s);
 // 1D fields: decode + reconstruct to floats in one launch (replaces the two above)
-hipError_t launch_decode_recon_1d(const uint8_t* payload, const uint32_t* tile_excl, const uint16_t* tail,
-                                  const Geom& g, float ebx2, float* out, hipStream_t s);
+hipError_t launch_decode_recon_1d(const uint8_t* payload, const uint32_t* tile_excl, const uint32_t* chunk_sums,
+                                  const uint16_t* tail, const Geom& g, float ebx2, float* out, hipStream_t s);
 hipError_t launch_reconstruct(const uint16_t* codes, float* out, const Geom& g, float ebx2, hipStream_t s);
 // reconstruct from u16 codes whose positions flagged in `bits` take their delta from `over`
 hipError_t launch_reconstruct_outlier_src(const uint16_t* codes, const uint32_t* bits, const int32_t* over,

@@ -604,11 +604,11 @@ __global__ __launch_bounds__(kWG) void k_tile_scan(const uint32_t* __restrict__ 
     if (t < T) tile_excl[t] = sh_excl + woff + incl - cnt;
 }
 
-// The same scan without look-back for up to 256 chunks of 256 tiles: k_tile_sums writes each
-// chunk's block count, k_tile_scan2 adds the counts of the chunks before its own (one per
-// thread, at most 256) to an in-chunk scan.  Nothing to clear and no waiting between
-// workgroups.
-constexpr uint32_t kTileScan2MaxChunks = kWG;
+// Tile offsets without look-back for up to 256 chunks of 256 tiles, in one launch:
+// k_tile_local writes each tile's offset within its chunk and each chunk's block count; the
+// decode kernels add the counts of the chunks before the tile's own (tile_block_offset).
+// Nothing to clear and no waiting between workgroups.
+constexpr uint32_t kTileLocalMaxChunks = 256;
 
 __device__ __forceinline__ uint32_t tile_popcount(const uint32_t* __restrict__ bitflags, uint32_t t)
 {
@@ -636,50 +636,54 @@ __device__ __forceinline__ uint32_t block_excl(uint32_t v, uint32_t& total)
     return pre + incl - v;
 }
 
-__global__ __launch_bounds__(kWG) void k_tile_sums(const uint32_t* __restrict__ bitflags, uint32_t T,
-                                                   uint32_t* __restrict__ chunk_sums)
+__global__ __launch_bounds__(kWG) void k_tile_local(const uint32_t* __restrict__ bitflags, uint32_t T,
+                                                    uint32_t* __restrict__ tile_excl, uint32_t* __restrict__ chunk_sums)
 {
     const uint32_t t = blockIdx.x * kWG + threadIdx.x;
     uint32_t       total;
-    (void)block_excl(t < T ? tile_popcount(bitflags, t) : 0u, total);
+    const uint32_t excl = block_excl(t < T ? tile_popcount(bitflags, t) : 0u, total);
+    if (t < T) tile_excl[t] = excl;
     if (threadIdx.x == 0) chunk_sums[blockIdx.x] = total;
-}
-
-__global__ __launch_bounds__(kWG) void k_tile_scan2(const uint32_t* __restrict__ bitflags, uint32_t T,
-                                                    const uint32_t* __restrict__ chunk_sums,
-                                                    uint32_t* __restrict__ tile_excl)
-{
-    __shared__ uint32_t sh_base;
-    const uint32_t t = blockIdx.x * kWG + threadIdx.x;
-    const uint32_t cnt = t < T ? tile_popcount(bitflags, t) : 0u;
-    uint32_t       base;
-    (void)block_excl(threadIdx.x < blockIdx.x ? chunk_sums[threadIdx.x] : 0u, base);
-    if (threadIdx.x == 0) sh_base = base;
-    __syncthreads();  // block_excl's LDS is reused below
-    uint32_t       total;
-    const uint32_t excl = block_excl(cnt, total);
-    if (t < T) tile_excl[t] = sh_base + excl;
 }
 
 hipError_t launch_tile_scan(const uint32_t* bitflags, uint32_t T, uint32_t* tile_excl, uint64_t* states,
                             uint32_t nstates, uint32_t* ticket, hipStream_t s, bool clear)
 {
     if (T == 0) return hipSuccess;
-    const uint32_t nchunks = (T + kWG - 1) / kWG;
-    // with the states already cleared by the caller (a launch it needed anyway) the look-back
-    // scan is one launch; otherwise the two-kernel scan saves the clearing launch
-    if (clear && nchunks <= kTileScan2MaxChunks) {  // the state words hold the chunk counts
-        uint32_t* sums = reinterpret_cast<uint32_t*>(states);
-        FZ_LAUNCH(k_tile_sums<<<nchunks, kWG, 0, s>>>(bitflags, T, sums));
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        FZ_LAUNCH(k_tile_scan2<<<nchunks, kWG, 0, s>>>(bitflags, T, sums, tile_excl));
-        return hipGetLastError();
-    }
     const hipError_t e = clear ? launch_scan_init(states, nstates, ticket, s) : hipSuccess;
     if (e != hipSuccess) return e;
     FZ_LAUNCH(k_tile_scan<<<(T + kWG - 1) / kWG, kWG, 0, s>>>(bitflags, T, tile_excl, states, ticket));
     return hipGetLastError();
+}
+
+hipError_t launch_tile_offsets(const uint32_t* bitflags, uint32_t T, uint32_t* tile_excl, uint64_t* states,
+                               uint32_t nstates, uint32_t* ticket, hipStream_t s, const uint32_t** chunk_sums)
+{
+    *chunk_sums = nullptr;
+    if (T == 0) return hipSuccess;
+    const uint32_t nchunks = (T + kWG - 1) / kWG;
+    if (nchunks > kTileLocalMaxChunks) return launch_tile_scan(bitflags, T, tile_excl, states, nstates, ticket, s, true);
+    uint32_t* sums = reinterpret_cast<uint32_t*>(states);  // the state words hold the chunk counts
+    FZ_LAUNCH(k_tile_local<<<nchunks, kWG, 0, s>>>(bitflags, T, tile_excl, sums));
+    *chunk_sums = sums;
+    return hipGetLastError();
+}
+
+// The payload block index of `tile`'s first nonzero block, by all 64 lanes of a wave:
+// tile_excl[tile], plus (with chunk_sums) the counts of the 256-tile chunks before its own.
+__device__ __forceinline__ uint32_t tile_block_offset(const uint32_t* __restrict__ tile_excl,
+                                                      const uint32_t* __restrict__ chunk_sums, uint32_t tile)
+{
+    const uint32_t v = tile_excl[tile];
+    if (!chunk_sums) return v;
+    const uint32_t c = tile / kWG;  // < kTileLocalMaxChunks
+    uint32_t       s = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kTileLocalMaxChunks / 64; i++) {
+        const uint32_t k = lane_id() + 64 * i;
+        s += k < c ? chunk_sums[k] : 0u;
+    }
+    return v + wave_sum(s);
 }
 
 // Inverse of the tile compress: the extra workgroup writes the tail codes (or zeros when
@@ -687,18 +691,19 @@ hipError_t launch_tile_scan(const uint32_t* bitflags, uint32_t T, uint32_t* tile
 // flight as with a workgroup per tile, and all-zero tiles only store zeros.
 __global__ __launch_bounds__(64) void k_decode_unshuffle(const uint8_t* __restrict__ payload,
                                                          const uint32_t* __restrict__ tile_excl,
+                                                         const uint32_t* __restrict__ chunk_sums,
                                                          const uint16_t* __restrict__ tail, Geom g,
                                                          uint32_t* __restrict__ codes)
 {
     const uint32_t tile = blockIdx.x;
     if (tile >= g.T) decode_tail(tail, g, reinterpret_cast<uint16_t*>(codes));
-    else decode_tile_w(payload, g.W, tile, tile_excl[tile], codes + (size_t)tile * kTileWords);
+    else decode_tile_w(payload, g.W, tile, tile_block_offset(tile_excl, chunk_sums, tile), codes + (size_t)tile * kTileWords);
 }
 
-hipError_t launch_decode_unshuffle(const uint8_t* payload, const uint32_t* tile_excl, const uint16_t* tail,
-                                   const Geom& g, uint32_t* codes, hipStream_t s)
+hipError_t launch_decode_unshuffle(const uint8_t* payload, const uint32_t* tile_excl, const uint32_t* chunk_sums,
+                                   const uint16_t* tail, const Geom& g, uint32_t* codes, hipStream_t s)
 {
-    FZ_LAUNCH(k_decode_unshuffle<<<g.T + 1, 64, 0, s>>>(payload, tile_excl, tail, g, codes));
+    FZ_LAUNCH(k_decode_unshuffle<<<g.T + 1, 64, 0, s>>>(payload, tile_excl, chunk_sums, tail, g, codes));
     return hipGetLastError();
 }
 
@@ -706,17 +711,19 @@ hipError_t launch_decode_unshuffle(const uint8_t* payload, const uint32_t* tile_
 // takes the trailer), straight to floats; no code array in between.
 __global__ __launch_bounds__(64) void k_decode_recon_1d(const uint8_t* __restrict__ payload,
                                                         const uint32_t* __restrict__ tile_excl,
+                                                        const uint32_t* __restrict__ chunk_sums,
                                                         const uint16_t* __restrict__ tail, Geom g, float ebx2,
                                                         float* __restrict__ out)
 {
     const uint32_t tile = blockIdx.x;
-    decode_recon_1d_w(payload, tail, g, tile, tile < g.T ? tile_excl[tile] : 0u, ebx2, out);
+    decode_recon_1d_w(payload, tail, g, tile, tile < g.T ? tile_block_offset(tile_excl, chunk_sums, tile) : 0u, ebx2,
+                      out);
 }
 
-hipError_t launch_decode_recon_1d(const uint8_t* payload, const uint32_t* tile_excl, const uint16_t* tail,
-                                  const Geom& g, float ebx2, float* out, hipStream_t s)
+hipError_t launch_decode_recon_1d(const uint8_t* payload, const uint32_t* tile_excl, const uint32_t* chunk_sums,
+                                  const uint16_t* tail, const Geom& g, float ebx2, float* out, hipStream_t s)
 {
-    FZ_LAUNCH(k_decode_recon_1d<<<g.T + 1, 64, 0, s>>>(payload, tile_excl, tail, g, ebx2, out));
+    FZ_LAUNCH(k_decode_recon_1d<<<g.T + 1, 64, 0, s>>>(payload, tile_excl, chunk_sums, tail, g, ebx2, out));
     return hipGetLastError();
 }
 
