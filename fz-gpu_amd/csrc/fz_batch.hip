@@ -416,7 +416,9 @@ __global__ __launch_bounds__(64) void k_batch_decode(BatchDev d, uint32_t nf)
 template <int NDIM>
 __global__ __launch_bounds__(kWG) void k_batch_recon(BatchDev d, uint32_t r_lo)
 {
-    const uint32_t wg = r_lo + blockIdx.x;
+    // 3D: XCD-sliced order, so that the workgroups of neighbouring narrow row segments (down
+    // to 32 bytes) share one L2 and fetch each line once
+    const uint32_t wg = r_lo + (NDIM == 3 ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x);
     const uint32_t f = d.rfield[wg];  // uniform: one scalar load (a search would chain two)
     const BField&  F = d.fields[f];
     const Geom     g = F.g;

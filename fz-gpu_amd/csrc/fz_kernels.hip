@@ -746,7 +746,8 @@ template <typename SRC>
 __global__ __launch_bounds__(kWG) void k_reconstruct_3d(const SRC codes, float* __restrict__ out, Geom g, float ebx2,
                                                         int q, uint32_t rx, uint32_t ry)
 {
-    recon_3d_any(q, codes, out, g, ebx2, blockIdx.x, rx, ry);
+    // XCD-sliced order: neighbouring narrow row segments (down to 32 bytes) share one L2
+    recon_3d_any(q, codes, out, g, ebx2, xcd_tile(blockIdx.x, gridDim.x), rx, ry);
 }
 
 template <typename SRC>
