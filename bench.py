@@ -24,6 +24,8 @@ sys.path.insert(0, os.path.join(ROOT, "fz-gpu_amd"))
 # thread placement of the CPU baseline's OpenMP runtime, fixed before any library starts one
 os.environ.setdefault("OMP_PROC_BIND", "close")
 os.environ.setdefault("OMP_PLACES", "cores")
+# the process's CPU set before an OpenMP runtime binds this thread to one core
+AFFINITY_AT_START = len(os.sched_getaffinity(0))
 
 HBM_PEAK_GBPS = 8000.0
 WORKLOAD_TEXT = {
@@ -409,7 +411,7 @@ def _host_info():
                 model = ln.split(":", 1)[1].strip()
     except Exception:
         pass
-    return {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)), "lscpu_model": model,
+    return {"nproc": os.cpu_count(), "affinity_cpus": AFFINITY_AT_START, "lscpu_model": model,
             "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"), "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"),
             "OMP_PLACES": os.environ.get("OMP_PLACES")}
 
