@@ -349,9 +349,10 @@ def test_error_bound_on_the_baseline_workloads(name, eb):
                                          ((65537 * 2048 + 100, 1, 1), 0),
                                          ((16384, 8192, 1), 65537 * 2048 + 100)])
 def test_decompress_at_the_tile_scan_switch(dims, length):
-    """Up to 65536 tiles the decompress scans tile counts in two kernels (chunk counts, then
-    chunk scans reading up to 255 chunk sums), beyond that with the look-back scan: the last
-    two-kernel case and the first look-back case, 1D and 2D, bit-exact against the oracle."""
+    """Up to 65536 tiles the decompress takes chunk-local tile offsets from one kernel and adds
+    up to 255 chunk counts in the decode kernels, beyond that the look-back scan's global
+    offsets: the last chunk-local case and the first look-back case, 1D and 2D, bit-exact
+    against the oracle."""
     n = length or dims[0] * dims[1] * dims[2]
     T = n // 2048
     assert T in (65536, 65537)
