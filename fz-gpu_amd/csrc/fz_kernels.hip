@@ -727,6 +727,234 @@ hipError_t launch_decode_recon_1d(const uint8_t* payload, const uint32_t* tile_e
     return hipGetLastError();
 }
 
+// 2D fields: decode and reconstruct in one 1024-thread workgroup per window of 16 rows x 1024
+// columns (one Lorenzo block row, 64 blocks), straight to floats; the codes never reach HBM.
+//
+// Decode.  The window's codes come into LDS by quad groups: QG q holds codes [256 q, 256 q +
+// 256), the four 32-word transpose groups of tile q / 8, whose bit planes sit in blocks
+// 8 b + q % 8 (b < 32) of that tile.  A half-wave (32 lanes) decodes one QG: lane b finds
+// block 8 b + m's flag and rank from the tile's 8 bitflag words, loads it, stages it as bit
+// plane row b in LDS, and after the in-register transpose lane j = 8 qq + L holds codes
+// [256 q + 8 j, + 8), which it puts at their window positions.  The window needs the QGs its
+// 16 row segments cover, each once (a QG shared by two rows of a narrow field is listed for
+// the first): at most 80, and the 32 half-waves take up to 3 each, all loads in flight
+// together.
+//
+// Reconstruct.  Thread (g, c) = (tid / 256, tid % 256) holds columns 4c .. 4c + 3 of rows
+// 4g .. 4g + 3: the column sums of its 4 rows go through LDS, so each group starts from the
+// sums of the groups above it (prefix over y), then its rows run as in recon_2d (prefix over
+// x in the thread and its quad, two DPP steps).
+constexpr uint32_t kD2Threads = 1024;
+constexpr uint32_t kD2Cols = 1024;
+constexpr uint32_t kD2MaxQ = 16 * (kD2Cols / 256 + 1);
+constexpr uint32_t kD2Batch = 3;  // QGs per half-wave in flight together (32 x 3 >= kD2MaxQ)
+constexpr uint32_t kQgPitch = 5;
+
+__global__ __launch_bounds__(kD2Threads) void k_decode_recon_2d(const uint8_t* __restrict__ payload,
+                                                                const uint32_t* __restrict__ tile_excl,
+                                                                const uint32_t* __restrict__ chunk_sums,
+                                                                const uint16_t* __restrict__ tail, Geom g, float ebx2,
+                                                                float* __restrict__ out, uint32_t rx)
+{
+    constexpr uint32_t NH = kD2Threads / 32;
+    __shared__ uint16_t win[16 * kD2Cols];            // the window's codes, row-major
+    __shared__ uint32_t stage[NH][32 * kQgPitch];      // one QG's bit-plane rows per half-wave
+    __shared__ int4     csum[3][kD2Cols / 4];          // column sums of row groups 0-2
+    __shared__ uint32_t cpre[kTileLocalMaxChunks];     // block counts of the chunks before each chunk
+    __shared__ uint32_t wtot[4], qtab[kD2MaxQ], nq_s;  // the QGs to decode
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint32_t bx = blockIdx.x % rx, by = blockIdx.x / rx;
+    const uint32_t xw0 = bx * kD2Cols, y0 = by * 16;
+    const uint32_t cw = min(kD2Cols, g.X - xw0), rows = min(16u, g.Y - y0);
+    // set-up: the chunk prefix (waves 0-3) and the QG list (wave 0)
+    uint32_t cv = 0, cincl = 0;
+    if (tid < kTileLocalMaxChunks) {
+        cv = (chunk_sums && tid < (g.T + kWG - 1) / kWG) ? chunk_sums[tid] : 0u;
+        cincl = wave_incl_scan(cv);
+        if (lane == 63) wtot[wv] = cincl;
+    }
+    if (wv == 0) {
+        // lane r < rows: QGs [a, b] of row segment r less those of row r - 1 (a and b grow with r)
+        const bool     valid = lane < rows;
+        const uint32_t p0 = (y0 + lane) * g.X + xw0;  // < x*y*z < 2^32 on the valid lanes
+        const uint32_t a = p0 >> 8, b = (p0 + cw - 1) >> 8;
+        const uint32_t bprev = __shfl_up(b, 1, 64);
+        const uint32_t start = lane > 0 ? max(a, bprev + 1) : a;
+        const uint32_t c = (valid && b + 1 > start) ? b + 1 - start : 0u;
+        const uint32_t incl = wave_incl_scan(c);
+        for (uint32_t j = 0; j < c; j++) qtab[incl - c + j] = start + j;
+        if (lane == 63) nq_s = incl;
+    }
+    __syncthreads();
+    if (tid < kTileLocalMaxChunks) {
+        uint32_t pre = 0;
+        for (uint32_t k = 0; k < wv; k++) pre += wtot[k];
+        cpre[tid] = pre + cincl - cv;
+    }
+    __syncthreads();
+    const uint32_t  nq = nq_s;
+    const uint32_t  half = lane >> 5, hl = lane & 31, H = 2 * wv + half;
+    uint32_t* const st = stage[H];
+    const uint32_t* bflags = reinterpret_cast<const uint32_t*>(payload);
+    // both half-waves of a wave step together (the transpose needs all 64 lanes active)
+    for (uint32_t i0 = 0; i0 < nq; i0 += kD2Batch * NH) {
+        uint32_t q[kD2Batch], fw[kD2Batch], ex[kD2Batch];
+        bool     live[kD2Batch];
+#pragma unroll
+        for (int b = 0; b < (int)kD2Batch; b++) {
+            const uint32_t i = i0 + H + b * NH;
+            live[b] = i < nq;
+            q[b] = live[b] ? qtab[i] : 0u;
+            const uint32_t t = q[b] >> 3;
+            const bool     in = live[b] && t < g.T;
+            fw[b] = in ? bflags[8 * t + (hl & 7u)] : 0u;
+            ex[b] = in ? tile_excl[t] + (chunk_sums ? cpre[t / kWG] : 0u) : 0u;
+        }
+        uint4 blk[kD2Batch];
+#pragma unroll
+        for (int b = 0; b < (int)kD2Batch; b++) {
+            // flag and rank of block 8 hl + m (flag word hl / 4, bit 8 (hl % 4) + m); word j is
+            // held by the half's lanes with hl % 8 == j
+            const uint32_t pc = __popc(fw[b]);
+            uint32_t       incl = pc;
+#pragma unroll
+            for (int d = 1; d < 8; d <<= 1) {
+                const uint32_t o = __shfl_up(incl, d, 8);
+                if ((hl & 7u) >= (uint32_t)d) incl += o;
+            }
+            const uint32_t src = (lane & 32u) + (hl >> 2);
+            const uint32_t wj = __shfl(fw[b], src, 64), ej = __shfl(incl - pc, src, 64);
+            const uint32_t bit = 8 * (hl & 3u) + (q[b] & 7u);
+            blk[b] = make_uint4(0, 0, 0, 0);
+            if ((wj >> bit) & 1u)
+                blk[b] = ld16(payload + 4ull * g.W + 16ull * ((uint64_t)ex[b] + ej + __popc(wj & ((1u << bit) - 1u))));
+        }
+#pragma unroll
+        for (int b = 0; b < (int)kD2Batch; b++) {
+            const uint32_t t = q[b] >> 3;
+            uint32_t       w[4] = {0, 0, 0, 0};
+            if (ballot64(live[b] && t < g.T) != 0) {  // uniform over the wave
+                uint32_t* row = st + kQgPitch * hl;
+                row[0] = blk[b].x;
+                row[1] = blk[b].y;
+                row[2] = blk[b].z;
+                row[3] = blk[b].w;
+                wave_lds_sync();
+                const uint32_t qq = hl >> 3, L = hl & 7u;
+#pragma unroll
+                for (int c = 0; c < 4; c++) w[c] = st[kQgPitch * (4 * L + c) + qq];
+                wave_lds_sync();  // the rows are rewritten by the next QG
+                transpose4(w, L);
+            }
+            if (!live[b]) continue;
+            const uint32_t p = 256 * q[b] + 8 * hl;
+            if (t >= g.T) {  // after the last full tile: the trailer's codes (0 without one)
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    uint32_t v = 0;
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        const uint32_t pp = p + 2 * c + h;
+                        if (tail && pp < g.len) v |= (uint32_t)tail[pp - kTileCodes * g.T] << (16 * h);
+                    }
+                    w[c] = v;
+                }
+            }
+            uint32_t row = fdiv(p, g.divX), col = p - row * g.X;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                if (row >= y0 && row < y0 + rows && col >= xw0 && col < xw0 + cw)
+                    win[(row - y0) * kD2Cols + (col - xw0)] = (uint16_t)(w[j >> 1] >> (16 * (j & 1)));
+                if (++col == g.X) {
+                    col = 0;
+                    row++;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // reconstruct: rows 4 gr .. 4 gr + 3 (gr uniform per wave), columns x0 .. x0 + 3
+    const uint32_t c = tid & (kD2Cols / 4 - 1), gr = tid / (kD2Cols / 4);
+    const uint32_t x0 = xw0 + 4 * c;
+    const bool     full = x0 + 4 <= g.X;
+    int32_t        dd[4][4];
+    int32_t        cs[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int r4 = 0; r4 < 4; r4++) {
+        const uint32_t r = 4 * gr + r4;
+        const uint16_t* wr = win + r * kD2Cols + 4 * c;
+        if (r < rows && full) {
+            const uint2 v = *reinterpret_cast<const uint2*>(wr);
+            dd[r4][0] = decode(v.x & 0xFFFFu);
+            dd[r4][1] = decode(v.x >> 16);
+            dd[r4][2] = decode(v.y & 0xFFFFu);
+            dd[r4][3] = decode(v.y >> 16);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; j++) dd[r4][j] = (r < rows && x0 + j < g.X) ? decode(wr[j]) : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) cs[j] += dd[r4][j];
+    }
+    if (gr < 3) csum[gr][c] = make_int4(cs[0], cs[1], cs[2], cs[3]);
+    __syncthreads();
+    int32_t col[4] = {0, 0, 0, 0};
+    for (uint32_t k = 0; k < gr; k++) {
+        const int4 v = csum[k][c];
+        col[0] += v.x;
+        col[1] += v.y;
+        col[2] += v.z;
+        col[3] += v.w;
+    }
+#pragma unroll
+    for (int r4 = 0; r4 < 4; r4++) {
+        const uint32_t r = 4 * gr + r4;
+        int32_t        p[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            col[j] += dd[r4][j];
+            p[j] = col[j] + (j ? p[j - 1] : 0);
+        }
+        const int32_t excl = quad_incl_scan(p[3]) - p[3];
+        if (r >= rows) continue;  // uniform over the wave
+        const size_t row = (size_t)(y0 + r) * g.X;
+        float        o[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) o[j] = (float)(p[j] + excl) * ebx2;
+        if (full) {
+            typedef float f32x4a4_t __attribute__((ext_vector_type(4), aligned(4)));
+            *reinterpret_cast<f32x4a4_t*>(out + row + x0) = f32x4a4_t{o[0], o[1], o[2], o[3]};
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (x0 + j < g.X) out[row + x0 + j] = o[j];
+        }
+    }
+}
+
+// Every workgroup runs its decode and its reconstruct one after the other, so the fused form
+// wins while the windows are resident at once or nearly (two 1024-thread workgroups per CU):
+// 3600 x 448 .. 3600 x 1800 (112 .. 452 windows) 12.6 / 18.8 us against 16.6 / 23.3 us for
+// decode + reconstruct, 26000 x 388 (650) 25.2 against 27.3 us, but 39200 x 388 (975) 31.4
+// against 30.0 us.
+constexpr uint64_t kD2MaxWindows = 768;
+
+bool decode_recon_2d_pays(const Geom& g)
+{
+    return (uint64_t)((g.X + kD2Cols - 1) / kD2Cols) * ((g.Y + 15) / 16) <= kD2MaxWindows;
+}
+
+hipError_t launch_decode_recon_2d(const uint8_t* payload, const uint32_t* tile_excl, const uint32_t* chunk_sums,
+                                  const uint16_t* tail, const Geom& g, float ebx2, float* out, hipStream_t s)
+{
+    const uint32_t rx = (g.X + kD2Cols - 1) / kD2Cols;
+    const uint64_t n = (uint64_t)rx * ((g.Y + 15) / 16);
+    if (n > 0xFFFFFFFFull) return hipErrorInvalidConfiguration;
+    FZ_LAUNCH(k_decode_recon_2d<<<(uint32_t)n, kD2Threads, 0, s>>>(payload, tile_excl, chunk_sums, tail, g, ebx2, out,
+                                                                   rx));
+    return hipGetLastError();
+}
+
 template <typename SRC>
 __global__ __launch_bounds__(kWG) void k_reconstruct_1d(const SRC codes, float* __restrict__ out, Geom g, float ebx2)
 {

@@ -343,6 +343,26 @@ def test_error_bound_on_the_baseline_workloads(name, eb):
         assert err <= bound + 4 * 2.0 ** -24 * float(torch.abs(d).max())
 
 
+# ---------------------------------------------------------------- the fused 2D decompress switch
+
+@pytest.mark.parametrize("dims", [(1024, 768 * 16, 1), (1024, 768 * 16 + 1, 1), (1000, 16 * 700 + 9, 1)])
+def test_decompress_at_the_fused_2d_switch(dims):
+    """2D fields of up to 768 windows of 16 x 1024 decompress in one fused launch, larger ones
+    by decode + reconstruct: the last fused case (768 windows), the first two-kernel case (769,
+    one row past a window row) and a ragged fused case, bit-exact against the oracle."""
+    n = dims[0] * dims[1]
+    say(f"fused 2D switch dims={dims}")
+    x = D.generate("waves", dims, 41 + dims[1])
+    ref = O.compress(x.ravel(), dims, 1e-3, O.MODE_FIXED, nthreads=16)
+    assert ref["status"] == 0
+    plan = fzgpu.Plan(*dims)
+    payload, tail, r = plan.compress(_to_dev(x.ravel()), 1e-3)
+    assert np.array_equal(_np(payload), ref["payload"])
+    xr = _np(plan.decompress(payload, tail, r["ebx2"]))
+    rc, xo = O.decompress(ref["payload"], ref["tail"], n, dims, ref["info"]["ebx2"], nthreads=16)
+    assert rc == 0 and np.array_equal(xr.view(np.uint32), xo.view(np.uint32))
+
+
 # ---------------------------------------------------------------- the decompress tile-scan switch
 
 @pytest.mark.parametrize("dims,length", [((65536 * 2048, 1, 1), 0), ((16384, 8192, 1), 0),
