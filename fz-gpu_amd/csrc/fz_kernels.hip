@@ -757,13 +757,16 @@ __global__ __launch_bounds__(kD2Threads) void k_decode_recon_2d(const uint8_t* _
                                                                 float* __restrict__ out, uint32_t rx)
 {
     constexpr uint32_t NH = kD2Threads / 32;
-    __shared__ uint16_t win[16 * kD2Cols];            // the window's codes, row-major
+    __shared__ __align__(16) uint16_t win[16 * kD2Cols];  // the window's codes, row-major
     __shared__ uint32_t stage[NH][32 * kQgPitch];      // one QG's bit-plane rows per half-wave
     __shared__ int4     csum[3][kD2Cols / 4];          // column sums of row groups 0-2
     __shared__ uint32_t cpre[kTileLocalMaxChunks];     // block counts of the chunks before each chunk
     __shared__ uint32_t wtot[4], qtab[kD2MaxQ], nq_s;  // the QGs to decode
     const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const uint32_t bx = blockIdx.x % rx, by = blockIdx.x / rx;
+    // XCD-sliced order: the windows that share a tile's blocks (neighbours along x, and along y
+    // for narrow fields) read them through one L2
+    const uint32_t wi = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t bx = wi % rx, by = wi / rx;
     const uint32_t xw0 = bx * kD2Cols, y0 = by * 16;
     const uint32_t cw = min(kD2Cols, g.X - xw0), rows = min(16u, g.Y - y0);
     // set-up: the chunk prefix (waves 0-3) and the QG list (wave 0)
@@ -861,6 +864,26 @@ __global__ __launch_bounds__(kD2Threads) void k_decode_recon_2d(const uint8_t* _
                 }
             }
             uint32_t row = fdiv(p, g.divX), col = p - row * g.X;
+            if ((g.X & 7u) == 0) {
+                // p % 8 == 0 and X % 8 == 0: the 8 codes sit in one row, 16-byte aligned in the window
+                if (row >= y0 && row < y0 + rows && col >= xw0 && col < xw0 + cw)
+                    *reinterpret_cast<uint4*>(win + (row - y0) * kD2Cols + (col - xw0)) = make_uint4(w[0], w[1], w[2], w[3]);
+                continue;
+            }
+            if ((g.X & 3u) == 0) {
+                // X % 4 == 0: each half of 4 codes sits in one row, 8-byte aligned in the window
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    if (row >= y0 && row < y0 + rows && col >= xw0 && col < xw0 + cw)
+                        *reinterpret_cast<uint2*>(win + (row - y0) * kD2Cols + (col - xw0)) = make_uint2(w[2 * h], w[2 * h + 1]);
+                    col += 4;
+                    if (col == g.X) {
+                        col = 0;
+                        row++;
+                    }
+                }
+                continue;
+            }
 #pragma unroll
             for (int j = 0; j < 8; j++) {
                 if (row >= y0 && row < y0 + rows && col >= xw0 && col < xw0 + cw)
@@ -932,16 +955,16 @@ __global__ __launch_bounds__(kD2Threads) void k_decode_recon_2d(const uint8_t* _
     }
 }
 
-// Every workgroup runs its decode and its reconstruct one after the other, so the fused form
-// wins while the windows are resident at once or nearly (two 1024-thread workgroups per CU):
-// 3600 x 448 .. 3600 x 1800 (112 .. 452 windows) 12.6 / 18.8 us against 16.6 / 23.3 us for
-// decode + reconstruct, 26000 x 388 (650) 25.2 against 27.3 us, but 39200 x 388 (975) 31.4
-// against 30.0 us.
+// With X % 4 == 0 a lane puts its codes into the window with one or two wide LDS stores, and
+// the fused form beats decode + reconstruct at every size measured (3600 x 1800: 16.0 against
+// 23.5 us; 16384 x 16384: 369 against 473 us).  Otherwise each code is a 2-byte store and the
+// fused form wins only while the windows are resident at once or nearly (two 1024-thread
+// workgroups per CU): up to 26000 x 388 (650 windows), not at 39200 x 388 (975).
 constexpr uint64_t kD2MaxWindows = 768;
 
 bool decode_recon_2d_pays(const Geom& g)
 {
-    return (uint64_t)((g.X + kD2Cols - 1) / kD2Cols) * ((g.Y + 15) / 16) <= kD2MaxWindows;
+    return (g.X & 3u) == 0 || (uint64_t)((g.X + kD2Cols - 1) / kD2Cols) * ((g.Y + 15) / 16) <= kD2MaxWindows;
 }
 
 hipError_t launch_decode_recon_2d(const uint8_t* payload, const uint32_t* tile_excl, const uint32_t* chunk_sums,

@@ -345,11 +345,13 @@ def test_error_bound_on_the_baseline_workloads(name, eb):
 
 # ---------------------------------------------------------------- the fused 2D decompress switch
 
-@pytest.mark.parametrize("dims", [(1024, 768 * 16, 1), (1024, 768 * 16 + 1, 1), (1000, 16 * 700 + 9, 1)])
+@pytest.mark.parametrize("dims", [(1022, 768 * 16, 1), (1022, 768 * 16 + 1, 1), (1020, 16 * 700 + 9, 1),
+                                  (3000, 2000, 1)])
 def test_decompress_at_the_fused_2d_switch(dims):
-    """2D fields of up to 768 windows of 16 x 1024 decompress in one fused launch, larger ones
-    by decode + reconstruct: the last fused case (768 windows), the first two-kernel case (769,
-    one row past a window row) and a ragged fused case, bit-exact against the oracle."""
+    """2D fields decompress in one fused launch when X % 4 == 0 (wide LDS stores: 8-byte at
+    X = 1020, 16-byte at X = 3000) and otherwise up to 768 windows of 16 x 1024, larger ones by
+    decode + reconstruct: X = 1022 at 768 windows (fused) and 769 (two kernels, one row past a
+    window row), bit-exact against the oracle."""
     n = dims[0] * dims[1]
     say(f"fused 2D switch dims={dims}")
     x = D.generate("waves", dims, 41 + dims[1])
