@@ -814,6 +814,7 @@ __global__ __launch_bounds__(kD2Threads) void k_decode_recon_2d(const uint8_t* _
             ex[b] = in ? tile_excl[t] + (chunk_sums ? cpre[t / kWG] : 0u) : 0u;
         }
         uint4 blk[kD2Batch];
+        bool  nzq[kD2Batch];  // some block of the wave's two QGs is nonzero (uniform)
 #pragma unroll
         for (int b = 0; b < (int)kD2Batch; b++) {
             // flag and rank of block 8 hl + m (flag word hl / 4, bit 8 (hl % 4) + m); word j is
@@ -828,15 +829,16 @@ __global__ __launch_bounds__(kD2Threads) void k_decode_recon_2d(const uint8_t* _
             const uint32_t src = (lane & 32u) + (hl >> 2);
             const uint32_t wj = __shfl(fw[b], src, 64), ej = __shfl(incl - pc, src, 64);
             const uint32_t bit = 8 * (hl & 3u) + (q[b] & 7u);
+            const bool     f = (wj >> bit) & 1u;
+            nzq[b] = ballot64(f) != 0;
             blk[b] = make_uint4(0, 0, 0, 0);
-            if ((wj >> bit) & 1u)
-                blk[b] = ld16(payload + 4ull * g.W + 16ull * ((uint64_t)ex[b] + ej + __popc(wj & ((1u << bit) - 1u))));
+            if (f) blk[b] = ld16(payload + 4ull * g.W + 16ull * ((uint64_t)ex[b] + ej + __popc(wj & ((1u << bit) - 1u))));
         }
 #pragma unroll
         for (int b = 0; b < (int)kD2Batch; b++) {
             const uint32_t t = q[b] >> 3;
             uint32_t       w[4] = {0, 0, 0, 0};
-            if (ballot64(live[b] && t < g.T) != 0) {  // uniform over the wave
+            if (nzq[b]) {  // all-zero blocks decode to zero codes without the transpose
                 uint32_t* row = st + kQgPitch * hl;
                 row[0] = blk[b].x;
                 row[1] = blk[b].y;
