@@ -216,6 +216,55 @@ def test_payload_check_refuses_inconsistent_payloads():
     lit.check_payload(ref_l["payload"], 0)
 
 
+@pytest.mark.parametrize("dims", [(300, 21, 1), (5000, 1, 1), (13, 12, 40)])
+def test_payload_check_agrees_with_the_oracle_on_random_damage(dims):
+    """Random damage to a valid archive (bitflag bits flipped, blocks cut or appended, bytes
+    appended, the tail length changed): fz_payload_check accepts exactly the payloads the
+    oracle's decompress accepts (fz_oracle.c decode_codes)."""
+    import numpy as np
+
+    import oracle as O
+    ref = _oracle_payload(dims, seed=11)
+    n = dims[0] * dims[1] * dims[2]
+    p = fzgpu.Plan(*dims, workspace=False)
+    good, tail = ref["payload"], ref["tail"]
+    W = p.sizes.flag_words
+    rng = np.random.default_rng(sum(dims))
+    seen = {True: 0, False: 0}
+    for trial in range(150):
+        pay, tl = good.copy(), len(tail)
+        kind = trial % 5
+        if kind == 0:    # flip 1-3 bitflag bits
+            for b in rng.integers(0, 32 * W, int(rng.integers(1, 4))):
+                pay[b // 8] ^= np.uint8(1 << (b % 8))
+        elif kind == 1:  # cut or append whole blocks
+            k = int(rng.integers(1, 4))
+            pay = pay[: max(0, pay.size - 16 * k)] if rng.integers(0, 2) else \
+                np.concatenate([pay, rng.integers(0, 256, 16 * k).astype(np.uint8)])
+        elif kind == 2:  # append a few stray bytes
+            pay = np.concatenate([pay, np.zeros(int(rng.integers(1, 16)), np.uint8)])
+        elif kind == 3:  # a flag moved: one set bit cleared and one clear bit set
+            bits = np.unpackbits(pay[: 4 * W], bitorder="little")
+            on, off = np.flatnonzero(bits), np.flatnonzero(bits == 0)
+            if on.size and off.size:
+                a, b = int(rng.choice(on)), int(rng.choice(off))
+                pay[a // 8] ^= np.uint8(1 << (a % 8))
+                pay[b // 8] ^= np.uint8(1 << (b % 8))
+        else:            # the tail length
+            tl = int(rng.integers(0, len(tail) + 3)) if len(tail) else 0
+        t_arr = np.resize(tail, tl).astype(np.uint16) if tl else np.zeros(0, np.uint16)
+        rc, _ = O.decompress(pay, t_arr if len(tail) else None, n, dims, ref["info"]["ebx2"])
+        try:
+            p.check_payload(pay, tl)
+            ok = True
+        except fzgpu.FzError as e:
+            assert e.status == 6
+            ok = False
+        assert ok == (rc == 0), (trial, kind, rc)
+        seen[ok] += 1
+    assert seen[True] > 0 and seen[False] > 0
+
+
 def test_side_check_follows_the_oracle_rules():
     import numpy as np
     dims = (100, 77, 1)
