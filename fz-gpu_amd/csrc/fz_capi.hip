@@ -339,6 +339,11 @@ fz_status fz_decompress_f32(const fz_plan plan, const uint8_t* d_payload, const 
         FZ_TRY(prof_mark(plan, stream, false));
         return FZ_SUCCESS;
     }
+    if (g.ndim == 3 && decode_recon_3d_pays(g)) {  // codes decoded into LDS per window: one launch
+        FZ_TRY(launch_decode_recon_3d(d_payload, texcl, csums, tail, g, ebx2, d_out, stream));
+        FZ_TRY(prof_mark(plan, stream, false));
+        return FZ_SUCCESS;
+    }
     FZ_TRY(launch_decode_unshuffle(d_payload, texcl, csums, tail, g, codes, stream));
     FZ_TRY(prof_mark(plan, stream, false));
     FZ_TRY(launch_reconstruct(reinterpret_cast<const uint16_t*>(codes), d_out, g, ebx2, stream));

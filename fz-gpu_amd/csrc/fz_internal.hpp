@@ -98,6 +98,11 @@ hipError_t launch_decode_recon_1d(const uint8_t* payload, const uint32_t* tile_e
 bool       decode_recon_2d_pays(const Geom& g);
 hipError_t launch_decode_recon_2d(const uint8_t* payload, const uint32_t* tile_excl, const uint32_t* chunk_sums,
                                   const uint16_t* tail, const Geom& g, float ebx2, float* out, hipStream_t s);
+// 3D fields: decode + reconstruct to floats in one launch (a workgroup per window of 8 planes x
+// R rows x 4Q columns)
+bool       decode_recon_3d_pays(const Geom& g);
+hipError_t launch_decode_recon_3d(const uint8_t* payload, const uint32_t* tile_excl, const uint32_t* chunk_sums,
+                                  const uint16_t* tail, const Geom& g, float ebx2, float* out, hipStream_t s);
 hipError_t launch_reconstruct(const uint16_t* codes, float* out, const Geom& g, float ebx2, hipStream_t s);
 // reconstruct from u16 codes whose positions flagged in `bits` take their delta from `over`
 hipError_t launch_reconstruct_outlier_src(const uint16_t* codes, const uint32_t* bits, const int32_t* over,

@@ -980,6 +980,367 @@ hipError_t launch_decode_recon_2d(const uint8_t* payload, const uint32_t* tile_e
     return hipGetLastError();
 }
 
+// 3D fields: decode and reconstruct in one NT-thread workgroup per window of 8 planes x R rows
+// x 4Q columns (R Q = NT: one Lorenzo block layer in z, R / 8 block rows in y), straight to
+// floats; the codes never reach HBM.
+//
+// Shape.  Q is the field's: the fewest quads per row that hold a whole row (4 Q >= X), at most
+// NT / 8; NT is 256 for X <= 128, 512 for X <= 256, else 1024 (then Q = 128, and rows wider
+// than 512 take several windows).
+//
+// Decode, as in k_decode_recon_2d.  The window's row segments list the quad groups they
+// touch, each once: with whole rows one segment per plane (a plane's rows are consecutive in
+// memory), else one per (plane, row); at most 8 (NT / 64 + 1) or 192 QGs.  Half-waves decode
+// them into the LDS window, all their loads in flight together.
+//
+// Reconstruct.  Thread (xq, yy) = (tid % Q, tid / Q) holds columns 4 xq .. 4 xq + 3 of row yy
+// over the 8 planes: prefix over z in registers, over x in the thread and with its pair lane
+// (one DPP step).  The prefix over y goes through LDS four planes at a time, in the window's
+// own memory once every code has been read: thread (xq, tz, yh, yb) = (tid % Q, rest % 4,
+// rest / 4 % 2, rest / 8) with rest = tid / Q sums rows 8 yb .. 8 yb + 4 yh + 3 of plane tz
+// and stores rows 8 yb + 4 yh .. + 3.
+constexpr uint32_t d3_maxq(uint32_t nt) { return nt == 1024 ? 192u : 8u * (nt / 64 + 1); }
+
+template <int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 8 : 7))) void k_decode_recon_3d(const uint8_t* __restrict__ payload,
+                                                        const uint32_t* __restrict__ tile_excl,
+                                                        const uint32_t* __restrict__ chunk_sums,
+                                                        const uint16_t* __restrict__ tail, Geom g, float ebx2,
+                                                        float* __restrict__ out, uint32_t qlog, uint32_t rx,
+                                                        uint32_t ry, uint32_t nwin)
+{
+    static_assert(NT == 256 || NT == 512 || NT == 1024, "threads per window");
+    constexpr uint32_t NH = NT / 32;                      // half-waves
+    constexpr uint32_t MAXQ = d3_maxq(NT);
+    // QGs per half-wave in flight together: half of them (two rounds), which keeps the kernel at
+    // 64 VGPRs without scratch
+    constexpr uint32_t NB = (MAXQ + 2 * NH - 1) / (2 * NH);
+    // 512 and 1024 threads: the two half-waves of a wave stage their QGs in turn through one
+    // buffer, which keeps the LDS at 38.8 / 75.8 KB (4 / 2 workgroups per CU) instead of 43.3 /
+    // 85.8 KB (3 / 1)
+    constexpr bool     SERIAL = NT >= 512;
+    __shared__ __align__(16) uint16_t win[32 * NT];       // codes [plane][row][column]; later int4 [4][R][Q]
+    __shared__ uint32_t stage[SERIAL ? NT / 64 : NH][32 * kQgPitch];  // bit-plane rows of a QG
+    __shared__ uint32_t cpre[kTileLocalMaxChunks];        // block counts of the chunks before each chunk
+    __shared__ uint32_t wtot[4], qtab[MAXQ], nq_s;        // the QGs to decode
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint32_t Q = 1u << qlog, R = (uint32_t)NT >> qlog, C = 4 * Q;
+    // the chunk prefix, once per workgroup (waves 0-3)
+    {
+        uint32_t cv = 0, cincl = 0;
+        if (tid < kTileLocalMaxChunks) {
+            cv = (chunk_sums && tid < (g.T + kWG - 1) / kWG) ? chunk_sums[tid] : 0u;
+            cincl = wave_incl_scan(cv);
+            if (lane == 63) wtot[wv] = cincl;
+        }
+        __syncthreads();
+        if (tid < kTileLocalMaxChunks) {
+            uint32_t pre = 0;
+            for (uint32_t k = 0; k < wv; k++) pre += wtot[k];
+            cpre[tid] = pre + cincl - cv;
+        }
+    }
+    // windows it, it + G, ... (G = gridDim.x, a multiple of 8 when G < nwin): a workgroup issues
+    // its next window's loads while the stores of the last one drain.  XCD-sliced order: G % 8 ==
+    // 0 keeps a workgroup on one XCD's windows, and windows that share a QG (neighbours along x,
+    // or along y / z when rows or planes are whole) read its blocks through one L2.
+    for (uint32_t it = blockIdx.x; it < nwin; it += gridDim.x) {
+    // the lane indices are recomputed in every window from an opaque copy of the thread index:
+    // values hoisted out of the loop would otherwise stay live across it (about 40 more VGPRs)
+    uint32_t tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const uint32_t lane = tid & 63, wv = tid >> 6;
+    const uint32_t wi = xcd_tile(it, nwin);
+    const uint32_t bx = wi % rx, t1 = wi / rx, by = t1 % ry, bz = t1 / ry;
+    const uint32_t xw0 = bx * C, y0 = by * R, z0 = bz * 8;
+    const uint32_t cw = min(C, g.X - xw0), rows = min(R, g.Y - y0), nz = min(8u, g.Z - z0);
+    const bool     whole = cw == g.X;
+    if (wv == 0) {
+        // lane i < nseg: the QGs [a, b] of segment i less those of segment i - 1 (a and b grow with i)
+        const uint32_t nseg = whole ? nz : nz * rows;  // rows <= 8 when rows are not whole
+        const bool     valid = lane < nseg;
+        uint32_t       p0 = 0, len = 1;
+        if (valid) {
+            if (whole) {
+                p0 = (z0 + lane) * g.XY + y0 * g.X;  // < x*y*z < 2^32
+                len = rows * g.X;
+            } else {
+                const uint32_t pz = lane / rows, r = lane - pz * rows;
+                p0 = (z0 + pz) * g.XY + (y0 + r) * g.X + xw0;
+                len = cw;
+            }
+        }
+        const uint32_t a = p0 >> 8, b = (p0 + len - 1) >> 8;
+        const uint32_t bprev = __shfl_up(b, 1, 64);
+        const uint32_t start = lane > 0 ? max(a, bprev + 1) : a;
+        const uint32_t c = (valid && b + 1 > start) ? b + 1 - start : 0u;
+        const uint32_t incl = wave_incl_scan(c);
+        for (uint32_t j = 0; j < c; j++) qtab[incl - c + j] = start + j;
+        if (lane == 63) nq_s = incl;
+    }
+    // the QG list and (first window) the chunk prefix are written; every access to the window's
+    // memory by the last window is done
+    __syncthreads();
+    const uint32_t  nq = nq_s;
+    const uint32_t  half = lane >> 5, hl = lane & 31, H = 2 * wv + half;
+    uint32_t* const st = stage[SERIAL ? wv : H];
+    const uint32_t* bflags = reinterpret_cast<const uint32_t*>(payload);
+    // a code at field position (x, y, z) inside the window goes to win[((z - z0) R + y - y0) C + x - xw0]
+    auto slot = [&](uint32_t x, uint32_t y, uint32_t z) -> int {
+        const uint32_t dz = z - z0, dy = y - y0, dx = x - xw0;  // wrap to large values below the window
+        return (dz < nz && dy < rows && dx < cw) ? (int)((dz * R + dy) * C + dx) : -1;
+    };
+    // both half-waves of a wave step together (the transpose needs all 64 lanes active)
+    for (uint32_t i0 = 0; i0 < nq; i0 += NB * NH) {
+        uint32_t q[NB], fw[NB], ex[NB];
+        bool     live[NB];
+#pragma unroll
+        for (int b = 0; b < (int)NB; b++) {
+            const uint32_t i = i0 + H + b * NH;
+            live[b] = i < nq;
+            q[b] = live[b] ? qtab[i] : 0u;
+            const uint32_t t = q[b] >> 3;
+            const bool     in = live[b] && t < g.T;
+            fw[b] = in ? bflags[8 * t + (hl & 7u)] : 0u;
+            ex[b] = in ? tile_excl[t] + (chunk_sums ? cpre[t / kWG] : 0u) : 0u;
+        }
+        uint4 blk[NB];
+        bool  nzq[NB];  // some block of the wave's two QGs is nonzero (uniform)
+#pragma unroll
+        for (int b = 0; b < (int)NB; b++) {
+            // flag and rank of block 8 hl + m (flag word hl / 4, bit 8 (hl % 4) + m); word j is
+            // held by the half's lanes with hl % 8 == j
+            const uint32_t pc = __popc(fw[b]);
+            uint32_t       incl = pc;
+#pragma unroll
+            for (int d = 1; d < 8; d <<= 1) {
+                const uint32_t o = __shfl_up(incl, d, 8);
+                if ((hl & 7u) >= (uint32_t)d) incl += o;
+            }
+            const uint32_t src = (lane & 32u) + (hl >> 2);
+            const uint32_t wj = __shfl(fw[b], src, 64), ej = __shfl(incl - pc, src, 64);
+            const uint32_t bit = 8 * (hl & 3u) + (q[b] & 7u);
+            const bool     f = (wj >> bit) & 1u;
+            nzq[b] = ballot64(f) != 0;
+            blk[b] = make_uint4(0, 0, 0, 0);
+            if (f) blk[b] = ld16(payload + 4ull * g.W + 16ull * ((uint64_t)ex[b] + ej + __popc(wj & ((1u << bit) - 1u))));
+        }
+#pragma unroll
+        for (int b = 0; b < (int)NB; b++) {
+            const uint32_t t = q[b] >> 3;
+            uint32_t       w[4] = {0, 0, 0, 0};
+            if (nzq[b]) {  // all-zero blocks decode to zero codes without the transpose
+                const uint32_t qq = hl >> 3, L = hl & 7u;
+#pragma unroll
+                for (uint32_t hh = 0; hh < (SERIAL ? 2u : 1u); hh++) {
+                    if (!SERIAL || half == hh) {
+                        uint32_t* row = st + kQgPitch * hl;
+                        row[0] = blk[b].x;
+                        row[1] = blk[b].y;
+                        row[2] = blk[b].z;
+                        row[3] = blk[b].w;
+                    }
+                    wave_lds_sync();
+                    if (!SERIAL || half == hh) {
+#pragma unroll
+                        for (int c = 0; c < 4; c++) w[c] = st[kQgPitch * (4 * L + c) + qq];
+                    }
+                    wave_lds_sync();  // the rows are rewritten by the other half or the next QG
+                }
+                transpose4(w, L);
+            }
+            if (!live[b]) continue;
+            const uint32_t p = 256 * q[b] + 8 * hl;
+            if (t >= g.T) {  // after the last full tile: the trailer's codes (0 without one)
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    uint32_t v = 0;
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        const uint32_t pp = p + 2 * c + h;
+                        if (tail && pp < g.len) v |= (uint32_t)tail[pp - kTileCodes * g.T] << (16 * h);
+                    }
+                    w[c] = v;
+                }
+            }
+            const uint32_t q1 = fdiv(p, g.divX);
+            uint32_t       x = p - q1 * g.X, z = fdiv(q1, g.divY), y = q1 - z * g.Y;
+            if ((g.X & 7u) == 0) {
+                // p % 8 == 0 and X % 8 == 0: the 8 codes sit in one row, 16-byte aligned in the window
+                const int s = slot(x, y, z);
+                if (s >= 0) *reinterpret_cast<uint4*>(win + s) = make_uint4(w[0], w[1], w[2], w[3]);
+                continue;
+            }
+            if ((g.X & 3u) == 0) {
+                // X % 4 == 0: each half of 4 codes sits in one row, 8-byte aligned in the window
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int s = slot(x, y, z);
+                    if (s >= 0) *reinterpret_cast<uint2*>(win + s) = make_uint2(w[2 * h], w[2 * h + 1]);
+                    x += 4;
+                    if (x == g.X) {
+                        x = 0;
+                        if (++y == g.Y) {
+                            y = 0;
+                            z++;
+                        }
+                    }
+                }
+                continue;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int s = slot(x, y, z);
+                if (s >= 0) win[s] = (uint16_t)(w[j >> 1] >> (16 * (j & 1)));
+                if (++x == g.X) {
+                    x = 0;
+                    if (++y == g.Y) {
+                        y = 0;
+                        z++;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // reconstruct, phase 1: columns xc .. xc + 3 of row yy, the 8 planes (z prefix in registers)
+    const uint32_t xq = tid & (Q - 1), yy = tid >> qlog;
+    const uint32_t xc = xw0 + 4 * xq;
+    const bool     rowok = yy < rows;
+    int32_t        r[8][4];
+    {
+        int32_t col[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int z = 0; z < 8; z++) {
+            int32_t dd[4] = {0, 0, 0, 0};
+            if (rowok && (uint32_t)z < nz) {
+                const uint2 v = *reinterpret_cast<const uint2*>(win + ((uint32_t)z * R + yy) * C + 4 * xq);
+                const uint32_t c4[4] = {v.x & 0xFFFFu, v.x >> 16, v.y & 0xFFFFu, v.y >> 16};
+#pragma unroll
+                for (int j = 0; j < 4; j++) dd[j] = xc + j < g.X ? decode(c4[j]) : 0;
+            }
+            int32_t p[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                col[j] += dd[j];                     // prefix over z
+                p[j] = col[j] + (j ? p[j - 1] : 0);  // prefix over the thread's 4 columns
+            }
+            // the odd lane of a pair holds the block's upper 4 columns: add the even lane's total
+            // (xq and the lane index have the same parity, Q being even)
+            const int32_t lower = __builtin_amdgcn_mov_dpp(p[3], 0xA0, 0xF, 0xF, false);  // quad_perm [0,0,2,2]
+            const int32_t add = (xq & 1u) ? lower : 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) r[z][j] = p[j] + add;
+        }
+    }
+    // phase 2: the prefix over y, four planes at a time through the window's memory
+    int4* const    ex4 = reinterpret_cast<int4*>(win);  // [4][R][Q]: 64 NT bytes, the window's size
+    const uint32_t tz = yy & 3u, yh = (yy >> 2) & 1u, yb = yy >> 3;
+    const bool     full = xc + 4 <= g.X;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        __syncthreads();  // every read of the window (h = 0) or of the previous planes (h = 1) is done
+#pragma unroll
+        for (int zz = 0; zz < 4; zz++)
+            ex4[((uint32_t)zz * R + yy) * Q + xq] = make_int4(r[4 * h + zz][0], r[4 * h + zz][1], r[4 * h + zz][2], r[4 * h + zz][3]);
+        __syncthreads();
+        int4 acc = make_int4(0, 0, 0, 0);
+        if (yh) {
+#pragma unroll
+            for (int y = 0; y < 4; y++) {
+                const int4 v = ex4[(tz * R + 8 * yb + (uint32_t)y) * Q + xq];
+                acc.x += v.x;
+                acc.y += v.y;
+                acc.z += v.z;
+                acc.w += v.w;
+            }
+        }
+        const uint32_t pz = 4 * h + tz;
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+            const uint32_t yr = 8 * yb + 4 * yh + (uint32_t)y;
+            const int4     v = ex4[(tz * R + yr) * Q + xq];
+            acc.x += v.x;
+            acc.y += v.y;
+            acc.z += v.z;
+            acc.w += v.w;
+            if (pz >= nz || yr >= rows) continue;
+            const size_t base = (size_t)(z0 + pz) * g.XY + (size_t)(y0 + yr) * g.X + xc;
+            const float  o[4] = {(float)acc.x * ebx2, (float)acc.y * ebx2, (float)acc.z * ebx2, (float)acc.w * ebx2};
+            if (full) {
+                typedef float f32x4a4_t __attribute__((ext_vector_type(4), aligned(4)));
+                *reinterpret_cast<f32x4a4_t*>(out + base) = f32x4a4_t{o[0], o[1], o[2], o[3]};
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    if (xc + j < g.X) out[base + j] = o[j];
+            }
+        }
+    }
+    }
+}
+
+// compute units of the current device (256 on MI355X when the query fails)
+static uint32_t device_cus()
+{
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        return 256;
+    }
+    return (uint32_t)n;
+}
+
+// threads per window and log2 of quads per row of the fused 3D decompress for a field
+static void d3_shape(const Geom& g, uint32_t& nt, uint32_t& qlog)
+{
+    if (g.X > 256) {
+        nt = 1024;
+        qlog = 7;
+    } else if (g.X > 128) {
+        nt = 512;
+        qlog = 6;
+    } else {
+        nt = 256;
+        qlog = 2;
+        while ((4u << qlog) < g.X) qlog++;  // 4 Q >= X, Q <= 32
+    }
+}
+
+// Measured against decode + reconstruct (back-to-back calls, waves fields at eb 1e-3): the fused
+// form wins on every field with X <= 256 (128 x 128 x 512: 19.4 against 29.3 us, 256^3: 28.0
+// against 37.8 us) and on wider ones whose windows use most of their columns (480 x 300 x 100:
+// 29.3 / 36.5 us, 1024 x 1024 x 64: 97 / 132 us, 500 x 500 x 100 even), and loses where a
+// 512-column window holds much less (300^3: 58.9 / 50.3 us, 700 x 500 x 60: 53.9 / 45.4 us).
+bool decode_recon_3d_pays(const Geom& g)
+{
+    if (g.ndim != 3) return false;
+    if (g.X <= 256) return true;
+    uint32_t nt, qlog;
+    d3_shape(g, nt, qlog);
+    const uint64_t C = 4u << qlog, rx = (g.X + C - 1) / C;
+    return 4 * (uint64_t)g.X >= 3 * rx * C;  // the field's columns fill 3/4 of the windows' width
+}
+
+hipError_t launch_decode_recon_3d(const uint8_t* payload, const uint32_t* tile_excl, const uint32_t* chunk_sums,
+                                  const uint16_t* tail, const Geom& g, float ebx2, float* out, hipStream_t s)
+{
+    uint32_t nt, qlog;
+    d3_shape(g, nt, qlog);
+    const uint32_t C = 4u << qlog, R = nt >> qlog;
+    const uint32_t rx = (g.X + C - 1) / C, ry = (g.Y + R - 1) / R;
+    const uint64_t n = (uint64_t)rx * ry * ((g.Z + 7) / 8);
+    if (n > 0xFFFFFFFFull) return hipErrorInvalidConfiguration;
+    // at most the workgroups the device holds at once (2 / 4 / 7 per CU by LDS), a multiple of 8
+    const uint32_t per_cu = nt == 1024 ? 2u : nt == 512 ? 4u : 7u;
+    const uint32_t slots = device_cus() * per_cu & ~7u;
+    const dim3     grid(n <= slots ? (uint32_t)n : slots);
+    if (nt == 1024) FZ_LAUNCH(k_decode_recon_3d<1024><<<grid, 1024, 0, s>>>(payload, tile_excl, chunk_sums, tail, g, ebx2, out, qlog, rx, ry, (uint32_t)n));
+    else if (nt == 512) FZ_LAUNCH(k_decode_recon_3d<512><<<grid, 512, 0, s>>>(payload, tile_excl, chunk_sums, tail, g, ebx2, out, qlog, rx, ry, (uint32_t)n));
+    else FZ_LAUNCH(k_decode_recon_3d<256><<<grid, 256, 0, s>>>(payload, tile_excl, chunk_sums, tail, g, ebx2, out, qlog, rx, ry, (uint32_t)n));
+    return hipGetLastError();
+}
+
 template <typename SRC>
 __global__ __launch_bounds__(kWG) void k_reconstruct_1d(const SRC codes, float* __restrict__ out, Geom g, float ebx2)
 {

@@ -392,3 +392,50 @@ def test_decompress_at_the_tile_scan_switch(dims, length):
     xr = _np(plan.decompress(payload, tail, r["ebx2"]))
     rc, xo = O.decompress(ref["payload"], ref["tail"], n, dims, ref["info"]["ebx2"], nthreads=16)
     assert rc == 0 and np.array_equal(xr.view(np.uint32), xo.view(np.uint32))
+
+
+# ---------------------------------------------------------------- the fused 3D decompress
+
+D3_SHAPES = [(16, 70, 9), (12, 40, 17), (17, 70, 9), (100, 30, 20), (129, 20, 9), (200, 24, 10), (257, 8, 8),
+             (400, 9, 9), (500, 11, 9), (512, 9, 9), (1000, 7, 17), (2000, 3, 9), (1, 1, 5000), (4, 3, 700),
+             (300, 41, 3), (700, 41, 3)]
+
+
+@pytest.mark.parametrize("dims", D3_SHAPES)
+@pytest.mark.parametrize("mode", [O.MODE_FIXED, O.MODE_LITERAL])
+def test_decompress_fused_3d_window_shapes(dims, mode):
+    """3D fields decode and reconstruct in one launch, a workgroup per window of 8 planes x R
+    rows x 4Q columns: 256 threads up to X = 128 (Q = 4 .. 32), 512 up to 256, 1024 beyond
+    (whole rows up to X = 512, several windows per row after), X % 8 == 0 / % 4 == 0 / odd,
+    partial windows at every edge, trailer codes from the tail (FIXED) or zero (LITERAL); fields
+    whose windows would sit mostly empty (X = 300, 700) take decode + reconstruct: bit-exact
+    against the oracle either way."""
+    x = _field(dims, sum(dims) + 11 + mode)
+    plan, payload, tail, r, ref = _check_against_oracle(x, dims, 1e-3, mode, fused=True)
+    say(f"fused 3D decompress dims={dims} mode={mode}")
+    xr = _np(plan.decompress(payload, tail if mode == O.MODE_FIXED else None, r["ebx2"]))
+    rc, xo = O.decompress(ref["payload"], ref["tail"] if mode == O.MODE_FIXED else None, x.size, dims,
+                          ref["info"]["ebx2"])
+    assert rc == 0
+    assert np.array_equal(xr.view(np.uint32), xo.view(np.uint32))
+
+
+def test_decompress_fused_3d_beyond_65536_tiles():
+    """A 3D field of 66560 tiles: the fused 3D decompress takes the look-back scan's global tile
+    offsets instead of chunk-local ones, bit-exact against the oracle."""
+    dims = (512, 512, 520)
+    n = dims[0] * dims[1] * dims[2]
+    say(f"fused 3D decompress dims={dims} T={n // 2048}")
+    rng = np.random.default_rng(77)
+    x = np.zeros(n, np.float32)
+    nz = rng.random(n) < 0.05
+    x[nz] = rng.normal(size=int(nz.sum())).astype(np.float32)
+    x[0], x[-1] = 3.0, -2.0
+    ref = O.compress(x, dims, 1e-4, O.MODE_FIXED, nthreads=16)
+    assert ref["status"] == 0
+    plan = fzgpu.Plan(*dims)
+    payload, tail, r = plan.compress(_to_dev(x), 1e-4)
+    assert np.array_equal(_np(payload), ref["payload"])
+    xr = _np(plan.decompress(payload, tail, r["ebx2"]))
+    rc, xo = O.decompress(ref["payload"], ref["tail"], n, dims, ref["info"]["ebx2"], nthreads=16)
+    assert rc == 0 and np.array_equal(xr.view(np.uint32), xo.view(np.uint32))
