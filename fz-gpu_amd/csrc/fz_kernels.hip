@@ -1309,17 +1309,23 @@ static void d3_shape(const Geom& g, uint32_t& nt, uint32_t& qlog)
 
 // Measured against decode + reconstruct (back-to-back calls, waves fields at eb 1e-3): the fused
 // form wins on every field with X <= 256 (128 x 128 x 512: 19.4 against 29.3 us, 256^3: 28.0
-// against 37.8 us) and on wider ones whose windows use most of their columns (480 x 300 x 100:
-// 29.3 / 36.5 us, 1024 x 1024 x 64: 97 / 132 us, 500 x 500 x 100 even), and loses where a
-// 512-column window holds much less (300^3: 58.9 / 50.3 us, 700 x 500 x 60: 53.9 / 45.4 us).
+// against 37.8 us).  Wider fields take 1024-thread windows, two per CU: the fused form wins
+// when the windows fit one round (480 x 300 x 100: 29.3 / 36.5 us, 400 x 400 x 64: 27.3 / 34.0)
+// or take three or more (2000 x 300 x 100: 101.8 / 125.6 us, 1024 x 1024 x 64: 97.4 / 132.2),
+// is even or slower in between, where a second round holds only part of the device and its
+// windows start in step (500 x 500 x 100, 819 windows: 52.7 / 51.0 us; 1000 x 1000 x 20: 41.7 /
+// 44.2), and loses where a 512-column window holds much less than 512 columns of the field
+// (300^3: 58.9 / 50.3 us, 700 x 500 x 60: 53.9 / 45.4 us).
 bool decode_recon_3d_pays(const Geom& g)
 {
     if (g.ndim != 3) return false;
     if (g.X <= 256) return true;
     uint32_t nt, qlog;
     d3_shape(g, nt, qlog);
-    const uint64_t C = 4u << qlog, rx = (g.X + C - 1) / C;
-    return 4 * (uint64_t)g.X >= 3 * rx * C;  // the field's columns fill 3/4 of the windows' width
+    const uint64_t C = 4u << qlog, R = nt >> qlog, rx = (g.X + C - 1) / C;
+    if (4 * (uint64_t)g.X < 3 * rx * C) return false;  // the field's columns fill < 3/4 of the windows' width
+    const uint64_t nwin = rx * ((g.Y + R - 1) / R) * ((g.Z + 7) / 8), slots = 2ull * device_cus();
+    return nwin <= slots || nwin >= 3 * slots;
 }
 
 hipError_t launch_decode_recon_3d(const uint8_t* payload, const uint32_t* tile_excl, const uint32_t* chunk_sums,
