@@ -257,23 +257,38 @@ def bench_serving(rates=(50, 100, 200, 500, 1000, 2000, 20000), duration_s=1.0, 
     # caching allocator would); warm-up: every pool field compressed and decompressed alone,
     # so that the caching allocator and the plan pool hold every request size before the
     # timed streams
-    be = S.DeviceBackend(plan_pool_bytes=16 << 30)
+    # workspaces and kept archives come from one 160 GB device arena: no request-stream
+    # allocation reaches the caching allocator (decompress outputs, which go to the client, do)
+    be = S.DeviceBackend(plan_pool_bytes=16 << 30, arena_bytes=int(160e9))
     warm = S.Server(be, hbm_budget_bytes=200e9, slots=3, max_fields=1, max_archives=len(pool))
     warm.run([S.Request(i, "compress", d, 1e-3, x, 0.0) for i, (d, x) in enumerate(pool)], realtime=False)
     arch = [r.result for r, _ in warm.archives]
     warm2 = S.Server(be, hbm_budget_bytes=200e9, slots=3, max_fields=1)
     warm2.run([S.Request(i, "decompress", a.dims, 1e-3, a, 0.0) for i, a in enumerate(arch)], realtime=False)
+    warm2.close()
+    warm.close()
     del warm, warm2, arch
+    def release(r):
+        # the bench reads only the timing of a finished request: a decompress request lets go of
+        # its output and of the archive it read, as a client that took its result would, so
+        # that device memory held outside the server stays bounded
+        if r.kind == "decompress":
+            r.result = r.source = None
+
     for k, rate in enumerate(rates):
         reqs = S.make_requests(rate, max(duration_s, min_requests / rate) if rate < 10000 else 0.25, pool,
                                seed=3000 + k)
         srv = S.Server(be, hbm_budget_bytes=200e9, slots=3, max_fields=256)
-        srv.run(reqs, realtime=True, on_arrival=S.pick_archive(4000 + k))
+        srv.run(reqs, realtime=True, on_arrival=S.pick_archive(4000 + k), on_complete=release)
         out[f"rate_{rate}"] = S.summarize(srv, reqs)
+        out[f"rate_{rate}"]["arena_peak_GB"] = round(be.arena.peak / 1e9, 3)
         log(f"  rate {rate}/s: {out[f'rate_{rate}']['latency']} batches {out[f'rate_{rate}']['batch_size_hist']}")
+        srv.close()
         del srv, reqs
         torch.cuda.empty_cache()
     out["plan_pool_GB"] = round(be.pool_bytes / 1e9, 3)
+    out["arena_GB"] = round(be.arena.size / 1e9, 3)
+    out["arena_misses"] = be.arena.misses
     return out
 
 

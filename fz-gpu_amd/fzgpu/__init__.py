@@ -237,6 +237,19 @@ def _empty(nbytes, device="cuda"):
     return torch.empty(max(int(nbytes), 16), dtype=torch.uint8, device=device)
 
 
+def _bucket(nbytes):
+    """nbytes rounded up to one of eight sizes per octave above 1 MiB (at most 12.5% more).
+    Batch workspaces and output arenas of a request stream come in as many sizes as there are
+    batches; allocated at these sizes, a block the caching allocator freed is found again by
+    the next batch of about the same size instead of splitting a larger one, so the allocator
+    does not fragment its segments until it must free them all (a device-wide wait)."""
+    n = max(int(nbytes), 16)
+    if n <= 1 << 20:
+        return n
+    step = 1 << (n.bit_length() - 4)
+    return (n + step - 1) // step * step
+
+
 class Plan:
     """Geometry + mode of one field and its device workspace (allocated once, reused)."""
 
@@ -594,10 +607,15 @@ class Graph:
 
 class BatchOutputs:
     """Device outputs of a batched compress: one payload arena and one tail arena with a
-    256-byte-aligned slice per field, and the result records."""
+    256-byte-aligned slice per field, and the result records.  alloc(nbytes) may supply the
+    arenas (a uint8 device tensor of at least nbytes, or None for a torch allocation)."""
 
-    def __init__(self, plan):
+    def __init__(self, plan, alloc=None):
         import torch
+
+        def arena(nbytes):
+            t = alloc(nbytes) if alloc is not None else None
+            return t if t is not None else torch.empty(_bucket(nbytes), dtype=torch.uint8, device=plan.device)
         self.payload_off, self.tail_off = [], []
         po = to = 0
         for i in range(plan.nf):
@@ -606,8 +624,8 @@ class BatchOutputs:
             self.tail_off.append((to, fs.tail_max_codes))
             po += _al(fs.payload_max_bytes + 16)
             to += _al(2 * fs.tail_max_codes + 2)
-        self.payload_arena = torch.empty(max(po, 256), dtype=torch.uint8, device=plan.device)
-        self.tail_arena = torch.empty(max(to, 256), dtype=torch.uint8, device=plan.device)
+        self.payload_arena = arena(max(po, 256))
+        self.tail_arena = arena(max(to, 256))
         self.payloads = [self.payload_arena[o: o + n + 16] for o, n in self.payload_off]
         self.tails = [self.tail_arena[o: o + 2 * n + 2].view(torch.int16) for o, n in self.tail_off]
         self.res = torch.zeros(64 * plan.nf, dtype=torch.uint8, device=plan.device)
@@ -619,7 +637,8 @@ class BatchOutputs:
             for c in caps:
                 offs.append(so)
                 so += _al(c)
-            self.side_arena = torch.zeros(max(so, 256), dtype=torch.uint8, device=plan.device)
+            self.side_arena = arena(max(so, 256))
+            self.side_arena.zero_()
             self.sides = [self.side_arena[o: o + c] for o, c in zip(offs, caps)]
             self.side_caps = caps
 
@@ -655,9 +674,16 @@ class BatchPlan:
         self.sizes = bs
         self.ws = None
         if workspace:
-            self.ws = _empty(bs.workspace_bytes, device)
-            _check(lib().fz_batch_workspace_init(h, _ptr(self.ws), self.ws.numel(), _stream()),
-                   "fz_batch_workspace_init")
+            self.attach_workspace(_empty(_bucket(bs.workspace_bytes), device))
+
+    def attach_workspace(self, ws, stream=None):
+        """Use the uint8 device tensor ws (at least sizes.workspace_bytes, 256-byte aligned) as
+        the plan's workspace and put it into its reset state (fz_batch_workspace_init)."""
+        if ws.numel() < self.sizes.workspace_bytes or ws.data_ptr() % 256:
+            raise ValueError("workspace too small or not 256-byte aligned")
+        self.ws = ws
+        _check(lib().fz_batch_workspace_init(self.handle, _ptr(ws), ws.numel(), _stream(stream)),
+               "fz_batch_workspace_init")
 
     def close(self):
         if self.handle is not None and _lib is not None:
@@ -701,8 +727,8 @@ class BatchPlan:
                 t.wave_tile[i] = wave_tile[i]
         _check(lib().fz_batch_set_tuning(self.handle, ctypes.byref(t)), "fz_batch_set_tuning")
 
-    def alloc_outputs(self):
-        return BatchOutputs(self)
+    def alloc_outputs(self, alloc=None):
+        return BatchOutputs(self, alloc)
 
     def _compress_args(self, inputs, ebs, out):
         assert len(inputs) == self.nf and len(ebs) == self.nf

@@ -12,6 +12,7 @@ archive (payload + tail + result record) in HBM for later decompress requests.
 The scheduler only talks to a backend (launch a batch, poll it, account memory), so its
 admission and ordering rules run on the CPU with a simulated backend in the tests.
 """
+import bisect
 import collections
 import dataclasses
 import math
@@ -65,11 +66,13 @@ class Archive:
 
 class OutputBuffer:
     """Device outputs of one finished compress batch.  Its archives are views into one
-    allocation, so its bytes stay in the budget until the last of them is dropped."""
+    allocation, so its bytes stay in the budget until the last of them is dropped; then
+    release() (if set) gives the memory back to the backend that allocated it."""
 
-    def __init__(self, nbytes):
+    def __init__(self, nbytes, release=None):
         self.nbytes = int(nbytes)
         self.refs = 0
+        self.release = release
 
 
 class HbmBudget:
@@ -191,6 +194,17 @@ class Server:
         buf.refs -= 1
         if buf.refs == 0:
             self.budget.give(buf.nbytes)
+            if buf.release is not None:
+                buf.release()
+                buf.release = None
+
+    def close(self):
+        """Drops every archive the server holds (their buffers go back to the backend); a
+        server that shares its backend with later servers is closed when it is done."""
+        for req, buf in list(self.archives):
+            req.result = None
+            self._drop_ref(buf)
+        self.archives.clear()
 
     def _evict(self):
         """Drops the oldest archive no decompress request is reading; its batch's output
@@ -264,7 +278,9 @@ class Server:
                         if isinstance(r.source, Archive):
                             r.source.pins -= 1
                 elif self.max_archives > 0:  # archives stay on the device, their buffer in the budget
-                    buf = OutputBuffer(out_bytes)
+                    keep = reqs[0].result.keep if isinstance(reqs[0].result, Archive) else None
+                    free = getattr(self.backend, "free_outputs", None)
+                    buf = OutputBuffer(out_bytes, (lambda k=keep: free(k)) if free and keep is not None else None)
                     buf.refs = 1  # held until every archive of the batch is registered
                     for r in reqs:
                         if isinstance(r.result, Archive):
@@ -290,6 +306,9 @@ class Server:
         (a decompress request picks its archive there); on_complete(req) sees each finished
         request before its archive can be evicted."""
         self.on_complete = on_complete
+        # archives the server keeps are its to free when it drops them; with max_archives = 0
+        # they go to the caller, so the backend must not take them from memory it reuses
+        self.backend.server_keeps_outputs = self.max_archives > 0
         pending = sorted(requests, key=lambda r: r.arrival)
         self.t0 = self.clock()
         i = 0
@@ -338,6 +357,67 @@ def poisson_arrivals(rate, duration_s, rng):
 
 # ---------------------------------------------------------------- the device backend
 
+class DeviceArena:
+    """One device allocation carved into 256-byte-aligned blocks: best fit from a free list kept
+    in address order, a freed block merged with free neighbours.  The serving backend takes its
+    batch workspaces and archive buffers from it, so a request stream of ever-new sizes never
+    reaches the caching allocator (whose splitting of cached segments fragments them until it
+    must free them all, a device-wide wait).  With buf=None only the offsets are managed (the
+    CPU tests)."""
+
+    ALIGN = 256
+
+    def __init__(self, nbytes, device="cuda", buf=None):
+        self.size = int(nbytes) // self.ALIGN * self.ALIGN
+        if buf is None and device is not None:
+            import torch
+            buf = torch.empty(self.size, dtype=torch.uint8, device=device)
+        self.buf = buf
+        self.free = [(0, self.size)]  # (offset, bytes), by offset, never two adjacent
+        self.used = 0
+        self.peak = 0
+        self.misses = 0               # requests no free block could hold (counted by the caller)
+
+    def alloc(self, nbytes):
+        """(offset, bytes) of a block of at least nbytes, or None when no free block holds it."""
+        n = max(self.ALIGN, (int(nbytes) + self.ALIGN - 1) // self.ALIGN * self.ALIGN)
+        best = -1
+        for i, (o, b) in enumerate(self.free):
+            if b >= n and (best < 0 or b < self.free[best][1]):
+                best = i
+                if b == n:
+                    break
+        if best < 0:
+            return None
+        o, b = self.free[best]
+        if b == n:
+            del self.free[best]
+        else:
+            self.free[best] = (o + n, b - n)
+        self.used += n
+        self.peak = max(self.peak, self.used)
+        return (o, n)
+
+    def release(self, block):
+        o, n = block
+        i = bisect.bisect_left(self.free, (o, 0))
+        if i > 0 and self.free[i - 1][0] + self.free[i - 1][1] > o or i < len(self.free) and o + n > self.free[i][0]:
+            raise ValueError("block released twice or not from this arena")
+        self.used -= n
+        if i < len(self.free) and o + n == self.free[i][0]:  # merge with the next free block
+            n += self.free[i][1]
+            del self.free[i]
+        if i > 0 and self.free[i - 1][0] + self.free[i - 1][1] == o:  # and with the previous one
+            po, pn = self.free[i - 1]
+            self.free[i - 1] = (po, pn + n)
+        else:
+            self.free.insert(i, (o, n))
+
+    def tensor(self, block, nbytes):
+        o, _ = block
+        return self.buf[o: o + int(nbytes)]
+
+
 class PlanPool:
     """Free plans of finished batches keyed by their field shapes: take() hands out a free plan
     of those shapes (or None), give() returns one, dropping the least recently used keys' plans
@@ -360,6 +440,18 @@ class PlanPool:
         if not lst:
             del self.free[shapes]
         return plan
+
+    def drop_lru(self):
+        """Closes the least recently used free plan; False when none is left."""
+        if not self.free:
+            return False
+        old_shapes, lst = next(iter(self.free.items()))
+        old = lst.pop(0)
+        self.bytes -= self.nbytes(old)
+        self.close(old)
+        if not lst:
+            del self.free[old_shapes]
+        return True
 
     def give(self, shapes, plan):
         nb = self.nbytes(plan)
@@ -387,7 +479,8 @@ class DeviceBackend:
     copied into pinned host memory behind its kernels, so finishing the batch reads host
     memory and waits for nothing."""
 
-    def __init__(self, mode=0, nstreams=3, device="cuda", plan_pool_bytes=4 << 30, side_fraction=1.0):
+    def __init__(self, mode=0, nstreams=3, device="cuda", plan_pool_bytes=4 << 30, side_fraction=1.0,
+                 arena_bytes=0):
         import torch
         import fzgpu
         self.torch = torch
@@ -397,18 +490,52 @@ class DeviceBackend:
         self.device = device
         self.streams = [torch.cuda.Stream(device=device) for _ in range(nstreams)]
         self.busy = [False] * nstreams
-        self.pool = PlanPool(plan_pool_bytes)
+        self.pool = PlanPool(plan_pool_bytes, close=self._close_plan)
+        # workspaces and kept archives from one arena (arena_bytes > 0), else from torch
+        self.arena = DeviceArena(arena_bytes, device) if arena_bytes else None
+        self.server_keeps_outputs = False  # set by the server: it frees the archives it keeps
 
     @property
     def pool_bytes(self):
         return self.pool.bytes
+
+    def _block(self, nbytes, blocks):
+        """A uint8 tensor of nbytes from the arena (its block appended to blocks), or None.  The
+        free plans' workspaces give way first, least recently used first."""
+        if self.arena is None:
+            return None
+        b = self.arena.alloc(nbytes)
+        while b is None and self.pool.drop_lru():
+            b = self.arena.alloc(nbytes)
+        if b is None:
+            self.arena.misses += 1
+            return None
+        blocks.append(b)
+        return self.arena.tensor(b, nbytes)
+
+    def _close_plan(self, plan):
+        for b in getattr(plan, "arena_blocks", ()):
+            self.arena.release(b)
+        plan.arena_blocks = []
+        plan.close()
+
+    def free_outputs(self, out):
+        """Returns a kept batch's archive buffers to the arena (called by the server when it
+        drops the batch's last archive)."""
+        for b in getattr(out, "arena_blocks", ()):
+            self.arena.release(b)
+        out.arena_blocks = []
 
     def _take_plan(self, shapes):
         import fzgpu
         plan = self.pool.take(shapes)
         # a new plan sets up its workspace on the current stream
         if plan is None:
-            plan = fzgpu.BatchPlan(list(shapes), mode=self.mode, side_fraction=self.side_fraction or 1.0)
+            plan = fzgpu.BatchPlan(list(shapes), mode=self.mode, side_fraction=self.side_fraction or 1.0,
+                                   workspace=False)
+            plan.arena_blocks = []
+            ws = self._block(plan.sizes.workspace_bytes, plan.arena_blocks)
+            plan.attach_workspace(ws if ws is not None else fzgpu._empty(fzgpu._bucket(plan.sizes.workspace_bytes)))
         return plan
 
     def launch(self, kind, reqs):
@@ -421,7 +548,10 @@ class DeviceBackend:
         with torch.cuda.stream(s):
             plan = self._take_plan(shapes)
             if kind == "compress":
-                out = plan.alloc_outputs()
+                blocks = []
+                keep_in_arena = self.arena is not None and self.server_keeps_outputs
+                out = plan.alloc_outputs((lambda n: self._block(n, blocks)) if keep_in_arena else None)
+                out.arena_blocks = blocks
                 if self.side_fraction:
                     plan.compress_outlier_async([r.source for r in reqs], [r.eb for r in reqs], out, stream=s)
                 else:

@@ -54,11 +54,18 @@ def test_served_results_equal_alone():
             assert float((r.result - x).abs().max()) <= tol, r.rid
 
 
-def _check_alone(done, kept):
-    """kept: rid -> the Archive a compress request produced (copied when it finished, since the
-    server may evict it later)."""
+def _copy(a):
+    """An Archive's tensors copied out of the server's memory (which it may reuse once it has
+    dropped the archive)."""
+    return S.Archive(a.dims, a.payload.clone(), a.tail.clone(), dict(a.record),
+                     side=a.side.clone() if a.side is not None else None)
+
+
+def _check_alone(done, kept, orig):
+    """kept: rid -> copy of the Archive a compress request produced, made when it finished;
+    orig: id of that served Archive -> rid (a decompress request's source is a served Archive)."""
     import torch
-    src_of = {id(kept[r.rid]): r for r in done if r.kind == "compress"}
+    src_of = {rid: r for r in done for rid in [r.rid] if r.kind == "compress"}
     for r in done:
         plan = fzgpu.Plan(*r.dims)
         if r.kind == "compress":
@@ -67,15 +74,17 @@ def _check_alone(done, kept):
             assert a.record == rec, r.rid
             assert torch.equal(a.payload, payload) and torch.equal(a.tail, tail), r.rid
         else:
-            a = r.source
+            rid = orig[id(r.source)]
+            a = kept[rid]
             alone = plan.decompress(a.payload, a.tail, a.record["ebx2"])
             assert torch.equal(r.result, alone), r.rid
-            x = src_of[id(a)].source.float()
+            x = src_of[rid].source.float()
             tol = a.record["eb_abs"] + 4 * 2.0 ** -24 * float(x.abs().max())
             assert float((r.result - x).abs().max()) <= tol, r.rid
 
 
-def test_saturating_stream_shares_batches_under_the_budget():
+@pytest.mark.parametrize("arena", [0, 3_000_000_000])
+def test_saturating_stream_shares_batches_under_the_budget(arena):
     """Arrivals far faster than the device serves them (one every 50 us), over a pool whose
     largest field is 268 MB (every 8th request), and a 0.6 GB budget smaller than what the waiting requests need
     together: waiting requests must share batches, several batches must be on the device at
@@ -95,13 +104,15 @@ def test_saturating_stream_shares_batches_under_the_budget():
     need_big = sum(S.request_bytes(S.Request(0, "compress", big)))
     assert 2 * need_big < budget < sum(sum(S.request_bytes(r)) for r in reqs)
     say(f"saturating stream: {len(reqs)} requests, budget {budget / 1e9} GB, big request {need_big / 1e6:.0f} MB")
-    srv = S.Server(S.DeviceBackend(nstreams=3), hbm_budget_bytes=budget, slots=3, max_fields=16, max_archives=24)
+    be = S.DeviceBackend(nstreams=3, arena_bytes=arena)
+    srv = S.Server(be, hbm_budget_bytes=budget, slots=3, max_fields=16, max_archives=24)
     # archives to decompress from the first arrival on: every pool field served once beforehand
-    kept = {}
+    kept, orig = {}, {}
 
-    def keep(r):  # the archive as served, before any eviction can drop it
+    def keep(r):  # the archive as served, copied before any eviction can drop it
         if r.kind == "compress":
-            kept[r.rid] = r.result
+            kept[r.rid] = _copy(r.result)
+            orig[id(r.result)] = r.rid
 
     warm = [S.Request(1000 + i, "compress", d, 1e-3, x, 0.0) for i, (d, x) in enumerate(pool)]
     srv.run(warm, realtime=False, on_complete=keep)
@@ -121,7 +132,13 @@ def test_saturating_stream_shares_batches_under_the_budget():
     assert srv.budget.peak <= srv.budget.total
     assert sum(r.dims == big for r in done) >= 5
     assert st["decompress"] >= 1
-    _check_alone(warm + done, kept)
+    if arena:  # workspaces and archives came from the arena, and its blocks came back
+        say(f"arena peak {be.arena.peak / 1e9:.3f} GB, misses {be.arena.misses}")
+        assert be.arena.peak > 0 and be.arena.misses == 0
+        srv.close()  # every archive dropped: only the pooled plans' workspaces stay in the arena
+        pooled = sum(b[1] for lst in be.pool.free.values() for p in lst for b in p.arena_blocks)
+        assert be.arena.used == pooled
+    _check_alone(warm + done, kept, orig)
 
 
 def test_outlier_mode_serving_handles_fields_the_fixed_mode_refuses():

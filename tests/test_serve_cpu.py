@@ -202,3 +202,69 @@ def test_a_request_that_can_never_fit_raises_instead_of_spinning():
     srv.budget.take(srv.budget.total - srv.budget.used)  # memory held outside the server's requests
     with pytest.raises(RuntimeError):
         srv.run([], realtime=False)
+
+
+# ---------------------------------------------------------------- device arena
+
+def test_arena_best_fit_merges_and_counts():
+    a = S.DeviceArena(1 << 20, device=None)
+    b1 = a.alloc(1000)            # rounded to 1024
+    b2 = a.alloc(256)
+    b3 = a.alloc(4096)
+    assert b1 == (0, 1024) and b2 == (1024, 256) and b3 == (1280, 4096)
+    assert a.used == 1024 + 256 + 4096
+    a.release(b2)                 # a 256-byte hole between b1 and b3
+    assert a.alloc(200) == (1024, 256)  # best fit takes the hole, not the big tail block
+    a.release((1024, 256))
+    a.release(b1)                 # merges with the hole
+    assert a.free[0] == (0, 1280)
+    a.release(b3)                 # merges everything back into one block
+    assert a.free == [(0, 1 << 20)] and a.used == 0
+    assert a.peak == 1024 + 256 + 4096
+    assert a.alloc(2 << 20) is None
+    with pytest.raises(ValueError):
+        a.release((0, 256))       # not allocated
+
+
+def test_arena_random_alloc_free_keeps_its_invariants():
+    rng = np.random.default_rng(5)
+    a = S.DeviceArena(64 << 20, device=None)
+    live = []
+    for _ in range(3000):
+        if live and (rng.random() < 0.45 or len(live) > 60):
+            a.release(live.pop(int(rng.integers(0, len(live)))))
+        else:
+            b = a.alloc(int(rng.integers(1, 4 << 20)))
+            if b is not None:
+                live.append(b)
+        # free blocks sorted, disjoint, never adjacent; used + free = size; live blocks disjoint
+        offs = sorted(live + a.free)
+        assert all(o1 + n1 <= o2 for (o1, n1), (o2, _) in zip(offs, offs[1:]))
+        assert all(o1 + n1 < o2 for (o1, n1), (o2, _) in zip(a.free, a.free[1:]))
+        assert a.used == sum(n for _, n in live) and a.used + sum(n for _, n in a.free) == a.size
+    for b in live:
+        a.release(b)
+    assert a.free == [(0, a.size)]
+
+
+def test_released_outputs_go_back_with_the_last_archive_of_their_batch():
+    srv, be, _ = _server(max_fields=4, max_archives=6)
+    freed = []
+    be.free_outputs = freed.append       # the device backend returns the batch's arena blocks
+
+    def finish(h, reqs, old=be.finish):
+        old(h, reqs)
+        keep = object()
+        for r in reqs:
+            if isinstance(r.result, S.Archive):
+                r.result.keep = keep
+    be.finish = finish
+    reqs = _reqs(24)
+    srv.run(reqs, realtime=False)
+    kept = {id(r.result.keep) for r, _ in srv.archives}
+    # every batch whose archives were all evicted has been released once, the others not yet
+    assert len(freed) == len({id(k) for k in freed})
+    assert not kept & {id(k) for k in freed}
+    assert len(srv.archives) <= 6
+    srv.close()
+    assert not srv.archives and {id(k) for k in freed} >= kept
