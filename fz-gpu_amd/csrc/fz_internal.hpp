@@ -96,6 +96,9 @@ hipError_t launch_decode_recon_1d(const uint8_t* payload, const uint32_t* tile_e
 // 2D fields: decode + reconstruct to floats in one launch (a workgroup per 16 x 1024 window),
 // for fields whose windows are few enough that it beats decode + reconstruct
 bool       decode_recon_2d_pays(const Geom& g);
+// ... and small enough that each window finds its tiles' offsets from the bitflags itself
+// (launch_decode_recon_2d with tile_excl = chunk_sums = nullptr, no tile-offset launch)
+bool       decode_recon_2d_direct(const Geom& g);
 hipError_t launch_decode_recon_2d(const uint8_t* payload, const uint32_t* tile_excl, const uint32_t* chunk_sums,
                                   const uint16_t* tail, const Geom& g, float ebx2, float* out, hipStream_t s);
 // 3D fields: decode + reconstruct to floats in one launch (a workgroup per window of 8 planes x

@@ -748,6 +748,11 @@ constexpr uint32_t kD2Threads = 1024;
 constexpr uint32_t kD2Cols = 1024;
 constexpr uint32_t kD2MaxQ = 16 * (kD2Cols / 256 + 1);
 constexpr uint32_t kD2Batch = 3;  // QGs per half-wave in flight together (32 x 3 >= kD2MaxQ)
+// Fields whose windows find their tile offsets themselves (no tile-offset launch): at most
+// 4096 tiles (the flag words before a tile fit 8 uint4 loads per thread) and X <= 4096 (a
+// window's 16 rows touch at most (15 X + 1024) / 2048 + 2 <= 64 tiles)
+constexpr uint32_t kD2DirectTiles = 64;
+constexpr uint32_t kD2DirectMaxT = 4 * kD2Threads;
 constexpr uint32_t kQgPitch = 5;
 
 __global__ __launch_bounds__(kD2Threads) void k_decode_recon_2d(const uint8_t* __restrict__ payload,
@@ -761,7 +766,8 @@ __global__ __launch_bounds__(kD2Threads) void k_decode_recon_2d(const uint8_t* _
     __shared__ uint32_t stage[NH][32 * kQgPitch];      // one QG's bit-plane rows per half-wave
     __shared__ int4     csum[3][kD2Cols / 4];          // column sums of row groups 0-2
     __shared__ uint32_t cpre[kTileLocalMaxChunks];     // block counts of the chunks before each chunk
-    __shared__ uint32_t wtot[4], qtab[kD2MaxQ], nq_s;  // the QGs to decode
+    __shared__ uint32_t wtot[kD2Threads / 64], qtab[kD2MaxQ], nq_s;  // the QGs to decode
+    __shared__ uint32_t toff[kD2DirectTiles];         // direct offsets: blocks before tiles t_lo ..
     const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     // XCD-sliced order: the windows that share a tile's blocks (neighbours along x, and along y
     // for narrow fields) read them through one L2
@@ -769,9 +775,28 @@ __global__ __launch_bounds__(kD2Threads) void k_decode_recon_2d(const uint8_t* _
     const uint32_t bx = wi % rx, by = wi / rx;
     const uint32_t xw0 = bx * kD2Cols, y0 = by * 16;
     const uint32_t cw = min(kD2Cols, g.X - xw0), rows = min(16u, g.Y - y0);
-    // set-up: the chunk prefix (waves 0-3) and the QG list (wave 0)
+    // without tile offsets (tile_excl null: small fields) the window finds the block offsets of
+    // its tiles t_lo .. from the bitflags: the popcount of every flag word before tile t_lo (at
+    // most 8 uint4 loads per thread, issued together), then a scan over the window's own tiles
+    const bool     direct = tile_excl == nullptr;
+    const uint32_t t_lo = min((y0 * g.X + xw0) >> 11, g.T);
+    // set-up: the chunk prefix (waves 0-3) or the direct prefix (all waves), and the QG list (wave 0)
     uint32_t cv = 0, cincl = 0;
-    if (tid < kTileLocalMaxChunks) {
+    if (direct) {
+        const uint4*   bf4 = reinterpret_cast<const uint4*>(payload);
+        const uint32_t n4 = 2 * t_lo;  // uint4s of flag words before tile t_lo (<= 8 kD2Threads)
+        uint4          v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const uint32_t i = tid + (uint32_t)k * kD2Threads;
+            v[k] = i < n4 ? bf4[i] : make_uint4(0, 0, 0, 0);
+        }
+        uint32_t c = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) c += __popc(v[k].x) + __popc(v[k].y) + __popc(v[k].z) + __popc(v[k].w);
+        c = wave_sum(c);
+        if (lane == 0) wtot[wv] = c;
+    } else if (tid < kTileLocalMaxChunks) {
         cv = (chunk_sums && tid < (g.T + kWG - 1) / kWG) ? chunk_sums[tid] : 0u;
         cincl = wave_incl_scan(cv);
         if (lane == 63) wtot[wv] = cincl;
@@ -789,7 +814,16 @@ __global__ __launch_bounds__(kD2Threads) void k_decode_recon_2d(const uint8_t* _
         if (lane == 63) nq_s = incl;
     }
     __syncthreads();
-    if (tid < kTileLocalMaxChunks) {
+    if (direct) {
+        if (wv == 0) {  // lane i: tile t_lo + i (the window's tiles span at most kD2DirectTiles)
+            uint32_t base = 0;
+#pragma unroll
+            for (int k = 0; k < (int)(kD2Threads / 64); k++) base += wtot[k];
+            const uint32_t t = t_lo + lane;
+            const uint32_t pc = t < g.T ? tile_popcount(reinterpret_cast<const uint32_t*>(payload), t) : 0u;
+            toff[lane] = base + wave_incl_scan(pc) - pc;
+        }
+    } else if (tid < kTileLocalMaxChunks) {
         uint32_t pre = 0;
         for (uint32_t k = 0; k < wv; k++) pre += wtot[k];
         cpre[tid] = pre + cincl - cv;
@@ -811,7 +845,7 @@ __global__ __launch_bounds__(kD2Threads) void k_decode_recon_2d(const uint8_t* _
             const uint32_t t = q[b] >> 3;
             const bool     in = live[b] && t < g.T;
             fw[b] = in ? bflags[8 * t + (hl & 7u)] : 0u;
-            ex[b] = in ? tile_excl[t] + (chunk_sums ? cpre[t / kWG] : 0u) : 0u;
+            ex[b] = !in ? 0u : direct ? toff[t - t_lo] : tile_excl[t] + (chunk_sums ? cpre[t / kWG] : 0u);
         }
         uint4 blk[kD2Batch];
         bool  nzq[kD2Batch];  // some block of the wave's two QGs is nonzero (uniform)
@@ -963,6 +997,11 @@ __global__ __launch_bounds__(kD2Threads) void k_decode_recon_2d(const uint8_t* _
 // fused form wins only while the windows are resident at once or nearly (two 1024-thread
 // workgroups per CU): up to 26000 x 388 (650 windows), not at 39200 x 388 (975).
 constexpr uint64_t kD2MaxWindows = 768;
+
+bool decode_recon_2d_direct(const Geom& g)
+{
+    return decode_recon_2d_pays(g) && g.T <= kD2DirectMaxT && g.X <= 4096;
+}
 
 bool decode_recon_2d_pays(const Geom& g)
 {

@@ -439,3 +439,23 @@ def test_decompress_fused_3d_beyond_65536_tiles():
     xr = _np(plan.decompress(payload, tail, r["ebx2"]))
     rc, xo = O.decompress(ref["payload"], ref["tail"], n, dims, ref["info"]["ebx2"], nthreads=16)
     assert rc == 0 and np.array_equal(xr.view(np.uint32), xo.view(np.uint32))
+
+
+@pytest.mark.parametrize("dims", [(4096, 2048, 1), (4096, 2049, 1), (4093, 2049, 1), (4097, 100, 1), (3600, 1800, 1),
+                                  (2048, 4096, 1), (5, 700, 1)])
+def test_decompress_2d_windows_finding_their_own_tile_offsets(dims):
+    """2D fields of at most 4096 tiles and X <= 4096 decompress in one launch, each window
+    finding the block offsets of its tiles from the bitflags (T = 4096 at X = 4096 and at
+    X = 4093, T = 4098 and X = 4097 past the limits, the 3600 x 1800 field): bit-exact against
+    the oracle."""
+    n = dims[0] * dims[1]
+    say(f"direct 2D dims={dims} T={n // 2048}")
+    x = D.generate("waves", dims, 17 + dims[1])
+    ref = O.compress(x.ravel(), dims, 1e-3, O.MODE_FIXED, nthreads=16)
+    assert ref["status"] == 0
+    plan = fzgpu.Plan(*dims)
+    payload, tail, r = plan.compress(_to_dev(x.ravel()), 1e-3)
+    assert np.array_equal(_np(payload), ref["payload"])
+    xr = _np(plan.decompress(payload, tail, r["ebx2"]))
+    rc, xo = O.decompress(ref["payload"], ref["tail"], n, dims, ref["info"]["ebx2"], nthreads=16)
+    assert rc == 0 and np.array_equal(xr.view(np.uint32), xo.view(np.uint32))
