@@ -326,7 +326,6 @@ fz_status fz_decompress_f32(const fz_plan plan, const uint8_t* d_payload, const 
     const uint16_t* tail = plan->mode != FZ_MODE_LITERAL ? d_tail : nullptr;
     FZ_TRY(prof_mark(plan, stream, true));
     if (g.ndim == 2 && decode_recon_2d_direct(g)) {  // small 2D fields: no tile-offset launch either
-        FZ_TRY(prof_mark(plan, stream, false));
         FZ_TRY(launch_decode_recon_2d(d_payload, nullptr, nullptr, tail, g, ebx2, d_out, stream));
         FZ_TRY(prof_mark(plan, stream, false));
         return FZ_SUCCESS;

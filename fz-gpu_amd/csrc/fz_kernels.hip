@@ -768,6 +768,7 @@ __global__ __launch_bounds__(kD2Threads) void k_decode_recon_2d(const uint8_t* _
     __shared__ uint32_t cpre[kTileLocalMaxChunks];     // block counts of the chunks before each chunk
     __shared__ uint32_t wtot[kD2Threads / 64], qtab[kD2MaxQ], nq_s;  // the QGs to decode
     __shared__ uint32_t toff[kD2DirectTiles];         // direct offsets: blocks before tiles t_lo ..
+    __shared__ uint4    tflg[2 * kD2DirectTiles];     // ... and their 8 bitflag words each
     const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     // XCD-sliced order: the windows that share a tile's blocks (neighbours along x, and along y
     // for narrow fields) read them through one L2
@@ -820,7 +821,15 @@ __global__ __launch_bounds__(kD2Threads) void k_decode_recon_2d(const uint8_t* _
 #pragma unroll
             for (int k = 0; k < (int)(kD2Threads / 64); k++) base += wtot[k];
             const uint32_t t = t_lo + lane;
-            const uint32_t pc = t < g.T ? tile_popcount(reinterpret_cast<const uint32_t*>(payload), t) : 0u;
+            uint4          a = make_uint4(0, 0, 0, 0), b = a;
+            if (t < g.T) {
+                a = reinterpret_cast<const uint4*>(payload)[2 * t];
+                b = reinterpret_cast<const uint4*>(payload)[2 * t + 1];
+            }
+            tflg[2 * lane] = a;
+            tflg[2 * lane + 1] = b;
+            const uint32_t pc = __popc(a.x) + __popc(a.y) + __popc(a.z) + __popc(a.w) + __popc(b.x) + __popc(b.y) +
+                                __popc(b.z) + __popc(b.w);
             toff[lane] = base + wave_incl_scan(pc) - pc;
         }
     } else if (tid < kTileLocalMaxChunks) {
@@ -844,7 +853,9 @@ __global__ __launch_bounds__(kD2Threads) void k_decode_recon_2d(const uint8_t* _
             q[b] = live[b] ? qtab[i] : 0u;
             const uint32_t t = q[b] >> 3;
             const bool     in = live[b] && t < g.T;
-            fw[b] = in ? bflags[8 * t + (hl & 7u)] : 0u;
+            fw[b] = !in ? 0u
+                    : direct ? reinterpret_cast<const uint32_t*>(tflg)[8 * (t - t_lo) + (hl & 7u)]
+                             : bflags[8 * t + (hl & 7u)];
             ex[b] = !in ? 0u : direct ? toff[t - t_lo] : tile_excl[t] + (chunk_sums ? cpre[t / kWG] : 0u);
         }
         uint4 blk[kD2Batch];
