@@ -719,7 +719,7 @@ struct fz_batch_plan_s {
     BatchShape            shape{};
     fz_batch_tuning       tune{};
     struct {
-        size_t fields, ustart, ufield, pstart[kNumChunks], rstart, rfield, tables_end, ptrs, prm, part, counts, dsts, texcl,
+        size_t fields, ustart, ufield, pstart[kNumChunks], rstart, rfield, wfield, tables_end, ptrs, prm, part, counts, dsts, texcl,
             staging, total;
     } L{};
     std::vector<BPtrs>      host_ptrs;  // staging of the call's pointer records (sorted order)
@@ -857,6 +857,7 @@ BatchDev batch_dev(const fz_batch_plan p, void* d_ws, fz_result* res)
     d.pstart = reinterpret_cast<const uint32_t*>(ws + p->L.pstart[chunk_index(p)]);
     d.rstart = reinterpret_cast<const uint32_t*>(ws + p->L.rstart);
     d.rfield = reinterpret_cast<const uint32_t*>(ws + p->L.rfield);
+    d.wfield = reinterpret_cast<const uint32_t*>(ws + p->L.wfield);
     d.ptrs = reinterpret_cast<BPtrs*>(ws + p->L.ptrs);
     d.prm = reinterpret_cast<BParams*>(ws + p->L.prm);
     d.part = reinterpret_cast<float4*>(ws + p->L.part);
@@ -928,7 +929,7 @@ fz_status batch_decompress_impl(const fz_batch_plan plan, const uint8_t* const* 
     hipEvent_t* ev = nullptr;
     if (timed && plan->prof) {
         ev = plan->ev;
-        plan->nslots = 5;
+        plan->nslots = 6;
     }
     FZ_TRY(launch_batch_decompress(d, plan->shape, ev, stream));
     return FZ_SUCCESS;
@@ -1002,6 +1003,7 @@ fz_status fz_batch_plan_create(fz_batch_plan* plan, const fz_field_shape* shapes
     p->rstart.resize(nfields + 1);
     for (int c = 0; c < kNumChunks; c++) p->pstart[c].resize(nfields + 1);
     uint64_t u = 0, r = 0, parts[kNumChunks] = {};
+    std::vector<std::pair<uint32_t, uint64_t>> fused[3];  // (field, windows) per window size
     BatchShape& b = p->shape;
     b.nf = nfields;
     for (uint32_t k = 0; k < nfields; k++) {
@@ -1022,6 +1024,17 @@ fz_status fz_batch_plan_create(fz_batch_plan* plan, const fz_field_shape* shapes
             F.rdims[0] = (g.X + 4 * F.rq - 1) / (4 * F.rq);
             F.rdims[1] = (g.Y + kWG / F.rq - 1) / (kWG / F.rq);
             F.rdims[2] = (g.Z + 7) / 8;
+            uint32_t nt, qlog, rx, ry;
+            uint64_t nwin;
+            if (mode != FZ_MODE_OUTLIER && batch_fused3d(g, nt, qlog, rx, ry, nwin)) {
+                // decoded and reconstructed by the fused window kernel: no reconstruct workgroups
+                F.wnt = nt;
+                F.wq = qlog;
+                F.wdims[0] = rx;
+                F.wdims[1] = ry;
+                F.rdims[0] = F.rdims[1] = F.rdims[2] = 0;
+                fused[nt == 256 ? 0 : nt == 512 ? 1 : 2].push_back({k, nwin});
+            }
         }
         const int nd = g.ndim;
         if (k == 0 || geo[order[k - 1]].ndim != nd) {
@@ -1049,6 +1062,19 @@ fz_status fz_batch_plan_create(fz_batch_plan* plan, const fz_field_shape* shapes
     }
     p->ustart[nfields] = (uint32_t)u;
     p->rstart[nfields] = (uint32_t)r;
+    uint64_t w = 0;  // the fused 3D windows, by window size
+    for (int c = 0; c < 3; c++) {
+        b.w_lo[c] = (uint32_t)w;
+        for (const auto& fw : fused[c]) {
+            p->fields[fw.first].wstart = (uint32_t)w;
+            w += fw.second;
+        }
+        b.w_hi[c] = (uint32_t)w;
+    }
+    if (w > 0xFFFFFFF0ull) {
+        delete p;
+        return FZ_ERR_CAPACITY;
+    }
     for (int c = 0; c < kNumChunks; c++) {
         p->pstart[c][nfields] = (uint32_t)parts[c];
         p->P[c] = (uint32_t)parts[c];
@@ -1066,6 +1092,7 @@ fz_status fz_batch_plan_create(fz_batch_plan* plan, const fz_field_shape* shapes
     }
     L.rstart = o; o = al256(o + 4ull * (nfields + 1));
     L.rfield = o; o = al256(o + 4ull * r);
+    L.wfield = o; o = al256(o + 4ull * w);
     L.tables_end = o;
     L.ptrs = o;    o = al256(o + sizeof(BPtrs) * nfields);
     L.prm = o;     o = al256(o + sizeof(BParams) * nfields);
@@ -1174,6 +1201,15 @@ fz_status fz_batch_workspace_init(const fz_batch_plan plan, void* d_ws, size_t w
         uint32_t* rf = reinterpret_cast<uint32_t*>(t.data() + L.rfield);
         for (uint32_t f = 0; f < plan->nf; f++)
             for (uint32_t w = plan->rstart[f]; w < plan->rstart[f + 1]; w++) rf[w] = f;
+    }
+    {
+        uint32_t* wf = reinterpret_cast<uint32_t*>(t.data() + L.wfield);
+        for (uint32_t f = 0; f < plan->nf; f++) {
+            const BField& F = plan->fields[f];
+            if (!F.wnt) continue;
+            const uint64_t nw = (uint64_t)F.wdims[0] * F.wdims[1] * ((F.g.Z + 7) / 8);
+            for (uint64_t w = 0; w < nw; w++) wf[F.wstart + w] = f;
+        }
     }
     FZ_TRY(hipMemcpyAsync(d_ws, t.data(), t.size(), hipMemcpyHostToDevice, stream));
     FZ_TRY(hipStreamSynchronize(stream));

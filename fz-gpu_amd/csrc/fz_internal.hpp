@@ -107,6 +107,12 @@ bool       decode_recon_3d_pays(const Geom& g);
 hipError_t launch_decode_recon_3d(const uint8_t* payload, const uint32_t* tile_excl, const uint32_t* chunk_sums,
                                   const uint16_t* tail, const Geom& g, float ebx2, float* out, hipStream_t s);
 hipError_t launch_reconstruct(const uint16_t* codes, float* out, const Geom& g, float ebx2, hipStream_t s);
+// Batched decompress: whether a 3D field is decoded and reconstructed by the fused window kernel
+// (then its windows' threads, log2 quads per row, windows along x and y, and window count)
+bool       batch_fused3d(const Geom& g, uint32_t& nt, uint32_t& qlog, uint32_t& rx, uint32_t& ry, uint64_t& nwin);
+struct BatchDev;
+struct BatchShape;
+hipError_t launch_batch_fused3d(const BatchDev& d, const BatchShape& b, hipStream_t s);
 // reconstruct from u16 codes whose positions flagged in `bits` take their delta from `over`
 hipError_t launch_reconstruct_outlier_src(const uint16_t* codes, const uint32_t* bits, const int32_t* over,
                                           float* out, const Geom& g, float ebx2, hipStream_t s);
@@ -154,7 +160,11 @@ struct BField {        // per field, uploaded once per workspace (128 B)
     uint32_t ustart;   // first global unit
     uint32_t rdims[3]; // reconstruct workgroups along x, y, z
     uint32_t rq;       // 3D: quads per row of its reconstruct workgroups (recon3d_q)
-    uint32_t pad[8];
+    uint32_t wnt;      // 3D decoded and reconstructed in one kernel: threads per window (0: not)
+    uint32_t wq;       // ... log2 of quads per window row
+    uint32_t wdims[2]; // ... windows along x, y
+    uint32_t wstart;   // ... first window (batch window space)
+    uint32_t pad[3];
 };
 static_assert(sizeof(BField) == 128, "BField layout");
 
@@ -185,6 +195,7 @@ struct BatchDev {
     const uint32_t* pstart;  // nf + 1, first range partial of each field (selected chunk)
     const uint32_t* rstart;  // nf + 1, first reconstruct workgroup of each field
     const uint32_t* rfield;  // per reconstruct workgroup: its field (sorted order)
+    const uint32_t* wfield;  // per fused 3D window: its field (sorted order)
     BPtrs*          ptrs;
     BParams*        prm;
     float4*         part;
@@ -206,6 +217,7 @@ struct BatchShape {
     uint32_t xcd[4];            // tile order per dimensionality: 1 XCD-sliced, 0 linear
     uint32_t wave_tile[4];      // tile layout per dimensionality: 1 one wave per tile, 0 four
     uint32_t gather_waves;      // units per gather workgroup
+    uint32_t w_lo[3], w_hi[3];  // fused 3D windows of 256 / 512 / 1024 threads (decompress)
 };
 
 // A group of consecutive fields (sorted order) and the units / range partials they own.

@@ -755,11 +755,11 @@ constexpr uint32_t kD2DirectTiles = 64;
 constexpr uint32_t kD2DirectMaxT = 4 * kD2Threads;
 constexpr uint32_t kQgPitch = 5;
 
-__global__ __launch_bounds__(kD2Threads) void k_decode_recon_2d(const uint8_t* __restrict__ payload,
-                                                                const uint32_t* __restrict__ tile_excl,
-                                                                const uint32_t* __restrict__ chunk_sums,
-                                                                const uint16_t* __restrict__ tail, Geom g, float ebx2,
-                                                                float* __restrict__ out, uint32_t rx)
+// Window wi of a 2D field (all kD2Threads threads of the workgroup); tile_excl / chunk_sums as
+// launch_tile_offsets gives them, or tile_excl = nullptr for the direct offsets.
+__device__ __forceinline__ void d2_window(const uint8_t* __restrict__ payload, const uint32_t* __restrict__ tile_excl,
+                                          const uint32_t* __restrict__ chunk_sums, const uint16_t* __restrict__ tail,
+                                          const Geom& g, float ebx2, float* __restrict__ out, uint32_t rx, uint32_t wi)
 {
     constexpr uint32_t NH = kD2Threads / 32;
     __shared__ __align__(16) uint16_t win[16 * kD2Cols];  // the window's codes, row-major
@@ -770,9 +770,6 @@ __global__ __launch_bounds__(kD2Threads) void k_decode_recon_2d(const uint8_t* _
     __shared__ uint32_t toff[kD2DirectTiles];         // direct offsets: blocks before tiles t_lo ..
     __shared__ uint4    tflg[2 * kD2DirectTiles];     // ... and their 8 bitflag words each
     const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    // XCD-sliced order: the windows that share a tile's blocks (neighbours along x, and along y
-    // for narrow fields) read them through one L2
-    const uint32_t wi = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t bx = wi % rx, by = wi / rx;
     const uint32_t xw0 = bx * kD2Cols, y0 = by * 16;
     const uint32_t cw = min(kD2Cols, g.X - xw0), rows = min(16u, g.Y - y0);
@@ -1002,6 +999,17 @@ __global__ __launch_bounds__(kD2Threads) void k_decode_recon_2d(const uint8_t* _
     }
 }
 
+__global__ __launch_bounds__(kD2Threads) void k_decode_recon_2d(const uint8_t* __restrict__ payload,
+                                                                const uint32_t* __restrict__ tile_excl,
+                                                                const uint32_t* __restrict__ chunk_sums,
+                                                                const uint16_t* __restrict__ tail, Geom g, float ebx2,
+                                                                float* __restrict__ out, uint32_t rx)
+{
+    // XCD-sliced order: the windows that share a tile's blocks (neighbours along x, and along y
+    // for narrow fields) read them through one L2
+    d2_window(payload, tile_excl, chunk_sums, tail, g, ebx2, out, rx, xcd_tile(blockIdx.x, gridDim.x));
+}
+
 // With X % 4 == 0 a lane puts its codes into the window with one or two wide LDS stores, and
 // the fused form beats decode + reconstruct at every size measured (3600 x 1800: 16.0 against
 // 23.5 us; 16384 x 16384: 369 against 473 us).  Otherwise each code is a 2-byte store and the
@@ -1051,13 +1059,13 @@ hipError_t launch_decode_recon_2d(const uint8_t* payload, const uint32_t* tile_e
 // and stores rows 8 yb + 4 yh .. + 3.
 constexpr uint32_t d3_maxq(uint32_t nt) { return nt == 1024 ? 192u : 8u * (nt / 64 + 1); }
 
+// Window wi of a 3D field (all NT threads of the workgroup): cpre is the chunk prefix in LDS
+// (chunk-local tile offsets) or nullptr (tile_excl holds global offsets).
 template <int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 8 : 7))) void k_decode_recon_3d(const uint8_t* __restrict__ payload,
-                                                        const uint32_t* __restrict__ tile_excl,
-                                                        const uint32_t* __restrict__ chunk_sums,
-                                                        const uint16_t* __restrict__ tail, Geom g, float ebx2,
-                                                        float* __restrict__ out, uint32_t qlog, uint32_t rx,
-                                                        uint32_t ry, uint32_t nwin)
+__device__ __forceinline__ void d3_window(const uint8_t* __restrict__ payload, const uint32_t* __restrict__ tile_excl,
+                                          const uint32_t* cpre, const uint16_t* __restrict__ tail, const Geom& g,
+                                          float ebx2, float* __restrict__ out, uint32_t qlog, uint32_t rx, uint32_t ry,
+                                          uint32_t wi)
 {
     static_assert(NT == 256 || NT == 512 || NT == 1024, "threads per window");
     constexpr uint32_t NH = NT / 32;                      // half-waves
@@ -1071,36 +1079,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 
     constexpr bool     SERIAL = NT >= 512;
     __shared__ __align__(16) uint16_t win[32 * NT];       // codes [plane][row][column]; later int4 [4][R][Q]
     __shared__ uint32_t stage[SERIAL ? NT / 64 : NH][32 * kQgPitch];  // bit-plane rows of a QG
-    __shared__ uint32_t cpre[kTileLocalMaxChunks];        // block counts of the chunks before each chunk
-    __shared__ uint32_t wtot[4], qtab[MAXQ], nq_s;        // the QGs to decode
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    __shared__ uint32_t qtab[MAXQ], nq_s;                 // the QGs to decode
     const uint32_t Q = 1u << qlog, R = (uint32_t)NT >> qlog, C = 4 * Q;
-    // the chunk prefix, once per workgroup (waves 0-3)
-    {
-        uint32_t cv = 0, cincl = 0;
-        if (tid < kTileLocalMaxChunks) {
-            cv = (chunk_sums && tid < (g.T + kWG - 1) / kWG) ? chunk_sums[tid] : 0u;
-            cincl = wave_incl_scan(cv);
-            if (lane == 63) wtot[wv] = cincl;
-        }
-        __syncthreads();
-        if (tid < kTileLocalMaxChunks) {
-            uint32_t pre = 0;
-            for (uint32_t k = 0; k < wv; k++) pre += wtot[k];
-            cpre[tid] = pre + cincl - cv;
-        }
-    }
-    // windows it, it + G, ... (G = gridDim.x, a multiple of 8 when G < nwin): a workgroup issues
-    // its next window's loads while the stores of the last one drain.  XCD-sliced order: G % 8 ==
-    // 0 keeps a workgroup on one XCD's windows, and windows that share a QG (neighbours along x,
-    // or along y / z when rows or planes are whole) read its blocks through one L2.
-    for (uint32_t it = blockIdx.x; it < nwin; it += gridDim.x) {
     // the lane indices are recomputed in every window from an opaque copy of the thread index:
     // values hoisted out of the loop would otherwise stay live across it (about 40 more VGPRs)
     uint32_t tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
     const uint32_t lane = tid & 63, wv = tid >> 6;
-    const uint32_t wi = xcd_tile(it, nwin);
     const uint32_t bx = wi % rx, t1 = wi / rx, by = t1 % ry, bz = t1 / ry;
     const uint32_t xw0 = bx * C, y0 = by * R, z0 = bz * 8;
     const uint32_t cw = min(C, g.X - xw0), rows = min(R, g.Y - y0), nz = min(8u, g.Z - z0);
@@ -1128,8 +1113,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 
         for (uint32_t j = 0; j < c; j++) qtab[incl - c + j] = start + j;
         if (lane == 63) nq_s = incl;
     }
-    // the QG list and (first window) the chunk prefix are written; every access to the window's
-    // memory by the last window is done
+    // the QG list (and, before the first window, the caller's chunk prefix) is written; every
+    // access to the window's memory by the workgroup's last window is done
     __syncthreads();
     const uint32_t  nq = nq_s;
     const uint32_t  half = lane >> 5, hl = lane & 31, H = 2 * wv + half;
@@ -1152,7 +1137,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 
             const uint32_t t = q[b] >> 3;
             const bool     in = live[b] && t < g.T;
             fw[b] = in ? bflags[8 * t + (hl & 7u)] : 0u;
-            ex[b] = in ? tile_excl[t] + (chunk_sums ? cpre[t / kWG] : 0u) : 0u;
+            ex[b] = in ? tile_excl[t] + (cpre ? cpre[t / kWG] : 0u) : 0u;
         }
         uint4 blk[NB];
         bool  nzq[NB];  // some block of the wave's two QGs is nonzero (uniform)
@@ -1327,7 +1312,41 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 
             }
         }
     }
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 8 : 7))) void k_decode_recon_3d(const uint8_t* __restrict__ payload,
+                                                        const uint32_t* __restrict__ tile_excl,
+                                                        const uint32_t* __restrict__ chunk_sums,
+                                                        const uint16_t* __restrict__ tail, Geom g, float ebx2,
+                                                        float* __restrict__ out, uint32_t qlog, uint32_t rx,
+                                                        uint32_t ry, uint32_t nwin)
+{
+    __shared__ uint32_t cpre[kTileLocalMaxChunks];  // block counts of the chunks before each chunk
+    __shared__ uint32_t wtot[4];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    // the chunk prefix, once per workgroup (waves 0-3)
+    {
+        uint32_t cv = 0, cincl = 0;
+        if (tid < kTileLocalMaxChunks) {
+            cv = (chunk_sums && tid < (g.T + kWG - 1) / kWG) ? chunk_sums[tid] : 0u;
+            cincl = wave_incl_scan(cv);
+            if (lane == 63) wtot[wv] = cincl;
+        }
+        __syncthreads();
+        if (tid < kTileLocalMaxChunks) {
+            uint32_t pre = 0;
+            for (uint32_t k = 0; k < wv; k++) pre += wtot[k];
+            cpre[tid] = pre + cincl - cv;
+        }
     }
+    // windows it, it + G, ... (G = gridDim.x, a multiple of 8 when G < nwin): a workgroup issues
+    // its next window's loads while the stores of the last one drain.  XCD-sliced order: G % 8 ==
+    // 0 keeps a workgroup on one XCD's windows, and windows that share a QG (neighbours along x,
+    // or along y / z when rows or planes are whole) read its blocks through one L2.
+    for (uint32_t it = blockIdx.x; it < nwin; it += gridDim.x)
+        d3_window<NT>(payload, tile_excl, chunk_sums ? cpre : nullptr, tail, g, ebx2, out, qlog, rx, ry,
+                      xcd_tile(it, nwin));
 }
 
 // compute units of the current device (256 on MI355X when the query fails)
@@ -1395,6 +1414,48 @@ hipError_t launch_decode_recon_3d(const uint8_t* payload, const uint32_t* tile_e
     else if (nt == 512) FZ_LAUNCH(k_decode_recon_3d<512><<<grid, 512, 0, s>>>(payload, tile_excl, chunk_sums, tail, g, ebx2, out, qlog, rx, ry, (uint32_t)n));
     else FZ_LAUNCH(k_decode_recon_3d<256><<<grid, 256, 0, s>>>(payload, tile_excl, chunk_sums, tail, g, ebx2, out, qlog, rx, ry, (uint32_t)n));
     return hipGetLastError();
+}
+
+// Batched decompress: the fused 3D windows of every field of one window size, a workgroup per
+// window; the field comes from the per-window table (one scalar load), its tile offsets are the
+// per-field scan's (global within the field, no chunk prefix).  Windows in XCD-sliced order.
+template <int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 8 : 7))) void k_batch_fused3d(BatchDev d, uint32_t w_lo)
+{
+    const uint32_t w = w_lo + xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t f = d.wfield[w];
+    const BField&  F = d.fields[f];
+    const Geom     g = F.g;
+    const BPtrs    q = d.ptrs[f];
+    d3_window<NT>(static_cast<const uint8_t*>(q.src), d.texcl + F.ustart, nullptr, g.mode == 0 ? q.tail : nullptr, g,
+                  (float)q.param, static_cast<float*>(q.dst), F.wq, F.wdims[0], F.wdims[1], w - F.wstart);
+}
+
+// In a batch the windows of many fields fill the device together, so the fused form is taken
+// wherever its windows hold most of their columns (X <= 256, or 3/4 of 512-column windows).
+bool batch_fused3d(const Geom& g, uint32_t& nt, uint32_t& qlog, uint32_t& rx, uint32_t& ry, uint64_t& nwin)
+{
+    if (g.ndim != 3) return false;
+    d3_shape(g, nt, qlog);
+    const uint64_t C = 4u << qlog, R = nt >> qlog;
+    rx = (uint32_t)((g.X + C - 1) / C);
+    ry = (uint32_t)((g.Y + R - 1) / R);
+    nwin = (uint64_t)rx * ry * ((g.Z + 7) / 8);
+    return g.X <= 256 || 4 * (uint64_t)g.X >= 3 * rx * C;
+}
+
+hipError_t launch_batch_fused3d(const BatchDev& d, const BatchShape& b, hipStream_t s)
+{
+    for (int c = 0; c < 3; c++) {
+        const uint32_t n = b.w_hi[c] - b.w_lo[c];
+        if (n == 0) continue;
+        if (c == 0) FZ_LAUNCH(k_batch_fused3d<256><<<n, 256, 0, s>>>(d, b.w_lo[c]));
+        else if (c == 1) FZ_LAUNCH(k_batch_fused3d<512><<<n, 512, 0, s>>>(d, b.w_lo[c]));
+        else FZ_LAUNCH(k_batch_fused3d<1024><<<n, 1024, 0, s>>>(d, b.w_lo[c]));
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 template <typename SRC>

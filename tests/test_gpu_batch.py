@@ -26,7 +26,7 @@ def say(*a):
 MIX = [(5000, 1, 1), (4096, 1, 1), (2047, 1, 1), (9, 1, 1), (100, 77, 1), (16, 16, 1), (17, 33, 1), (3600, 20, 1),
        (1, 3000, 1), (33, 20, 17), (64, 64, 64), (500, 50, 10), (1, 1, 5000), (7, 1, 900), (70000, 1, 1),
        (300, 299, 1), (41, 37, 29), (1000, 3, 1, 6000), (2048, 1, 1, 4100), (3601, 37, 1), (9, 50, 40),
-       (13, 1, 700)]
+       (13, 1, 700), (200, 30, 12), (1000, 9, 9), (1100, 9, 9)]
 EBS = (1e-2, 1e-3, 1e-4)
 
 
