@@ -114,6 +114,7 @@ struct fz_plan_s {
     Layout     L;
     int        mode;
     int        wave_tile = 0;               // tile-kernel layout of the fused compress
+    int        inverse_form = FZ_INVERSE_AUTO;  // kernel form of the 2D / 3D decompress
     int        prof = 0;                    // per-kernel events enabled
     int        nev = 0;                     // events recorded by the last call
     int        paired = 0;                  // last call recorded (start, stop) pairs per kernel
@@ -185,6 +186,13 @@ fz_status fz_plan_set_tile_layout(fz_plan plan, int wave_tile)
 {
     if (!plan || (wave_tile != 0 && wave_tile != 1)) return FZ_ERR_INVALID_ARG;
     plan->wave_tile = wave_tile;
+    return FZ_SUCCESS;
+}
+
+fz_status fz_plan_set_inverse_form(fz_plan plan, int form)
+{
+    if (!plan || form < FZ_INVERSE_AUTO || form > FZ_INVERSE_SPLIT) return FZ_ERR_INVALID_ARG;
+    plan->inverse_form = form;
     return FZ_SUCCESS;
 }
 
@@ -324,8 +332,11 @@ fz_status fz_decompress_f32(const fz_plan plan, const uint8_t* d_payload, const 
     uint32_t*     texcl = reinterpret_cast<uint32_t*>(ws + L.texcl);
     uint32_t*     codes = reinterpret_cast<uint32_t*>(ws + L.codes);
     const uint16_t* tail = plan->mode != FZ_MODE_LITERAL ? d_tail : nullptr;
+    const int  form = plan->inverse_form;
+    const bool win2 = g.ndim == 2 && (form == FZ_INVERSE_WINDOWS || (form == FZ_INVERSE_AUTO && decode_recon_2d_pays(g)));
+    const bool win3 = g.ndim == 3 && (form == FZ_INVERSE_WINDOWS || (form == FZ_INVERSE_AUTO && decode_recon_3d_pays(g)));
     FZ_TRY(prof_mark(plan, stream, true));
-    if (g.ndim == 2 && decode_recon_2d_direct(g)) {  // small 2D fields: no tile-offset launch either
+    if (win2 && decode_recon_2d_direct(g)) {  // small 2D fields: no tile-offset launch either
         FZ_TRY(launch_decode_recon_2d(d_payload, nullptr, nullptr, tail, g, ebx2, d_out, stream));
         FZ_TRY(prof_mark(plan, stream, false));
         return FZ_SUCCESS;
@@ -339,12 +350,12 @@ fz_status fz_decompress_f32(const fz_plan plan, const uint8_t* d_payload, const 
         FZ_TRY(prof_mark(plan, stream, false));
         return FZ_SUCCESS;
     }
-    if (g.ndim == 2 && decode_recon_2d_pays(g)) {  // codes decoded into LDS per window: one launch
+    if (win2) {  // codes decoded into LDS per window: one launch
         FZ_TRY(launch_decode_recon_2d(d_payload, texcl, csums, tail, g, ebx2, d_out, stream));
         FZ_TRY(prof_mark(plan, stream, false));
         return FZ_SUCCESS;
     }
-    if (g.ndim == 3 && decode_recon_3d_pays(g)) {  // codes decoded into LDS per window: one launch
+    if (win3) {  // codes decoded into LDS per window: one launch
         FZ_TRY(launch_decode_recon_3d(d_payload, texcl, csums, tail, g, ebx2, d_out, stream));
         FZ_TRY(prof_mark(plan, stream, false));
         return FZ_SUCCESS;
@@ -1003,7 +1014,7 @@ fz_status fz_batch_plan_create(fz_batch_plan* plan, const fz_field_shape* shapes
     p->rstart.resize(nfields + 1);
     for (int c = 0; c < kNumChunks; c++) p->pstart[c].resize(nfields + 1);
     uint64_t u = 0, r = 0, parts[kNumChunks] = {};
-    std::vector<std::pair<uint32_t, uint64_t>> fused[3];  // (field, windows) per window size
+    std::vector<std::pair<uint32_t, uint64_t>> fused[kWinClasses];  // (field, windows) per window size
     BatchShape& b = p->shape;
     b.nf = nfields;
     for (uint32_t k = 0; k < nfields; k++) {
@@ -1024,16 +1035,18 @@ fz_status fz_batch_plan_create(fz_batch_plan* plan, const fz_field_shape* shapes
             F.rdims[0] = (g.X + 4 * F.rq - 1) / (4 * F.rq);
             F.rdims[1] = (g.Y + kWG / F.rq - 1) / (kWG / F.rq);
             F.rdims[2] = (g.Z + 7) / 8;
-            uint32_t nt, qlog, rx, ry;
+            uint32_t wq, wr, rx, ry;
             uint64_t nwin;
-            if (mode != FZ_MODE_OUTLIER && batch_fused3d(g, nt, qlog, rx, ry, nwin)) {
+            if (mode != FZ_MODE_OUTLIER && batch_fused3d(g, wq, wr, rx, ry, nwin)) {
                 // decoded and reconstructed by the fused window kernel: no reconstruct workgroups
-                F.wnt = nt;
-                F.wq = qlog;
+                F.wnt = wq * wr;  // a multiple of 64, at most 1024
+                F.wq = wq;
+                F.wr = wr;
+                F.wm = d3_magic(wq);
                 F.wdims[0] = rx;
                 F.wdims[1] = ry;
                 F.rdims[0] = F.rdims[1] = F.rdims[2] = 0;
-                fused[nt == 256 ? 0 : nt == 512 ? 1 : 2].push_back({k, nwin});
+                fused[F.wnt / 64 - 1].push_back({k, nwin});
             }
         }
         const int nd = g.ndim;
@@ -1063,7 +1076,7 @@ fz_status fz_batch_plan_create(fz_batch_plan* plan, const fz_field_shape* shapes
     p->ustart[nfields] = (uint32_t)u;
     p->rstart[nfields] = (uint32_t)r;
     uint64_t w = 0;  // the fused 3D windows, by window size
-    for (int c = 0; c < 3; c++) {
+    for (int c = 0; c < kWinClasses; c++) {
         b.w_lo[c] = (uint32_t)w;
         for (const auto& fw : fused[c]) {
             p->fields[fw.first].wstart = (uint32_t)w;

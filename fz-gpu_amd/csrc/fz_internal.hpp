@@ -96,8 +96,9 @@ hipError_t launch_decode_recon_1d(const uint8_t* payload, const uint32_t* tile_e
 // 2D fields: decode + reconstruct to floats in one launch (a workgroup per 16 x 1024 window),
 // for fields whose windows are few enough that it beats decode + reconstruct
 bool       decode_recon_2d_pays(const Geom& g);
-// ... and small enough that each window finds its tiles' offsets from the bitflags itself
-// (launch_decode_recon_2d with tile_excl = chunk_sums = nullptr, no tile-offset launch)
+// 2D fields small enough (T <= 4096, X <= 4096) that each window finds its tiles' offsets from
+// the bitflags itself (launch_decode_recon_2d with tile_excl = chunk_sums = nullptr, no
+// tile-offset launch)
 bool       decode_recon_2d_direct(const Geom& g);
 hipError_t launch_decode_recon_2d(const uint8_t* payload, const uint32_t* tile_excl, const uint32_t* chunk_sums,
                                   const uint16_t* tail, const Geom& g, float ebx2, float* out, hipStream_t s);
@@ -108,8 +109,10 @@ hipError_t launch_decode_recon_3d(const uint8_t* payload, const uint32_t* tile_e
                                   const uint16_t* tail, const Geom& g, float ebx2, float* out, hipStream_t s);
 hipError_t launch_reconstruct(const uint16_t* codes, float* out, const Geom& g, float ebx2, hipStream_t s);
 // Batched decompress: whether a 3D field is decoded and reconstructed by the fused window kernel
-// (then its windows' threads, log2 quads per row, windows along x and y, and window count)
-bool       batch_fused3d(const Geom& g, uint32_t& nt, uint32_t& qlog, uint32_t& rx, uint32_t& ry, uint64_t& nwin);
+// (then its windows' quads per row, rows, windows along x and y, and window count)
+// floor(t / Q) = umulhi(t, d3_magic(Q)) for t < 2^32 / Q (thread indices of a window)
+inline uint32_t d3_magic(uint32_t Q) { return 0xFFFFFFFFu / Q + 1; }
+bool       batch_fused3d(const Geom& g, uint32_t& Q, uint32_t& R, uint32_t& rx, uint32_t& ry, uint64_t& nwin);
 struct BatchDev;
 struct BatchShape;
 hipError_t launch_batch_fused3d(const BatchDev& d, const BatchShape& b, hipStream_t s);
@@ -152,6 +155,7 @@ hipError_t launch_verify(const float* x, const float* o, uint64_t n, void* temp,
 constexpr uint32_t kBatchMaxFields = 4096;  // two wave_upper rounds
 constexpr int      kPtrChunk = 56;          // pointer records per k_batch_set launch (1792 B of arguments)
 constexpr int      kNumChunks = 3;          // range-partial sizes the workspace holds tables for
+constexpr int      kWinClasses = 16;        // fused 3D decompress window sizes: 64 (c + 1) threads
 constexpr uint32_t kRangeChunks[kNumChunks] = {16384, 65536, 262144};
 
 struct BField {        // per field, uploaded once per workspace (128 B)
@@ -161,10 +165,12 @@ struct BField {        // per field, uploaded once per workspace (128 B)
     uint32_t rdims[3]; // reconstruct workgroups along x, y, z
     uint32_t rq;       // 3D: quads per row of its reconstruct workgroups (recon3d_q)
     uint32_t wnt;      // 3D decoded and reconstructed in one kernel: threads per window (0: not)
-    uint32_t wq;       // ... log2 of quads per window row
+    uint32_t wq;       // ... quads per window row
+    uint32_t wr;       // ... rows per window
     uint32_t wdims[2]; // ... windows along x, y
     uint32_t wstart;   // ... first window (batch window space)
-    uint32_t pad[3];
+    uint32_t wm;       // ... d3_magic(wq)
+    uint32_t pad;
 };
 static_assert(sizeof(BField) == 128, "BField layout");
 
@@ -217,7 +223,7 @@ struct BatchShape {
     uint32_t xcd[4];            // tile order per dimensionality: 1 XCD-sliced, 0 linear
     uint32_t wave_tile[4];      // tile layout per dimensionality: 1 one wave per tile, 0 four
     uint32_t gather_waves;      // units per gather workgroup
-    uint32_t w_lo[3], w_hi[3];  // fused 3D windows of 256 / 512 / 1024 threads (decompress)
+    uint32_t w_lo[kWinClasses], w_hi[kWinClasses];  // fused 3D windows of 64 (c + 1) threads (decompress)
 };
 
 // A group of consecutive fields (sorted order) and the units / range partials they own.

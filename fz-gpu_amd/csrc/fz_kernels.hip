@@ -1017,10 +1017,7 @@ __global__ __launch_bounds__(kD2Threads) void k_decode_recon_2d(const uint8_t* _
 // workgroups per CU): up to 26000 x 388 (650 windows), not at 39200 x 388 (975).
 constexpr uint64_t kD2MaxWindows = 768;
 
-bool decode_recon_2d_direct(const Geom& g)
-{
-    return decode_recon_2d_pays(g) && g.T <= kD2DirectMaxT && g.X <= 4096;
-}
+bool decode_recon_2d_direct(const Geom& g) { return g.T <= kD2DirectMaxT && g.X <= 4096; }
 
 bool decode_recon_2d_pays(const Geom& g)
 {
@@ -1038,13 +1035,17 @@ hipError_t launch_decode_recon_2d(const uint8_t* payload, const uint32_t* tile_e
     return hipGetLastError();
 }
 
-// 3D fields: decode and reconstruct in one NT-thread workgroup per window of 8 planes x R rows
-// x 4Q columns (R Q = NT: one Lorenzo block layer in z, R / 8 block rows in y), straight to
-// floats; the codes never reach HBM.
+// 3D fields: decode and reconstruct in one workgroup of R Q threads per window of 8 planes x R
+// rows x 4Q columns (one Lorenzo block layer in z, R / 8 block rows in y), straight to floats;
+// the codes never reach HBM.
 //
-// Shape.  Q is the field's: the fewest quads per row that hold a whole row (4 Q >= X), at most
-// NT / 8; NT is 256 for X <= 128, 512 for X <= 256, else 1024 (then Q = 128, and rows wider
-// than 512 take several windows).
+// Shape.  X <= 128: Q is the fewest quads per row that hold a whole row (4 Q >= X, a power of
+// two <= 32) and R = 256 / Q.  Wider rows: R = 8, and a row takes the fewest windows of at most
+// 512 columns, Q the quads of its share rounded up to a multiple of 8 (whole waves); so a
+// window holds within 32 columns of what it covers of the field (d3_shape: a single field with
+// X <= 256 takes Q = 64).  The template NT (256, 512 or
+// 1024) bounds R Q, and sizes the QG list and the staging buffers; the window's memory is
+// dynamic (64 R Q bytes).
 //
 // Decode, as in k_decode_recon_2d.  The window's row segments list the quad groups they
 // touch, each once: with whole rows one segment per plane (a plane's rows are consecutive in
@@ -1059,33 +1060,37 @@ hipError_t launch_decode_recon_2d(const uint8_t* payload, const uint32_t* tile_e
 // and stores rows 8 yb + 4 yh .. + 3.
 constexpr uint32_t d3_maxq(uint32_t nt) { return nt == 1024 ? 192u : 8u * (nt / 64 + 1); }
 
-// Window wi of a 3D field (all NT threads of the workgroup): cpre is the chunk prefix in LDS
-// (chunk-local tile offsets) or nullptr (tile_excl holds global offsets).
+// Window wi of a 3D field (all R Q threads of the workgroup): cpre is the chunk prefix in LDS
+// (chunk-local tile offsets) or nullptr (tile_excl holds global offsets); mq = d3_magic(Q).
 template <int NT>
 __device__ __forceinline__ void d3_window(const uint8_t* __restrict__ payload, const uint32_t* __restrict__ tile_excl,
                                           const uint32_t* cpre, const uint16_t* __restrict__ tail, const Geom& g,
-                                          float ebx2, float* __restrict__ out, uint32_t qlog, uint32_t rx, uint32_t ry,
-                                          uint32_t wi)
+                                          float ebx2, float* __restrict__ out, uint32_t Q, uint32_t R, uint32_t mq,
+                                          uint32_t rx, uint32_t ry, uint32_t wi)
 {
     static_assert(NT == 256 || NT == 512 || NT == 1024, "threads per window");
     constexpr uint32_t NH = NT / 32;                      // half-waves
     constexpr uint32_t MAXQ = d3_maxq(NT);
-    // QGs per half-wave in flight together: half of them (two rounds), which keeps the kernel at
-    // 64 VGPRs without scratch
+    // QGs per half-wave in flight together: half of them at NT threads (two rounds), which keeps
+    // the kernel at 64 VGPRs without scratch
     constexpr uint32_t NB = (MAXQ + 2 * NH - 1) / (2 * NH);
     // 512 and 1024 threads: the two half-waves of a wave stage their QGs in turn through one
-    // buffer, which keeps the LDS at 38.8 / 75.8 KB (4 / 2 workgroups per CU) instead of 43.3 /
-    // 85.8 KB (3 / 1)
+    // buffer (LDS 38.8 / 75.8 KB at NT threads, 4 / 2 workgroups per CU, instead of 43.3 / 85.8 KB)
     constexpr bool     SERIAL = NT >= 512;
-    __shared__ __align__(16) uint16_t win[32 * NT];       // codes [plane][row][column]; later int4 [4][R][Q]
+    extern __shared__ __align__(16) uint16_t win[];       // codes [plane][row][column]; later int4 [4][R][Q]
     __shared__ uint32_t stage[SERIAL ? NT / 64 : NH][32 * kQgPitch];  // bit-plane rows of a QG
     __shared__ uint32_t qtab[MAXQ], nq_s;                 // the QGs to decode
-    const uint32_t Q = 1u << qlog, R = (uint32_t)NT >> qlog, C = 4 * Q;
+    const uint32_t C = 4 * Q, nh = (R * Q) >> 5;
     // the lane indices are recomputed in every window from an opaque copy of the thread index:
     // values hoisted out of the loop would otherwise stay live across it (about 40 more VGPRs)
     uint32_t tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
     const uint32_t lane = tid & 63, wv = tid >> 6;
+    // lane exchanges by the opaque lane index (the builtins' own lane index would stay live
+    // across the window loop)
+    auto bperm = [](uint32_t src, uint32_t v) -> uint32_t {
+        return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v);
+    };
     const uint32_t bx = wi % rx, t1 = wi / rx, by = t1 % ry, bz = t1 / ry;
     const uint32_t xw0 = bx * C, y0 = by * R, z0 = bz * 8;
     const uint32_t cw = min(C, g.X - xw0), rows = min(R, g.Y - y0), nz = min(8u, g.Z - z0);
@@ -1106,10 +1111,15 @@ __device__ __forceinline__ void d3_window(const uint8_t* __restrict__ payload, c
             }
         }
         const uint32_t a = p0 >> 8, b = (p0 + len - 1) >> 8;
-        const uint32_t bprev = __shfl_up(b, 1, 64);
+        const uint32_t bprev = bperm(lane - 1, b);
         const uint32_t start = lane > 0 ? max(a, bprev + 1) : a;
         const uint32_t c = (valid && b + 1 > start) ? b + 1 - start : 0u;
-        const uint32_t incl = wave_incl_scan(c);
+        uint32_t       incl = c;
+#pragma unroll
+        for (uint32_t d = 1; d < 64; d <<= 1) {
+            const uint32_t o = bperm(lane - d, incl);
+            if (lane >= d) incl += o;
+        }
         for (uint32_t j = 0; j < c; j++) qtab[incl - c + j] = start + j;
         if (lane == 63) nq_s = incl;
     }
@@ -1126,12 +1136,12 @@ __device__ __forceinline__ void d3_window(const uint8_t* __restrict__ payload, c
         return (dz < nz && dy < rows && dx < cw) ? (int)((dz * R + dy) * C + dx) : -1;
     };
     // both half-waves of a wave step together (the transpose needs all 64 lanes active)
-    for (uint32_t i0 = 0; i0 < nq; i0 += NB * NH) {
+    for (uint32_t i0 = 0; i0 < nq; i0 += NB * nh) {
         uint32_t q[NB], fw[NB], ex[NB];
         bool     live[NB];
 #pragma unroll
         for (int b = 0; b < (int)NB; b++) {
-            const uint32_t i = i0 + H + b * NH;
+            const uint32_t i = i0 + H + b * nh;
             live[b] = i < nq;
             q[b] = live[b] ? qtab[i] : 0u;
             const uint32_t t = q[b] >> 3;
@@ -1149,11 +1159,11 @@ __device__ __forceinline__ void d3_window(const uint8_t* __restrict__ payload, c
             uint32_t       incl = pc;
 #pragma unroll
             for (int d = 1; d < 8; d <<= 1) {
-                const uint32_t o = __shfl_up(incl, d, 8);
+                const uint32_t o = bperm(lane - d, incl);
                 if ((hl & 7u) >= (uint32_t)d) incl += o;
             }
             const uint32_t src = (lane & 32u) + (hl >> 2);
-            const uint32_t wj = __shfl(fw[b], src, 64), ej = __shfl(incl - pc, src, 64);
+            const uint32_t wj = bperm(src, fw[b]), ej = bperm(src, incl - pc);
             const uint32_t bit = 8 * (hl & 3u) + (q[b] & 7u);
             const bool     f = (wj >> bit) & 1u;
             nzq[b] = ballot64(f) != 0;
@@ -1239,7 +1249,7 @@ __device__ __forceinline__ void d3_window(const uint8_t* __restrict__ payload, c
     }
     __syncthreads();
     // reconstruct, phase 1: columns xc .. xc + 3 of row yy, the 8 planes (z prefix in registers)
-    const uint32_t xq = tid & (Q - 1), yy = tid >> qlog;
+    const uint32_t yy = __umulhi(tid, mq), xq = tid - yy * Q;
     const uint32_t xc = xw0 + 4 * xq;
     const bool     rowok = yy < rows;
     int32_t        r[8][4];
@@ -1269,7 +1279,7 @@ __device__ __forceinline__ void d3_window(const uint8_t* __restrict__ payload, c
         }
     }
     // phase 2: the prefix over y, four planes at a time through the window's memory
-    int4* const    ex4 = reinterpret_cast<int4*>(win);  // [4][R][Q]: 64 NT bytes, the window's size
+    int4* const    ex4 = reinterpret_cast<int4*>(win);  // [4][R][Q]: 64 R Q bytes, the window's size
     const uint32_t tz = yy & 3u, yh = (yy >> 2) & 1u, yb = yy >> 3;
     const bool     full = xc + 4 <= g.X;
 #pragma unroll
@@ -1319,13 +1329,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 
                                                         const uint32_t* __restrict__ tile_excl,
                                                         const uint32_t* __restrict__ chunk_sums,
                                                         const uint16_t* __restrict__ tail, Geom g, float ebx2,
-                                                        float* __restrict__ out, uint32_t qlog, uint32_t rx,
-                                                        uint32_t ry, uint32_t nwin)
+                                                        float* __restrict__ out, uint32_t Q, uint32_t R,
+                                                        uint32_t mq, uint32_t rx, uint32_t ry, uint32_t nwin)
 {
     __shared__ uint32_t cpre[kTileLocalMaxChunks];  // block counts of the chunks before each chunk
     __shared__ uint32_t wtot[4];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    // the chunk prefix, once per workgroup (waves 0-3)
+    // the chunk prefix, once per workgroup (waves 0-3; R Q >= 256)
     {
         uint32_t cv = 0, cincl = 0;
         if (tid < kTileLocalMaxChunks) {
@@ -1345,7 +1355,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 
     // 0 keeps a workgroup on one XCD's windows, and windows that share a QG (neighbours along x,
     // or along y / z when rows or planes are whole) read its blocks through one L2.
     for (uint32_t it = blockIdx.x; it < nwin; it += gridDim.x)
-        d3_window<NT>(payload, tile_excl, chunk_sums ? cpre : nullptr, tail, g, ebx2, out, qlog, rx, ry,
+        d3_window<NT>(payload, tile_excl, chunk_sums ? cpre : nullptr, tail, g, ebx2, out, Q, R, mq, rx, ry,
                       xcd_tile(it, nwin));
 }
 
@@ -1360,65 +1370,98 @@ static uint32_t device_cus()
     return (uint32_t)n;
 }
 
-// threads per window and log2 of quads per row of the fused 3D decompress for a field
-static void d3_shape(const Geom& g, uint32_t& nt, uint32_t& qlog)
+// tight = false (single field): 128 < X <= 256 takes 256-column windows of 512 threads, whose
+// decode has more half-waves per QG list (one round where a tight window of 320-448 threads
+// takes two); a lone field's windows mostly fit one round of the device, so their latency is
+// its time (130^3: 19.4 against 21.0 us).  Batches take tight windows (more fields per round).
+static void d3_shape(const Geom& g, uint32_t& Q, uint32_t& R, bool tight)
 {
-    if (g.X > 256) {
-        nt = 1024;
-        qlog = 7;
-    } else if (g.X > 128) {
-        nt = 512;
-        qlog = 6;
+    if (g.X <= 128) {
+        Q = 4;
+        while (4 * Q < g.X) Q *= 2;  // 4 Q >= X, Q <= 32
+        R = 256 / Q;
+    } else if (g.X <= 256 && !tight) {
+        Q = 64;
+        R = 8;
     } else {
-        nt = 256;
-        qlog = 2;
-        while ((4u << qlog) < g.X) qlog++;  // 4 Q >= X, Q <= 32
+        const uint32_t nwx = (g.X + 511) / 512, w = (g.X + nwx - 1) / nwx;  // windows per row, columns of each
+        Q = ((w + 3) / 4 + 7) & ~7u;                                         // <= 128
+        R = 8;
     }
 }
 
+// the kernel template for windows of n = R Q threads
+static int d3_nt(uint32_t n) { return n <= 256 ? 256 : n <= 512 ? 512 : 1024; }
+
+// windows of the fused 3D decompress the device holds at once, a multiple of 8: per CU the
+// fewer of what its LDS (160 KB) and its wave slots (4 SIMDs x 8 waves at 64 VGPRs, 7 for the
+// 256-thread kernel) allow
+static uint32_t d3_slots(uint32_t Q, uint32_t R)
+{
+    static uint32_t lds_static[3] = {0, 0, 0};  // the kernels' own LDS bytes
+    const uint32_t  n = Q * R;
+    const int       nt = d3_nt(n), c = nt == 256 ? 0 : nt == 512 ? 1 : 2;
+    if (!lds_static[c]) {
+        hipFuncAttributes a{};
+        const void* f = nt == 256 ? reinterpret_cast<const void*>(&k_decode_recon_3d<256>)
+                        : nt == 512 ? reinterpret_cast<const void*>(&k_decode_recon_3d<512>)
+                                    : reinterpret_cast<const void*>(&k_decode_recon_3d<1024>);
+        if (hipFuncGetAttributes(&a, f) == hipSuccess && a.sharedSizeBytes > 0) lds_static[c] = (uint32_t)a.sharedSizeBytes;
+        else {
+            (void)hipGetLastError();
+            lds_static[c] = 12288;
+        }
+    }
+    const uint32_t by_lds = 163840u / (lds_static[c] + 64 * n);
+    const uint32_t by_waves = 4 * (nt == 256 ? 7u : 8u) / (n / 64);
+    const uint32_t per_cu = by_lds < by_waves ? by_lds : by_waves;
+    return device_cus() * (per_cu ? per_cu : 1u) & ~7u;
+}
+
 // Measured against decode + reconstruct (back-to-back calls, waves fields at eb 1e-3): the fused
-// form wins on every field with X <= 256 (128 x 128 x 512: 19.4 against 29.3 us, 256^3: 28.0
-// against 37.8 us).  Wider fields take 1024-thread windows, two per CU: the fused form wins
-// when the windows fit one round (480 x 300 x 100: 29.3 / 36.5 us, 400 x 400 x 64: 27.3 / 34.0)
-// or take three or more (2000 x 300 x 100: 101.8 / 125.6 us, 1024 x 1024 x 64: 97.4 / 132.2),
-// is even or slower in between, where a second round holds only part of the device and its
-// windows start in step (500 x 500 x 100, 819 windows: 52.7 / 51.0 us; 1000 x 1000 x 20: 41.7 /
-// 44.2), and loses where a 512-column window holds much less than 512 columns of the field
-// (300^3: 58.9 / 50.3 us, 700 x 500 x 60: 53.9 / 45.4 us).
+// form wins on every field with X <= 256 (128 x 128 x 512: 19.4 against 28.2 us, 256^3: 29.1
+// against 38.3 us).  Wider fields win when their windows fill one round of the device with
+// at least 1.5 windows per CU (270 x 300 x 100, 494 windows at 768 slots: 29.4 / 41.3 us;
+// 480 x 300 x 100: 29.6 / 36.8; 400 x 400 x 64: 28.3 / 34.2) or take three rounds or more
+// (2000 x 300 x 100: 104.7 / 122.8 us, 1024 x 1024 x 64: 98.7 / 130.1), and lose with fewer
+// windows, whose latency is then the kernel's time (354 x 200 x 90, 300 windows: 30.6 / 27.7
+// us), and in between, where a second round holds only part of the device and its windows
+// start in step (300^3, 1444 windows at 768 slots: 61.5 / 51.3 us; 500 x 500 x 100: 53.8 /
+// 50.5; 1000 x 1000 x 20: 42.6 / 44.0).
 bool decode_recon_3d_pays(const Geom& g)
 {
     if (g.ndim != 3) return false;
     if (g.X <= 256) return true;
-    uint32_t nt, qlog;
-    d3_shape(g, nt, qlog);
-    const uint64_t C = 4u << qlog, R = nt >> qlog, rx = (g.X + C - 1) / C;
-    if (4 * (uint64_t)g.X < 3 * rx * C) return false;  // the field's columns fill < 3/4 of the windows' width
-    const uint64_t nwin = rx * ((g.Y + R - 1) / R) * ((g.Z + 7) / 8), slots = 2ull * device_cus();
-    return nwin <= slots || nwin >= 3 * slots;
+    uint32_t Q, R;
+    d3_shape(g, Q, R, false);
+    const uint64_t C = 4 * Q, rx = (g.X + C - 1) / C;
+    const uint64_t nwin = rx * ((g.Y + R - 1) / R) * ((g.Z + 7) / 8), slots = d3_slots(Q, R);
+    return (2 * nwin >= 3ull * device_cus() && nwin <= slots) || nwin >= 3 * slots;
 }
 
 hipError_t launch_decode_recon_3d(const uint8_t* payload, const uint32_t* tile_excl, const uint32_t* chunk_sums,
                                   const uint16_t* tail, const Geom& g, float ebx2, float* out, hipStream_t s)
 {
-    uint32_t nt, qlog;
-    d3_shape(g, nt, qlog);
-    const uint32_t C = 4u << qlog, R = nt >> qlog;
+    uint32_t Q, R;
+    d3_shape(g, Q, R, false);
+    const uint32_t C = 4 * Q, n_t = Q * R;
     const uint32_t rx = (g.X + C - 1) / C, ry = (g.Y + R - 1) / R;
     const uint64_t n = (uint64_t)rx * ry * ((g.Z + 7) / 8);
     if (n > 0xFFFFFFFFull) return hipErrorInvalidConfiguration;
-    // at most the workgroups the device holds at once (2 / 4 / 7 per CU by LDS), a multiple of 8
-    const uint32_t per_cu = nt == 1024 ? 2u : nt == 512 ? 4u : 7u;
-    const uint32_t slots = device_cus() * per_cu & ~7u;
+    const uint32_t slots = d3_slots(Q, R);
     const dim3     grid(n <= slots ? (uint32_t)n : slots);
-    if (nt == 1024) FZ_LAUNCH(k_decode_recon_3d<1024><<<grid, 1024, 0, s>>>(payload, tile_excl, chunk_sums, tail, g, ebx2, out, qlog, rx, ry, (uint32_t)n));
-    else if (nt == 512) FZ_LAUNCH(k_decode_recon_3d<512><<<grid, 512, 0, s>>>(payload, tile_excl, chunk_sums, tail, g, ebx2, out, qlog, rx, ry, (uint32_t)n));
-    else FZ_LAUNCH(k_decode_recon_3d<256><<<grid, 256, 0, s>>>(payload, tile_excl, chunk_sums, tail, g, ebx2, out, qlog, rx, ry, (uint32_t)n));
+    const size_t   lds = 64ull * n_t;  // the window
+    const int      nt = d3_nt(n_t);
+    if (nt == 1024) FZ_LAUNCH(k_decode_recon_3d<1024><<<grid, n_t, lds, s>>>(payload, tile_excl, chunk_sums, tail, g, ebx2, out, Q, R, d3_magic(Q), rx, ry, (uint32_t)n));
+    else if (nt == 512) FZ_LAUNCH(k_decode_recon_3d<512><<<grid, n_t, lds, s>>>(payload, tile_excl, chunk_sums, tail, g, ebx2, out, Q, R, d3_magic(Q), rx, ry, (uint32_t)n));
+    else FZ_LAUNCH(k_decode_recon_3d<256><<<grid, n_t, lds, s>>>(payload, tile_excl, chunk_sums, tail, g, ebx2, out, Q, R, d3_magic(Q), rx, ry, (uint32_t)n));
     return hipGetLastError();
 }
 
-// Batched decompress: the fused 3D windows of every field of one window size, a workgroup per
-// window; the field comes from the per-window table (one scalar load), its tile offsets are the
-// per-field scan's (global within the field, no chunk prefix).  Windows in XCD-sliced order.
+// Batched decompress: the fused 3D windows of every field of one window size (R Q threads), a
+// workgroup per window; the field comes from the per-window table (one scalar load), its tile
+// offsets are the per-field scan's (global within the field, no chunk prefix).  Windows in
+// XCD-sliced order.
 template <int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 8 : 7))) void k_batch_fused3d(BatchDev d, uint32_t w_lo)
 {
@@ -1428,30 +1471,33 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 
     const Geom     g = F.g;
     const BPtrs    q = d.ptrs[f];
     d3_window<NT>(static_cast<const uint8_t*>(q.src), d.texcl + F.ustart, nullptr, g.mode == 0 ? q.tail : nullptr, g,
-                  (float)q.param, static_cast<float*>(q.dst), F.wq, F.wdims[0], F.wdims[1], w - F.wstart);
+                  (float)q.param, static_cast<float*>(q.dst), F.wq, F.wr, F.wm, F.wdims[0], F.wdims[1], w - F.wstart);
 }
 
-// In a batch the windows of many fields fill the device together, so the fused form is taken
-// wherever its windows hold most of their columns (X <= 256, or 3/4 of 512-column windows).
-bool batch_fused3d(const Geom& g, uint32_t& nt, uint32_t& qlog, uint32_t& rx, uint32_t& ry, uint64_t& nwin)
+// In a batch the windows of many fields fill the device together, so every 3D field takes the
+// fused form, in tight windows (within 32 columns of what they cover): the 256-field batch's
+// 3D decode + reconstruct 3.25 -> 3.19 ms against fused windows for X <= 256 or 3/4-full
+// power-of-two windows only.
+bool batch_fused3d(const Geom& g, uint32_t& Q, uint32_t& R, uint32_t& rx, uint32_t& ry, uint64_t& nwin)
 {
     if (g.ndim != 3) return false;
-    d3_shape(g, nt, qlog);
-    const uint64_t C = 4u << qlog, R = nt >> qlog;
-    rx = (uint32_t)((g.X + C - 1) / C);
-    ry = (uint32_t)((g.Y + R - 1) / R);
+    d3_shape(g, Q, R, true);
+    rx = (g.X + 4 * Q - 1) / (4 * Q);
+    ry = (g.Y + R - 1) / R;
     nwin = (uint64_t)rx * ry * ((g.Z + 7) / 8);
-    return g.X <= 256 || 4 * (uint64_t)g.X >= 3 * rx * C;
+    return true;
 }
 
 hipError_t launch_batch_fused3d(const BatchDev& d, const BatchShape& b, hipStream_t s)
 {
-    for (int c = 0; c < 3; c++) {
-        const uint32_t n = b.w_hi[c] - b.w_lo[c];
+    for (int c = 0; c < kWinClasses; c++) {
+        const uint32_t n = b.w_hi[c] - b.w_lo[c], n_t = 64u * (c + 1);
         if (n == 0) continue;
-        if (c == 0) FZ_LAUNCH(k_batch_fused3d<256><<<n, 256, 0, s>>>(d, b.w_lo[c]));
-        else if (c == 1) FZ_LAUNCH(k_batch_fused3d<512><<<n, 512, 0, s>>>(d, b.w_lo[c]));
-        else FZ_LAUNCH(k_batch_fused3d<1024><<<n, 1024, 0, s>>>(d, b.w_lo[c]));
+        const size_t lds = 64ull * n_t;
+        const int    nt = d3_nt(n_t);
+        if (nt == 256) FZ_LAUNCH(k_batch_fused3d<256><<<n, n_t, lds, s>>>(d, b.w_lo[c]));
+        else if (nt == 512) FZ_LAUNCH(k_batch_fused3d<512><<<n, n_t, lds, s>>>(d, b.w_lo[c]));
+        else FZ_LAUNCH(k_batch_fused3d<1024><<<n, n_t, lds, s>>>(d, b.w_lo[c]));
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

@@ -25,6 +25,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libfzgpu.so")
 MODE_FIXED = 0
 MODE_LITERAL = 1
 MODE_OUTLIER = 2
+INVERSE_AUTO, INVERSE_WINDOWS, INVERSE_SPLIT = 0, 1, 2
 
 STATUS = {0: "success", 1: "invalid argument", 2: "workspace too small", 3: "HIP runtime error",
           4: "constant field", 5: "non-finite value", 6: "corrupt payload", 7: "capacity exceeded",
@@ -64,7 +65,7 @@ EXPORTS = ["fz_version", "fz_status_string", "fz_last_error", "fz_plan_create", 
            "fz_workspace_init", "fz_compress_f32", "fz_compress_plain_f32", "fz_decompress_f32",
            "fz_compress_f32_sync", "fz_lorenzo_construct_f32", "fz_lorenzo_reconstruct_f32",
            "fz_bitshuffle_bitflag", "fz_bitunshuffle", "fz_exclusive_sum_u32", "fz_encode_blocks",
-           "fz_plan_set_tile_layout", "fz_plan_profile", "fz_plan_kernel_times",
+           "fz_plan_set_tile_layout", "fz_plan_set_inverse_form", "fz_plan_profile", "fz_plan_kernel_times",
            "fz_batch_plan_create", "fz_batch_plan_destroy", "fz_batch_plan_sizes", "fz_batch_field_sizes",
            "fz_batch_get_tuning", "fz_batch_set_tuning", "fz_batch_workspace_init", "fz_batch_compress_f32",
            "fz_batch_decompress_f32", "fz_batch_autotune", "fz_batch_profile", "fz_batch_kernel_times",
@@ -169,6 +170,7 @@ def lib():
     L.fz_encode_blocks.argtypes = [vp, vp, vp, u64, vp]
     L.fz_plan_profile.argtypes = [vp, i32]
     L.fz_plan_set_tile_layout.argtypes = [vp, i32]
+    L.fz_plan_set_inverse_form.argtypes = [vp, i32]
     L.fz_plan_kernel_times.argtypes = [vp, ctypes.POINTER(f32), i32, ctypes.POINTER(i32)]
     L.fz_batch_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(FieldShape), u32, i32]
     L.fz_batch_plan_destroy.argtypes = [vp]
@@ -287,6 +289,12 @@ class Plan:
     def set_tile_layout(self, wave_tile):
         """Tile-kernel layout of the fused compress: 1 = one wave per tile, 0 = one workgroup."""
         _check(lib().fz_plan_set_tile_layout(self.handle, int(wave_tile)), "fz_plan_set_tile_layout")
+
+    def set_inverse_form(self, form):
+        """Kernel form of decompress for 2D / 3D fields (same floats either way): INVERSE_AUTO,
+        INVERSE_WINDOWS (decode into LDS, reconstruct straight to floats) or INVERSE_SPLIT
+        (decode to codes, then reconstruct)."""
+        _check(lib().fz_plan_set_inverse_form(self.handle, int(form)), "fz_plan_set_inverse_form")
 
     def profile(self, enable=True):
         _check(lib().fz_plan_profile(self.handle, 1 if enable else 0), "fz_plan_profile")

@@ -192,6 +192,16 @@ fz_status fz_compress_f32_sync(const fz_plan plan, const float* d_in, double eb_
  * per tile, four tiles per workgroup, no workgroup barrier inside a tile. */
 fz_status fz_plan_set_tile_layout(fz_plan plan, int wave_tile);
 
+/* Kernel form of fz_decompress_f32 for 2D and 3D fields (no counterpart in the reference; the
+ * reference has no inverse); every form gives the same floats.  FZ_INVERSE_AUTO (the default):
+ * the form measured faster for the field's shape.  FZ_INVERSE_WINDOWS: windows of the field
+ * decode into LDS and reconstruct straight to floats (one kernel after the tile offsets, none
+ * before it for 2D fields of at most 4096 tiles and 4096 columns).  FZ_INVERSE_SPLIT: tile
+ * offsets, decode to u16 codes in the workspace, reconstruct.  1D fields always decode and
+ * reconstruct in one kernel. */
+typedef enum fz_inverse_form { FZ_INVERSE_AUTO = 0, FZ_INVERSE_WINDOWS = 1, FZ_INVERSE_SPLIT = 2 } fz_inverse_form;
+fz_status fz_plan_set_inverse_form(fz_plan plan, int form);
+
 /* Per-kernel timing (the reference's cuda_timer_t around its Lorenzo launch, timer.hh:45-85,
  * extended to every kernel): when enabled, the compress/decompress calls on this plan record
  * an event before their first kernel and after each kernel; fz_plan_kernel_times waits for the

@@ -351,7 +351,7 @@ def test_decompress_at_the_fused_2d_switch(dims):
     """2D fields decompress in one fused launch when X % 4 == 0 (wide LDS stores: 8-byte at
     X = 1020, 16-byte at X = 3000) and otherwise up to 768 windows of 16 x 1024, larger ones by
     decode + reconstruct: X = 1022 at 768 windows (fused) and 769 (two kernels, one row past a
-    window row), bit-exact against the oracle."""
+    window row), bit-exact against the oracle; so are the two forms forced either way."""
     n = dims[0] * dims[1]
     say(f"fused 2D switch dims={dims}")
     x = D.generate("waves", dims, 41 + dims[1])
@@ -360,9 +360,12 @@ def test_decompress_at_the_fused_2d_switch(dims):
     plan = fzgpu.Plan(*dims)
     payload, tail, r = plan.compress(_to_dev(x.ravel()), 1e-3)
     assert np.array_equal(_np(payload), ref["payload"])
-    xr = _np(plan.decompress(payload, tail, r["ebx2"]))
     rc, xo = O.decompress(ref["payload"], ref["tail"], n, dims, ref["info"]["ebx2"], nthreads=16)
-    assert rc == 0 and np.array_equal(xr.view(np.uint32), xo.view(np.uint32))
+    assert rc == 0
+    for form in FORMS:
+        plan.set_inverse_form(form)
+        xr = _np(plan.decompress(payload, tail, r["ebx2"]))
+        assert np.array_equal(xr.view(np.uint32), xo.view(np.uint32)), form
 
 
 # ---------------------------------------------------------------- the decompress tile-scan switch
@@ -398,26 +401,30 @@ def test_decompress_at_the_tile_scan_switch(dims, length):
 
 D3_SHAPES = [(16, 70, 9), (12, 40, 17), (17, 70, 9), (100, 30, 20), (129, 20, 9), (200, 24, 10), (257, 8, 8),
              (400, 9, 9), (500, 11, 9), (512, 9, 9), (1000, 7, 17), (2000, 3, 9), (1, 1, 5000), (4, 3, 700),
-             (300, 41, 3), (700, 41, 3)]
+             (300, 41, 3), (700, 41, 3), (354, 9, 10), (1030, 9, 9), (1500, 17, 9), (270, 300, 100)]
+FORMS = [fzgpu.INVERSE_AUTO, fzgpu.INVERSE_WINDOWS, fzgpu.INVERSE_SPLIT]
 
 
 @pytest.mark.parametrize("dims", D3_SHAPES)
 @pytest.mark.parametrize("mode", [O.MODE_FIXED, O.MODE_LITERAL])
 def test_decompress_fused_3d_window_shapes(dims, mode):
     """3D fields decode and reconstruct in one launch, a workgroup per window of 8 planes x R
-    rows x 4Q columns: 256 threads up to X = 128 (Q = 4 .. 32), 512 up to 256, 1024 beyond
-    (whole rows up to X = 512, several windows per row after), X % 8 == 0 / % 4 == 0 / odd,
-    partial windows at every edge, trailer codes from the tail (FIXED) or zero (LITERAL); fields
-    whose windows would sit mostly empty (X = 300, 700) take decode + reconstruct: bit-exact
-    against the oracle either way."""
+    rows x 4Q columns: Q = 4 .. 32 (R = 256 / Q) up to X = 128, Q = 64 up to 256, beyond it
+    the fewest windows of at most 512 columns per row with Q a multiple of 8 (X = 300: Q = 80,
+    700: two windows of Q = 88, 1030: three of 88, 1500: three of 128), X % 8 == 0 / % 4 == 0
+    / odd, partial windows at every edge, trailer codes from the tail (FIXED) or zero (LITERAL).
+    Each kernel form (the automatic choice, windows, decode + reconstruct) is bit-exact against
+    the oracle."""
     x = _field(dims, sum(dims) + 11 + mode)
     plan, payload, tail, r, ref = _check_against_oracle(x, dims, 1e-3, mode, fused=True)
-    say(f"fused 3D decompress dims={dims} mode={mode}")
-    xr = _np(plan.decompress(payload, tail if mode == O.MODE_FIXED else None, r["ebx2"]))
     rc, xo = O.decompress(ref["payload"], ref["tail"] if mode == O.MODE_FIXED else None, x.size, dims,
                           ref["info"]["ebx2"])
     assert rc == 0
-    assert np.array_equal(xr.view(np.uint32), xo.view(np.uint32))
+    for form in FORMS:
+        say(f"3D decompress dims={dims} mode={mode} form={form}")
+        plan.set_inverse_form(form)
+        xr = _np(plan.decompress(payload, tail if mode == O.MODE_FIXED else None, r["ebx2"]))
+        assert np.array_equal(xr.view(np.uint32), xo.view(np.uint32)), form
 
 
 def test_decompress_fused_3d_beyond_65536_tiles():
@@ -447,7 +454,7 @@ def test_decompress_2d_windows_finding_their_own_tile_offsets(dims):
     """2D fields of at most 4096 tiles and X <= 4096 decompress in one launch, each window
     finding the block offsets of its tiles from the bitflags (T = 4096 at X = 4096 and at
     X = 4093, T = 4098 and X = 4097 past the limits, the 3600 x 1800 field): bit-exact against
-    the oracle."""
+    the oracle, and so is decode + reconstruct forced on the same fields."""
     n = dims[0] * dims[1]
     say(f"direct 2D dims={dims} T={n // 2048}")
     x = D.generate("waves", dims, 17 + dims[1])
@@ -456,6 +463,9 @@ def test_decompress_2d_windows_finding_their_own_tile_offsets(dims):
     plan = fzgpu.Plan(*dims)
     payload, tail, r = plan.compress(_to_dev(x.ravel()), 1e-3)
     assert np.array_equal(_np(payload), ref["payload"])
-    xr = _np(plan.decompress(payload, tail, r["ebx2"]))
     rc, xo = O.decompress(ref["payload"], ref["tail"], n, dims, ref["info"]["ebx2"], nthreads=16)
-    assert rc == 0 and np.array_equal(xr.view(np.uint32), xo.view(np.uint32))
+    assert rc == 0
+    for form in FORMS:
+        plan.set_inverse_form(form)
+        xr = _np(plan.decompress(payload, tail, r["ebx2"]))
+        assert np.array_equal(xr.view(np.uint32), xo.view(np.uint32)), form
