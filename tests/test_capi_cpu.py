@@ -53,6 +53,16 @@ def test_tile_layout_knob_is_validated():
         assert e.value.status == 1
 
 
+def test_inverse_form_knob_is_validated():
+    p = fzgpu.Plan(64, 64, 64, workspace=False)
+    for form in (fzgpu.INVERSE_WINDOWS, fzgpu.INVERSE_SPLIT, fzgpu.INVERSE_AUTO):
+        p.set_inverse_form(form)
+    for bad in (3, -1):
+        with pytest.raises(fzgpu.FzError) as e:
+            p.set_inverse_form(bad)
+        assert e.value.status == 1
+
+
 def test_scan_size_query_needs_no_device():
     n = ctypes.c_size_t(0)
     assert fzgpu.lib().fz_exclusive_sum_u32(None, ctypes.byref(n), None, None, 1 << 20, None) == 0
