@@ -469,3 +469,65 @@ def test_decompress_2d_windows_finding_their_own_tile_offsets(dims):
         plan.set_inverse_form(form)
         xr = _np(plan.decompress(payload, tail, r["ebx2"]))
         assert np.array_equal(xr.view(np.uint32), xo.view(np.uint32)), form
+
+
+# ---------------------------------------------------------------- seeded random sweep
+
+def _random_case(seed):
+    """A seeded random field: dimensionality, shape (2 .. 2M floats, a quarter of the seeds
+    down to 2, the rest from 50K; any residues), content
+    (random walk with zeros, plane waves, a large offset with small noise, mixed signs) and eb
+    spread so that every dual-quant level and path family comes up over the seeds."""
+    rng = np.random.default_rng(1000 + seed)
+    nd = int(rng.integers(1, 4))
+    total = int(np.exp(rng.uniform(np.log(2) if seed % 4 == 0 else np.log(5e4), np.log(2e6))))
+    if nd == 1:
+        dims = (total, 1, 1)
+    elif nd == 2:
+        x = int(np.clip(np.exp(rng.uniform(0, np.log(total))), 1, total))
+        dims = (x, max(2, total // max(x, 1)), 1)
+    else:
+        x = int(np.clip(np.exp(rng.uniform(0, np.log(total) / 1.5)), 1, total))
+        y = int(np.clip(np.exp(rng.uniform(0, np.log(max(total // x, 1)))), 1, total))
+        dims = (x, y, max(2, total // max(x * y, 1)))
+    n = dims[0] * dims[1] * dims[2]
+    kind = int(rng.integers(0, 4))
+    if kind == 0:
+        v = np.cumsum(rng.normal(size=n))
+        v[rng.random(n) < 0.3] = 0
+    elif kind == 1:
+        i = np.arange(n)
+        v = np.sin(i * 0.001) + 0.5 * np.cos(i * 0.037) + 0.01 * rng.normal(size=n)
+    elif kind == 2:
+        v = 1000.0 + 0.05 * rng.normal(size=n)
+    else:
+        v = rng.normal(size=n) * np.where(rng.random(n) < 0.5, 1.0, 100.0)
+    x = v.astype(np.float32)
+    x[0] = 1.5 if kind != 2 else 1000.5  # never a constant field
+    eb = float(10.0 ** rng.uniform(-5, -1.5))
+    mode = O.MODE_FIXED if rng.random() < 0.7 else O.MODE_LITERAL
+    return dims, x, eb, mode
+
+
+@pytest.mark.parametrize("seed", range(64))
+def test_random_fields_every_path_bit_exact(seed):
+    """Seeded random fields, each compressed by the fused path (both tile layouts) and by the
+    plain per-stage pipeline, and decompressed in every inverse form: payload, tail, result
+    record and floats all bit-exact against the oracle."""
+    dims, x, eb, mode = _random_case(seed)
+    say(f"random seed={seed} dims={dims} eb={eb:.3g} mode={mode}")
+    plan, payload, tail, r, ref = _check_against_oracle(x, dims, eb, mode, fused=True)
+    for fused, wave_tile in ((True, 1), (False, 0)):
+        plan.set_tile_layout(wave_tile)
+        p2, t2, r2 = plan.compress(_to_dev(x), eb, fused=fused)
+        assert r2 == r, (fused, wave_tile)
+        assert np.array_equal(_np(p2), ref["payload"]), (fused, wave_tile)
+        if mode == O.MODE_FIXED:
+            assert np.array_equal(_np(t2, np.uint16), ref["tail"]), (fused, wave_tile)
+    rc, xo = O.decompress(ref["payload"], ref["tail"] if mode == O.MODE_FIXED else None, x.size, dims,
+                          ref["info"]["ebx2"])
+    assert rc == 0
+    for form in FORMS:
+        plan.set_inverse_form(form)
+        xr = _np(plan.decompress(payload, tail if mode == O.MODE_FIXED else None, r["ebx2"]))
+        assert np.array_equal(xr.view(np.uint32), xo.view(np.uint32)), form
