@@ -288,3 +288,48 @@ def test_batch_with_the_largest_field():
         del one
     del xs, out, outs, singles, plan
     torch.cuda.empty_cache()
+
+
+def _random_batch(seed, nfields):
+    """Seeded random shapes for a batch: a third each 1D / 2D / 3D, 2 .. 600K floats, any
+    residues; the 3D rows span every fused-window width (X from 1 to 1500)."""
+    rng = np.random.default_rng(4000 + seed)
+    shapes = []
+    for i in range(nfields):
+        nd = 1 + i % 3
+        total = int(np.exp(rng.uniform(0, np.log(6e5))))
+        if nd == 1:
+            shapes.append((max(total, 2), 1, 1))
+        elif nd == 2:
+            x = int(np.exp(rng.uniform(0, np.log(max(total, 2)))))
+            shapes.append((max(x, 1), max(2, total // max(x, 1)), 1))
+        else:
+            x = int(np.exp(rng.uniform(0, np.log(1500))))
+            y = int(np.exp(rng.uniform(0, np.log(max(total // x, 2)))))
+            shapes.append((x, max(y, 1), max(2, total // max(x * y, 1))))
+    return shapes
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_batches_equal_single_field(seed):
+    """Seeded random batches of 48 fields: every field's archive equals the single-field fused
+    compress, and the batched decompress (3D fields in fused windows of every size class)
+    equals the single-field decompress, bit for bit."""
+    shapes = _random_batch(seed, 48)
+    xs = [_field(s, 9000 + 64 * seed + i) for i, s in enumerate(shapes)]
+    ebs = [float(10.0 ** np.random.default_rng(seed * 100 + i).uniform(-4.5, -1.5)) for i in range(len(shapes))]
+    say(f"random batch seed={seed}: {len(shapes)} fields, 3D X {sorted(s[0] for s in shapes if s[2] > 1)}")
+    plan = fzgpu.BatchPlan(shapes)
+    out, res = plan.compress([_to_dev(x) for x in xs], ebs)
+    assert all(r["status"] == 0 for r in res)
+    pays = [out.payloads[i] for i in range(len(shapes))]
+    tails = [out.tails[i][: res[i]["tail_len"]] for i in range(len(shapes))]
+    outs = plan.decompress(pays, tails, [r["ebx2"] for r in res])
+    for i, (s, x) in enumerate(zip(shapes, xs)):
+        single = fzgpu.Plan(*s)
+        payload, tail, r = single.compress(_to_dev(x), ebs[i])
+        assert r == res[i], (i, s)
+        p, t = _field_bytes(out, res, i)
+        assert np.array_equal(p, _np(payload)) and np.array_equal(t, _np(tail, np.uint16)), (i, s)
+        one = single.decompress(payload, tail, r["ebx2"])
+        assert np.array_equal(_np(one).view(np.uint32), _np(outs[i]).view(np.uint32)), (i, s)
