@@ -531,3 +531,37 @@ def test_random_fields_every_path_bit_exact(seed):
         plan.set_inverse_form(form)
         xr = _np(plan.decompress(payload, tail if mode == O.MODE_FIXED else None, r["ebx2"]))
         assert np.array_equal(xr.view(np.uint32), xo.view(np.uint32)), form
+
+
+@pytest.mark.parametrize("form", [fzgpu.INVERSE_WINDOWS, fzgpu.INVERSE_SPLIT])
+def test_3d_decompress_replays_in_a_graph(form):
+    """The 3D decompress (fused windows with their dynamic LDS, or decode + reconstruct) launches
+    kernels only: captured into a graph, every replay (eager work and syncs between replays)
+    gives the eager floats, which equal the oracle's."""
+    import torch
+    dims = (300, 41, 20)
+    x = _field(dims, 1234)
+    plan, payload, tail, r, ref = _check_against_oracle(x, dims, 1e-3, O.MODE_FIXED, fused=True)
+    rc, xo = O.decompress(ref["payload"], ref["tail"], x.size, dims, ref["info"]["ebx2"])
+    assert rc == 0
+    say(f"graph capture of the 3D decompress, form {form}")
+    plan.set_inverse_form(form)
+    out = torch.empty(x.size, dtype=torch.float32, device="cuda")
+    plan.decompress(payload, tail, r["ebx2"], out=out)
+    assert np.array_equal(_np(out).view(np.uint32), xo.view(np.uint32))
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        plan.decompress(payload, tail, r["ebx2"], out=out)
+    torch.cuda.synchronize()
+    other = fzgpu.Plan(64, 64, 64)
+    op, ot, orr = other.compress(_to_dev(_field((64, 64, 64), 99)), 1e-3)
+    for rep in range(3):
+        out.fill_(-3.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(_np(out).view(np.uint32), xo.view(np.uint32)), rep
+        other.decompress(op, ot, orr["ebx2"])  # unrelated eager work between replays
+        torch.cuda.synchronize()
