@@ -928,10 +928,14 @@ __device__ __forceinline__ constexpr uint32_t tmask()
     return J == 16 ? 0x0000FFFFu : J == 8 ? 0x00FF00FFu : J == 4 ? 0x0F0F0F0Fu : J == 2 ? 0x33333333u : 0x55555555u;
 }
 
-// lane-exchange stage: `upper` lanes hold row k+J, the others row k.
+// lane-exchange stage: `upper` lanes hold row k+J, the others row k.  J = 16 and 8 move whole
+// bytes: one byte permute of (mine, other) (upper: [o2 o3 m2 m3] / [o1 m1 o3 m3], lower:
+// [m0 m1 o0 o1] / [m0 o0 m2 o2], byte 0 first); J <= 4 rotates the other row and merges.
 template <int J>
 __device__ __forceinline__ uint32_t xstage(uint32_t mine, uint32_t other, bool upper)
 {
+    if constexpr (J == 16) return __builtin_amdgcn_perm(mine, other, upper ? 0x07060302u : 0x01000504u);
+    if constexpr (J == 8) return __builtin_amdgcn_perm(mine, other, upper ? 0x07030501u : 0x02060004u);
     constexpr uint32_t M = tmask<J>();
     const uint32_t rot = __builtin_amdgcn_alignbit(other, other, upper ? J : 32 - J);
     const uint32_t sel = upper ? ~M : M;

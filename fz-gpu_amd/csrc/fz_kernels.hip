@@ -947,19 +947,14 @@ __device__ __forceinline__ void d2_window(const uint8_t* __restrict__ payload, c
     int32_t        dd[4][4];
     int32_t        cs[4] = {0, 0, 0, 0};
 #pragma unroll
+    // rows past the field (r >= rows) and columns past it (x0 + j >= X) hold stale codes: the
+    // sums carry them only down the column and right along the row, into positions that are
+    // never stored
     for (int r4 = 0; r4 < 4; r4++) {
         const uint32_t r = 4 * gr + r4;
-        const uint16_t* wr = win + r * kD2Cols + 4 * c;
-        if (r < rows && full) {
-            const uint2 v = *reinterpret_cast<const uint2*>(wr);
-            dd[r4][0] = decode(v.x & 0xFFFFu);
-            dd[r4][1] = decode(v.x >> 16);
-            dd[r4][2] = decode(v.y & 0xFFFFu);
-            dd[r4][3] = decode(v.y >> 16);
-        } else {
-#pragma unroll
-            for (int j = 0; j < 4; j++) dd[r4][j] = (r < rows && x0 + j < g.X) ? decode(wr[j]) : 0;
-        }
+        const uint2    v = *reinterpret_cast<const uint2*>(win + r * kD2Cols + 4 * c);
+        decode2(v.x, dd[r4][0], dd[r4][1]);
+        decode2(v.y, dd[r4][2], dd[r4][3]);
 #pragma unroll
         for (int j = 0; j < 4; j++) cs[j] += dd[r4][j];
     }
@@ -1251,19 +1246,17 @@ __device__ __forceinline__ void d3_window(const uint8_t* __restrict__ payload, c
     // reconstruct, phase 1: columns xc .. xc + 3 of row yy, the 8 planes (z prefix in registers)
     const uint32_t yy = __umulhi(tid, mq), xq = tid - yy * Q;
     const uint32_t xc = xw0 + 4 * xq;
-    const bool     rowok = yy < rows;
     int32_t        r[8][4];
     {
         int32_t col[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int z = 0; z < 8; z++) {
-            int32_t dd[4] = {0, 0, 0, 0};
-            if (rowok && (uint32_t)z < nz) {
-                const uint2 v = *reinterpret_cast<const uint2*>(win + ((uint32_t)z * R + yy) * C + 4 * xq);
-                const uint32_t c4[4] = {v.x & 0xFFFFu, v.x >> 16, v.y & 0xFFFFu, v.y >> 16};
-#pragma unroll
-                for (int j = 0; j < 4; j++) dd[j] = xc + j < g.X ? decode(c4[j]) : 0;
-            }
+            // planes, rows and columns past the field hold stale codes: the sums carry them
+            // only forward in z, down in y and right in x, into positions that are never stored
+            int32_t     dd[4];
+            const uint2 v = *reinterpret_cast<const uint2*>(win + ((uint32_t)z * R + yy) * C + 4 * xq);
+            decode2(v.x, dd[0], dd[1]);
+            decode2(v.y, dd[2], dd[3]);
             int32_t p[4];
 #pragma unroll
             for (int j = 0; j < 4; j++) {
@@ -1419,15 +1412,14 @@ static uint32_t d3_slots(uint32_t Q, uint32_t R)
 }
 
 // Measured against decode + reconstruct (back-to-back calls, waves fields at eb 1e-3): the fused
-// form wins on every field with X <= 256 (128 x 128 x 512: 19.4 against 28.2 us, 256^3: 29.1
-// against 38.3 us).  Wider fields win when their windows fill one round of the device with
-// at least 1.5 windows per CU (270 x 300 x 100, 494 windows at 768 slots: 29.4 / 41.3 us;
-// 480 x 300 x 100: 29.6 / 36.8; 400 x 400 x 64: 28.3 / 34.2) or take three rounds or more
-// (2000 x 300 x 100: 104.7 / 122.8 us, 1024 x 1024 x 64: 98.7 / 130.1), and lose with fewer
-// windows, whose latency is then the kernel's time (354 x 200 x 90, 300 windows: 30.6 / 27.7
-// us), and in between, where a second round holds only part of the device and its windows
-// start in step (300^3, 1444 windows at 768 slots: 61.5 / 51.3 us; 500 x 500 x 100: 53.8 /
-// 50.5; 1000 x 1000 x 20: 42.6 / 44.0).
+// form wins on every field with X <= 256 (128 x 128 x 512: 18.3 against 28.0 us, 256^3: 26.3
+// against 38.6 us, 130^3: 18.0 / 18.3).  Wider fields win when their windows take up to 1.75
+// rounds of the device's window slots (270 x 300 x 100, 494 windows at 768 slots: 26.9 / 41.2
+// us; 1000 x 1000 x 20, 1.46 rounds: 39.0 / 43.9; 500 x 500 x 100, 1.6 rounds: 48.6 / 50.3) or
+// three rounds or more (2000 x 300 x 100: 101.0 / 121.2 us, 1024 x 1024 x 64: 94.1 / 127.7), and
+// lose in between, where the last round holds only part of the device and the windows of a
+// round run in step (300^3, 1.88 rounds: 55.4 / 51.6 us; 700 x 500 x 60, 1.97: 50.1 / 46.7;
+// 600 x 600 x 100, 2.54: 80.7 / 58.9).
 bool decode_recon_3d_pays(const Geom& g)
 {
     if (g.ndim != 3) return false;
@@ -1436,7 +1428,7 @@ bool decode_recon_3d_pays(const Geom& g)
     d3_shape(g, Q, R, false);
     const uint64_t C = 4 * Q, rx = (g.X + C - 1) / C;
     const uint64_t nwin = rx * ((g.Y + R - 1) / R) * ((g.Z + 7) / 8), slots = d3_slots(Q, R);
-    return (2 * nwin >= 3ull * device_cus() && nwin <= slots) || nwin >= 3 * slots;
+    return 4 * nwin <= 7 * slots || nwin >= 3 * slots;
 }
 
 hipError_t launch_decode_recon_3d(const uint8_t* payload, const uint32_t* tile_excl, const uint32_t* chunk_sums,

@@ -789,6 +789,19 @@ __device__ __forceinline__ int32_t decode(uint32_t c)
     return (c & 0x8000u) ? -m : m;
 }
 
+// both codes of a word (low half first) -> deltas, on int16 pairs: m = -1 where the sign bit
+// is set, d = (magnitude ^ m) - m per half (no half overflows: magnitudes are <= 32767); five
+// or six instructions for the pair instead of about ten
+__device__ __forceinline__ void decode2(uint32_t w, int32_t& lo, int32_t& hi)
+{
+    typedef short v2s __attribute__((ext_vector_type(2)));
+    const v2s      m = __builtin_bit_cast(v2s, w) >> (v2s){15, 15};
+    const v2s      x = __builtin_bit_cast(v2s, (w & 0x7FFF7FFFu) ^ __builtin_bit_cast(uint32_t, m));
+    const uint32_t d = __builtin_bit_cast(uint32_t, (v2s)(x - m));
+    lo = (int32_t)(int16_t)(d & 0xFFFFu);
+    hi = (int32_t)d >> 16;
+}
+
 // Scatter a tile's nonzero blocks back (block i at payload + 4W + 16 (excl + rank)) and
 // unshuffle, in one wave (64 threads): lane l ends with words 4 (l + 64 v) .. + 3 of the tile in
 // words[v], v < 4.  Lane l owns blocks 4l .. 4l + 3
@@ -883,7 +896,9 @@ __device__ __forceinline__ void decode_recon_1d_w(const uint8_t* __restrict__ pa
     for (int v = 0; v < 4; v++) {
         int32_t s[8];
 #pragma unroll
-        for (int i = 0; i < 8; i++) s[i] = decode((words[v][i >> 1] >> (16 * (i & 1))) & 0xFFFFu) + (i ? s[i - 1] : 0);
+        for (int i = 0; i < 8; i += 2) decode2(words[v][i >> 1], s[i], s[i + 1]);
+#pragma unroll
+        for (int i = 1; i < 8; i++) s[i] += s[i - 1];
         int32_t t = s[7];
 #pragma unroll
         for (int dd = 1; dd < 32; dd <<= 1) {
@@ -923,10 +938,8 @@ __device__ __forceinline__ int32_t delta1(const int32_t* __restrict__ c, size_t 
 __device__ __forceinline__ void delta4(const uint16_t* __restrict__ c, size_t p, int32_t d[4])
 {
     const uint2 v = *reinterpret_cast<const uint2*>(c + p);
-    d[0] = decode(v.x & 0xFFFFu);
-    d[1] = decode(v.x >> 16);
-    d[2] = decode(v.y & 0xFFFFu);
-    d[3] = decode(v.y >> 16);
+    decode2(v.x, d[0], d[1]);
+    decode2(v.y, d[2], d[3]);
 }
 __device__ __forceinline__ void delta4(const int32_t* __restrict__ c, size_t p, int32_t d[4])
 {
@@ -942,10 +955,7 @@ __device__ __forceinline__ void delta16(const uint16_t* __restrict__ c, size_t p
     const uint4 b = *reinterpret_cast<const uint4*>(c + p + 8);
     const uint32_t wd[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-        d[2 * j] = decode(wd[j] & 0xffffu);
-        d[2 * j + 1] = decode(wd[j] >> 16);
-    }
+    for (int j = 0; j < 8; j++) decode2(wd[j], d[2 * j], d[2 * j + 1]);
 }
 __device__ __forceinline__ void delta16(const int32_t* __restrict__ c, size_t p, int32_t d[16])
 {
