@@ -939,7 +939,9 @@ __device__ __forceinline__ uint32_t xstage(uint32_t mine, uint32_t other, bool u
     constexpr uint32_t M = tmask<J>();
     const uint32_t rot = __builtin_amdgcn_alignbit(other, other, upper ? J : 32 - J);
     const uint32_t sel = upper ? ~M : M;
-    return (mine & sel) | (rot & ~sel);
+    // mine where sel is set, else rot: one three-input bit operation (truth table S0 ? S1 : S2,
+    // operand bytes 0xF0 / 0xCC / 0xAA)
+    return __builtin_amdgcn_bitop3_b32(sel, mine, rot, 0xCA);
 }
 
 template <int J>
