@@ -756,7 +756,8 @@ constexpr uint32_t kD2DirectMaxT = 4 * kD2Threads;
 constexpr uint32_t kQgPitch = 5;
 
 // Window wi of a 2D field (all kD2Threads threads of the workgroup); tile_excl / chunk_sums as
-// launch_tile_offsets gives them, or tile_excl = nullptr for the direct offsets.
+// launch_tile_offsets gives them, or (DIRECT) the window finds its tiles' offsets itself.
+template <bool DIRECT>
 __device__ __forceinline__ void d2_window(const uint8_t* __restrict__ payload, const uint32_t* __restrict__ tile_excl,
                                           const uint32_t* __restrict__ chunk_sums, const uint16_t* __restrict__ tail,
                                           const Geom& g, float ebx2, float* __restrict__ out, uint32_t rx, uint32_t wi)
@@ -769,6 +770,7 @@ __device__ __forceinline__ void d2_window(const uint8_t* __restrict__ payload, c
     __shared__ uint32_t wtot[kD2Threads / 64], qtab[kD2MaxQ], nq_s;  // the QGs to decode
     __shared__ uint32_t toff[kD2DirectTiles];         // direct offsets: blocks before tiles t_lo ..
     __shared__ uint4    tflg[2 * kD2DirectTiles];     // ... and their 8 bitflag words each
+    __shared__ uint32_t twoff[8 * kD2DirectTiles];    // ... and the blocks before each of those words
     const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const uint32_t bx = wi % rx, by = wi / rx;
     const uint32_t xw0 = bx * kD2Cols, y0 = by * 16;
@@ -776,11 +778,11 @@ __device__ __forceinline__ void d2_window(const uint8_t* __restrict__ payload, c
     // without tile offsets (tile_excl null: small fields) the window finds the block offsets of
     // its tiles t_lo .. from the bitflags: the popcount of every flag word before tile t_lo (at
     // most 8 uint4 loads per thread, issued together), then a scan over the window's own tiles
-    const bool     direct = tile_excl == nullptr;
+    constexpr bool direct = DIRECT;
     const uint32_t t_lo = min((y0 * g.X + xw0) >> 11, g.T);
     // set-up: the chunk prefix (waves 0-3) or the direct prefix (all waves), and the QG list (wave 0)
     uint32_t cv = 0, cincl = 0;
-    if (direct) {
+    if constexpr (direct) {
         const uint4*   bf4 = reinterpret_cast<const uint4*>(payload);
         const uint32_t n4 = 2 * t_lo;  // uint4s of flag words before tile t_lo (<= 8 kD2Threads)
         uint4          v[8];
@@ -812,7 +814,7 @@ __device__ __forceinline__ void d2_window(const uint8_t* __restrict__ payload, c
         if (lane == 63) nq_s = incl;
     }
     __syncthreads();
-    if (direct) {
+    if constexpr (direct) {
         if (wv == 0) {  // lane i: tile t_lo + i (the window's tiles span at most kD2DirectTiles)
             uint32_t base = 0;
 #pragma unroll
@@ -825,9 +827,17 @@ __device__ __forceinline__ void d2_window(const uint8_t* __restrict__ payload, c
             }
             tflg[2 * lane] = a;
             tflg[2 * lane + 1] = b;
-            const uint32_t pc = __popc(a.x) + __popc(a.y) + __popc(a.z) + __popc(a.w) + __popc(b.x) + __popc(b.y) +
-                                __popc(b.z) + __popc(b.w);
-            toff[lane] = base + wave_incl_scan(pc) - pc;
+            const uint32_t fw8[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+            uint32_t       pw[8], pc = 0;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                pw[j] = pc;
+                pc += __popc(fw8[j]);
+            }
+            const uint32_t o = base + wave_incl_scan(pc) - pc;
+            toff[lane] = o;
+#pragma unroll
+            for (int j = 0; j < 8; j++) twoff[8 * lane + j] = o + pw[j];
         }
     } else if (tid < kTileLocalMaxChunks) {
         uint32_t pre = 0;
@@ -850,26 +860,32 @@ __device__ __forceinline__ void d2_window(const uint8_t* __restrict__ payload, c
             q[b] = live[b] ? qtab[i] : 0u;
             const uint32_t t = q[b] >> 3;
             const bool     in = live[b] && t < g.T;
+            // direct: the flag word of this lane's block (word hl / 4) and the blocks before that
+            // word, from the window's tables; else lane hl % 8's word of the tile, scanned below
             fw[b] = !in ? 0u
-                    : direct ? reinterpret_cast<const uint32_t*>(tflg)[8 * (t - t_lo) + (hl & 7u)]
+                    : direct ? reinterpret_cast<const uint32_t*>(tflg)[8 * (t - t_lo) + (hl >> 2)]
                              : bflags[8 * t + (hl & 7u)];
-            ex[b] = !in ? 0u : direct ? toff[t - t_lo] : tile_excl[t] + (chunk_sums ? cpre[t / kWG] : 0u);
+            ex[b] = !in ? 0u : direct ? twoff[8 * (t - t_lo) + (hl >> 2)] : tile_excl[t] + (chunk_sums ? cpre[t / kWG] : 0u);
         }
         uint4 blk[kD2Batch];
         bool  nzq[kD2Batch];  // some block of the wave's two QGs is nonzero (uniform)
 #pragma unroll
         for (int b = 0; b < (int)kD2Batch; b++) {
-            // flag and rank of block 8 hl + m (flag word hl / 4, bit 8 (hl % 4) + m); word j is
-            // held by the half's lanes with hl % 8 == j
-            const uint32_t pc = __popc(fw[b]);
-            uint32_t       incl = pc;
+            // flag and rank of block 8 hl + m (flag word hl / 4, bit 8 (hl % 4) + m); without
+            // the direct tables word j is held by the half's lanes with hl % 8 == j
+            uint32_t wj = fw[b], ej = 0;
+            if constexpr (!direct) {
+                const uint32_t pc = __popc(fw[b]);
+                uint32_t       incl = pc;
 #pragma unroll
-            for (int d = 1; d < 8; d <<= 1) {
-                const uint32_t o = __shfl_up(incl, d, 8);
-                if ((hl & 7u) >= (uint32_t)d) incl += o;
+                for (int d = 1; d < 8; d <<= 1) {
+                    const uint32_t o = __shfl_up(incl, d, 8);
+                    if ((hl & 7u) >= (uint32_t)d) incl += o;
+                }
+                const uint32_t src = (lane & 32u) + (hl >> 2);
+                wj = __shfl(fw[b], src, 64);
+                ej = __shfl(incl - pc, src, 64);
             }
-            const uint32_t src = (lane & 32u) + (hl >> 2);
-            const uint32_t wj = __shfl(fw[b], src, 64), ej = __shfl(incl - pc, src, 64);
             const uint32_t bit = 8 * (hl & 3u) + (q[b] & 7u);
             const bool     f = (wj >> bit) & 1u;
             nzq[b] = ballot64(f) != 0;
@@ -994,6 +1010,7 @@ __device__ __forceinline__ void d2_window(const uint8_t* __restrict__ payload, c
     }
 }
 
+template <bool DIRECT>
 __global__ __launch_bounds__(kD2Threads) void k_decode_recon_2d(const uint8_t* __restrict__ payload,
                                                                 const uint32_t* __restrict__ tile_excl,
                                                                 const uint32_t* __restrict__ chunk_sums,
@@ -1002,7 +1019,7 @@ __global__ __launch_bounds__(kD2Threads) void k_decode_recon_2d(const uint8_t* _
 {
     // XCD-sliced order: the windows that share a tile's blocks (neighbours along x, and along y
     // for narrow fields) read them through one L2
-    d2_window(payload, tile_excl, chunk_sums, tail, g, ebx2, out, rx, xcd_tile(blockIdx.x, gridDim.x));
+    d2_window<DIRECT>(payload, tile_excl, chunk_sums, tail, g, ebx2, out, rx, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // With X % 4 == 0 a lane puts its codes into the window with one or two wide LDS stores, and
@@ -1025,8 +1042,8 @@ hipError_t launch_decode_recon_2d(const uint8_t* payload, const uint32_t* tile_e
     const uint32_t rx = (g.X + kD2Cols - 1) / kD2Cols;
     const uint64_t n = (uint64_t)rx * ((g.Y + 15) / 16);
     if (n > 0xFFFFFFFFull) return hipErrorInvalidConfiguration;
-    FZ_LAUNCH(k_decode_recon_2d<<<(uint32_t)n, kD2Threads, 0, s>>>(payload, tile_excl, chunk_sums, tail, g, ebx2, out,
-                                                                   rx));
+    if (!tile_excl) FZ_LAUNCH(k_decode_recon_2d<true><<<(uint32_t)n, kD2Threads, 0, s>>>(payload, nullptr, nullptr, tail, g, ebx2, out, rx));
+    else FZ_LAUNCH(k_decode_recon_2d<false><<<(uint32_t)n, kD2Threads, 0, s>>>(payload, tile_excl, chunk_sums, tail, g, ebx2, out, rx));
     return hipGetLastError();
 }
 
