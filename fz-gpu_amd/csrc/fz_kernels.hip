@@ -1055,9 +1055,8 @@ hipError_t launch_decode_recon_2d(const uint8_t* payload, const uint32_t* tile_e
 // two <= 32) and R = 256 / Q.  Wider rows: R = 8, and a row takes the fewest windows of at most
 // 512 columns, Q the quads of its share rounded up to a multiple of 8 (whole waves); so a
 // window holds within 32 columns of what it covers of the field (d3_shape: a single field with
-// X <= 256 takes Q = 64).  The template NT (256, 512 or
-// 1024) bounds R Q, and sizes the QG list and the staging buffers; the window's memory is
-// dynamic (64 R Q bytes).
+// X <= 256 takes Q = 64).  The template NT (256, 512 or 1024) bounds R Q, and sizes the QG
+// list and the staging buffers; the window's memory is dynamic (64 R Q bytes).
 //
 // Decode, as in k_decode_recon_2d.  The window's row segments list the quad groups they
 // touch, each once: with whole rows one segment per plane (a plane's rows are consecutive in
