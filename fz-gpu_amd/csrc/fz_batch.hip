@@ -400,7 +400,7 @@ __global__ __launch_bounds__(64) void k_batch_decode(BatchDev d, uint32_t nf)
     const Geom     g = d.fields[f].g;
     const uint32_t u0 = d.ustart[f], tile = gu - u0;
     const BPtrs    q = d.ptrs[f];
-    if (d.fields[f].wnt) return;  // decoded by the fused 3D window kernel (uniform)
+    if (d.fields[f].wnt) return;  // decoded by a fused window kernel (uniform)
     if (g.ndim == 1) {  // uniform
         decode_recon_1d_w(static_cast<const uint8_t*>(q.src), g.mode == 0 ? q.tail : nullptr, g, tile,
                           tile < g.T ? d.texcl[gu] : 0u, (float)q.param, static_cast<float*>(q.dst));
@@ -454,10 +454,12 @@ hipError_t launch_batch_decompress(const BatchDev& d, const BatchShape& b, hipEv
         else
             e = launch_pair(k_batch_recon<3>, dim3(n), dim3(kWG), s, ev, 4, d, b.r_lo[3]);
     }
-    // the 3D fields whose windows decode and reconstruct in one kernel (one launch per window size)
+    // the 3D fields whose windows decode and reconstruct in one kernel (one launch per window
+    // size), then the 2D fields whose rows are a multiple of 4 wide (one launch)
     const bool timed = ev && ev[10] && ev[11];
     if (e == hipSuccess && timed) e = hipEventRecord(ev[10], s);
     if (e == hipSuccess) e = launch_batch_fused3d(d, b, s);
+    if (e == hipSuccess) e = launch_batch_fused2d(d, b, s);
     if (e == hipSuccess && timed) e = hipEventRecord(ev[11], s);
     return e;
 }

@@ -1015,6 +1015,7 @@ fz_status fz_batch_plan_create(fz_batch_plan* plan, const fz_field_shape* shapes
     for (int c = 0; c < kNumChunks; c++) p->pstart[c].resize(nfields + 1);
     uint64_t u = 0, r = 0, parts[kNumChunks] = {};
     std::vector<std::pair<uint32_t, uint64_t>> fused[kWinClasses];  // (field, windows) per window size
+    std::vector<std::pair<uint32_t, uint64_t>> fused2d;  // (field, windows): fused 2D
     BatchShape& b = p->shape;
     b.nf = nfields;
     for (uint32_t k = 0; k < nfields; k++) {
@@ -1030,6 +1031,15 @@ fz_status fz_batch_plan_create(fz_batch_plan* plan, const fz_field_shape* shapes
             F.rdims[0] = (g.X + kRecon2DCols - 1) / kRecon2DCols;
             F.rdims[1] = (g.Y + 15) / 16;
             F.rdims[2] = 1;
+            uint32_t rx, ry;
+            if (mode != FZ_MODE_OUTLIER && batch_fused2d(g, rx, ry)) {
+                // decoded and reconstructed by the fused 2D window kernel: no reconstruct workgroups
+                F.wnt = 1024;
+                F.wdims[0] = rx;
+                F.wdims[1] = ry;
+                F.rdims[0] = F.rdims[1] = F.rdims[2] = 0;
+                fused2d.push_back({k, (uint64_t)rx * ry});
+            }
         } else {
             F.rq = recon3d_q(g.X, g.Y);
             F.rdims[0] = (g.X + 4 * F.rq - 1) / (4 * F.rq);
@@ -1075,7 +1085,7 @@ fz_status fz_batch_plan_create(fz_batch_plan* plan, const fz_field_shape* shapes
     }
     p->ustart[nfields] = (uint32_t)u;
     p->rstart[nfields] = (uint32_t)r;
-    uint64_t w = 0;  // the fused 3D windows, by window size
+    uint64_t w = 0;  // the fused 3D windows, by window size, then the fused 2D ones
     for (int c = 0; c < kWinClasses; c++) {
         b.w_lo[c] = (uint32_t)w;
         for (const auto& fw : fused[c]) {
@@ -1084,6 +1094,12 @@ fz_status fz_batch_plan_create(fz_batch_plan* plan, const fz_field_shape* shapes
         }
         b.w_hi[c] = (uint32_t)w;
     }
+    b.w2_lo = (uint32_t)w;  // the fused 2D windows after them
+    for (const auto& fw : fused2d) {
+        p->fields[fw.first].wstart = (uint32_t)w;
+        w += fw.second;
+    }
+    b.w2_hi = (uint32_t)w;
     if (w > 0xFFFFFFF0ull) {
         delete p;
         return FZ_ERR_CAPACITY;

@@ -116,6 +116,10 @@ bool       batch_fused3d(const Geom& g, uint32_t& Q, uint32_t& R, uint32_t& rx, 
 struct BatchDev;
 struct BatchShape;
 hipError_t launch_batch_fused3d(const BatchDev& d, const BatchShape& b, hipStream_t s);
+// Batched decompress: whether a 2D field is decoded and reconstructed by the fused window
+// kernel (then its windows along x and y), and the launch over every such window
+bool       batch_fused2d(const Geom& g, uint32_t& rx, uint32_t& ry);
+hipError_t launch_batch_fused2d(const BatchDev& d, const BatchShape& b, hipStream_t s);
 // reconstruct from u16 codes whose positions flagged in `bits` take their delta from `over`
 hipError_t launch_reconstruct_outlier_src(const uint16_t* codes, const uint32_t* bits, const int32_t* over,
                                           float* out, const Geom& g, float ebx2, hipStream_t s);
@@ -164,7 +168,7 @@ struct BField {        // per field, uploaded once per workspace (128 B)
     uint32_t ustart;   // first global unit
     uint32_t rdims[3]; // reconstruct workgroups along x, y, z
     uint32_t rq;       // 3D: quads per row of its reconstruct workgroups (recon3d_q)
-    uint32_t wnt;      // 3D decoded and reconstructed in one kernel: threads per window (0: not)
+    uint32_t wnt;      // decoded and reconstructed in one kernel: threads per window (0: not)
     uint32_t wq;       // ... quads per window row
     uint32_t wr;       // ... rows per window
     uint32_t wdims[2]; // ... windows along x, y
@@ -224,6 +228,7 @@ struct BatchShape {
     uint32_t wave_tile[4];      // tile layout per dimensionality: 1 one wave per tile, 0 four
     uint32_t gather_waves;      // units per gather workgroup
     uint32_t w_lo[kWinClasses], w_hi[kWinClasses];  // fused 3D windows of 64 (c + 1) threads (decompress)
+    uint32_t w2_lo, w2_hi;                          // fused 2D windows (decompress)
 };
 
 // A group of consecutive fields (sorted order) and the units / range partials they own.

@@ -1512,6 +1512,37 @@ hipError_t launch_batch_fused3d(const BatchDev& d, const BatchShape& b, hipStrea
     return hipSuccess;
 }
 
+// Batched decompress: the fused 2D windows of every field whose rows are a multiple of 4 wide (a
+// lane's codes go into the window with wide LDS stores; then the fused form beats decode +
+// reconstruct at every size measured), a workgroup per window over the per-window field
+// table, with the per-field scan's tile offsets (global within the field).
+__global__ __launch_bounds__(kD2Threads) void k_batch_fused2d(BatchDev d, uint32_t w_lo)
+{
+    const uint32_t w = w_lo + xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t f = d.wfield[w];
+    const BField&  F = d.fields[f];
+    const Geom     g = F.g;
+    const BPtrs    q = d.ptrs[f];
+    d2_window<false>(static_cast<const uint8_t*>(q.src), d.texcl + F.ustart, nullptr, g.mode == 0 ? q.tail : nullptr, g,
+                     (float)q.param, static_cast<float*>(q.dst), F.wdims[0], w - F.wstart);
+}
+
+bool batch_fused2d(const Geom& g, uint32_t& rx, uint32_t& ry)
+{
+    if (g.ndim != 2 || (g.X & 3u) != 0) return false;
+    rx = (g.X + kD2Cols - 1) / kD2Cols;
+    ry = (g.Y + 15) / 16;
+    return true;
+}
+
+hipError_t launch_batch_fused2d(const BatchDev& d, const BatchShape& b, hipStream_t s)
+{
+    const uint32_t n = b.w2_hi - b.w2_lo;
+    if (n == 0) return hipSuccess;
+    FZ_LAUNCH(k_batch_fused2d<<<n, kD2Threads, 0, s>>>(d, b.w2_lo));
+    return hipGetLastError();
+}
+
 template <typename SRC>
 __global__ __launch_bounds__(kWG) void k_reconstruct_1d(const SRC codes, float* __restrict__ out, Geom g, float ebx2)
 {
