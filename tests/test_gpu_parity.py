@@ -313,7 +313,7 @@ def test_rounding_ties_on_the_packed_paths(dims, sign, wave_tile):
 
 # ---------------------------------------------------------------- the error bound, without slack
 
-# max|x - x'| <= eb * (max - min) holds exactly on these BASELINE workloads.  On the others the
+# max|x - x'| <= eb * (max - min) holds exactly on these full-size workloads.  On the others the
 # reference's own binary32 arithmetic exceeds it by less than 4 * 2^-24 * max|x|: the prequant
 # product fl(x * fl(1/(2 eb_abs))) and the dequant product fl(s * fl(2 eb_abs)) each round by up
 # to 2^-24 relative, so a value at |x| ~ 1000 eb_abs can land 2^-24 * |x| outside the cell; the
