@@ -252,19 +252,30 @@ __global__ __launch_bounds__(kGatherThreads) void k_scan_gather(const uint32_t* 
         sh_off[tid] = wave_prefix(sh_w, run) + incl - cnt;
         __syncthreads();
         uint8_t* out = payload + 4ull * W + 16ull * ((uint64_t)excl + base);
+        // tiles of this pass (>= 1) and the halvings that narrow [0, ntl) to one tile
+        const uint32_t ntl = min((uint32_t)kGatherThreads, t_hi - r0);
+        const int      steps = 32 - __clz(ntl - 1);
         for (uint32_t j0 = 0; j0 < run; j0 += 4 * kGatherThreads) {
+            // last tile of the pass whose first block is <= j, four searches in step
+            uint32_t lo[4], hi[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                lo[u] = 0;
+                hi[u] = ntl - 1;
+            }
+            for (int it = 0; it < steps; it++) {
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const uint32_t j = j0 + tid + u * kGatherThreads, mid = (lo[u] + hi[u] + 1) >> 1;
+                    if (sh_off[mid] <= j) lo[u] = mid;
+                    else hi[u] = mid - 1;
+                }
+            }
             uint64_t src[4];  // staging byte offsets pass 2^32 beyond 2^20 tiles
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const uint32_t j = j0 + tid + u * kGatherThreads;
-                uint32_t lo = 0, hi = kGatherThreads - 1;  // last tile of the run whose first block is <= j
-#pragma unroll
-                for (int it = 0; it < 10; it++) {
-                    const uint32_t mid = (lo + hi + 1) >> 1;
-                    if (sh_off[mid] <= j) lo = mid;
-                    else hi = mid - 1;
-                }
-                src[u] = (uint64_t)(r0 + lo) * 4096u + 16u * (j - sh_off[lo]);
+                src[u] = (uint64_t)(r0 + lo[u]) * 4096u + 16u * (j - sh_off[lo[u]]);
             }
             uint4 v[4];
 #pragma unroll
